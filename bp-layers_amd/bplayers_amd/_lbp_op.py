@@ -1,0 +1,123 @@
+"""The min-sum LBP op behind the three ``pytorch_cuda_lbp_op`` modules.
+
+Reference interface (ops/lbp_{stereo,semantic_pw,semantic_pw_pixel}/src/lbp.cpp):
+  forward_minsum(cost, jump, edge, messages, delta) -> [messages, messages_argmin, message_scale]
+  backward_minsum(cost, jump, edge, in_grad, messages, messages_argmin, message_scale)
+      -> [grad_unary, grad_pairwise, grad_edge, grad_messages]
+  (the semantic-pixel build's backward has no ``jump`` argument).
+
+Behaviour kept from the reference: inputs must be CUDA, contiguous (and float32);
+``messages`` is updated in place and returned; the backward's grad_messages is a copy of
+in_grad (the reference semantics, ``mode="reference"``).
+
+Deliberate differences (all internal to the autograd Functions that carry these tensors):
+  * messages_argmin is uint8 [N,4,H,W,C]: the winning sender label, or 255 when the value
+    came from the truncation term (whose label, the global argmin, is kept per receiver);
+  * message_scale is float32 [N,4,H,W] = sum_c exp(msg) as in the reference; its storage
+    additionally carries the per-receiver global argmins (uint8) after the floats, which is
+    how backward_minsum receives them through the reference's argument list;
+  * every batch element is processed; batch-shared pairwise gradients are summed;
+  * ``mode="exact"`` (keyword) returns the exact adjoint w.r.t. the input messages;
+  * ``out=`` (keyword) writes the messages out of place, leaving ``messages`` untouched.
+"""
+import torch
+
+from . import _native as nat
+
+
+def _alloc_scale_and_gargmin(N, H, W, device):
+    n = N * 4 * H * W
+    buf = torch.empty(n * 5, dtype=torch.uint8, device=device)
+    scale = buf[: n * 4].view(torch.float32).view(N, 4, H, W)
+    garg = buf[n * 4:].view(N, 4, H, W)
+    return scale, garg
+
+
+def _gargmin_of(message_scale):
+    n = message_scale.numel()
+    st = message_scale.untyped_storage()
+    if message_scale.dtype != torch.float32 or message_scale.storage_offset() != 0 or st.nbytes() < 5 * n:
+        raise RuntimeError("message_scale was not produced by this library's forward_minsum")
+    g = torch.empty(0, dtype=torch.uint8, device=message_scale.device)
+    g.set_(st, 4 * n, (n,))
+    return g.view(message_scale.shape)
+
+
+def _check_common(kind, cost, jump, edge, messages):
+    nat.check_input(cost, "cost")
+    if jump is not None:
+        nat.check_input(jump, "jump")
+    nat.check_input(edge, "edge")
+    nat.check_input(messages, "messages")
+    if cost.dim() != 4:
+        raise RuntimeError(f"cost must be [N,H,W,C], got {tuple(cost.shape)}")
+    N, H, W, C = cost.shape
+    if tuple(messages.shape) != (N, 4, H, W, C):
+        raise RuntimeError(f"messages must be {(N, 4, H, W, C)}, got {tuple(messages.shape)}")
+    if tuple(edge.shape) != (N, 4, H, W):
+        raise RuntimeError(f"edge must be {(N, 4, H, W)}, got {tuple(edge.shape)}")
+    for t, nm in ((jump, "jump"), (edge, "edge"), (messages, "messages")):
+        if t is not None and t.device != cost.device:
+            raise RuntimeError(f"{nm} is on {t.device}, cost on {cost.device}")
+    K, Nj = 0, 1
+    if kind == nat.LBP_STEREO:
+        if jump.dim() != 5 or jump.shape[1] != 4 or tuple(jump.shape[3:]) != (H, W) or jump.shape[0] not in (1, N):
+            raise RuntimeError(f"stereo jump must be [N|1,4,K,H,W], got {tuple(jump.shape)}")
+        K, Nj = int(jump.shape[2]), int(jump.shape[0])
+    elif kind == nat.LBP_SEMANTIC_GLOBAL:
+        if tuple(jump.shape[1:]) != (4, C, C) or jump.shape[0] < 1:
+            raise RuntimeError(f"semantic jump must be [1,4,C,C], got {tuple(jump.shape)}")
+    elif jump is not None:
+        if tuple(jump.shape) != (4, H, W, C, C):
+            raise RuntimeError(f"pixel jump must be [4,H,W,C,C], got {tuple(jump.shape)}")
+    return N, H, W, C, K, Nj
+
+
+def forward_minsum(kind, cost, jump, edge, messages, delta, out=None):
+    N, H, W, C, K, Nj = _check_common(kind, cost, jump, edge, messages)
+    msg_out = messages
+    if out is not None:
+        nat.check_input(out, "out")
+        if out.shape != messages.shape:
+            raise RuntimeError("out must have the shape of messages")
+        msg_out = out
+    argmin = torch.empty((N, 4, H, W, C), dtype=torch.uint8, device=cost.device)
+    scale, garg = _alloc_scale_and_gargmin(N, H, W, cost.device)
+    delta = max(0, min(int(delta), 65535))
+    nat.call("bpl_lbp_fwd", kind, nat.ptr(cost), nat.ptr(jump), K, Nj, nat.ptr(edge), nat.ptr(messages),
+             nat.ptr(msg_out), nat.ptr(argmin), nat.ptr(garg), nat.ptr(scale), N, H, W, C, delta,
+             nat.stream_of(cost))
+    return [msg_out, argmin, scale]
+
+
+def backward_minsum(kind, cost, jump, edge, in_grad, messages, messages_argmin, message_scale,
+                    mode="reference", delta=None):
+    N, H, W, C, K, Nj = _check_common(kind, cost, jump, edge, messages)
+    nat.check_input(in_grad, "in_grad")
+    if in_grad.shape != messages.shape:
+        raise RuntimeError("in_grad must have the shape of messages")
+    if messages_argmin.dtype != torch.uint8 or messages_argmin.shape != messages.shape:
+        raise RuntimeError("messages_argmin was not produced by this library's forward_minsum")
+    garg = _gargmin_of(message_scale)
+    m = {"reference": nat.GRAD_REFERENCE, "exact": nat.GRAD_EXACT}[mode]
+    dev = cost.device
+    grad_cost = torch.empty_like(cost)
+    if kind == nat.LBP_STEREO:
+        grad_jump = torch.empty_like(jump)
+        gj_ret = grad_jump
+    elif kind == nat.LBP_SEMANTIC_GLOBAL:
+        grad_jump = torch.empty((1, 4, C, C), dtype=torch.float32, device=dev)
+        # reference shape [N,4,C,C] with every batch element accumulated into index 0
+        gj_ret = torch.cat([grad_jump, torch.zeros((N - 1, 4, C, C), device=dev)]) if N > 1 else grad_jump
+    else:
+        grad_jump = torch.empty((4, H, W, C, C), dtype=torch.float32, device=dev)
+        gj_ret = grad_jump
+    grad_edge = torch.empty((N, 4, H, W), dtype=torch.float32, device=dev)
+    grad_msg = torch.empty_like(messages)
+    ws_bytes = int(nat.lib().bpl_lbp_bwd_workspace_size(kind, N, H, W, C))
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+    nat.call("bpl_lbp_bwd", kind, m, nat.ptr(jump), K, Nj, nat.ptr(edge), nat.ptr(in_grad), nat.ptr(messages),
+             nat.ptr(messages_argmin), nat.ptr(garg), nat.ptr(grad_cost), nat.ptr(grad_jump),
+             nat.ptr(grad_edge), nat.ptr(grad_msg), nat.ptr(ws), ws_bytes, N, H, W, C,
+             -1 if delta is None else int(delta), nat.stream_of(cost))
+    return [grad_cost, gj_ret, grad_edge, grad_msg]

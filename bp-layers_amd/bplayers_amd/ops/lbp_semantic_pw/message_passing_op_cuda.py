@@ -1,0 +1,80 @@
+"""Semantic min-sum LBP with a global C x C pairwise matrix per direction
+(reference ops/lbp_semantic_pw/message_passing_op_cuda.py:23-120)."""
+import torch
+import torch.nn as nn
+
+from ... import pytorch_cuda_lbp_op_semantic_pw as lbp
+from ...corenet import TemperatureSoftmin
+from ..lbp_stereo.message_passing_op_cuda import construct_pw_tensor  # noqa: F401  (same helper)
+
+
+def pack_pairwise(jump, single_pw):
+    """pw_mat [1,2,C,C] -> jump_in [1,4,C,C]: LEFT=M_h, RIGHT=M_h^T, UP=M_v, DOWN=M_v^T; with
+    single_pw all four are jump[0,0] (message_passing_op_cuda.py:27-40)."""
+    if not bool(single_pw):
+        j = torch.stack([jump[0, 0], jump[0, 0].t(), jump[0, 1], jump[0, 1].t()])[None]
+    else:
+        j = torch.stack([jump[0, 0]] * 4)[None]
+    return j.contiguous()
+
+
+def unpack_pairwise_grad(grad_jump, single_pw):
+    """Adjoint of pack_pairwise (message_passing_op_cuda.py:57-70)."""
+    g = grad_jump[0]
+    if not bool(single_pw):
+        return torch.stack([g[0] + g[1].t(), g[2] + g[3].t()])[None]
+    return (g[0] + g[1] + g[2] + g[3])[None, None]
+
+
+class LBPMinSumFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, cost, L1, L2, edge, messages, delta, jump, single_pw, grad_mode='reference'):
+        jump_in = pack_pairwise(jump, single_pw)
+        out = None if grad_mode == 'reference' else torch.empty_like(messages)
+        messages, messages_argmin, message_scale = lbp.forward_minsum(cost, jump_in, edge, messages.contiguous(),
+                                                                      delta, out=out)
+        ctx.save_for_backward(cost, jump_in, edge, messages, messages_argmin, message_scale)
+        ctx.single_pw = bool(single_pw)
+        ctx.grad_mode = grad_mode
+        return messages
+
+    @staticmethod
+    def backward(ctx, in_grad):
+        cost, jump, edge, messages, messages_argmin, message_scale = ctx.saved_tensors
+        grad_cost, grad_jump, grad_edge, grad_message = lbp.backward_minsum(
+            cost, jump.contiguous(), edge, in_grad.contiguous(), messages, messages_argmin, message_scale,
+            mode=ctx.grad_mode)
+        return (grad_cost, None, None, grad_edge, grad_message, None, unpack_pairwise_grad(grad_jump, ctx.single_pw),
+                None, None)
+
+
+class MessagePassing(nn.Module):
+    def __init__(self, device, max_iter, num_labels, delta, mode='min-sum', grad_mode='reference'):
+        super(MessagePassing, self).__init__()
+        self.device = device
+        self.max_iter = max_iter
+        if mode != 'min-sum':
+            raise ValueError("Unknown message parsing mode " + mode)
+        self.mode = mode
+        self.grad_mode = grad_mode
+        self.L1 = nn.Parameter(torch.tensor(0.1, device=device), requires_grad=True)
+        self.L2 = nn.Parameter(torch.tensor(2.5, device=device), requires_grad=True)
+        self.softmin = TemperatureSoftmin(dim=3, init_temp=1.0, device=device)
+        self.delta = delta
+        self.rescaleT = None
+
+    def projectL1L2(self):
+        self.L2.data = torch.max(self.L1.data, self.L2.data)
+
+    def forward(self, prob_vol, edge_weights, messages, jump, single_pw=False):
+        N, H, W, C = prob_vol.shape
+        if edge_weights is None:
+            edge_weights = torch.ones((N, 4, H, W), device=prob_vol.device)
+        if self.mode != 'min-sum':
+            raise NotImplementedError("message parsing mode " + self.mode + " is currently not implemented!")
+        cost = -prob_vol
+        for it in range(self.max_iter):
+            messages = LBPMinSumFunction.apply(cost, self.L1, self.L2, edge_weights, messages, self.delta, jump,
+                                               torch.tensor(single_pw), self.grad_mode)
+        beliefs = messages.sum(dim=1) + cost
+        return self.softmin.forward(beliefs)

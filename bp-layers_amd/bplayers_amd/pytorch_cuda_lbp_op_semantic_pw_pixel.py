@@ -1,0 +1,15 @@
+"""Drop-in for the ops/lbp_semantic_pw_pixel build of the reference's ``pytorch_cuda_lbp_op`` extension
+(ops/lbp_semantic_pw_pixel/src/lbp.cpp:15-44), backed by libbpl_amd.  Semantics and deviations: see bplayers_amd._lbp_op."""
+from . import _lbp_op
+from ._native import LBP_SEMANTIC_PIXEL as KIND
+
+
+def forward_minsum(cost, jump, edge, messages, delta, *, out=None):
+    """One LBP iteration (RIGHT, LEFT, DOWN, UP); ``messages`` is updated in place unless
+    ``out`` is given.  Returns [messages, messages_argmin, message_scale]."""
+    return _lbp_op.forward_minsum(KIND, cost, jump, edge, messages, delta, out=out)
+
+
+def backward_minsum(cost, edge, in_grad, messages, messages_argmin, message_scale, *, mode="reference"):
+    """Gradients [grad_unary, grad_pairwise, grad_edge, grad_messages] of one iteration."""
+    return _lbp_op.backward_minsum(KIND, cost, None, edge, in_grad, messages, messages_argmin, message_scale, mode=mode)

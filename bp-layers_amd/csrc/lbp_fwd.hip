@@ -1,0 +1,350 @@
+// Min-sum LBP forward sweeps on gfx950.
+//
+// Semantics follow the reference kernel lbp_cuda_forward_kernel_reduction_min_sum
+// (ops/lbp_stereo/src/lbp_min_sum_kernel.cu:9-213 and its semantic twins) and the host
+// loop (:424-500); the design is our own:
+//   * one launch per phase (horizontal: RIGHT+LEFT, vertical: DOWN+UP) instead of one
+//     launch per column/row; each wave walks whole scanlines, carrying the chain message
+//     in registers;
+//   * labels live on lanes (SEG lanes per scanline, LPL labels per lane); small label
+//     counts pack 64/SEG scanlines into one wave;
+//   * reductions over labels are DPP / swizzle all-reduces, the candidate window is read
+//     from a per-segment LDS copy of h;
+//   * the next PF steps' streamed inputs (cost and the two cross messages) are prefetched
+//     into a register ring.
+#include "bpl_common.h"
+
+namespace bpl {
+namespace {
+
+struct FwdArgs {
+  const float *cost, *jump, *edge, *msg_in;
+  float *msg_out;
+  uint8_t *argmin, *gargmin;
+  float *scale;
+  int N, H, W, C, K, Nj, delta, bits, mat_lds;
+};
+
+__device__ __forceinline__ int64_t px(int n, int y, int x, int H, int W) {
+  return ((int64_t)n * H + y) * (int64_t)W + x;
+}
+__device__ __forceinline__ int64_t sl_px(int n, int s, int y, int x, int H, int W) {
+  return (((int64_t)n * 4 + s) * H + y) * (int64_t)W + x;
+}
+__device__ __forceinline__ int sat_u16(float v) { return (int)fminf(fmaxf(v, 0.f), 65535.f); }
+
+template <int KIND, int SEG, int LPL, bool VERT, int PF>
+__global__ __launch_bounds__(64) void lbp_fwd_phase(FwdArgs a) {
+  constexpr int SPW = kWave / SEG;
+  constexpr int CP = SEG * LPL;
+  __shared__ float lds_h[2][SPW][CP];
+  __shared__ float lds_p[2][SPW][16];
+  extern __shared__ float lds_m[];  // semantic global: the C x C matrix of this direction
+
+  const int H = a.H, W = a.W, C = a.C, K = a.K;
+  const int L = VERT ? W : H;               // scanlines per (n, direction)
+  const int len = VERT ? H : W;             // senders + 1 per scanline
+  const int nsteps = len - 1;
+  const int wpg = (L + SPW - 1) / SPW;
+  const int g = blockIdx.x / wpg;
+  const int wl = blockIdx.x - g * wpg;
+  const int n = g >> 1;
+  const int dsel = g & 1;
+  const int dir = VERT ? (dsel ? UP : DOWN) : (dsel ? LEFT : RIGHT);
+  const int rslot = dir == RIGHT ? LEFT : dir == LEFT ? RIGHT : dir == DOWN ? UP : DOWN;
+  const bool fwd = (dir == RIGHT || dir == DOWN);
+  const int lane = threadIdx.x;
+  const int seg = lane / SEG;
+  const int sl = lane - seg * SEG;
+  const int line = wl * SPW + seg;
+  const bool line_ok = line < L;
+  const int ln = line_ok ? line : 0;
+  const bool trunc = a.delta < C;
+  const int delta = a.delta;
+  const int nj = a.Nj > 1 ? n : 0;
+  const int s1 = VERT ? RIGHT : UP;         // cross slots, in the reference's summation order
+  const int s2 = VERT ? LEFT : DOWN;
+  const float *cross = VERT ? a.msg_out : a.msg_in;
+
+  if (KIND == BPL_LBP_SEMANTIC_GLOBAL && a.mat_lds) {
+    const float *M = a.jump + (int64_t)dir * C * C;
+    for (int i = lane; i < C * C; i += kWave) lds_m[i] = M[i];
+    __syncthreads();
+  }
+
+  auto pos_of = [&](int s) { return fwd ? s : len - 1 - s; };
+  auto yx_of = [&](int p, int &y, int &x) {
+    if (VERT) { y = p; x = ln; } else { y = ln; x = p; }
+  };
+
+  // ---- boundary: the first sender's chain slot is never written by this pass
+  float chain[LPL];
+  {
+    int y, x;
+    yx_of(pos_of(0), y, x);
+    const int64_t o = sl_px(a.N > 0 ? n : 0, rslot, y, x, H, W);
+#pragma unroll
+    for (int k = 0; k < LPL; ++k) {
+      const int c = sl + SEG * k;
+      float v = 0.f;
+      if (line_ok && c < C) {
+        v = a.msg_in[o * C + c];
+        a.msg_out[o * C + c] = v;
+        a.argmin[o * C + c] = 0;
+      }
+      chain[k] = v;
+    }
+    if (line_ok && sl == 0) {
+      if (a.scale) a.scale[o] = 0.f;
+      if (a.gargmin) a.gargmin[o] = 0;
+    }
+  }
+
+  // ---- register prefetch ring
+  float pc[PF][LPL], p1[PF][LPL], p2[PF][LPL], pp[PF];
+  auto load_step = [&](int s, float *dc, float *d1, float *d2, float &dp) {
+    int y, x;
+    yx_of(pos_of(s), y, x);
+    const int64_t cp = px(n, y, x, H, W) * C;
+    const int64_t o1 = sl_px(n, s1, y, x, H, W) * C;
+    const int64_t o2 = sl_px(n, s2, y, x, H, W) * C;
+#pragma unroll
+    for (int k = 0; k < LPL; ++k) {
+      const int c = sl + SEG * k;
+      const bool ok = c < C;
+      dc[k] = ok ? a.cost[cp + c] : 0.f;
+      d1[k] = ok ? cross[o1 + c] : 0.f;
+      d2[k] = ok ? cross[o2 + c] : 0.f;
+    }
+    const int64_t eo = sl_px(n, dir, y, x, H, W);
+    if (KIND == BPL_LBP_STEREO) {
+      if (sl < K)
+        dp = a.jump[(((int64_t)nj * 4 + dir) * K + sl) * (int64_t)H * W + (int64_t)y * W + x];
+      else
+        dp = a.edge[eo];
+    } else {
+      dp = a.edge[eo];
+    }
+  };
+
+#pragma unroll
+  for (int i = 0; i < PF; ++i) load_step(i < nsteps ? i : (nsteps > 0 ? nsteps - 1 : 0), pc[i], p1[i], p2[i], pp[i]);
+
+  for (int s0 = 0; s0 < nsteps; s0 += PF) {
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      const int s = s0 + i;
+      if (s >= nsteps) break;
+      float cst[LPL], x1[LPL], x2[LPL];
+#pragma unroll
+      for (int k = 0; k < LPL; ++k) { cst[k] = pc[i][k]; x1[k] = p1[i][k]; x2[k] = p2[i][k]; }
+      const float pv = pp[i];
+      {
+        const int sn = s + PF < nsteps ? s + PF : nsteps - 1;
+        load_step(sn, pc[i], p1[i], p2[i], pp[i]);
+      }
+      const int buf = s & 1;
+      int y, x;
+      yx_of(pos_of(s), y, x);
+
+      float h[LPL];
+#pragma unroll
+      for (int k = 0; k < LPL; ++k) {
+        h[k] = VERT ? ((cst[k] + x1[k]) + x2[k]) + chain[k] : ((cst[k] + chain[k]) + x1[k]) + x2[k];
+        const int c = sl + SEG * k;
+        if (c < C) lds_h[buf][seg][c] = h[k];
+      }
+      if (sl < 16) lds_p[buf][seg][sl] = pv;
+      __syncthreads();
+
+      const float w = lds_p[buf][seg][KIND == BPL_LBP_STEREO ? K : 0];
+      float msg[LPL];
+      int arg[LPL];
+      int lstar = 0;
+      if (trunc) {
+        float bv = __builtin_inff();
+        int bk = -1;
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) {
+          const int c = sl + SEG * k;
+          if (c < C) argmin_step(bv, bk, h[k], tie_key(c, a.bits));
+        }
+        seg_argmin<SEG>(bv, bk);
+        lstar = tie_key(bk, a.bits);  // bit reversal is an involution
+        float JK;
+        if (KIND == BPL_LBP_STEREO) JK = lds_p[buf][seg][K - 1];
+        else if (KIND == BPL_LBP_SEMANTIC_GLOBAL) JK = a.mat_lds ? lds_m[C - 1] : a.jump[(int64_t)dir * C * C + C - 1];
+        else JK = a.jump[px(dir, y, x, H, W) * C * C + C - 1];
+        const float gv = fmaf(JK, w, bv);
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) { msg[k] = gv; arg[k] = kFromTrunc; }
+        if (KIND == BPL_LBP_STEREO) {
+          const float o = lds_p[buf][seg][0];
+          int k_lo = -delta, k_hi = delta;
+          if (o != 0.f) { k_lo = (int)floorf(o - 0.5f) - delta; k_hi = (int)ceilf(o + 0.5f) + delta; }
+          int st[LPL], sp[LPL];
+#pragma unroll
+          for (int k = 0; k < LPL; ++k) {
+            const int c = sl + SEG * k;
+            st[k] = sat_u16(roundf((float)(c - delta + 1) + o - 1.0f));
+            sp[k] = min(sat_u16(roundf((float)(c + delta) + o + 1.0f)), C);
+          }
+          for (int kk = k_lo; kk <= k_hi; ++kk) {
+            const float fk = lds_p[buf][seg][min(max(K / 2 - kk, 1), K - 1)];
+#pragma unroll
+            for (int k = 0; k < LPL; ++k) {
+              const int c = sl + SEG * k;
+              const int l = c + kk;
+              if (c < C && l >= st[k] && l < sp[k]) {
+                const float nv = fmaf(fk, w, lds_h[buf][seg][l]);
+                if (nv <= msg[k]) { msg[k] = nv; arg[k] = l; }
+              }
+            }
+          }
+        } else {
+          for (int kk = 1 - delta; kk <= delta - 1; ++kk) {
+#pragma unroll
+            for (int k = 0; k < LPL; ++k) {
+              const int c = sl + SEG * k;
+              const int l = c + kk;
+              if (c < C && l >= 0 && l < C) {
+                float fv;
+                if (KIND == BPL_LBP_SEMANTIC_GLOBAL)
+                  fv = a.mat_lds ? lds_m[l * C + c] : a.jump[((int64_t)dir * C + l) * C + c];
+                else
+                  fv = a.jump[(px(dir, y, x, H, W) * C + l) * C + c];
+                const float nv = fmaf(fv, w, lds_h[buf][seg][l]);
+                if (nv <= msg[k]) { msg[k] = nv; arg[k] = l; }
+              }
+            }
+          }
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) { msg[k] = 1e15f; arg[k] = 0; }
+        const float *Mpix = (KIND == BPL_LBP_SEMANTIC_PIXEL) ? a.jump + px(dir, y, x, H, W) * C * C : nullptr;
+        for (int l = 0; l < C; ++l) {
+          const float hv = lds_h[buf][seg][l];
+#pragma unroll
+          for (int k = 0; k < LPL; ++k) {
+            const int c = sl + SEG * k;
+            if (c < C) {
+              float fv;
+              if (KIND == BPL_LBP_STEREO) fv = lds_p[buf][seg][min(max(K / 2 + c - l, 1), K - 1)];
+              else if (KIND == BPL_LBP_SEMANTIC_GLOBAL)
+                fv = a.mat_lds ? lds_m[l * C + c] : a.jump[((int64_t)dir * C + l) * C + c];
+              else fv = Mpix[l * C + c];
+              const float nv = fmaf(fv, w, hv);
+              if (nv <= msg[k]) { msg[k] = nv; arg[k] = l; }
+            }
+          }
+        }
+      }
+
+      // ---- normalisation: m - log sum_c exp(m), max-shifted
+      float mx = -__builtin_inff();
+#pragma unroll
+      for (int k = 0; k < LPL; ++k)
+        if (sl + SEG * k < C) mx = fmaxf(mx, msg[k]);
+      mx = seg_reduce<SEG>(mx, OpMax());
+      float se = 0.f;
+#pragma unroll
+      for (int k = 0; k < LPL; ++k)
+        if (sl + SEG * k < C) se += fast_exp(msg[k] - mx);
+      se = seg_reduce<SEG>(se, OpSum());
+      const float lse = mx + fast_log(se);
+
+      int yq = y, xq = x;
+      if (VERT) yq += fwd ? 1 : -1; else xq += fwd ? 1 : -1;
+      const int64_t oq = sl_px(n, rslot, yq, xq, H, W);
+#pragma unroll
+      for (int k = 0; k < LPL; ++k) {
+        const int c = sl + SEG * k;
+        const float m = msg[k] - lse;
+        chain[k] = m;
+        if (line_ok && c < C) {
+          a.msg_out[oq * C + c] = m;
+          a.argmin[oq * C + c] = (uint8_t)arg[k];
+        }
+      }
+      if (line_ok && sl == 0) {
+        if (a.scale) a.scale[oq] = fast_exp(lse);
+        if (a.gargmin) a.gargmin[oq] = (uint8_t)(trunc ? lstar : 0);
+      }
+    }
+  }
+}
+
+using FwdKernel = void (*)(FwdArgs);
+
+template <int KIND, int SEG, int LPL>
+void launch_phase(const FwdArgs &a, bool vert, hipStream_t st, size_t shm) {
+  constexpr int PF = 4;
+  constexpr int SPW = kWave / SEG;
+  const int L = vert ? a.W : a.H;
+  const int wpg = (L + SPW - 1) / SPW;
+  const dim3 grid((unsigned)(a.N * 2 * wpg));
+  if (vert)
+    hipLaunchKernelGGL((lbp_fwd_phase<KIND, SEG, LPL, true, PF>), grid, dim3(kWave), shm, st, a);
+  else
+    hipLaunchKernelGGL((lbp_fwd_phase<KIND, SEG, LPL, false, PF>), grid, dim3(kWave), shm, st, a);
+}
+
+template <int KIND>
+void launch_kind(const FwdArgs &a, bool vert, hipStream_t st, size_t shm) {
+  const int C = a.C;
+  if (C <= 16) launch_phase<KIND, 16, 1>(a, vert, st, shm);
+  else if (C <= 32) launch_phase<KIND, 32, 1>(a, vert, st, shm);
+  else if (C <= 64) launch_phase<KIND, 64, 1>(a, vert, st, shm);
+  else if (C <= 128) launch_phase<KIND, 64, 2>(a, vert, st, shm);
+  else if (C <= 192) launch_phase<KIND, 64, 3>(a, vert, st, shm);
+  else launch_phase<KIND, 64, 4>(a, vert, st, shm);
+}
+
+}  // namespace
+}  // namespace bpl
+
+using namespace bpl;
+
+extern "C" int bpl_lbp_fwd(int kind, const float *cost, const float *jump, int64_t jump_k, int64_t jump_n,
+                           const float *edge, const float *msg_in, float *msg_out,
+                           uint8_t *argmin, uint8_t *gargmin, float *scale,
+                           int64_t N, int64_t H, int64_t W, int64_t C, int delta, void *stream) {
+  BPL_REQUIRE(kind >= 0 && kind <= 2, BPL_E_ARG, "bpl_lbp_fwd: unknown kind %d", kind);
+  BPL_REQUIRE(N >= 1 && H >= 1 && W >= 1 && C >= 1, BPL_E_ARG,
+              "bpl_lbp_fwd: sizes must be positive (N=%lld H=%lld W=%lld C=%lld)",
+              (long long)N, (long long)H, (long long)W, (long long)C);
+  BPL_REQUIRE(C <= 255, BPL_E_LIMIT, "bpl_lbp_fwd: C=%lld labels exceeds the supported 255", (long long)C);
+  BPL_REQUIRE(N * 2 * (H > W ? H : W) < (1LL << 31) && H < (1 << 30) && W < (1 << 30), BPL_E_LIMIT,
+              "bpl_lbp_fwd: grid too large");
+  BPL_REQUIRE(delta >= 0, BPL_E_ARG, "bpl_lbp_fwd: delta must be >= 0");
+  BPL_REQUIRE(cost && jump && edge && msg_in && msg_out && argmin, BPL_E_ARG, "bpl_lbp_fwd: null pointer");
+  BPL_REQUIRE(delta >= C || gargmin, BPL_E_ARG, "bpl_lbp_fwd: gargmin is required when delta < C");
+  if (kind == BPL_LBP_STEREO) {
+    BPL_REQUIRE(jump_k >= 2 && jump_k <= 15, BPL_E_LIMIT, "bpl_lbp_fwd: stereo jump K=%lld not in [2,15]",
+                (long long)jump_k);
+    BPL_REQUIRE(jump_n == 1 || jump_n == N, BPL_E_ARG, "bpl_lbp_fwd: jump batch %lld must be 1 or N",
+                (long long)jump_n);
+  }
+  FwdArgs a;
+  a.cost = cost; a.jump = jump; a.edge = edge; a.msg_in = msg_in; a.msg_out = msg_out;
+  a.argmin = argmin; a.gargmin = gargmin; a.scale = scale;
+  a.N = (int)N; a.H = (int)H; a.W = (int)W; a.C = (int)C;
+  a.K = kind == BPL_LBP_STEREO ? (int)jump_k : 0;
+  a.Nj = kind == BPL_LBP_STEREO ? (int)jump_n : 1;
+  a.delta = delta; a.bits = tie_bits((int)C);
+  a.mat_lds = (kind == BPL_LBP_SEMANTIC_GLOBAL && C <= 64) ? 1 : 0;
+  const size_t shm = a.mat_lds ? (size_t)C * C * sizeof(float) : 0;
+  hipStream_t st = as_stream(stream);
+  for (int phase = 0; phase < 2; ++phase) {
+    const bool vert = phase == 1;
+    switch (kind) {
+      case BPL_LBP_STEREO: launch_kind<BPL_LBP_STEREO>(a, vert, st, shm); break;
+      case BPL_LBP_SEMANTIC_GLOBAL: launch_kind<BPL_LBP_SEMANTIC_GLOBAL>(a, vert, st, shm); break;
+      default: launch_kind<BPL_LBP_SEMANTIC_PIXEL>(a, vert, st, shm); break;
+    }
+    int rc = check_hip(hipGetLastError(), "bpl_lbp_fwd launch");
+    if (rc) return rc;
+  }
+  return BPL_OK;
+}

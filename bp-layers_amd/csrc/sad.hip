@@ -1,0 +1,442 @@
+// SAD matching-cost kernels on gfx950: stereo cost volume and fused flow window SAD with
+// min-projections, forward and (deterministic, gather-based) backward.
+//
+// Semantics follow ops/sad/src/stereo_sad_kernel.cu:16-108 and
+// ops/flow_mp_sad/src/flow_mp_sad_kernel.cu:10-200 (fill value 40 outside the image,
+// sign(diff) with a 1e-15 dead zone in the backward, first-index argmins in the flow
+// min-projections); every batch element is computed.
+#include "bpl_common.h"
+
+namespace bpl {
+namespace {
+
+constexpr float kFill = 40.f;
+constexpr float kSignEps = 1e-15f;
+
+__device__ __forceinline__ float sgn_eps(float t) { return fabsf(t) > kSignEps ? (t > 0.f ? 1.f : -1.f) : 0.f; }
+
+// ----------------------------------------------------------------------------- stereo fwd
+// Block: 64 pixels of one row x 64 disparities; thread = 4 pixels x 4 disparities.  Rows of
+// f0 and f1 are staged channel-chunk by channel-chunk in LDS; each f1 value is reused by up
+// to 16 (pixel, disparity) pairs of a thread.
+constexpr int ST_TX = 64, ST_TD = 64, ST_CB = 16;
+
+__global__ __launch_bounds__(256) void sad_stereo_fwd_k(const float *__restrict__ f0, const float *__restrict__ f1,
+                                                        float *__restrict__ cost, int C, int H, int W, int D,
+                                                        int mind, int dpasses) {
+  __shared__ float s0[ST_CB][ST_TX];
+  __shared__ float s1[ST_CB][ST_TX + ST_TD];
+  const int x0 = blockIdx.x * ST_TX;
+  const int y = blockIdx.y;
+  const int n = blockIdx.z / dpasses;
+  const int dp = blockIdx.z - n * dpasses;
+  const int d0 = mind + dp * ST_TD;
+  const int t = threadIdx.x;
+  const int dg = t & 15, xg = t >> 4;
+  const int xs = x0 - d0 - (ST_TD - 1);  // f1 column of s1[.][0]
+  const int64_t HW = (int64_t)H * W;
+  const float *F0 = f0 + (int64_t)n * C * HW + (int64_t)y * W;
+  const float *F1 = f1 + (int64_t)n * C * HW + (int64_t)y * W;
+  float acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = 0.f;
+
+  for (int c0 = 0; c0 < C; c0 += ST_CB) {
+    for (int i = t; i < ST_CB * ST_TX; i += 256) {
+      const int cc = i / ST_TX, xx = i - cc * ST_TX;
+      const int c = c0 + cc, xc = x0 + xx;
+      s0[cc][xx] = (c < C && xc < W) ? F0[(int64_t)c * HW + xc] : 0.f;
+    }
+    for (int i = t; i < ST_CB * (ST_TX + ST_TD); i += 256) {
+      const int cc = i / (ST_TX + ST_TD), j = i - cc * (ST_TX + ST_TD);
+      const int c = c0 + cc, xc = xs + j;
+      s1[cc][j] = (c < C && xc >= 0 && xc < W) ? F1[(int64_t)c * HW + xc] : 0.f;
+    }
+    __syncthreads();
+    const int nc = min(ST_CB, C - c0);
+    const int jb = xg * 4 - dg * 4 + ST_TD - 4;
+    for (int cc = 0; cc < nc; ++cc) {
+      float a4[4], v[7];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) a4[a] = s0[cc][xg * 4 + a];
+#pragma unroll
+      for (int j = 0; j < 7; ++j) v[j] = s1[cc][jb + j];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] += fabsf(a4[a] - v[a - b + 3]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int x = x0 + xg * 4 + a;
+    if (x >= W) continue;
+    float *out = cost + (((int64_t)n * H + y) * W + x) * D;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int i = dp * ST_TD + dg * 4 + b;
+      const int d = mind + i;
+      if (i < D) out[i] = (x - d >= 0 && x - d < W) ? acc[a][b] : kFill;
+    }
+  }
+}
+
+// general step: one thread per output, d = int(i * step + min_disp) (stereo_sad_kernel.cu:35)
+__global__ void sad_stereo_fwd_step_k(const float *__restrict__ f0, const float *__restrict__ f1, float *__restrict__ cost,
+                                      int N, int C, int H, int W, int D, int mind, float step) {
+  const int64_t total = (int64_t)N * H * W * D;
+  const int64_t HW = (int64_t)H * W;
+  for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < total; o += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(o % D);
+    const int64_t p = o / D;
+    const int x = (int)(p % W);
+    const int y = (int)((p / W) % H);
+    const int n = (int)(p / HW);
+    const int d = (int)((float)i * step + (float)mind);
+    float r = kFill;
+    if (x - d >= 0 && x - d < W) {
+      r = 0.f;
+      for (int c = 0; c < C; ++c) {
+        const int64_t b = ((int64_t)n * C + c) * HW + (int64_t)y * W;
+        r += fabsf(f0[b + x] - f1[b + x - d]);
+      }
+    }
+    cost[o] = r;
+  }
+}
+
+// ----------------------------------------------------------------------------- stereo bwd
+// Block: one row, 64 pixels x 4 channel lanes.  The two gradient volumes needed by a pixel
+// (g[y,x,:] for df0 and the diagonal g[y,x+d,d] for df1) are staged in LDS once and reused by
+// every channel; each (channel, pixel) sums its D terms in disparity order.
+constexpr int SB_TX = 64, SB_CL = 4;
+
+__global__ __launch_bounds__(256) void sad_stereo_bwd_k(const float *__restrict__ f0, const float *__restrict__ f1,
+                                                        const float *__restrict__ g, float *__restrict__ df0,
+                                                        float *__restrict__ df1, int C, int H, int W, int D, int mind) {
+  extern __shared__ float sm[];
+  float *gx = sm;                       // [SB_TX][D+1]
+  float *gd = sm + SB_TX * (D + 1);     // [SB_TX][D+1]
+  const int x0 = blockIdx.x * SB_TX;
+  const int y = blockIdx.y;
+  const int n = blockIdx.z;
+  const int t = threadIdx.x;
+  const int64_t HW = (int64_t)H * W;
+  const float *G = g + ((int64_t)n * H + y) * (int64_t)W * D;
+  for (int i = t; i < SB_TX * D; i += 256) {
+    const int xl = i / D, k = i - xl * D;
+    const int x = x0 + xl;
+    const int d = mind + k;
+    gx[xl * (D + 1) + k] = (x < W) ? G[(int64_t)x * D + k] : 0.f;
+    const int xd = x + d;
+    gd[xl * (D + 1) + k] = (x < W && xd >= 0 && xd < W) ? G[(int64_t)xd * D + k] : 0.f;
+  }
+  __syncthreads();
+  const int xl = t % SB_TX;
+  const int cl = t / SB_TX;
+  const int x = x0 + xl;
+  if (x >= W) return;
+  for (int c = cl; c < C; c += SB_CL) {
+    const int64_t b = ((int64_t)n * C + c) * HW + (int64_t)y * W;
+    const float a0 = f0[b + x], b1 = f1[b + x];
+    float r0 = 0.f, r1 = 0.f;
+    for (int k = 0; k < D; ++k) {
+      const int d = mind + k;
+      const int xm = x - d, xp = x + d;
+      if (xm >= 0 && xm < W) r0 += sgn_eps(a0 - f1[b + xm]) * gx[xl * (D + 1) + k];
+      if (xp >= 0 && xp < W) r1 -= sgn_eps(f0[b + xp] - b1) * gd[xl * (D + 1) + k];
+    }
+    df0[b + x] = r0;
+    df1[b + x] = r1;
+  }
+}
+
+// ----------------------------------------------------------------------------- flow fwd
+// Block: 8 rows x 64 pixels, thread = 2 adjacent pixels.  For each vertical displacement v
+// the f1 rows y+ov+v-h of the tile (plus the horizontal halo) are staged in LDS channel chunk
+// by chunk; S(u,v) for all u is accumulated in registers, then cv_v[v] (min over u) is
+// written and the running cv_u (min over v) updated -- both with first-index ties.
+constexpr int FL_TY = 8, FL_TX = 64, FL_CB = 8, FL_KMAX = 25;
+
+template <int K>
+__global__ __launch_bounds__(256) void sad_flow_fwd_k(const float *__restrict__ f0, const float *__restrict__ f1,
+                                                      float *__restrict__ cv_u, float *__restrict__ cv_v,
+                                                      float *__restrict__ u_star, float *__restrict__ v_star,
+                                                      int C, int H, int W, int sws, int ou, int ov, int bu, int bv) {
+  constexpr int SW = FL_TX + K - 1;
+  __shared__ float s1[FL_CB][FL_TY][SW];
+  __shared__ float s0[FL_CB][FL_TY][FL_TX];
+  const int hs = sws / 2;
+  const int x0 = blockIdx.x * FL_TX;
+  const int y0 = blockIdx.y * FL_TY;
+  const int n = blockIdx.z;
+  const int t = threadIdx.x;
+  const int ty = t / 32, tx = (t % 32) * 2;
+  const int y = y0 + ty;
+  const int64_t HW = (int64_t)H * W;
+  const float *F0 = f0 + (int64_t)n * C * HW;
+  const float *F1 = f1 + (int64_t)n * C * HW;
+  float bu_v[2][K];
+  int bu_i[2][K];
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int u = 0; u < K; ++u) { bu_v[p][u] = 9999999.f; bu_i[p][u] = 0; }
+
+  for (int vi = 0; vi < K; ++vi) {
+    const int dv = ov + vi - hs;
+    float S[2][K];
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int u = 0; u < K; ++u) S[p][u] = 0.f;
+    for (int c0 = 0; c0 < C; c0 += FL_CB) {
+      __syncthreads();
+      for (int i = t; i < FL_CB * FL_TY * SW; i += 256) {
+        const int cc = i / (FL_TY * SW);
+        const int r = i - cc * FL_TY * SW;
+        const int ry = r / SW, j = r - ry * SW;
+        const int c = c0 + cc, yy = y0 + ry + dv, xx = x0 + ou - hs + j;
+        s1[cc][ry][j] = (c < C && yy >= 0 && yy < H && xx >= 0 && xx < W) ? F1[(int64_t)c * HW + (int64_t)yy * W + xx] : 0.f;
+      }
+      for (int i = t; i < FL_CB * FL_TY * FL_TX; i += 256) {
+        const int cc = i / (FL_TY * FL_TX);
+        const int r = i - cc * FL_TY * FL_TX;
+        const int ry = r / FL_TX, j = r - ry * FL_TX;
+        const int c = c0 + cc, yy = y0 + ry, xx = x0 + j;
+        s0[cc][ry][j] = (c < C && yy < H && xx < W) ? F0[(int64_t)c * HW + (int64_t)yy * W + xx] : 0.f;
+      }
+      __syncthreads();
+      const int nc = min(FL_CB, C - c0);
+      for (int cc = 0; cc < nc; ++cc) {
+        const float a0 = s0[cc][ty][tx], a1 = s0[cc][ty][tx + 1];
+        float prev = s1[cc][ty][tx];
+#pragma unroll
+        for (int u = 0; u < K; ++u) {
+          const float nx = s1[cc][ty][tx + u + 1];
+          S[0][u] += fabsf(a0 - prev);
+          S[1][u] += fabsf(a1 - nx);
+          prev = nx;
+        }
+      }
+    }
+    const bool rowok = (y + dv >= 0) && (y + dv < H);
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const int x = x0 + tx + p;
+      float mv = 9999999.f;
+      int mi = 0;
+#pragma unroll
+      for (int u = 0; u < K; ++u) {
+        const int xx = x + ou + u - hs;
+        const float s = (rowok && xx >= 0 && xx < W) ? S[p][u] : kFill;
+        if (s < mv) { mv = s; mi = u; }
+        if (s < bu_v[p][u]) { bu_v[p][u] = s; bu_i[p][u] = vi; }
+      }
+      if (y < H && x < W) {
+        const int64_t o = (((int64_t)n * H + y) * W + x) * K + vi;
+        cv_v[o] = mv;
+        v_star[o] = (float)(mi + bu * sws);
+      }
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    const int x = x0 + tx + p;
+    if (y < H && x < W) {
+      const int64_t o = (((int64_t)n * H + y) * W + x) * K;
+#pragma unroll
+      for (int u = 0; u < K; ++u) {
+        cv_u[o + u] = bu_v[p][u];
+        u_star[o + u] = (float)(bu_i[p][u] + bv * sws);
+      }
+    }
+  }
+}
+
+// Generic window size: one thread per (pixel), S(u,v) evaluated on the fly, O(K^2 C).
+__global__ void sad_flow_fwd_generic_k(const float *__restrict__ f0, const float *__restrict__ f1, float *__restrict__ cv_u,
+                                       float *__restrict__ cv_v, float *__restrict__ u_star, float *__restrict__ v_star,
+                                       int N, int C, int H, int W, int sws, int ou, int ov, int bu, int bv) {
+  const int K = sws + 1, hs = sws / 2;
+  const int64_t HW = (int64_t)H * W;
+  const int64_t total = (int64_t)N * HW;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < total; p += (int64_t)gridDim.x * blockDim.x) {
+    const int n = (int)(p / HW);
+    const int y = (int)((p / W) % H), x = (int)(p % W);
+    const float *F0 = f0 + (int64_t)n * C * HW + (int64_t)y * W + x;
+    const float *F1 = f1 + (int64_t)n * C * HW;
+    float *cu = cv_u + p * K, *cvv = cv_v + p * K, *us = u_star + p * K, *vs = v_star + p * K;
+    for (int k = 0; k < K; ++k) { cu[k] = 9999999.f; us[k] = (float)(bv * sws); }
+    for (int vi = 0; vi < K; ++vi) {
+      const int yy = y + ov + vi - hs;
+      float mv = 9999999.f;
+      int mi = 0;
+      for (int ui = 0; ui < K; ++ui) {
+        const int xx = x + ou + ui - hs;
+        float s = kFill;
+        if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
+          s = 0.f;
+          for (int c = 0; c < C; ++c) s += fabsf(F0[(int64_t)c * HW] - F1[(int64_t)c * HW + (int64_t)yy * W + xx]);
+        }
+        if (s < mv) { mv = s; mi = ui; }
+        if (s < cu[ui]) { cu[ui] = s; us[ui] = (float)(vi + bv * sws); }
+      }
+      cvv[vi] = mv;
+      vs[vi] = (float)(mi + bu * sws);
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------- flow bwd
+// df0 at (y,x): 2K terms along the stored argmins.  df1 at (y',x') gathers every source
+// (y,x,k) whose argmin lands on it: for the u-part x = x'-u and y = y'-v with
+// u_star[y,x,u] = v+h; for the v-part y = y'-v and x = x'-u with v_star[y,x,v] = u+h.
+// Terms are summed in a fixed order, so the result is deterministic.
+__global__ __launch_bounds__(256) void sad_flow_bwd_k(const float *__restrict__ f0, const float *__restrict__ f1,
+                                                      const float *__restrict__ gu, const float *__restrict__ gv,
+                                                      const float *__restrict__ us, const float *__restrict__ vs,
+                                                      float *__restrict__ df0, float *__restrict__ df1,
+                                                      int N, int C, int H, int W, int sws) {
+  const int K = sws + 1, hs = sws / 2;
+  const int64_t HW = (int64_t)H * W;
+  const int64_t total = (int64_t)N * HW;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < total; p += (int64_t)gridDim.x * blockDim.x) {
+    const int n = (int)(p / HW);
+    const int y = (int)((p / W) % H), x = (int)(p % W);
+    const int64_t pix = p - (int64_t)n * HW;
+    const float *F0 = f0 + (int64_t)n * C * HW;
+    const float *F1 = f1 + (int64_t)n * C * HW;
+    float *D0 = df0 + (int64_t)n * C * HW;
+    float *D1 = df1 + (int64_t)n * C * HW;
+    for (int c = 0; c < C; ++c) {
+      const float a = F0[(int64_t)c * HW + pix];
+      const float b = F1[(int64_t)c * HW + pix];
+      float r0 = 0.f, r1 = 0.f;
+      // df0: u-part then v-part (kernel order of flow_mp_sad_kernel.cu:153-196)
+      for (int k = 0; k < K; ++k) {
+        const int64_t o = p * K + k;
+        {
+          const int u = k - hs, v = (int)us[o] - hs;
+          const int xx = x + u, yy = y + v;
+          if (xx >= 0 && xx < W && yy >= 0 && yy < H) r0 += sgn_eps(a - F1[(int64_t)c * HW + (int64_t)yy * W + xx]) * gu[o];
+        }
+      }
+      for (int k = 0; k < K; ++k) {
+        const int64_t o = p * K + k;
+        const int v = k - hs, u = (int)vs[o] - hs;
+        const int xx = x + u, yy = y + v;
+        if (xx >= 0 && xx < W && yy >= 0 && yy < H) r0 += sgn_eps(a - F1[(int64_t)c * HW + (int64_t)yy * W + xx]) * gv[o];
+      }
+      // df1: sources landing on (y, x)
+      for (int k = 0; k < K; ++k) {
+        const int u = k - hs;
+        const int xs = x - u;
+        if (xs < 0 || xs >= W) continue;
+        for (int v = -hs; v <= hs; ++v) {
+          const int ys = y - v;
+          if (ys < 0 || ys >= H) continue;
+          const int64_t o = (((int64_t)n * H + ys) * W + xs) * K + k;
+          if ((int)us[o] - hs == v) r1 -= sgn_eps(F0[(int64_t)c * HW + (int64_t)ys * W + xs] - b) * gu[o];
+        }
+      }
+      for (int k = 0; k < K; ++k) {
+        const int v = k - hs;
+        const int ys = y - v;
+        if (ys < 0 || ys >= H) continue;
+        for (int u = -hs; u <= hs; ++u) {
+          const int xs = x - u;
+          if (xs < 0 || xs >= W) continue;
+          const int64_t o = (((int64_t)n * H + ys) * W + xs) * K + k;
+          if ((int)vs[o] - hs == u) r1 -= sgn_eps(F0[(int64_t)c * HW + (int64_t)ys * W + xs] - b) * gv[o];
+        }
+      }
+      D0[(int64_t)c * HW + pix] = r0;
+      D1[(int64_t)c * HW + pix] = r1;
+    }
+  }
+}
+
+unsigned grid_for(int64_t total, int threads) {
+  int64_t b = (total + threads - 1) / threads;
+  if (b > 1 << 20) b = 1 << 20;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+}  // namespace
+}  // namespace bpl
+
+using namespace bpl;
+
+extern "C" int bpl_sad_stereo_fwd(const float *f0, const float *f1, float *cost, int64_t N, int64_t C, int64_t H,
+                                  int64_t W, int min_disp, int max_disp, float step, void *stream) {
+  BPL_REQUIRE(f0 && f1 && cost, BPL_E_ARG, "bpl_sad_stereo_fwd: null pointer");
+  BPL_REQUIRE(N >= 1 && C >= 1 && H >= 1 && W >= 1, BPL_E_ARG, "bpl_sad_stereo_fwd: sizes must be positive");
+  BPL_REQUIRE(step > 0.f, BPL_E_ARG, "bpl_sad_stereo_fwd: step must be positive");
+  BPL_REQUIRE(H < 65536 && N * C * H * W < (1LL << 40), BPL_E_LIMIT, "bpl_sad_stereo_fwd: tensor too large");
+  const int D = (int)((float)(max_disp - min_disp + 1) / step);
+  BPL_REQUIRE(D >= 1, BPL_E_ARG, "bpl_sad_stereo_fwd: empty disparity range");
+  hipStream_t st = as_stream(stream);
+  if (step == 1.0f) {
+    const int dpasses = (D + ST_TD - 1) / ST_TD;
+    BPL_REQUIRE(N * dpasses < 65536, BPL_E_LIMIT, "bpl_sad_stereo_fwd: N * ceil(D/64) too large");
+    dim3 grid((unsigned)((W + ST_TX - 1) / ST_TX), (unsigned)H, (unsigned)(N * dpasses));
+    hipLaunchKernelGGL(sad_stereo_fwd_k, grid, dim3(256), 0, st, f0, f1, cost, (int)C, (int)H, (int)W, D, min_disp,
+                       dpasses);
+  } else {
+    hipLaunchKernelGGL(sad_stereo_fwd_step_k, dim3(grid_for(N * H * W * D, 256)), dim3(256), 0, st, f0, f1, cost,
+                       (int)N, (int)C, (int)H, (int)W, D, min_disp, step);
+  }
+  return check_hip(hipGetLastError(), "bpl_sad_stereo_fwd launch");
+}
+
+extern "C" int bpl_sad_stereo_bwd(const float *f0, const float *f1, const float *grad_cost, float *df0, float *df1,
+                                  int64_t N, int64_t C, int64_t H, int64_t W, int min_disp, int max_disp,
+                                  void *stream) {
+  BPL_REQUIRE(f0 && f1 && grad_cost && df0 && df1, BPL_E_ARG, "bpl_sad_stereo_bwd: null pointer");
+  BPL_REQUIRE(N >= 1 && C >= 1 && H >= 1 && W >= 1, BPL_E_ARG, "bpl_sad_stereo_bwd: sizes must be positive");
+  const int D = max_disp - min_disp + 1;
+  BPL_REQUIRE(D >= 1, BPL_E_ARG, "bpl_sad_stereo_bwd: empty disparity range");
+  const size_t shm = (size_t)2 * SB_TX * (D + 1) * sizeof(float);
+  BPL_REQUIRE(shm <= 160 * 1024, BPL_E_LIMIT, "bpl_sad_stereo_bwd: D=%d needs %zu bytes of LDS", D, shm);
+  BPL_REQUIRE(H < 65536 && N < 65536, BPL_E_LIMIT, "bpl_sad_stereo_bwd: tensor too large");
+  dim3 grid((unsigned)((W + SB_TX - 1) / SB_TX), (unsigned)H, (unsigned)N);
+  hipLaunchKernelGGL(sad_stereo_bwd_k, grid, dim3(256), shm, as_stream(stream), f0, f1, grad_cost, df0, df1,
+                     (int)C, (int)H, (int)W, D, min_disp);
+  return check_hip(hipGetLastError(), "bpl_sad_stereo_bwd launch");
+}
+
+extern "C" int bpl_sad_flow_fwd(const float *f0, const float *f1, float *cv_u, float *cv_v, float *u_star,
+                                float *v_star, int64_t N, int64_t C, int64_t H, int64_t W, int sws, int offset_u,
+                                int offset_v, int block_u, int block_v, void *stream) {
+  BPL_REQUIRE(f0 && f1 && cv_u && cv_v && u_star && v_star, BPL_E_ARG, "bpl_sad_flow_fwd: null pointer");
+  BPL_REQUIRE(N >= 1 && C >= 1 && H >= 1 && W >= 1, BPL_E_ARG, "bpl_sad_flow_fwd: sizes must be positive");
+  BPL_REQUIRE(sws >= 0 && sws <= 1024, BPL_E_ARG, "bpl_sad_flow_fwd: sws=%d out of range", sws);
+  hipStream_t st = as_stream(stream);
+  if (sws == 24) {
+    dim3 grid((unsigned)((W + FL_TX - 1) / FL_TX), (unsigned)((H + FL_TY - 1) / FL_TY), (unsigned)N);
+    hipLaunchKernelGGL(sad_flow_fwd_k<25>, grid, dim3(256), 0, st, f0, f1, cv_u, cv_v, u_star, v_star, (int)C,
+                       (int)H, (int)W, sws, offset_u, offset_v, block_u, block_v);
+  } else {
+    hipLaunchKernelGGL(sad_flow_fwd_generic_k, dim3(grid_for(N * H * W, 128)), dim3(128), 0, st, f0, f1, cv_u,
+                       cv_v, u_star, v_star, (int)N, (int)C, (int)H, (int)W, sws, offset_u, offset_v, block_u,
+                       block_v);
+  }
+  return check_hip(hipGetLastError(), "bpl_sad_flow_fwd launch");
+}
+
+extern "C" int bpl_sad_flow_bwd(const float *f0, const float *f1, const float *grad_u, const float *grad_v,
+                                const float *u_star, const float *v_star, float *df0, float *df1, int64_t N,
+                                int64_t C, int64_t H, int64_t W, int sws, void *stream) {
+  BPL_REQUIRE(f0 && f1 && grad_u && grad_v && u_star && v_star && df0 && df1, BPL_E_ARG,
+              "bpl_sad_flow_bwd: null pointer");
+  BPL_REQUIRE(N >= 1 && C >= 1 && H >= 1 && W >= 1, BPL_E_ARG, "bpl_sad_flow_bwd: sizes must be positive");
+  BPL_REQUIRE(sws >= 0 && sws <= 1024, BPL_E_ARG, "bpl_sad_flow_bwd: sws=%d out of range", sws);
+  hipLaunchKernelGGL(sad_flow_bwd_k, dim3(grid_for(N * H * W, 256)), dim3(256), 0, as_stream(stream), f0, f1,
+                     grad_u, grad_v, u_star, v_star, df0, df1, (int)N, (int)C, (int)H, (int)W, sws);
+  return check_hip(hipGetLastError(), "bpl_sad_flow_bwd launch");
+}

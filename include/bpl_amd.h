@@ -1,0 +1,133 @@
+/*
+ * bpl_amd.h -- C ABI of the MI355X (gfx950) BP-Layers hot path.
+ *
+ * One shared library (libbpl_amd.so) exports the SAD matching-cost ops and the
+ * min-sum loopy-belief-propagation ops of VLOGroup/bp-layers.  Every entry point
+ * takes device pointers allocated by the caller, 64-bit sizes and a HIP stream
+ * (passed as void*; NULL = the default stream), launches asynchronously on that
+ * stream and returns 0 on success or a negative BPL_E* code.  The library never
+ * allocates device memory; scratch space is passed in as a caller-owned workspace
+ * whose size is queried first.  bpl_last_error() returns a thread-local message
+ * for the last failing call on the calling thread.
+ *
+ * Tensor layouts are the reference's (all float32 and contiguous unless noted):
+ *   features      [N,C,H,W]
+ *   stereo cost   [N,H,W,D]             flow volumes [N,H,W,K]
+ *   LBP cost      [N,H,W,C]             messages     [N,4,H,W,C]
+ *   edge weights  [N,4,H,W]             message slot order LEFT=0 RIGHT=1 UP=2 DOWN=3
+ *   jump          stereo [Nj,4,K,H,W] (Nj in {1,N}); semantic global [1,4,C,C];
+ *                 semantic pixel [4,H,W,C,C]
+ *   argmin        uint8 [N,4,H,W,C]: winning sender label per receiver label, or
+ *                 255 when the value came from the truncation term, whose label is
+ *                 then gargmin (uint8 [N,4,H,W]) at the same receiver slot.
+ */
+#ifndef BPL_AMD_H
+#define BPL_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BPL_OK 0
+#define BPL_E_ARG -1      /* invalid size, pointer or parameter */
+#define BPL_E_LIMIT -2    /* a size beyond what the kernels support */
+#define BPL_E_HIP -3      /* a HIP runtime error (launch failure, ...) */
+
+/* pairwise-term variants of the LBP op */
+#define BPL_LBP_STEREO 0          /* ops/lbp_stereo: per-pixel K-vector over label differences */
+#define BPL_LBP_SEMANTIC_GLOBAL 1 /* ops/lbp_semantic_pw: one C x C matrix per direction      */
+#define BPL_LBP_SEMANTIC_PIXEL 2  /* ops/lbp_semantic_pw_pixel: C x C matrix per pixel+dir.  */
+
+/* backward modes */
+#define BPL_GRAD_REFERENCE 0  /* the reference's backward_minsum semantics              */
+#define BPL_GRAD_EXACT 1      /* exact adjoint of one iteration (input-message gradient) */
+
+/* Library identification and diagnostics. */
+int bpl_version(void);
+const char *bpl_last_error(void);
+
+/*
+ * Stereo SAD cost volume.  Replaces pytorch_cuda_stereo_sad_op.forward
+ * (reference ops/sad/src/stereo_sad.cpp:10-15; kernel ops/sad/src/stereo_sad_kernel.cu:16-62).
+ * cost[n,y,x,i] = sum_c |f0[n,c,y,x] - f1[n,c,y,x-d]|, d = int(i*step) + min_disp,
+ * or 40 where x-d is outside the image; D = int((max_disp-min_disp+1)/step).
+ */
+int bpl_sad_stereo_fwd(const float *f0, const float *f1, float *cost,
+                       int64_t N, int64_t C, int64_t H, int64_t W,
+                       int min_disp, int max_disp, float step, void *stream);
+
+/*
+ * Gradient of the stereo SAD (step 1).  Replaces pytorch_cuda_stereo_sad_op.backward
+ * (reference ops/sad/src/stereo_sad.cpp:17-24; kernel stereo_sad_kernel.cu:64-108).
+ * Deterministic gather over disparities; df0/df1 are fully overwritten.
+ */
+int bpl_sad_stereo_bwd(const float *f0, const float *f1, const float *grad_cost,
+                       float *df0, float *df1,
+                       int64_t N, int64_t C, int64_t H, int64_t W,
+                       int min_disp, int max_disp, void *stream);
+
+/*
+ * Fused flow SAD over a (sws+1)^2 window with min-projections.  Replaces
+ * pytorch_cuda_flow_mp_sad_op.forward (reference ops/flow_mp_sad/src/flow_mp_sad.cpp:10-16;
+ * kernel flow_mp_sad_kernel.cu:10-124).  Outputs [N,H,W,sws+1]; argmins as float label ids.
+ */
+int bpl_sad_flow_fwd(const float *f0, const float *f1,
+                     float *cv_u, float *cv_v, float *u_star, float *v_star,
+                     int64_t N, int64_t C, int64_t H, int64_t W,
+                     int sws, int offset_u, int offset_v, int block_u, int block_v,
+                     void *stream);
+
+/*
+ * Gradient of the flow SAD through the stored argmins.  Replaces
+ * pytorch_cuda_flow_mp_sad_op.backward (reference flow_mp_sad.cpp:18-30; kernel
+ * flow_mp_sad_kernel.cu:126-200).  Deterministic gather (no float atomics).
+ */
+int bpl_sad_flow_bwd(const float *f0, const float *f1, const float *grad_u,
+                     const float *grad_v, const float *u_star, const float *v_star,
+                     float *df0, float *df1,
+                     int64_t N, int64_t C, int64_t H, int64_t W, int sws, void *stream);
+
+/*
+ * One min-sum LBP iteration: passes RIGHT, LEFT, DOWN, UP.  Replaces
+ * pytorch_cuda_lbp_op.forward_minsum of the three reference builds
+ * (ops/lbp_{stereo,semantic_pw,semantic_pw_pixel}/src/lbp.cpp:16-27; kernels
+ * lbp_min_sum_kernel.cu:9-213 and host loop :424-500).
+ * msg_in and msg_out may be the same buffer (in-place, as in the reference).
+ * jump_k is K for the stereo variant and ignored otherwise; jump_n is the leading
+ * dimension of a stereo jump (1 = shared by the batch, or N).
+ * scale (float [N,4,H,W], sum_c exp(message)) may be NULL; gargmin may be NULL when
+ * delta >= C (no truncation term).
+ */
+int bpl_lbp_fwd(int kind, const float *cost, const float *jump, int64_t jump_k, int64_t jump_n,
+                const float *edge, const float *msg_in, float *msg_out,
+                uint8_t *argmin, uint8_t *gargmin, float *scale,
+                int64_t N, int64_t H, int64_t W, int64_t C, int delta, void *stream);
+
+/* Workspace bytes needed by bpl_lbp_bwd for these sizes. */
+int64_t bpl_lbp_bwd_workspace_size(int kind, int64_t N, int64_t H, int64_t W, int64_t C);
+
+/*
+ * Backward of one LBP iteration.  Replaces pytorch_cuda_lbp_op.backward_minsum
+ * (reference ops/lbp_stereo/src/lbp.cpp:29-46 and its semantic twins; kernels
+ * lbp_min_sum_kernel.cu:215-417, host loop :513-621).
+ * messages: the message tensor the probabilities exp(m) are read from (the
+ * reference passes the in-place-updated tensor); argmin/gargmin as written by
+ * bpl_lbp_fwd.  Outputs are fully overwritten: grad_cost [N,H,W,C]; grad_jump
+ * (shape of jump); grad_edge [N,4,H,W] (zeros for SEMANTIC_PIXEL, as in the
+ * reference); grad_msg [N,4,H,W,C] (BPL_GRAD_REFERENCE: a copy of in_grad;
+ * BPL_GRAD_EXACT: the gradient w.r.t. the iteration's input messages).
+ */
+int bpl_lbp_bwd(int kind, int mode, const float *jump, int64_t jump_k, int64_t jump_n,
+                const float *edge, const float *in_grad, const float *messages,
+                const uint8_t *argmin, const uint8_t *gargmin,
+                float *grad_cost, float *grad_jump, float *grad_edge, float *grad_msg,
+                void *workspace, int64_t workspace_bytes,
+                int64_t N, int64_t H, int64_t W, int64_t C, int delta, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BPL_AMD_H */

@@ -213,7 +213,7 @@ def tree_argmin(h):
     return v[:, 0], lab[:, 0]
 
 
-def send(h, pw, n, d, ys, xs, w, delta):
+def send(h, pw, n, d, ys, xs, w, delta, want_margin=False):
     """One message computation for a scanline of senders (lbp_min_sum_kernel.cu:62-212).
 
     h [P,C] is cost + the three incoming messages other than the one from the receiver;
@@ -223,6 +223,7 @@ def send(h, pw, n, d, ys, xs, w, delta):
     """
     P, C = h.shape
     cs = np.arange(C)
+    cands = []  # (value, label, is_global) for the margin diagnostics
     if delta >= C:
         # full branch (:88-111): running fminf over ascending labels, argmin = last attaining
         msg = np.full((P, C), MAX_FLOAT)
@@ -231,6 +232,8 @@ def send(h, pw, n, d, ys, xs, w, delta):
             new = h[:, l:l + 1] + pw.f(n, d, ys, xs, cs, np.full(C, l)) * w[:, None]
             msg = np.minimum(msg, new)
             arg = np.where(msg == new, l, arg)
+            if want_margin:
+                cands.append((new, np.full((P, C), l), False))
         from_global = np.zeros((P, C), dtype=bool)
         lstar = np.full(P, -1, dtype=np.int64)
     else:
@@ -239,6 +242,13 @@ def send(h, pw, n, d, ys, xs, w, delta):
         msg = np.broadcast_to((Hs + pw.trunc(n, d, ys, xs) * w)[:, None], (P, C)).copy()
         arg = np.broadcast_to(lstar[:, None], (P, C)).copy()
         from_global = np.ones((P, C), dtype=bool)
+        if want_margin:
+            cands.append((msg.copy(), arg.copy(), True))
+            # other labels attaining (almost) the global min are alternative global winners
+            Hs_sorted = np.sort(h, axis=1)
+            second = Hs_sorted[:, 1] if C > 1 else np.full(P, np.inf)
+            cands.append((np.broadcast_to((second + pw.trunc(n, d, ys, xs) * w)[:, None], (P, C)),
+                          np.full((P, C), -2), True))
         start, stop = pw.window(n, d, ys, xs, delta, C)
         kmin = int((start - cs[None, :]).min())
         kmax = int((stop - 1 - cs[None, :]).max())
@@ -251,9 +261,17 @@ def send(h, pw, n, d, ys, xs, w, delta):
             msg = np.where(take, new, msg)
             arg = np.where(take, lc[None, :], arg)
             from_global &= ~take
+            if want_margin:
+                cands.append((np.where(valid, new, np.inf), np.broadcast_to(lc[None, :], (P, C)), False))
     # normalisation (:166-181), max-shifted
     m = msg.max(axis=1, keepdims=True)
     lse = m + np.log(np.exp(msg - m).sum(axis=1, keepdims=True))
+    if want_margin:
+        margin = np.full((P, C), np.inf)
+        for v, lab, isg in cands:
+            same = (lab == arg) & (isg == from_global)
+            margin = np.minimum(margin, np.where(same, np.inf, v - msg))
+        return msg - lse, arg, np.exp(lse[:, 0]), from_global, lstar, margin
     return msg - lse, arg, np.exp(lse[:, 0]), from_global, lstar
 
 
@@ -280,7 +298,7 @@ def _line(d, s, H, W):
     return ys, xs, ys + dy, xs + dx
 
 
-def lbp_forward(cost, pw, edge, messages, delta):
+def lbp_forward(cost, pw, edge, messages, delta, want_margin=False):
     """One LBP iteration: passes RIGHT, LEFT, DOWN, UP (lbp_min_sum_kernel.cu:424-500).
 
     Messages are updated in place in a copy of ``messages`` (the reference mutates its
@@ -296,6 +314,7 @@ def lbp_forward(cost, pw, edge, messages, delta):
     scale = np.zeros((N, 4, H, W))
     from_global = np.zeros((N, 4, H, W, C), dtype=bool)
     gargmin = np.full((N, 4, H, W), -1, dtype=np.int64)
+    margin = np.full((N, 4, H, W, C), np.inf)
     for n in range(N):
         for d in PASS_ORDER:
             r = RECV[d]
@@ -306,12 +325,17 @@ def lbp_forward(cost, pw, edge, messages, delta):
                     if slot != d:
                         h += M[n, slot, ys, xs, :]
                 w = edge[n, d, ys, xs]
-                mn, a, S, fg, ls = send(h, pw, n, d, ys, xs, w, delta)
+                res = send(h, pw, n, d, ys, xs, w, delta, want_margin)
+                mn, a, S, fg, ls = res[:5]
+                if want_margin:
+                    margin[n, r, yq, xq, :] = res[5]
                 M[n, r, yq, xq, :] = mn
                 argmin[n, r, yq, xq, :] = a
                 scale[n, r, yq, xq] = S
                 from_global[n, r, yq, xq, :] = fg
                 gargmin[n, r, yq, xq] = ls
+    if want_margin:
+        return M, argmin, scale, from_global, gargmin, margin
     return M, argmin, scale, from_global, gargmin
 
 

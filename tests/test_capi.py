@@ -1,0 +1,89 @@
+"""CPU checks of the C ABI: the library loads, exports every function include/bpl_amd.h
+declares, and rejects bad arguments with error codes before touching a GPU."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "bpl_amd.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(bpl_\w+)\s*\(", text, flags=re.M)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from bplayers_amd import _native
+    if not os.path.exists(_native.library_path()):
+        import importlib.util
+        spec = importlib.util.spec_from_file_location("bpl_build", os.path.join(ROOT, "bp-layers_amd", "build.py"))
+        mod = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(mod)
+        mod.build()
+    return _native.lib()
+
+
+def test_header_declares_the_expected_surface():
+    names = declared_functions()
+    for n in ("bpl_version", "bpl_last_error", "bpl_sad_stereo_fwd", "bpl_sad_stereo_bwd", "bpl_sad_flow_fwd",
+              "bpl_sad_flow_bwd", "bpl_lbp_fwd", "bpl_lbp_bwd", "bpl_lbp_bwd_workspace_size"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol(lib):
+    missing = [n for n in declared_functions() if getattr(lib, n, None) is None]
+    assert not missing, f"not exported: {missing}"
+
+
+def test_version_and_error_string(lib):
+    assert lib.bpl_version() == 1
+    assert isinstance(lib.bpl_last_error(), bytes)
+
+
+def test_argument_validation_returns_codes(lib):
+    from bplayers_amd import _native as nat
+    P = ctypes.c_void_p(1)
+    rc = lib.bpl_lbp_fwd(7, P, P, 6, 1, P, P, P, P, P, P, 1, 4, 4, 8, 3, None)
+    assert rc == -1 and b"unknown kind" in lib.bpl_last_error()
+    rc = lib.bpl_lbp_fwd(nat.LBP_STEREO, P, P, 6, 1, P, P, P, P, P, P, 1, 4, 4, 300, 3, None)
+    assert rc == -2 and b"exceeds" in lib.bpl_last_error()
+    rc = lib.bpl_lbp_fwd(nat.LBP_STEREO, P, P, 20, 1, P, P, P, P, P, P, 1, 4, 4, 8, 3, None)
+    assert rc == -2
+    rc = lib.bpl_lbp_fwd(nat.LBP_STEREO, P, P, 6, 1, P, P, P, P, None, P, 1, 4, 4, 8, 3, None)
+    assert rc == -1 and b"gargmin" in lib.bpl_last_error()
+    rc = lib.bpl_lbp_bwd(nat.LBP_STEREO, 5, P, 6, 1, P, P, P, P, P, P, P, P, P, P, 1 << 40, 1, 4, 4, 8, 3, None)
+    assert rc == -1 and b"mode" in lib.bpl_last_error()
+    need = lib.bpl_lbp_bwd_workspace_size(nat.LBP_STEREO, 1, 4, 4, 8)
+    assert need == 4 * 1 * 4 * 4 * 8 * 4
+    rc = lib.bpl_lbp_bwd(nat.LBP_STEREO, 1, P, 6, 1, P, P, P, P, P, P, P, P, P, P, need - 1, 1, 4, 4, 8, 3, None)
+    assert rc == -1 and b"workspace" in lib.bpl_last_error()
+    rc = lib.bpl_sad_stereo_fwd(P, P, P, 1, 3, 4, 4, 5, 2, 1.0, None)
+    assert rc == -1 and b"empty" in lib.bpl_last_error()
+    rc = lib.bpl_sad_flow_fwd(P, P, P, P, P, P, 1, 3, 4, 4, -2, 0, 0, 0, 0, None)
+    assert rc == -1
+
+
+def test_python_ops_require_cuda_tensors(lib):
+    import torch
+    from bplayers_amd import pytorch_cuda_lbp_op_stereo as st
+    from bplayers_amd import pytorch_cuda_stereo_sad_op as sad
+    x = torch.zeros((1, 4, 4, 8))
+    with pytest.raises(RuntimeError, match="CUDA"):
+        st.forward_minsum(x, torch.zeros((1, 4, 6, 4, 4)), torch.zeros((1, 4, 4, 4)), torch.zeros((1, 4, 4, 4, 8)), 3)
+    with pytest.raises(RuntimeError, match="CUDA"):
+        sad.forward(torch.zeros((1, 2, 4, 4)), torch.zeros((1, 2, 4, 4)), 0, 3)
+
+
+def test_reference_aliases():
+    import sys
+    import bplayers_amd
+    bplayers_amd.install_reference_aliases("semantic_pw")
+    import pytorch_cuda_lbp_op
+    assert pytorch_cuda_lbp_op.__name__.endswith("semantic_pw")
+    assert hasattr(pytorch_cuda_lbp_op, "forward_minsum") and hasattr(pytorch_cuda_lbp_op, "backward_minsum")
+    for k in ("pytorch_cuda_lbp_op", "pytorch_cuda_stereo_sad_op", "pytorch_cuda_flow_mp_sad_op"):
+        sys.modules.pop(k)

@@ -1,0 +1,130 @@
+"""GPU parity of the LBP forward (libbpl_amd via bplayers_amd) against the FP64 oracle."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import lbp as O
+from _lbp_cases import random_stereo_problem, semantic_matrix, pack_semantic_global, LEFT, RIGHT, UP, DOWN
+
+pytestmark = pytest.mark.gpu
+
+# Tolerances (fp32 kernel vs FP64 oracle on identical fp32 inputs):
+MSG_TOL = 2e-5     # |gpu - oracle| / max(1, |oracle|) on messages
+TIE_TOL = 1e-4     # argmins are compared where the oracle's winner leads every other candidate by this
+
+
+def _mods():
+    from bplayers_amd import pytorch_cuda_lbp_op_stereo as st
+    from bplayers_amd import pytorch_cuda_lbp_op_semantic_pw as sg
+    from bplayers_amd import pytorch_cuda_lbp_op_semantic_pw_pixel as sp
+    return {"stereo": st, "semantic_global": sg, "semantic_pixel": sp}
+
+
+def make_case(kind, N, H, W, C, delta, seed, K_in=3, offset=False, ordered=False):
+    rng = np.random.default_rng(seed)
+    if kind == "stereo":
+        cost, aff, jump, edge = random_stereo_problem(rng, N, H, W, C, K_in=K_in, ordered=ordered)
+        if offset:
+            jump[:, :, 0] = rng.choice([0.0, 0.5, -1.0, 1.3, 2.5], size=jump[:, :, 0].shape)
+    elif kind == "semantic_global":
+        cost = rng.normal(size=(N, H, W, C))
+        jump = pack_semantic_global(np.stack([semantic_matrix(rng, C), semantic_matrix(rng, C)])[None])
+        edge = rng.uniform(0.5, 1.5, size=(N, 4, H, W))
+    else:
+        cost = rng.normal(size=(N, H, W, C))
+        jump = rng.uniform(0, 1, size=(4, H, W, C, C))
+        edge = rng.uniform(0.5, 1.5, size=(N, 4, H, W))
+    M0 = rng.normal(size=(N, 4, H, W, C)) * 0.3
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    return f32(cost), f32(jump), f32(edge), f32(M0)
+
+
+def run_gpu(kind, cost, jump, edge, M0, delta, inplace=True):
+    from bplayers_amd._lbp_op import _gargmin_of
+    mod = _mods()[kind]
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.from_numpy(a).to(dev)
+    m = t(M0)
+    out = None if inplace else torch.empty_like(m)
+    msgs, am, sc = mod.forward_minsum(t(cost), t(jump), t(edge), m, delta, out=out)
+    torch.cuda.synchronize()
+    if inplace:
+        assert msgs.data_ptr() == m.data_ptr()
+    else:
+        assert torch.equal(m.cpu(), torch.from_numpy(M0)), "input messages modified"
+    return (msgs.cpu().numpy().astype(np.float64), am.cpu().numpy(), sc.cpu().numpy().astype(np.float64),
+            _gargmin_of(sc).cpu().numpy())
+
+
+def check_forward(kind, cost, jump, edge, M0, delta, inplace=True):
+    Mg, ag, sg, gg = run_gpu(kind, cost, jump, edge, M0, delta, inplace)
+    pw = O.make_pairwise(kind, jump)
+    Mo, ao, so, fgo, lso, margin = O.lbp_forward(cost, pw, edge, M0, delta, want_margin=True)
+    err = np.abs(Mg - Mo) / np.maximum(1.0, np.abs(Mo))
+    assert err.max() <= MSG_TOL, f"message error {err.max():.3g}"
+    N, _, H, W, C = Mo.shape
+    written = np.ones((N, 4, H, W), dtype=bool)
+    written[:, LEFT, :, 0] = written[:, RIGHT, :, -1] = False
+    written[:, UP, 0, :] = written[:, DOWN, -1, :] = False
+    sc_err = np.abs(sg - so)[written] / so[written]
+    assert sc_err.max() <= 1e-4, f"scale error {sc_err.max():.3g}"
+    gpu_fg = ag == 255
+    gpu_lab = np.where(gpu_fg, np.broadcast_to(gg[..., None], ag.shape), ag).astype(np.int64)
+    ora_lab = np.where(fgo, np.broadcast_to(lso[..., None], ao.shape), ao)
+    clear = (margin > TIE_TOL) & written[..., None]
+    assert np.array_equal(gpu_lab[clear], ora_lab[clear]), \
+        f"argmin mismatches: {(gpu_lab[clear] != ora_lab[clear]).sum()} of {clear.sum()}"
+    assert np.array_equal(gpu_fg[clear], fgo[clear]), "truncation-term flags differ"
+    assert clear.mean() > 0.9, f"too many near-ties ({1 - clear.mean():.3f})"
+    # unwritten boundary slots are copies of the input
+    np.testing.assert_array_equal(Mg[~written], M0.astype(np.float64)[~written])
+    return err.max(), 1 - clear[written].mean()
+
+
+CASES = [
+    # kind, N, H, W, C, delta, extra
+    ("stereo", 2, 9, 13, 64, 3, {}),
+    ("stereo", 1, 6, 7, 192, 3, {}),
+    ("stereo", 1, 5, 9, 100, 3, {}),
+    ("stereo", 1, 4, 5, 255, 3, {}),
+    ("stereo", 2, 7, 11, 25, 3, {}),
+    ("stereo", 1, 6, 6, 9, 3, {}),
+    ("stereo", 1, 5, 6, 12, 12, {}),
+    ("stereo", 1, 6, 7, 40, 3, {"K_in": 5}),
+    ("stereo", 1, 6, 7, 40, 3, {"offset": True}),
+    ("stereo", 1, 6, 7, 30, 2, {"ordered": True}),
+    ("stereo", 1, 1, 9, 16, 3, {}),
+    ("stereo", 1, 9, 1, 16, 3, {}),
+    ("semantic_global", 2, 6, 7, 19, 19, {}),
+    ("semantic_global", 1, 5, 6, 19, 3, {}),
+    ("semantic_global", 1, 4, 5, 70, 70, {}),
+    ("semantic_pixel", 1, 5, 6, 7, 7, {}),
+    ("semantic_pixel", 1, 4, 5, 19, 19, {}),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c[0]}-N{c[1]}H{c[2]}W{c[3]}C{c[4]}d{c[5]}{''.join(c[6])}")
+@pytest.mark.parametrize("inplace", [True, False])
+def test_lbp_forward_matches_oracle(case, inplace):
+    kind, N, H, W, C, delta, extra = case
+    print(f"case {case} inplace={inplace}", flush=True)
+    cost, jump, edge, M0 = make_case(kind, N, H, W, C, delta, seed=hash((N, H, W, C)) % 1000, **extra)
+    e, tie = check_forward(kind, cost, jump, edge, M0, delta, inplace)
+    print(f"  max rel err {e:.3g}, near-tie fraction {tie:.4f}")
+
+
+def test_lbp_forward_two_iterations_long_rows():
+    """Chains long enough for fp32 drift to show (W=300)."""
+    cost, jump, edge, M0 = make_case("stereo", 1, 4, 300, 64, 3, seed=5)
+    from bplayers_amd import pytorch_cuda_lbp_op_stereo as st
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.from_numpy(a).to(dev)
+    m = t(M0)
+    for _ in range(2):
+        st.forward_minsum(t(cost), t(jump), t(edge), m, 3)
+    M = M0.astype(np.float64)
+    for _ in range(2):
+        M, *_ = O.lbp_forward(cost, O.StereoPairwise(jump), edge, M, 3)
+    err = np.abs(m.cpu().numpy() - M) / np.maximum(1.0, np.abs(M))
+    print(f"max rel err after 2 iterations, W=300: {err.max():.3g}")
+    assert err.max() <= MSG_TOL
