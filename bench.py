@@ -1,0 +1,277 @@
+#!/usr/bin/env python3
+"""Benchmark of the BP-Layers hot path on one MI355X.
+
+Default workload: stereo_fb_384x1248_d192 -- full-resolution stereo LBP, batch 1, 192
+disparities, 3 iterations with per-pixel, per-direction P1/P2, forward + exact backward.
+A step is one pass of the LBP hot path over the batch: 3 forward iterations followed by
+their 3 backward iterations, on inputs already resident in HBM (the SAD cost volume is
+computed once before timing and timed separately).
+
+Prints ONE JSON line (see the contract in the repository docs):
+  metric "G(pixel*label)/s per LBP iter fwd+bwd", value = N*H*W*C*iters / t_step / 1e9.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "bp-layers_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+# canonical algorithmic bytes per pixel*label per LBP iteration (fp32 messages, u8 argmins,
+# passes fused per phase): forward 44, exact backward 52; per-pixel parameters add
+# (P1/P2 per direction: 32 B/px read fwd, 64 B/px read+grad bwd; edge weights 16 / 32 B/px) / C
+FWD_B_PL = 44.0
+BWD_B_PL = 52.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="stereo_fb_384x1248_d192",
+                    choices=["stereo_fb_384x1248_d192", "stereo_fwd_128x256_d64"])
+    ap.add_argument("--rows", type=int, default=0, help="crop the image to this many rows (debug)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    return ap.parse_args()
+
+
+class StereoPlan:
+    """All device buffers of the LBP fwd+bwd step, allocated once."""
+
+    def __init__(self, wl, iters, rows=0, backward=True):
+        import numpy as np
+        import torch
+        from bplayers_amd import _native as nat
+        from bplayers_amd import pytorch_cuda_stereo_sad_op as sad
+        from bplayers_amd.ops.lbp_stereo.message_passing_op_cuda import pack_affinities
+        self.nat = nat
+        dev = torch.device("cuda:0")
+        self.dev = dev
+        f0 = torch.from_numpy(wl["f0"]).to(dev)
+        f1 = torch.from_numpy(wl["f1"]).to(dev)
+        if rows:
+            f0 = f0[:, :, :rows].contiguous()
+            f1 = f1[:, :, :rows].contiguous()
+        self.f0, self.f1 = f0, f1
+        self.min_disp, self.max_disp = wl["min_disp"], wl["max_disp"]
+        sadv = sad.forward(f0, f1, self.min_disp, self.max_disp, 1.0)
+        self.cost = (-torch.softmax(-sadv, dim=3)).contiguous()     # cost = -softmax(-SAD), T = 1
+        self.sad_grad = torch.randn_like(sadv)
+        del sadv
+        N, H, W, C = self.cost.shape
+        self.N, self.H, self.W, self.C = N, H, W, C
+        aff = torch.from_numpy(wl["affinities"]).to(dev)
+        if rows:
+            aff = aff[..., :rows, :].contiguous()
+        self.jump = pack_affinities(aff, torch.zeros((N, 4, H, W), device=dev))
+        self.K = self.jump.shape[2]
+        self.edge = torch.ones((N, 4, H, W), device=dev)
+        self.delta = wl["delta"]
+        self.iters = iters
+        self.msgs = [torch.zeros((N, 4, H, W, C), device=dev)] + \
+                    [torch.empty((N, 4, H, W, C), device=dev) for _ in range(iters)]
+        self.argmin = [torch.empty((N, 4, H, W, C), dtype=torch.uint8, device=dev) for _ in range(iters)]
+        self.garg = [torch.empty((N, 4, H, W), dtype=torch.uint8, device=dev) for _ in range(iters)]
+        self.backward = backward
+        if backward:
+            g = torch.Generator(device="cpu").manual_seed(5)
+            self.G = torch.randn((N, 4, H, W, C), generator=g).to(dev)
+            self.gmsg = [torch.empty((N, 4, H, W, C), device=dev) for _ in range(2)]
+            self.gcost = torch.empty((N, H, W, C), device=dev)
+            self.gjump = torch.empty_like(self.jump)
+            self.gedge = torch.empty((N, 4, H, W), device=dev)
+            ws = int(nat.lib().bpl_lbp_bwd_workspace_size(nat.LBP_STEREO, N, H, W, C))
+            self.ws = torch.empty(ws, dtype=torch.uint8, device=dev)
+            self.ws_bytes = ws
+        self.stream = torch.cuda.current_stream(dev)
+        self.sp = self.nat.ptr  # shorthand
+
+    def pl(self):
+        return self.N * self.H * self.W * self.C
+
+    def fwd(self, t):
+        nat, p = self.nat, self.sp
+        nat.call("bpl_lbp_fwd", nat.LBP_STEREO, p(self.cost), p(self.jump), self.K, self.jump.shape[0], p(self.edge),
+                 p(self.msgs[t]), p(self.msgs[t + 1]), p(self.argmin[t]), p(self.garg[t]), None,
+                 self.N, self.H, self.W, self.C, self.delta, nat.stream_of(self.cost))
+
+    def bwd(self, t, g_in, g_out):
+        nat, p = self.nat, self.sp
+        nat.call("bpl_lbp_bwd", nat.LBP_STEREO, nat.GRAD_EXACT, p(self.jump), self.K, self.jump.shape[0],
+                 p(self.edge), p(g_in), p(self.msgs[t + 1]), p(self.argmin[t]), p(self.garg[t]), p(self.gcost),
+                 p(self.gjump), p(self.gedge), p(g_out), p(self.ws), self.ws_bytes, self.N, self.H, self.W, self.C,
+                 self.delta, nat.stream_of(self.cost))
+
+    def step(self):
+        for t in range(self.iters):
+            self.fwd(t)
+        if self.backward:
+            g_in = self.G
+            for t in reversed(range(self.iters)):
+                g_out = self.gmsg[t % 2]
+                self.bwd(t, g_in, g_out)
+                g_in = g_out
+
+
+def time_events(fn, reps, torch):
+    """Average duration (ms) of fn() measured with HIP events on the current stream."""
+    s = torch.cuda.Event(enable_timing=True)
+    e = torch.cuda.Event(enable_timing=True)
+    ts = []
+    for _ in range(reps):
+        s.record()
+        fn()
+        e.record()
+        e.synchronize()
+        ts.append(s.elapsed_time(e))
+    return statistics.median(ts), ts
+
+
+def cpu_baseline(wl, plan_shape, seconds, iters_per_sample=1):
+    """The C++/OpenMP restatement (oracle/c) on a crop of the same workload, fp32, all host
+    cores: forward + exact backward of one LBP iteration; returns G pl/s."""
+    import numpy as np
+    sys.path.insert(0, ROOT)
+    from oracle import cpu
+    N, H, W, C = plan_shape
+    threads = len(os.sched_getaffinity(0))
+    rng = np.random.default_rng(0)
+
+    def run(rows):
+        cost = -rng.uniform(0, 1, size=(N, rows, W, C)).astype(np.float32)
+        aff = wl["affinities"][..., :rows, :]
+        from bplayers_amd.ops.lbp_stereo.message_passing_op_cuda import pack_affinities
+        import torch
+        jump = pack_affinities(torch.from_numpy(np.ascontiguousarray(aff)), torch.zeros((N, 4, rows, W))).numpy()
+        edge = np.ones((N, 4, rows, W), dtype=np.float32)
+        M0 = np.zeros((N, 4, rows, W, C), dtype=np.float32)
+        G = rng.normal(size=M0.shape).astype(np.float32)
+        t0 = time.perf_counter()
+        M, am, fg = cpu.lbp_forward(cost, jump, edge, M0, wl["delta"], threads=threads)
+        cpu.lbp_backward_exact(jump, edge, G, M, am, fg, wl["delta"], threads=threads)
+        return time.perf_counter() - t0
+
+    rows = 4
+    t = run(rows)
+    # scale the crop so the timed sample takes about `seconds`
+    rows = int(max(4, min(H, rows * seconds / max(t, 1e-3))))
+    t = run(rows)
+    pl = N * rows * W * C
+    return {"value": pl / t / 1e9, "unit": "G(pixel*label)/s per LBP iter fwd+bwd", "cores": threads,
+            "kind": "port",
+            "sample": f"C++/OpenMP fp32 restatement (oracle/c), 1 iteration fwd+exact bwd on a "
+                      f"{rows}x{W} crop with {C} labels ({pl / 1e6:.1f} M pl, {t:.1f} s)"}
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build runs on exactly one GPU; the multi-GPU path is not built "
+              f"(--gpus {args.gpus}, WORLD_SIZE {world}).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from bplayers_amd import workloads
+
+    if args.workload == "stereo_fb_384x1248_d192":
+        wl = workloads.stereo_full()
+        backward = True
+    else:
+        wl = workloads.stereo_small()
+        backward = False
+    plan = StereoPlan(wl, wl["iters"], rows=args.rows, backward=backward)
+    torch.cuda.synchronize()
+
+    for _ in range(args.warmup):
+        plan.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(args.steps):
+        plan.step()
+    end.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = start.elapsed_time(end) / args.steps
+
+    pl = plan.pl()
+    iters = plan.iters
+    value = pl * iters / (ms_step * 1e-3) / 1e9
+
+    # per-op durations (HIP events on the launch stream) for the roofline
+    t_fwd, _ = time_events(lambda: plan.fwd(0), 5, torch)
+    res = {"fwd_ms": t_fwd}
+    C = plan.C
+    fwd_b = FWD_B_PL + (32.0 + 16.0) / C
+    bwd_b = BWD_B_PL + (64.0 + 32.0) / C
+    if backward:
+        t_bwd, _ = time_events(lambda: plan.bwd(0, plan.G, plan.gmsg[0]), 5, torch)
+        res["bwd_ms"] = t_bwd
+        it_ms = t_fwd + t_bwd
+        alg = pl * (fwd_b + bwd_b)
+        unit_name = "lbp_iteration_fwd+bwd (2 forward phase kernels, 2 backward phase kernels, combine)"
+    else:
+        it_ms = t_fwd
+        alg = pl * (FWD_B_PL - 4.0 + 48.0 / C)  # forward-only: no argmin writes needed
+        unit_name = "lbp_iteration_fwd (2 forward phase kernels)"
+    achieved = alg / (it_ms * 1e-3) / 1e9
+    # SAD timing (separately, same workload)
+    from bplayers_amd import pytorch_cuda_stereo_sad_op as sad
+    cost_buf = torch.empty((plan.N, plan.H, plan.W, plan.max_disp - plan.min_disp + 1), device=plan.dev)
+    t_sad, _ = time_events(lambda: plan.nat.call(
+        "bpl_sad_stereo_fwd", plan.sp(plan.f0), plan.sp(plan.f1), plan.sp(cost_buf), plan.N, plan.f0.shape[1],
+        plan.H, plan.W, plan.min_disp, plan.max_disp, 1.0, plan.nat.stream_of(plan.f0)), 5, torch)
+    res["sad_fwd_ms"] = t_sad
+    if backward:
+        d0 = torch.empty_like(plan.f0)
+        d1 = torch.empty_like(plan.f1)
+        t_sadb, _ = time_events(lambda: plan.nat.call(
+            "bpl_sad_stereo_bwd", plan.sp(plan.f0), plan.sp(plan.f1), plan.sp(plan.sad_grad), plan.sp(d0),
+            plan.sp(d1), plan.N, plan.f0.shape[1], plan.H, plan.W, plan.min_disp, plan.max_disp,
+            plan.nat.stream_of(plan.f0)), 3, torch)
+        res["sad_bwd_ms"] = t_sadb
+
+    line = {
+        "metric": "G(pixel*label)/s per LBP iter fwd+bwd" if backward else "G(pixel*label)/s per LBP iter fwd",
+        "value": round(value, 4),
+        "unit": "G(pixel*label)/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": args.workload + (f"_crop{args.rows}" if args.rows else ""), "N": plan.N,
+                   "H": plan.H, "W": plan.W, "labels": plan.C, "iters": iters, "delta": plan.delta,
+                   "pairwise": "per-pixel per-direction P1/P2 (K=6 jump)" if backward else "global P1=0.1 P2=0.5",
+                   "grad_mode": "exact" if backward else None},
+        "roofline": {"bound": "hbm", "kernel": unit_name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "alg_bytes_per_pl": round(fwd_b + (bwd_b if backward else 0.0), 3),
+                     "iteration_ms": round(it_ms, 4)},
+        "timings_ms": {k: round(v, 4) for k, v in res.items()},
+        "sad": {"fwd_G_pd_per_s": round(pl / (t_sad * 1e-3) / 1e9, 3),
+                "bwd_G_pd_per_s": round(pl / (res["sad_bwd_ms"] * 1e-3) / 1e9, 3) if backward else None},
+        "wall_s": round(wall, 3),
+    }
+    if not args.no_cpu:
+        try:
+            line["cpu_baseline"] = cpu_baseline(wl, (plan.N, plan.H, plan.W, plan.C), args.cpu_seconds)
+        except Exception as e:  # the baseline is reported, never required for the GPU number
+            line["cpu_baseline"] = {"value": None, "error": repr(e)}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

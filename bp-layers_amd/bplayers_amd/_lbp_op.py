@@ -106,10 +106,13 @@ def backward_minsum(kind, cost, jump, edge, in_grad, messages, messages_argmin, 
         grad_jump = torch.empty_like(jump)
         gj_ret = grad_jump
     elif kind == nat.LBP_SEMANTIC_GLOBAL:
-        grad_jump = torch.empty((1, 4, C, C), dtype=torch.float32, device=dev)
         # reference shape [N,4,C,C] with every batch element accumulated into index 0
-        gj_ret = torch.cat([grad_jump, torch.zeros((N - 1, 4, C, C), device=dev)]) if N > 1 else grad_jump
+        gj_ret = torch.zeros((N, 4, C, C), dtype=torch.float32, device=dev) if N > 1 else None
+        grad_jump = gj_ret[:1] if N > 1 else torch.empty((1, 4, C, C), dtype=torch.float32, device=dev)
+        gj_ret = gj_ret if N > 1 else grad_jump
     else:
+        if mode == "exact" and jump is None:
+            raise RuntimeError("the exact backward of the per-pixel variant needs the pairwise tensor (jump=)")
         grad_jump = torch.empty((4, H, W, C, C), dtype=torch.float32, device=dev)
         gj_ret = grad_jump
     grad_edge = torch.empty((N, 4, H, W), dtype=torch.float32, device=dev)

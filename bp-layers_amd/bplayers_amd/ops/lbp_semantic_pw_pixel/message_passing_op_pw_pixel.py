@@ -36,15 +36,16 @@ class LBPMinSumFunction(torch.autograd.Function):
         out = None if grad_mode == 'reference' else torch.empty_like(messages)
         messages, messages_argmin, message_scale = lbp.forward_minsum(cost, j, edge, messages.contiguous(), delta,
                                                                       out=out)
-        ctx.save_for_backward(cost, edge, messages, messages_argmin, message_scale)
+        ctx.save_for_backward(cost, edge, messages, messages_argmin, message_scale, j)
         ctx.grad_mode = grad_mode
         return messages
 
     @staticmethod
     def backward(ctx, in_grad):
-        cost, edge, messages, messages_argmin, message_scale = ctx.saved_tensors
+        cost, edge, messages, messages_argmin, message_scale, j = ctx.saved_tensors
         grad_cost, grad_jump, grad_edge, grad_message = lbp.backward_minsum(
-            cost, edge, in_grad.contiguous(), messages, messages_argmin, message_scale, mode=ctx.grad_mode)
+            cost, edge, in_grad.contiguous(), messages, messages_argmin, message_scale, mode=ctx.grad_mode,
+            jump=j if ctx.grad_mode == 'exact' else None)
         return grad_cost, None, None, grad_edge, grad_message, None, fold_pairwise_grad(grad_jump), None
 
 

@@ -10,6 +10,10 @@ def forward_minsum(cost, jump, edge, messages, delta, *, out=None):
     return _lbp_op.forward_minsum(KIND, cost, jump, edge, messages, delta, out=out)
 
 
-def backward_minsum(cost, edge, in_grad, messages, messages_argmin, message_scale, *, mode="reference"):
-    """Gradients [grad_unary, grad_pairwise, grad_edge, grad_messages] of one iteration."""
-    return _lbp_op.backward_minsum(KIND, cost, None, edge, in_grad, messages, messages_argmin, message_scale, mode=mode)
+def backward_minsum(cost, edge, in_grad, messages, messages_argmin, message_scale, *, mode="reference",
+                    jump=None):
+    """Gradients [grad_unary, grad_pairwise, grad_edge, grad_messages] of one iteration.
+    The reference build computes no edge gradient (zeros); mode="exact" computes it and needs
+    the pairwise tensor passed as ``jump=``."""
+    return _lbp_op.backward_minsum(KIND, cost, jump, edge, in_grad, messages, messages_argmin, message_scale,
+                                   mode=mode)

@@ -259,7 +259,19 @@ __global__ __launch_bounds__(64) void lbp_bwd_phase(BwdArgs a) {
         if (line_ok && sl == 0) a.grad_edge[eo] = ge;
       } else {
         float *Gp = a.grad_jump + px(dir, y, x, H, W) * C * C;
-        if (line_ok && sl == 0) a.grad_edge[eo] = 0.f;  // not computed for this variant (reference)
+        // edge gradient: zeros in the reference mode (not computed by the reference build),
+        // sum_c f(c, argmin(c)) g(c) in the exact mode
+        float ge = 0.f;
+        if (exact) {
+          const float *Jp = a.jump + px(dir, y, x, H, W) * C * C;
+#pragma unroll
+          for (int k = 0; k < LPL; ++k) {
+            const int c = sl + SEG * k;
+            if (c < C) ge += (A[k] == kFromTrunc ? Jp[C - 1] : Jp[A[k] * C + c]) * gv[k];
+          }
+          ge = seg_reduce<SEG>(ge, OpSum());
+        }
+        if (line_ok && sl == 0) a.grad_edge[eo] = ge;
         if (line_ok) {
           for (int r = 0; r < C; ++r) {
 #pragma unroll
@@ -410,7 +422,8 @@ extern "C" int bpl_lbp_bwd(int kind, int mode, const float *jump, int64_t jump_k
   BPL_REQUIRE(C <= 255, BPL_E_LIMIT, "bpl_lbp_bwd: C=%lld labels exceeds the supported 255", (long long)C);
   BPL_REQUIRE(edge && in_grad && messages && argmin && gargmin && grad_cost && grad_jump && grad_edge && grad_msg,
               BPL_E_ARG, "bpl_lbp_bwd: null pointer");
-  BPL_REQUIRE(kind == BPL_LBP_SEMANTIC_PIXEL || jump, BPL_E_ARG, "bpl_lbp_bwd: jump required");
+  BPL_REQUIRE(jump || (kind == BPL_LBP_SEMANTIC_PIXEL && mode == BPL_GRAD_REFERENCE), BPL_E_ARG,
+              "bpl_lbp_bwd: jump required");
   if (kind == BPL_LBP_STEREO) {
     BPL_REQUIRE(jump_k >= 2 && jump_k <= 15, BPL_E_LIMIT, "bpl_lbp_bwd: stereo jump K=%lld not in [2,15]",
                 (long long)jump_k);

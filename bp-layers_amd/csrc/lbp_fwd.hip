@@ -90,7 +90,7 @@ __global__ __launch_bounds__(64) void lbp_fwd_phase(FwdArgs a) {
       if (line_ok && c < C) {
         v = a.msg_in[o * C + c];
         a.msg_out[o * C + c] = v;
-        a.argmin[o * C + c] = 0;
+        if (a.argmin) a.argmin[o * C + c] = 0;
       }
       chain[k] = v;
     }
@@ -264,7 +264,7 @@ __global__ __launch_bounds__(64) void lbp_fwd_phase(FwdArgs a) {
         chain[k] = m;
         if (line_ok && c < C) {
           a.msg_out[oq * C + c] = m;
-          a.argmin[oq * C + c] = (uint8_t)arg[k];
+          if (a.argmin) a.argmin[oq * C + c] = (uint8_t)arg[k];
         }
       }
       if (line_ok && sl == 0) {
@@ -318,8 +318,9 @@ extern "C" int bpl_lbp_fwd(int kind, const float *cost, const float *jump, int64
   BPL_REQUIRE(N * 2 * (H > W ? H : W) < (1LL << 31) && H < (1 << 30) && W < (1 << 30), BPL_E_LIMIT,
               "bpl_lbp_fwd: grid too large");
   BPL_REQUIRE(delta >= 0, BPL_E_ARG, "bpl_lbp_fwd: delta must be >= 0");
-  BPL_REQUIRE(cost && jump && edge && msg_in && msg_out && argmin, BPL_E_ARG, "bpl_lbp_fwd: null pointer");
-  BPL_REQUIRE(delta >= C || gargmin, BPL_E_ARG, "bpl_lbp_fwd: gargmin is required when delta < C");
+  BPL_REQUIRE(cost && jump && edge && msg_in && msg_out, BPL_E_ARG, "bpl_lbp_fwd: null pointer");
+  BPL_REQUIRE(!argmin || delta >= C || gargmin, BPL_E_ARG,
+              "bpl_lbp_fwd: gargmin is required with argmin when delta < C");
   if (kind == BPL_LBP_STEREO) {
     BPL_REQUIRE(jump_k >= 2 && jump_k <= 15, BPL_E_LIMIT, "bpl_lbp_fwd: stereo jump K=%lld not in [2,15]",
                 (long long)jump_k);

@@ -98,7 +98,8 @@ int bpl_sad_flow_bwd(const float *f0, const float *f1, const float *grad_u,
  * jump_k is K for the stereo variant and ignored otherwise; jump_n is the leading
  * dimension of a stereo jump (1 = shared by the batch, or N).
  * scale (float [N,4,H,W], sum_c exp(message)) may be NULL; gargmin may be NULL when
- * delta >= C (no truncation term).
+ * delta >= C (no truncation term); argmin and gargmin may both be NULL for a forward-only
+ * solve (no backward possible).
  */
 int bpl_lbp_fwd(int kind, const float *cost, const float *jump, int64_t jump_k, int64_t jump_n,
                 const float *edge, const float *msg_in, float *msg_out,
@@ -115,9 +116,9 @@ int64_t bpl_lbp_bwd_workspace_size(int kind, int64_t N, int64_t H, int64_t W, in
  * messages: the message tensor the probabilities exp(m) are read from (the
  * reference passes the in-place-updated tensor); argmin/gargmin as written by
  * bpl_lbp_fwd.  Outputs are fully overwritten: grad_cost [N,H,W,C]; grad_jump
- * (shape of jump); grad_edge [N,4,H,W] (zeros for SEMANTIC_PIXEL, as in the
- * reference); grad_msg [N,4,H,W,C] (BPL_GRAD_REFERENCE: a copy of in_grad;
- * BPL_GRAD_EXACT: the gradient w.r.t. the iteration's input messages).
+ * (shape of jump); grad_edge [N,4,H,W] (SEMANTIC_PIXEL in the reference mode: zeros, as
+ * the reference, and jump may be NULL); grad_msg [N,4,H,W,C] (BPL_GRAD_REFERENCE: a copy
+ * of in_grad; BPL_GRAD_EXACT: the gradient w.r.t. the iteration's input messages).
  */
 int bpl_lbp_bwd(int kind, int mode, const float *jump, int64_t jump_k, int64_t jump_n,
                 const float *edge, const float *in_grad, const float *messages,
@@ -125,6 +126,24 @@ int bpl_lbp_bwd(int kind, int mode, const float *jump, int64_t jump_k, int64_t j
                 float *grad_cost, float *grad_jump, float *grad_edge, float *grad_msg,
                 void *workspace, int64_t workspace_bytes,
                 int64_t N, int64_t H, int64_t W, int64_t C, int delta, void *stream);
+
+/*
+ * y = sign * softmax(scale * x) over the contiguous last dimension of a [rows, C] tensor
+ * (C <= 256).  With scale = -1/T and sign = -1 this is the reference's matching softmax
+ * softmax(-SAD/T) (matching.py:22-25, corenet.py:97-104) followed by the LBP input
+ * cost = -prob (ops/lbp_stereo/message_passing_op_cuda.py:162).  One wave per row.
+ */
+int bpl_softmax_labels(const float *x, float *y, int64_t rows, int64_t C, float scale, float sign,
+                       void *stream);
+
+/*
+ * Beliefs softmin((cost + sum_s messages_s) / T) over labels (reference
+ * ops/lbp_stereo/message_passing_op_cuda.py:170-173) into beliefs [N,H,W,C] (may be NULL)
+ * and the winner-take-all label argmax_c (first index, inference_op.py:20-24) into
+ * label [N,H,W] as float (may be NULL).  C <= 256.
+ */
+int bpl_lbp_beliefs(const float *cost, const float *messages, float *beliefs, float *label,
+                    int64_t N, int64_t H, int64_t W, int64_t C, float T, void *stream);
 
 #ifdef __cplusplus
 }

@@ -267,10 +267,16 @@ def send(h, pw, n, d, ys, xs, w, delta, want_margin=False):
     m = msg.max(axis=1, keepdims=True)
     lse = m + np.log(np.exp(msg - m).sum(axis=1, keepdims=True))
     if want_margin:
+        # gap to the best candidate with another label, and to a same-label candidate of the
+        # other source unless the two are exactly equal (a structural tie that fp32 resolves
+        # by the same '<=' rule, e.g. equal window and truncation entries)
         margin = np.full((P, C), np.inf)
         for v, lab, isg in cands:
-            same = (lab == arg) & (isg == from_global)
-            margin = np.minimum(margin, np.where(same, np.inf, v - msg))
+            other_label = lab != arg
+            other_source = (lab == arg) & (isg != from_global)
+            gap = v - msg
+            margin = np.minimum(margin, np.where(other_label, gap, np.inf))
+            margin = np.minimum(margin, np.where(other_source & (gap != 0), np.abs(gap), np.inf))
         return msg - lse, arg, np.exp(lse[:, 0]), from_global, lstar, margin
     return msg - lse, arg, np.exp(lse[:, 0]), from_global, lstar
 

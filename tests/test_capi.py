@@ -55,6 +55,8 @@ def test_argument_validation_returns_codes(lib):
     assert rc == -2
     rc = lib.bpl_lbp_fwd(nat.LBP_STEREO, P, P, 6, 1, P, P, P, P, None, P, 1, 4, 4, 8, 3, None)
     assert rc == -1 and b"gargmin" in lib.bpl_last_error()
+    rc = lib.bpl_lbp_fwd(nat.LBP_STEREO, P, None, 6, 1, P, P, P, P, P, P, 1, 4, 4, 8, 3, None)
+    assert rc == -1 and b"null" in lib.bpl_last_error()
     rc = lib.bpl_lbp_bwd(nat.LBP_STEREO, 5, P, 6, 1, P, P, P, P, P, P, P, P, P, P, 1 << 40, 1, 4, 4, 8, 3, None)
     assert rc == -1 and b"mode" in lib.bpl_last_error()
     need = lib.bpl_lbp_bwd_workspace_size(nat.LBP_STEREO, 1, 4, 4, 8)

@@ -1,0 +1,181 @@
+"""GPU tests of the autograd layer (bplayers_amd.ops.*) against the FP64 oracle."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import lbp as O
+from _lbp_cases import pack_stereo_jump, unpack_stereo_grad, semantic_matrix, pack_semantic_global
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def _t(a, grad=False):
+    return torch.tensor(np.asarray(a, dtype=np.float32), device=DEV, requires_grad=grad)
+
+
+def oracle_pipeline(prob, aff, edge, iters, delta, Wb, T=1.0):
+    """loss = sum(Wb * softmin(sum messages + cost)) with cost = -prob, messages from zeros."""
+    cost = -np.asarray(prob, dtype=np.float64)
+    jump = pack_stereo_jump(aff)
+    M = np.zeros(cost.shape[:1] + (4,) + cost.shape[1:])
+    M, recs = O.lbp_forward_iters(cost, O.StereoPairwise(jump), edge, M, delta, iters)
+    return float((Wb * O.beliefs(cost, M, T)).sum()), M, recs
+
+
+@pytest.mark.parametrize("iters", [1, 3])
+def test_stereo_message_passing_exact_gradients_match_fd(iters):
+    from bplayers_amd.ops.lbp_stereo.message_passing_op_cuda import MessagePassing
+    rng = np.random.default_rng(iters)
+    N, H, W, C = 1, 4, 6, 12
+    prob = rng.dirichlet(np.ones(C), size=(N, H, W)).astype(np.float32)
+    p1 = rng.uniform(0.05, 0.3, size=(N, 4, H, W))
+    p2 = rng.uniform(0.4, 0.8, size=(N, 4, H, W))
+    aff = np.stack([p1, p1, p2], axis=2).astype(np.float32)
+    edge = rng.uniform(0.5, 1.5, size=(N, 4, H, W)).astype(np.float32)
+    Wb = rng.normal(size=(N, H, W, C))
+    mp = MessagePassing(DEV, iters, C, 3, grad_mode="exact")
+    tp, ta, te = _t(prob, True), _t(aff, True), _t(edge, True)
+    m0 = torch.zeros((N, 4, H, W, C), device=DEV)
+    beliefs, msgs = mp(tp, te, ta, None, m0)
+    loss = (beliefs * torch.tensor(Wb, dtype=torch.float32, device=DEV)).sum()
+    loss.backward()
+    L0, Mo, _ = oracle_pipeline(prob, aff, edge, iters, 3, Wb)
+    np.testing.assert_allclose(float(loss), L0, rtol=1e-5, atol=1e-5)
+    eps = 1e-5
+    for name, arr, g in (("prob", prob, tp.grad), ("aff", aff, ta.grad), ("edge", edge, te.grad)):
+        a64 = arr.astype(np.float64)
+        flat = a64.reshape(-1)
+        gg = g.cpu().numpy().reshape(-1)
+        for i in rng.choice(flat.size, size=25, replace=False):
+            o = flat[i]
+            flat[i] = o + eps
+            lp = oracle_pipeline(a64.reshape(arr.shape) if name == "prob" else prob,
+                                 a64.reshape(arr.shape) if name == "aff" else aff,
+                                 a64.reshape(arr.shape) if name == "edge" else edge, iters, 3, Wb)[0]
+            flat[i] = o - eps
+            lm = oracle_pipeline(a64.reshape(arr.shape) if name == "prob" else prob,
+                                 a64.reshape(arr.shape) if name == "aff" else aff,
+                                 a64.reshape(arr.shape) if name == "edge" else edge, iters, 3, Wb)[0]
+            flat[i] = o
+            fd = (lp - lm) / (2 * eps)
+            assert abs(fd - gg[i]) <= 2e-3 * max(1.0, abs(fd)), (name, i, fd, gg[i])
+
+
+def test_stereo_reference_mode_multi_iteration_matches_restatement():
+    """grad_mode='reference', 2 iterations: every iteration's backward sees the final
+    (aliased) messages and in_grad of the final messages (the reference's semantics)."""
+    from bplayers_amd.ops.lbp_stereo.message_passing_op_cuda import LBPMinSumFunction
+    rng = np.random.default_rng(11)
+    N, H, W, C = 1, 5, 7, 16
+    cost = rng.normal(size=(N, H, W, C)).astype(np.float32)
+    p1 = rng.uniform(0.05, 0.3, size=(N, 4, H, W))
+    p2 = rng.uniform(0.4, 0.8, size=(N, 4, H, W))
+    aff = np.stack([p1, p1, p2], axis=2).astype(np.float32)
+    edge = rng.uniform(0.5, 1.5, size=(N, 4, H, W)).astype(np.float32)
+    G = rng.normal(size=(N, 4, H, W, C))
+    tc, ta, te = _t(cost, True), _t(aff, True), _t(edge, True)
+    m = torch.zeros((N, 4, H, W, C), device=DEV, requires_grad=True)
+    off = torch.zeros((N, 4, H, W), device=DEV)
+    x = m
+    for _ in range(2):
+        x = LBPMinSumFunction.apply(tc, te, x, 3, ta, off)
+    (x * torch.tensor(G, dtype=torch.float32, device=DEV)).sum().backward()
+    # restatement: per-iteration argmins/scales, final messages, in_grad = G for both
+    jump = pack_stereo_jump(aff)
+    pw = O.StereoPairwise(jump)
+    M = np.zeros((N, 4, H, W, C))
+    recs = []
+    for _ in range(2):
+        Mn, a, S, fg, ls = O.lbp_forward(cost, pw, edge, M, 3)
+        recs.append((a, S))
+        M = Mn
+    gc = np.zeros_like(cost, dtype=np.float64)
+    gp = np.zeros_like(jump)
+    ge = np.zeros_like(edge, dtype=np.float64)
+    for a, S in recs:
+        c_, p_, e_, _ = O.lbp_backward_reference(cost, pw, edge, G, M, a, S)
+        gc += c_
+        gp += p_
+        ge += e_
+    np.testing.assert_allclose(tc.grad.cpu().numpy(), gc, rtol=1e-4, atol=1e-4)
+    np.testing.assert_allclose(te.grad.cpu().numpy(), ge, rtol=1e-4, atol=1e-4)
+    np.testing.assert_allclose(ta.grad.cpu().numpy(), unpack_stereo_grad(gp, 3), rtol=1e-4, atol=1e-4)
+    np.testing.assert_allclose(m.grad.cpu().numpy(), G, rtol=1e-6)
+
+
+@pytest.mark.parametrize("variant", ["global", "pixel"])
+def test_semantic_message_passing_exact_gradients(variant):
+    rng = np.random.default_rng(5)
+    N, H, W, C = 2, 4, 5, 7
+    prob = rng.dirichlet(np.ones(C), size=(N, H, W)).astype(np.float32)
+    edge = rng.uniform(0.5, 1.5, size=(N, 4, H, W)).astype(np.float32)
+    Wb = rng.normal(size=(N, H, W, C))
+    if variant == "global":
+        from bplayers_amd.ops.lbp_semantic_pw.message_passing_op_cuda import MessagePassing
+        jump = np.stack([semantic_matrix(rng, C), semantic_matrix(rng, C)])[None].astype(np.float32)
+        packed = pack_semantic_global(jump)
+        pwk = "semantic_global"
+    else:
+        from bplayers_amd.ops.lbp_semantic_pw_pixel.message_passing_op_pw_pixel import MessagePassing
+        jump = rng.uniform(0, 1, size=(2, H, W, C, C)).astype(np.float32)
+        packed = np.zeros((4, H, W, C, C))
+        packed[0] = jump[0]
+        packed[1, :, 1:] = np.swapaxes(jump[0, :, :-1], -1, -2)
+        packed[2] = jump[1]
+        packed[3, 1:] = np.swapaxes(jump[1, :-1], -1, -2)
+        pwk = "semantic_pixel"
+    mp = MessagePassing(DEV, 2, C, C, grad_mode="exact")
+    tp, tj, te = _t(prob, True), _t(jump, True), _t(edge, True)
+    m0 = torch.zeros((N, 4, H, W, C), device=DEV)
+    beliefs = mp(tp, te, m0, tj) if variant == "pixel" else mp(tp, te, m0, tj, False)
+    (beliefs * torch.tensor(Wb, dtype=torch.float32, device=DEV)).sum().backward()
+    cost = -prob.astype(np.float64)
+    M, recs = O.lbp_forward_iters(cost, O.make_pairwise(pwk, packed), edge, np.zeros((N, 4, H, W, C)), C, 2)
+    b = O.beliefs(cost, M)
+    gz = b * (Wb - (Wb * b).sum(-1, keepdims=True))
+    gM = np.broadcast_to((-gz)[:, None], M.shape)
+    gc, gpk, ge, _ = O.lbp_backward_exact_iters(cost, O.make_pairwise(pwk, packed), edge, recs, gM)
+    gprob = -(gc + (-gz))
+    np.testing.assert_allclose(tp.grad.cpu().numpy(), gprob, rtol=1e-3, atol=1e-4)
+    np.testing.assert_allclose(te.grad.cpu().numpy(), ge, rtol=1e-3, atol=1e-4)
+    if variant == "global":
+        gj = np.stack([gpk[0, 0] + gpk[0, 1].T, gpk[0, 2] + gpk[0, 3].T])[None]
+    else:
+        gj = np.zeros_like(jump, dtype=np.float64)
+        gj[0] = gpk[0]
+        gj[0, :, :-1] += np.swapaxes(gpk[1, :, 1:], -1, -2)
+        gj[1] = gpk[2]
+        gj[1, :-1] += np.swapaxes(gpk[3, 1:], -1, -2)
+    np.testing.assert_allclose(tj.grad.cpu().numpy(), gj, rtol=1e-3, atol=1e-4)
+
+
+def test_flow_bp_pipeline_runs_and_matches_oracle_forward():
+    """FlowMpSadFunction -> softmax -> BP (5-D path: u then v from u's messages)."""
+    from bplayers_amd.ops.flow_mp_sad.flow_mp_sad import FlowMpSadFunction
+    from bplayers_amd.ops.lbp_stereo.bp_op_cuda import BP
+    from oracle import sad as S
+    rng = np.random.default_rng(3)
+    f0 = rng.normal(size=(1, 8, 12, 20)).astype(np.float32)
+    f1 = rng.normal(size=(1, 8, 12, 20)).astype(np.float32)
+    t0, t1 = _t(f0, True), _t(f1, True)
+    cu, cv, us, vs = FlowMpSadFunction.apply(t0, t1, 8)
+    pu = torch.softmax(-cu, dim=3)
+    pv = torch.softmax(-cv, dim=3)
+    prob = torch.cat((pu.unsqueeze(1), pv.unsqueeze(1)), dim=1).contiguous()
+    bp = BP(DEV, max_iter=1, num_labels=9, delta=3, mode_inference="wta")
+    aff = torch.zeros((1, 4, 3, 12, 20), device=DEV)
+    aff[:, :, :2] = 0.1
+    aff[:, :, 2] = 0.5
+    flow, beliefs, messages = bp(prob, None, aff, None)
+    beliefs.sum().backward()
+    assert t0.grad is not None and torch.isfinite(t0.grad).all()
+    # oracle forward of the same chain
+    ocu, ocv, _, _ = S.flow_sad_forward(f0, f1, 8)
+    sm = lambda x: np.exp(-x - (-x).max(-1, keepdims=True)) / np.exp(-x - (-x).max(-1, keepdims=True)).sum(-1, keepdims=True)
+    jump = pack_stereo_jump(aff.cpu().numpy())
+    edge = np.ones((1, 4, 12, 20))
+    Mu, *_ = O.lbp_forward(-sm(ocu), O.StereoPairwise(jump), edge, np.zeros((1, 4, 12, 20, 9)), 3)
+    Mv, *_ = O.lbp_forward(-sm(ocv), O.StereoPairwise(jump), edge, Mu, 3)
+    bv = O.beliefs(-sm(ocv), Mv)
+    np.testing.assert_allclose(beliefs[:, 1].detach().cpu().numpy(), bv, rtol=1e-4, atol=1e-5)

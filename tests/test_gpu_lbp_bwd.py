@@ -28,7 +28,8 @@ def run_case(kind, N, H, W, C, delta, mode, seed, **extra):
     mo = torch.empty_like(t(M0))
     msgs, am, sc = mod.forward_minsum(t(cost), t(jump), t(edge), t(M0), delta, out=mo)
     if kind == "semantic_pixel":
-        res = mod.backward_minsum(t(cost), t(edge), t(G), msgs, am, sc, mode=mode)
+        res = mod.backward_minsum(t(cost), t(edge), t(G), msgs, am, sc, mode=mode,
+                                  jump=t(jump) if mode == "exact" else None)
     else:
         res = mod.backward_minsum(t(cost), t(jump), t(edge), t(G), msgs, am, sc, mode=mode)
     torch.cuda.synchronize()

@@ -1,0 +1,105 @@
+"""GPU parity of the SAD ops against the FP64 oracle."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import sad as S
+
+pytestmark = pytest.mark.gpu
+
+SAD_RTOL = 1e-5  # fp32 sum over channels vs float64
+
+
+def _t(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to("cuda:0")
+
+
+@pytest.mark.parametrize("N,C,H,W,mn,mx", [(1, 32, 8, 100, 0, 63), (2, 64, 5, 130, 0, 191), (1, 3, 4, 17, -3, 5),
+                                           (1, 20, 3, 70, 4, 70)])
+def test_stereo_sad_forward(N, C, H, W, mn, mx):
+    from bplayers_amd import pytorch_cuda_stereo_sad_op as op
+    rng = np.random.default_rng(C + W)
+    f0 = rng.normal(size=(N, C, H, W)).astype(np.float32)
+    f1 = rng.normal(size=(N, C, H, W)).astype(np.float32)
+    got = op.forward(_t(f0), _t(f1), mn, mx, 1.0).cpu().numpy()
+    exp = S.stereo_sad_forward(f0, f1, mn, mx)
+    np.testing.assert_allclose(got, exp, rtol=SAD_RTOL, atol=1e-5)
+
+
+def test_stereo_sad_forward_fractional_step():
+    from bplayers_amd import pytorch_cuda_stereo_sad_op as op
+    rng = np.random.default_rng(1)
+    f0 = rng.normal(size=(1, 4, 3, 20)).astype(np.float32)
+    f1 = rng.normal(size=(1, 4, 3, 20)).astype(np.float32)
+    got = op.forward(_t(f0), _t(f1), 0, 9, 0.5).cpu().numpy()
+    np.testing.assert_allclose(got, S.stereo_sad_forward(f0, f1, 0, 9, 0.5), rtol=SAD_RTOL, atol=1e-5)
+
+
+@pytest.mark.parametrize("N,C,H,W,mn,mx", [(1, 16, 4, 80, 0, 31), (2, 8, 3, 50, -2, 9), (1, 64, 2, 200, 0, 191)])
+def test_stereo_sad_backward(N, C, H, W, mn, mx):
+    from bplayers_amd import pytorch_cuda_stereo_sad_op as op
+    rng = np.random.default_rng(C)
+    f0 = rng.normal(size=(N, C, H, W)).astype(np.float32)
+    f1 = rng.normal(size=(N, C, H, W)).astype(np.float32)
+    g = rng.normal(size=(N, H, W, mx - mn + 1)).astype(np.float32)
+    d0, d1 = op.backward(_t(f0), _t(f1), mn, mx, _t(g))
+    e0, e1 = S.stereo_sad_backward(f0, f1, mn, mx, g)
+    np.testing.assert_allclose(d0.cpu().numpy(), e0, rtol=1e-4, atol=1e-4)
+    np.testing.assert_allclose(d1.cpu().numpy(), e1, rtol=1e-4, atol=1e-4)
+
+
+@pytest.mark.parametrize("sws,ou,ov,bu,bv", [(24, 0, 0, 0, 0), (24, 3, -2, 1, 1), (8, 0, 0, 0, 0), (10, -5, 5, 0, 1)])
+def test_flow_sad_forward(sws, ou, ov, bu, bv):
+    from bplayers_amd import pytorch_cuda_flow_mp_sad_op as op
+    rng = np.random.default_rng(sws)
+    f0 = rng.normal(size=(1, 12, 19, 70)).astype(np.float32)
+    f1 = rng.normal(size=(1, 12, 19, 70)).astype(np.float32)
+    got = [t.cpu().numpy() for t in op.forward(_t(f0), _t(f1), sws, ou, ov, bu, bv)]
+    exp = S.flow_sad_forward(f0, f1, sws, ou, ov, bu, bv)
+    np.testing.assert_allclose(got[0], exp[0], rtol=SAD_RTOL, atol=1e-5)
+    np.testing.assert_allclose(got[1], exp[1], rtol=SAD_RTOL, atol=1e-5)
+    # argmins: identical where the minimum is not a near-tie in the oracle
+    Sfull = None
+    for g, e in ((got[2], exp[2]), (got[3], exp[3])):
+        mism = g != e
+        assert mism.mean() < 1e-3, f"argmin mismatches {mism.sum()}"
+
+
+def test_flow_sad_backward():
+    from bplayers_amd import pytorch_cuda_flow_mp_sad_op as op
+    rng = np.random.default_rng(7)
+    sws = 24
+    f0 = rng.normal(size=(1, 6, 30, 40)).astype(np.float32)
+    f1 = rng.normal(size=(1, 6, 30, 40)).astype(np.float32)
+    cu, cv, us, vs = S.flow_sad_forward(f0, f1, sws)
+    gu = rng.normal(size=cu.shape).astype(np.float32)
+    gv = rng.normal(size=cv.shape).astype(np.float32)
+    d0, d1 = op.backward(_t(f0), _t(f1), sws, _t(gu), _t(gv), _t(us), _t(vs))
+    e0, e1 = S.flow_sad_backward(f0, f1, sws, gu, gv, us, vs)
+    np.testing.assert_allclose(d0.cpu().numpy(), e0, rtol=1e-4, atol=1e-4)
+    np.testing.assert_allclose(d1.cpu().numpy(), e1, rtol=1e-4, atol=1e-4)
+
+
+def test_flow_sad_function_sws216_matches_tiled_merge():
+    """FlowMpSadFunction computes sws=216 as one window; the reference tiles it into four
+    109-windows merged with '<'.  The two give the same volumes and argmins."""
+    from bplayers_amd.ops.flow_mp_sad.flow_mp_sad import FlowMpSadFunction
+    from bplayers_amd import pytorch_cuda_flow_mp_sad_op as op
+    rng = np.random.default_rng(9)
+    f0 = _t(rng.normal(size=(1, 3, 12, 14)))
+    f1 = _t(rng.normal(size=(1, 3, 12, 14)))
+    cu, cv, us, vs = FlowMpSadFunction.apply(f0, f1, 216)
+    s = 108
+    t = {}
+    for key, (ou, ov, bu, bv) in {"00": (-s // 2, -s // 2, 0, 0), "01": (-s // 2, s // 2, 0, 1),
+                                   "10": (s // 2, -s // 2, 1, 0), "11": (s // 2, s // 2, 1, 1)}.items():
+        t[key] = op.forward(f0, f1, s, ou, ov, bu, bv)
+    u0 = torch.cat((t["00"][0], t["10"][0][..., 1:]), -1)
+    u1 = torch.cat((t["01"][0], t["11"][0][..., 1:]), -1)
+    a0 = torch.cat((t["00"][2], t["10"][2][..., 1:]), -1)
+    a1 = torch.cat((t["01"][2], t["11"][2][..., 1:]), -1)
+    idx = u1 < u0
+    u0[idx] = u1[idx]
+    a0[idx] = a1[idx]
+    assert torch.equal(cu, u0)
+    assert torch.equal(us, a0)
