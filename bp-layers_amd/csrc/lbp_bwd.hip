@@ -109,25 +109,22 @@ __global__ __launch_bounds__(64) void lbp_bwd_phase(BwdArgs a) {
     const int64_t pq = px(n, yq, xq, H, W) * C;
 #pragma unroll
     for (int k = 0; k < LPL; ++k) {
-      const int c = sl + SEG * k;
-      const bool ok = c < C;
-      dg[k] = ok ? a.in_grad[oq * C + c] : 0.f;
-      dm[k] = ok ? a.msgs[oq * C + c] : 0.f;
-      da[k] = ok ? (int)a.argmin[oq * C + c] : 0;
+      const int c = min(sl + SEG * k, C - 1);  // clamped: loads stay unconditional
+      dg[k] = a.in_grad[oq * C + c];
+      dm[k] = a.msgs[oq * C + c];
+      da[k] = (int)a.argmin[oq * C + c];
       if (!VERT) {
-        dd[k] = ok ? a.X[DOWN][pq + c] : 0.f;
-        du[k] = ok ? a.X[UP][pq + c] : 0.f;
+        dd[k] = a.X[DOWN][pq + c];
+        du[k] = a.X[UP][pq + c];
       } else {
         dd[k] = 0.f;
         du[k] = 0.f;
       }
     }
-    dl = a.gargmin ? (int)a.gargmin[oq] : 0;
+    dl = (int)a.gargmin[oq];
     const int64_t eo = sl_px(n, dir, y, x, H, W);
-    if (KIND == BPL_LBP_STEREO)
-      dp = sl < K ? a.jump[(((int64_t)nj * 4 + dir) * K + sl) * HW + (int64_t)y * W + x] : a.edge[eo];
-    else
-      dp = a.edge[eo];
+    const int j = (KIND == BPL_LBP_STEREO && sl < K) ? sl : -1;
+    dp = j >= 0 ? a.jump[(((int64_t)nj * 4 + dir) * K + j) * HW + (int64_t)y * W + x] : a.edge[eo];
   };
 
   if (nsteps > 0) {  // a single-pixel scanline has no receiver to read
@@ -148,16 +145,10 @@ __global__ __launch_bounds__(64) void lbp_bwd_phase(BwdArgs a) {
       const int t = t0 + i;
       if (t >= nsteps) break;
       const int s = nsteps - 1 - t;  // adjoint order
-      float G[LPL], Mq[LPL], XD[LPL], XU[LPL];
-      int A[LPL];
-#pragma unroll
-      for (int k = 0; k < LPL; ++k) { G[k] = pg[i][k]; Mq[k] = pm[i][k]; A[k] = pa[i][k]; XD[k] = pd[i][k]; XU[k] = pu[i][k]; }
+      float *G = pg[i], *Mq = pm[i], *XD = pd[i], *XU = pu[i];
+      int *A = pa[i];
       const float pv = pp[i];
       const int gl = pl[i];
-      {
-        const int sn = s - PF >= 0 ? s - PF : 0;
-        load_step(sn, pg[i], pm[i], pa[i], pd[i], pu[i], pp[i], pl[i]);
-      }
       const int buf = t & 1;
       int y, x;
       yx_of(pos_of(s), y, x);
@@ -299,6 +290,8 @@ __global__ __launch_bounds__(64) void lbp_bwd_phase(BwdArgs a) {
         const int c = sl + SEG * k;
         if (line_ok && c < C) X[op + c] = gh[k];
       }
+      // refill ring slot i after its last use (no register copies, no drain of the ring)
+      load_step(s - PF >= 0 ? s - PF : 0, pg[i], pm[i], pa[i], pd[i], pu[i], pp[i], pl[i]);
     }
   }
 }
