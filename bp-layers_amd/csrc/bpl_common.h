@@ -37,8 +37,6 @@ constexpr int kDppXor1 = 0xB1;
 constexpr int kDppXor2 = 0x4E;
 constexpr int kDppHalfMirror = 0x141;
 constexpr int kDppMirror = 0x140;
-// ds_swizzle bit-mask mode: and=0x1f, or=0, xor=0x10 -> lane ^ 16 inside 32-lane halves
-constexpr int kSwzXor16 = 0x401F;
 
 template <int CTRL>
 __device__ __forceinline__ int dpp_i(int v) {
@@ -48,10 +46,85 @@ template <int CTRL>
 __device__ __forceinline__ float dpp_f(float v) {
   return __int_as_float(dpp_i<CTRL>(__float_as_int(v)));
 }
-__device__ __forceinline__ int swz16_i(int v) { return __builtin_amdgcn_ds_swizzle(v, kSwzXor16); }
-__device__ __forceinline__ float swz16_f(float v) { return __int_as_float(swz16_i(__float_as_int(v))); }
 __device__ __forceinline__ float readlane_f(float v, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// gfx950 row/half swaps with both operands = v: `lo` receives the even 16-lane row (or the
+// lower 32-lane half) of each pair, `hi` the odd row (upper half), in every lane of the pair.
+__device__ __forceinline__ void swap_rows(int v, int &lo, int &hi) {
+  const auto r = __builtin_amdgcn_permlane16_swap((unsigned)v, (unsigned)v, false, false);
+  lo = (int)r[0];
+  hi = (int)r[1];
+}
+__device__ __forceinline__ void swap_halves(int v, int &lo, int &hi) {
+  const auto r = __builtin_amdgcn_permlane32_swap((unsigned)v, (unsigned)v, false, false);
+  lo = (int)r[0];
+  hi = (int)r[1];
+}
+__device__ __forceinline__ void swap_rows(float v, float &lo, float &hi) {
+  int a, b;
+  swap_rows(__float_as_int(v), a, b);
+  lo = __int_as_float(a);
+  hi = __int_as_float(b);
+}
+__device__ __forceinline__ void swap_halves(float v, float &lo, float &hi) {
+  int a, b;
+  swap_halves(__float_as_int(v), a, b);
+  lo = __int_as_float(a);
+  hi = __int_as_float(b);
+}
+
+// Order-preserving integer key of a float (signed compare of keys == float compare, with
+// -0 < +0); key_to_float inverts it bit for bit.
+__device__ __forceinline__ int float_key(float f) {
+  const int b = __float_as_int(f);
+  return b ^ ((b >> 31) & 0x7FFFFFFF);
+}
+__device__ __forceinline__ float key_to_float(int k) { return __int_as_float(k ^ ((k >> 31) & 0x7FFFFFFF)); }
+
+// Full-wave all-reduces whose row stages fold the DPP move into the arithmetic instruction
+// (bound_ctrl set: every lane of these patterns has a valid source).
+template <int CTRL>
+__device__ __forceinline__ int dpp_src_i(int v) {
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, true);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_src_f(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+__device__ __forceinline__ int wave_min_i(int v) {
+  v = min(v, dpp_src_i<kDppXor1>(v));
+  v = min(v, dpp_src_i<kDppXor2>(v));
+  v = min(v, dpp_src_i<kDppHalfMirror>(v));
+  v = min(v, dpp_src_i<kDppMirror>(v));
+  int lo, hi;
+  swap_rows(v, lo, hi);
+  v = min(lo, hi);
+  swap_halves(v, lo, hi);
+  return min(lo, hi);
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+  v = max(v, dpp_src_i<kDppXor1>(v));
+  v = max(v, dpp_src_i<kDppXor2>(v));
+  v = max(v, dpp_src_i<kDppHalfMirror>(v));
+  v = max(v, dpp_src_i<kDppMirror>(v));
+  int lo, hi;
+  swap_rows(v, lo, hi);
+  v = max(lo, hi);
+  swap_halves(v, lo, hi);
+  return max(lo, hi);
+}
+__device__ __forceinline__ float wave_sum_f(float v) {
+  v = v + dpp_src_f<kDppXor1>(v);
+  v = v + dpp_src_f<kDppXor2>(v);
+  v = v + dpp_src_f<kDppHalfMirror>(v);
+  v = v + dpp_src_f<kDppMirror>(v);
+  float lo, hi;
+  swap_rows(v, lo, hi);
+  v = lo + hi;
+  swap_halves(v, lo, hi);
+  return lo + hi;
 }
 
 // All-reduce over aligned segments of SEG lanes (16, 32 or 64); every lane of a segment
@@ -73,8 +146,16 @@ __device__ __forceinline__ float seg_reduce(float v, Op op) {
   v = op(v, dpp_f<kDppXor2>(v));
   v = op(v, dpp_f<kDppHalfMirror>(v));
   v = op(v, dpp_f<kDppMirror>(v));
-  if constexpr (SEG >= 32) v = op(v, swz16_f(v));
-  if constexpr (SEG == 64) v = op(readlane_f(v, 0), readlane_f(v, 32));
+  if constexpr (SEG >= 32) {
+    float lo, hi;
+    swap_rows(v, lo, hi);
+    v = op(lo, hi);
+  }
+  if constexpr (SEG == 64) {
+    float lo, hi;
+    swap_halves(v, lo, hi);
+    v = op(lo, hi);
+  }
   return v;
 }
 
@@ -91,10 +172,20 @@ __device__ __forceinline__ void seg_argmin(float &v, int &k) {
   argmin_step(v, k, dpp_f<kDppXor2>(v), dpp_i<kDppXor2>(k));
   argmin_step(v, k, dpp_f<kDppHalfMirror>(v), dpp_i<kDppHalfMirror>(k));
   argmin_step(v, k, dpp_f<kDppMirror>(v), dpp_i<kDppMirror>(k));
-  if constexpr (SEG >= 32) argmin_step(v, k, swz16_f(v), swz16_i(k));
+  if constexpr (SEG >= 32) {
+    float v0, v1;
+    int k0, k1;
+    swap_rows(v, v0, v1);
+    swap_rows(k, k0, k1);
+    argmin_step(v0, k0, v1, k1);
+    v = v0;
+    k = k0;
+  }
   if constexpr (SEG == 64) {
-    float v0 = readlane_f(v, 0), v1 = readlane_f(v, 32);
-    int k0 = __builtin_amdgcn_readlane(k, 0), k1 = __builtin_amdgcn_readlane(k, 32);
+    float v0, v1;
+    int k0, k1;
+    swap_halves(v, v0, v1);
+    swap_halves(k, k0, k1);
     argmin_step(v0, k0, v1, k1);
     v = v0;
     k = k0;

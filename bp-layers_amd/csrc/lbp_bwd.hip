@@ -132,6 +132,8 @@ __global__ __launch_bounds__(64) void lbp_bwd_phase(BwdArgs a) {
     for (int i = 0; i < PF; ++i) {
       const int s = nsteps - 1 - i;
       load_step(s >= 0 ? s : 0, pg[i], pm[i], pa[i], pd[i], pu[i], pp[i], pl[i]);
+      // keep the slots in issue order: the loop's wait counts assume slot 0 is the oldest
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
 
@@ -143,152 +145,153 @@ __global__ __launch_bounds__(64) void lbp_bwd_phase(BwdArgs a) {
 #pragma unroll
     for (int i = 0; i < PF; ++i) {
       const int t = t0 + i;
-      if (t >= nsteps) break;
       const int s = nsteps - 1 - t;  // adjoint order
-      float *G = pg[i], *Mq = pm[i], *XD = pd[i], *XU = pu[i];
-      int *A = pa[i];
-      const float pv = pp[i];
-      const int gl = pl[i];
-      const int buf = t & 1;
-      int y, x;
-      yx_of(pos_of(s), y, x);
+      if (t < nsteps) {
+        float *G = pg[i], *Mq = pm[i], *XD = pd[i], *XU = pu[i];
+        int *A = pa[i];
+        const float pv = pp[i];
+        const int gl = pl[i];
+        const int buf = t & 1;
+        int y, x;
+        yx_of(pos_of(s), y, x);
 
-      // normalisation Jacobian: g = t - softmax(msg) * sum(t), t = in_grad + chain
-      float tt[LPL], ts = 0.f;
-#pragma unroll
-      for (int k = 0; k < LPL; ++k) {
-        tt[k] = G[k] + acc[k];
-        if (sl + SEG * k < C) ts += tt[k];
-      }
-      ts = seg_reduce<SEG>(ts, OpSum());
-      float gv[LPL];
-      float dmax = 0.f, fsum = 0.f;
-#pragma unroll
-      for (int k = 0; k < LPL; ++k) {
-        const int c = sl + SEG * k;
-        const bool ok = c < C;
-        float gk = tt[k] - fast_exp(Mq[k]) * ts;
-        if (!VERT) gk += XD[k] + XU[k];
-        gk = ok ? gk : 0.f;
-        gv[k] = gk;
-        const bool mk = A[k] == kFromTrunc;
-        if (ok) {
-          lds_g[buf][seg][c] = gk;
-          lds_a[buf][seg][c] = mk ? -1 : A[k];
-          if (mk) fsum += gk;
-          else dmax = fmaxf(dmax, (float)abs(A[k] - c));
-        }
-      }
-      if (sl < 16) lds_p[buf][seg][sl] = pv;
-      fsum = seg_reduce<SEG>(fsum, OpSum());
-      dmax = seg_reduce<64>(dmax, OpMax());
-      __syncthreads();
-      const float w = lds_p[buf][seg][KIND == BPL_LBP_STEREO ? K : 0];
-
-      // ---- route to the sender labels (deterministic gather)
-      const int D = (int)dmax;
-      float gh[LPL];
-#pragma unroll
-      for (int k = 0; k < LPL; ++k) gh[k] = 0.f;
-      for (int kk = -D; kk <= D; ++kk) {
+        // normalisation Jacobian: g = t - softmax(msg) * sum(t), t = in_grad + chain
+        float tt[LPL], ts = 0.f;
 #pragma unroll
         for (int k = 0; k < LPL; ++k) {
-          const int l = sl + SEG * k;
-          const int c = l - kk;
-          if (l < C && c >= 0 && c < C && lds_a[buf][seg][c] == l) gh[k] += lds_g[buf][seg][c];
+          tt[k] = G[k] + acc[k];
+          if (sl + SEG * k < C) ts += tt[k];
         }
-      }
-#pragma unroll
-      for (int k = 0; k < LPL; ++k)
-        if (sl + SEG * k == gl) gh[k] += fsum;
-
-      // ---- parameter gradients of (dir, sender)
-      const int64_t eo = sl_px(n, dir, y, x, H, W);
-      if (KIND == BPL_LBP_STEREO) {
-        float ge = 0.f;
-        for (int b = 1; b < K; ++b) {
-          float v = 0.f;
-#pragma unroll
-          for (int k = 0; k < LPL; ++k) {
-            const int c = sl + SEG * k;
-            if (c < C) {
-              const bool mk = A[k] == kFromTrunc;
-              const int src = mk ? gl : A[k];
-              const int bin = (mk && exact) ? K - 1 : min(max(K / 2 + c - src, 1), K - 1);
-              v += bin == b ? gv[k] : 0.f;
-            }
-          }
-          v = seg_reduce<SEG>(v, OpSum());
-          ge += lds_p[buf][seg][b] * v;
-          if (line_ok && sl == b) {
-            float *dst = a.grad_jump + (((int64_t)nj * 4 + dir) * K + b) * HW + (int64_t)y * W + x;
-            *dst = a.accum ? *dst + w * v : w * v;
-          }
-        }
-        if (line_ok && sl == 0) {
-          a.grad_edge[eo] = ge;
-          if (!a.accum) a.grad_jump[(((int64_t)nj * 4 + dir) * K) * HW + (int64_t)y * W + x] = 0.f;
-        }
-      } else if (KIND == BPL_LBP_SEMANTIC_GLOBAL) {
-        // entry [r][c] of this direction's matrix; every truncation-term winner uses the
-        // truncation entry [0][C-1] in the exact mode and [gargmin][c] in the reference mode
-        const float *Md = a.jump + (int64_t)dir * C * C;
-        float ge = 0.f;
+        ts = seg_reduce<SEG>(ts, OpSum());
+        float gv[LPL];
+        float dmax = 0.f, fsum = 0.f;
 #pragma unroll
         for (int k = 0; k < LPL; ++k) {
           const int c = sl + SEG * k;
-          if (c < C) {
-            const bool mk = A[k] == kFromTrunc;
-            const bool to_trunc = mk && exact;
-            const int r = mk ? gl : A[k];
-            ge += (to_trunc ? Md[C - 1] : Md[r * C + c]) * gv[k];
-            if (line_ok && !to_trunc) part[r * C + c] += w * gv[k];
-            if (line_ok && exact && c == C - 1) part[C - 1] += w * fsum;  // lane owning column C-1
+          const bool ok = c < C;
+          float gk = tt[k] - fast_exp(Mq[k]) * ts;
+          if (!VERT) gk += XD[k] + XU[k];
+          gk = ok ? gk : 0.f;
+          gv[k] = gk;
+          const bool mk = A[k] == kFromTrunc;
+          if (ok) {
+            lds_g[buf][seg][c] = gk;
+            lds_a[buf][seg][c] = mk ? -1 : A[k];
+            if (mk) fsum += gk;
+            else dmax = fmaxf(dmax, (float)abs(A[k] - c));
           }
         }
-        ge = seg_reduce<SEG>(ge, OpSum());
-        if (line_ok && sl == 0) a.grad_edge[eo] = ge;
-      } else {
-        float *Gp = a.grad_jump + px(dir, y, x, H, W) * C * C;
-        // edge gradient: zeros in the reference mode (not computed by the reference build),
-        // sum_c f(c, argmin(c)) g(c) in the exact mode
-        float ge = 0.f;
-        if (exact) {
-          const float *Jp = a.jump + px(dir, y, x, H, W) * C * C;
+        if (sl < 16) lds_p[buf][seg][sl] = pv;
+        fsum = seg_reduce<SEG>(fsum, OpSum());
+        dmax = seg_reduce<64>(dmax, OpMax());
+        __syncthreads();
+        const float w = lds_p[buf][seg][KIND == BPL_LBP_STEREO ? K : 0];
+
+        // ---- route to the sender labels (deterministic gather)
+        const int D = (int)dmax;
+        float gh[LPL];
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) gh[k] = 0.f;
+        for (int kk = -D; kk <= D; ++kk) {
 #pragma unroll
           for (int k = 0; k < LPL; ++k) {
-            const int c = sl + SEG * k;
-            if (c < C) ge += (A[k] == kFromTrunc ? Jp[C - 1] : Jp[A[k] * C + c]) * gv[k];
+            const int l = sl + SEG * k;
+            const int c = l - kk;
+            if (l < C && c >= 0 && c < C && lds_a[buf][seg][c] == l) gh[k] += lds_g[buf][seg][c];
           }
-          ge = seg_reduce<SEG>(ge, OpSum());
         }
-        if (line_ok && sl == 0) a.grad_edge[eo] = ge;
-        if (line_ok) {
-          for (int r = 0; r < C; ++r) {
+#pragma unroll
+        for (int k = 0; k < LPL; ++k)
+          if (sl + SEG * k == gl) gh[k] += fsum;
+
+        // ---- parameter gradients of (dir, sender)
+        const int64_t eo = sl_px(n, dir, y, x, H, W);
+        if (KIND == BPL_LBP_STEREO) {
+          float ge = 0.f;
+          for (int b = 1; b < K; ++b) {
+            float v = 0.f;
 #pragma unroll
             for (int k = 0; k < LPL; ++k) {
               const int c = sl + SEG * k;
               if (c < C) {
                 const bool mk = A[k] == kFromTrunc;
-                float v = 0.f;
-                if (!mk && A[k] == r) v = w * gv[k];
-                if (mk && !exact && gl == r) v = w * gv[k];
-                if (exact && r == 0 && c == C - 1) v += w * fsum;
-                float *dst = Gp + r * C + c;
-                *dst = a.accum ? *dst + v : v;
+                const int src = mk ? gl : A[k];
+                const int bin = (mk && exact) ? K - 1 : min(max(K / 2 + c - src, 1), K - 1);
+                v += bin == b ? gv[k] : 0.f;
+              }
+            }
+            v = seg_reduce<SEG>(v, OpSum());
+            ge += lds_p[buf][seg][b] * v;
+            if (line_ok && sl == b) {
+              float *dst = a.grad_jump + (((int64_t)nj * 4 + dir) * K + b) * HW + (int64_t)y * W + x;
+              *dst = a.accum ? *dst + w * v : w * v;
+            }
+          }
+          if (line_ok && sl == 0) {
+            a.grad_edge[eo] = ge;
+            if (!a.accum) a.grad_jump[(((int64_t)nj * 4 + dir) * K) * HW + (int64_t)y * W + x] = 0.f;
+          }
+        } else if (KIND == BPL_LBP_SEMANTIC_GLOBAL) {
+          // entry [r][c] of this direction's matrix; every truncation-term winner uses the
+          // truncation entry [0][C-1] in the exact mode and [gargmin][c] in the reference mode
+          const float *Md = a.jump + (int64_t)dir * C * C;
+          float ge = 0.f;
+#pragma unroll
+          for (int k = 0; k < LPL; ++k) {
+            const int c = sl + SEG * k;
+            if (c < C) {
+              const bool mk = A[k] == kFromTrunc;
+              const bool to_trunc = mk && exact;
+              const int r = mk ? gl : A[k];
+              ge += (to_trunc ? Md[C - 1] : Md[r * C + c]) * gv[k];
+              if (line_ok && !to_trunc) part[r * C + c] += w * gv[k];
+              if (line_ok && exact && c == C - 1) part[C - 1] += w * fsum;  // lane owning column C-1
+            }
+          }
+          ge = seg_reduce<SEG>(ge, OpSum());
+          if (line_ok && sl == 0) a.grad_edge[eo] = ge;
+        } else {
+          float *Gp = a.grad_jump + px(dir, y, x, H, W) * C * C;
+          // edge gradient: zeros in the reference mode (not computed by the reference build),
+          // sum_c f(c, argmin(c)) g(c) in the exact mode
+          float ge = 0.f;
+          if (exact) {
+            const float *Jp = a.jump + px(dir, y, x, H, W) * C * C;
+#pragma unroll
+            for (int k = 0; k < LPL; ++k) {
+              const int c = sl + SEG * k;
+              if (c < C) ge += (A[k] == kFromTrunc ? Jp[C - 1] : Jp[A[k] * C + c]) * gv[k];
+            }
+            ge = seg_reduce<SEG>(ge, OpSum());
+          }
+          if (line_ok && sl == 0) a.grad_edge[eo] = ge;
+          if (line_ok) {
+            for (int r = 0; r < C; ++r) {
+#pragma unroll
+              for (int k = 0; k < LPL; ++k) {
+                const int c = sl + SEG * k;
+                if (c < C) {
+                  const bool mk = A[k] == kFromTrunc;
+                  float v = 0.f;
+                  if (!mk && A[k] == r) v = w * gv[k];
+                  if (mk && !exact && gl == r) v = w * gv[k];
+                  if (exact && r == 0 && c == C - 1) v += w * fsum;
+                  float *dst = Gp + r * C + c;
+                  *dst = a.accum ? *dst + v : v;
+                }
               }
             }
           }
         }
-      }
 
-      // ---- chain and sender h-gradient
-      const int64_t op = px(n, y, x, H, W) * C;
+        // ---- chain and sender h-gradient
+        const int64_t op = px(n, y, x, H, W) * C;
 #pragma unroll
-      for (int k = 0; k < LPL; ++k) {
-        acc[k] = gh[k];
-        const int c = sl + SEG * k;
-        if (line_ok && c < C) X[op + c] = gh[k];
+        for (int k = 0; k < LPL; ++k) {
+          acc[k] = gh[k];
+          const int c = sl + SEG * k;
+          if (line_ok && c < C) X[op + c] = gh[k];
+        }
       }
       // refill ring slot i after its last use (no register copies, no drain of the ring)
       load_step(s - PF >= 0 ? s - PF : 0, pg[i], pm[i], pa[i], pd[i], pu[i], pp[i], pl[i]);

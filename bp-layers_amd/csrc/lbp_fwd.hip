@@ -123,141 +123,146 @@ __global__ __launch_bounds__(64) void lbp_fwd_phase(FwdArgs a) {
   };
 
 #pragma unroll
-  for (int i = 0; i < PF; ++i) load_step(i < nsteps ? i : (nsteps > 0 ? nsteps - 1 : 0), pc[i], p1[i], p2[i], pp[i]);
+  for (int i = 0; i < PF; ++i) {
+    load_step(i < nsteps ? i : (nsteps > 0 ? nsteps - 1 : 0), pc[i], p1[i], p2[i], pp[i]);
+    // keep the slots in issue order: the loop's wait counts assume slot 0 is the oldest
+    __builtin_amdgcn_sched_barrier(0);
+  }
 
   for (int s0 = 0; s0 < nsteps; s0 += PF) {
 #pragma unroll
     for (int i = 0; i < PF; ++i) {
       const int s = s0 + i;
-      if (s >= nsteps) break;
-      const float pv = pp[i];
-      const int buf = s & 1;
-      int y, x;
-      yx_of(pos_of(s), y, x);
+      if (s < nsteps) {
+        const float pv = pp[i];
+        const int buf = s & 1;
+        int y, x;
+        yx_of(pos_of(s), y, x);
 
-      float h[LPL];
-#pragma unroll
-      for (int k = 0; k < LPL; ++k) {
-        h[k] = VERT ? ((pc[i][k] + p1[i][k]) + p2[i][k]) + chain[k] : ((pc[i][k] + chain[k]) + p1[i][k]) + p2[i][k];
-        const int c = sl + SEG * k;
-        if (c < C) lds_h[buf][seg][c] = h[k];
-      }
-      if (sl < 16) lds_p[buf][seg][sl] = pv;
-      __syncthreads();
-
-      const float w = lds_p[buf][seg][KIND == BPL_LBP_STEREO ? K : 0];
-      float msg[LPL];
-      int arg[LPL];
-      int lstar = 0;
-      if (trunc) {
-        float bv = __builtin_inff();
-        int bk = -1;
+        float h[LPL];
 #pragma unroll
         for (int k = 0; k < LPL; ++k) {
+          h[k] = VERT ? ((pc[i][k] + p1[i][k]) + p2[i][k]) + chain[k] : ((pc[i][k] + chain[k]) + p1[i][k]) + p2[i][k];
           const int c = sl + SEG * k;
-          if (c < C) argmin_step(bv, bk, h[k], tie_key(c, a.bits));
+          if (c < C) lds_h[buf][seg][c] = h[k];
         }
-        seg_argmin<SEG>(bv, bk);
-        lstar = tie_key(bk, a.bits);  // bit reversal is an involution
-        float JK;
-        if (KIND == BPL_LBP_STEREO) JK = lds_p[buf][seg][K - 1];
-        else if (KIND == BPL_LBP_SEMANTIC_GLOBAL) JK = a.mat_lds ? lds_m[C - 1] : a.jump[(int64_t)dir * C * C + C - 1];
-        else JK = a.jump[px(dir, y, x, H, W) * C * C + C - 1];
-        const float gv = fmaf(JK, w, bv);
-#pragma unroll
-        for (int k = 0; k < LPL; ++k) { msg[k] = gv; arg[k] = kFromTrunc; }
-        if (KIND == BPL_LBP_STEREO) {
-          const float o = lds_p[buf][seg][0];
-          int k_lo = -delta, k_hi = delta;
-          if (o != 0.f) { k_lo = (int)floorf(o - 0.5f) - delta; k_hi = (int)ceilf(o + 0.5f) + delta; }
-          int st[LPL], sp[LPL];
+        if (sl < 16) lds_p[buf][seg][sl] = pv;
+        __syncthreads();
+
+        const float w = lds_p[buf][seg][KIND == BPL_LBP_STEREO ? K : 0];
+        float msg[LPL];
+        int arg[LPL];
+        int lstar = 0;
+        if (trunc) {
+          float bv = __builtin_inff();
+          int bk = -1;
 #pragma unroll
           for (int k = 0; k < LPL; ++k) {
             const int c = sl + SEG * k;
-            st[k] = sat_u16(roundf((float)(c - delta + 1) + o - 1.0f));
-            sp[k] = min(sat_u16(roundf((float)(c + delta) + o + 1.0f)), C);
+            if (c < C) argmin_step(bv, bk, h[k], tie_key(c, a.bits));
           }
-          for (int kk = k_lo; kk <= k_hi; ++kk) {
-            const float fk = lds_p[buf][seg][min(max(K / 2 - kk, 1), K - 1)];
+          seg_argmin<SEG>(bv, bk);
+          lstar = tie_key(bk, a.bits);  // bit reversal is an involution
+          float JK;
+          if (KIND == BPL_LBP_STEREO) JK = lds_p[buf][seg][K - 1];
+          else if (KIND == BPL_LBP_SEMANTIC_GLOBAL) JK = a.mat_lds ? lds_m[C - 1] : a.jump[(int64_t)dir * C * C + C - 1];
+          else JK = a.jump[px(dir, y, x, H, W) * C * C + C - 1];
+          const float gv = fmaf(JK, w, bv);
+#pragma unroll
+          for (int k = 0; k < LPL; ++k) { msg[k] = gv; arg[k] = kFromTrunc; }
+          if (KIND == BPL_LBP_STEREO) {
+            const float o = lds_p[buf][seg][0];
+            int k_lo = -delta, k_hi = delta;
+            if (o != 0.f) { k_lo = (int)floorf(o - 0.5f) - delta; k_hi = (int)ceilf(o + 0.5f) + delta; }
+            int st[LPL], sp[LPL];
 #pragma unroll
             for (int k = 0; k < LPL; ++k) {
               const int c = sl + SEG * k;
-              const int l = c + kk;
-              if (c < C && l >= st[k] && l < sp[k]) {
-                const float nv = fmaf(fk, w, lds_h[buf][seg][l]);
-                if (nv <= msg[k]) { msg[k] = nv; arg[k] = l; }
+              st[k] = sat_u16(roundf((float)(c - delta + 1) + o - 1.0f));
+              sp[k] = min(sat_u16(roundf((float)(c + delta) + o + 1.0f)), C);
+            }
+            for (int kk = k_lo; kk <= k_hi; ++kk) {
+              const float fk = lds_p[buf][seg][min(max(K / 2 - kk, 1), K - 1)];
+#pragma unroll
+              for (int k = 0; k < LPL; ++k) {
+                const int c = sl + SEG * k;
+                const int l = c + kk;
+                if (c < C && l >= st[k] && l < sp[k]) {
+                  const float nv = fmaf(fk, w, lds_h[buf][seg][l]);
+                  if (nv <= msg[k]) { msg[k] = nv; arg[k] = l; }
+                }
+              }
+            }
+          } else {
+            for (int kk = 1 - delta; kk <= delta - 1; ++kk) {
+#pragma unroll
+              for (int k = 0; k < LPL; ++k) {
+                const int c = sl + SEG * k;
+                const int l = c + kk;
+                if (c < C && l >= 0 && l < C) {
+                  float fv;
+                  if (KIND == BPL_LBP_SEMANTIC_GLOBAL)
+                    fv = a.mat_lds ? lds_m[l * C + c] : a.jump[((int64_t)dir * C + l) * C + c];
+                  else
+                    fv = a.jump[(px(dir, y, x, H, W) * C + l) * C + c];
+                  const float nv = fmaf(fv, w, lds_h[buf][seg][l]);
+                  if (nv <= msg[k]) { msg[k] = nv; arg[k] = l; }
+                }
               }
             }
           }
         } else {
-          for (int kk = 1 - delta; kk <= delta - 1; ++kk) {
+#pragma unroll
+          for (int k = 0; k < LPL; ++k) { msg[k] = 1e15f; arg[k] = 0; }
+          const float *Mpix = (KIND == BPL_LBP_SEMANTIC_PIXEL) ? a.jump + px(dir, y, x, H, W) * C * C : nullptr;
+          for (int l = 0; l < C; ++l) {
+            const float hv = lds_h[buf][seg][l];
 #pragma unroll
             for (int k = 0; k < LPL; ++k) {
               const int c = sl + SEG * k;
-              const int l = c + kk;
-              if (c < C && l >= 0 && l < C) {
+              if (c < C) {
                 float fv;
-                if (KIND == BPL_LBP_SEMANTIC_GLOBAL)
+                if (KIND == BPL_LBP_STEREO) fv = lds_p[buf][seg][min(max(K / 2 + c - l, 1), K - 1)];
+                else if (KIND == BPL_LBP_SEMANTIC_GLOBAL)
                   fv = a.mat_lds ? lds_m[l * C + c] : a.jump[((int64_t)dir * C + l) * C + c];
-                else
-                  fv = a.jump[(px(dir, y, x, H, W) * C + l) * C + c];
-                const float nv = fmaf(fv, w, lds_h[buf][seg][l]);
+                else fv = Mpix[l * C + c];
+                const float nv = fmaf(fv, w, hv);
                 if (nv <= msg[k]) { msg[k] = nv; arg[k] = l; }
               }
             }
           }
         }
-      } else {
+
+        // ---- normalisation: m - log sum_c exp(m), max-shifted
+        float mx = -__builtin_inff();
 #pragma unroll
-        for (int k = 0; k < LPL; ++k) { msg[k] = 1e15f; arg[k] = 0; }
-        const float *Mpix = (KIND == BPL_LBP_SEMANTIC_PIXEL) ? a.jump + px(dir, y, x, H, W) * C * C : nullptr;
-        for (int l = 0; l < C; ++l) {
-          const float hv = lds_h[buf][seg][l];
+        for (int k = 0; k < LPL; ++k)
+          if (sl + SEG * k < C) mx = fmaxf(mx, msg[k]);
+        mx = seg_reduce<SEG>(mx, OpMax());
+        float se = 0.f;
 #pragma unroll
-          for (int k = 0; k < LPL; ++k) {
-            const int c = sl + SEG * k;
-            if (c < C) {
-              float fv;
-              if (KIND == BPL_LBP_STEREO) fv = lds_p[buf][seg][min(max(K / 2 + c - l, 1), K - 1)];
-              else if (KIND == BPL_LBP_SEMANTIC_GLOBAL)
-                fv = a.mat_lds ? lds_m[l * C + c] : a.jump[((int64_t)dir * C + l) * C + c];
-              else fv = Mpix[l * C + c];
-              const float nv = fmaf(fv, w, hv);
-              if (nv <= msg[k]) { msg[k] = nv; arg[k] = l; }
-            }
+        for (int k = 0; k < LPL; ++k)
+          if (sl + SEG * k < C) se += fast_exp(msg[k] - mx);
+        se = seg_reduce<SEG>(se, OpSum());
+        const float lse = mx + fast_log(se);
+
+        int yq = y, xq = x;
+        if (VERT) yq += fwd ? 1 : -1; else xq += fwd ? 1 : -1;
+        const int64_t oq = sl_px(n, rslot, yq, xq, H, W);
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) {
+          const int c = sl + SEG * k;
+          const float m = msg[k] - lse;
+          chain[k] = m;
+          if (line_ok && c < C) {
+            a.msg_out[oq * C + c] = m;
+            if (a.argmin) a.argmin[oq * C + c] = (uint8_t)arg[k];
           }
         }
-      }
-
-      // ---- normalisation: m - log sum_c exp(m), max-shifted
-      float mx = -__builtin_inff();
-#pragma unroll
-      for (int k = 0; k < LPL; ++k)
-        if (sl + SEG * k < C) mx = fmaxf(mx, msg[k]);
-      mx = seg_reduce<SEG>(mx, OpMax());
-      float se = 0.f;
-#pragma unroll
-      for (int k = 0; k < LPL; ++k)
-        if (sl + SEG * k < C) se += fast_exp(msg[k] - mx);
-      se = seg_reduce<SEG>(se, OpSum());
-      const float lse = mx + fast_log(se);
-
-      int yq = y, xq = x;
-      if (VERT) yq += fwd ? 1 : -1; else xq += fwd ? 1 : -1;
-      const int64_t oq = sl_px(n, rslot, yq, xq, H, W);
-#pragma unroll
-      for (int k = 0; k < LPL; ++k) {
-        const int c = sl + SEG * k;
-        const float m = msg[k] - lse;
-        chain[k] = m;
-        if (line_ok && c < C) {
-          a.msg_out[oq * C + c] = m;
-          if (a.argmin) a.argmin[oq * C + c] = (uint8_t)arg[k];
+        if (line_ok && sl == 0) {
+          if (a.scale) a.scale[oq] = fast_exp(lse);
+          if (a.gargmin) a.gargmin[oq] = (uint8_t)(trunc ? lstar : 0);
         }
-      }
-      if (line_ok && sl == 0) {
-        if (a.scale) a.scale[oq] = fast_exp(lse);
-        if (a.gargmin) a.gargmin[oq] = (uint8_t)(trunc ? lstar : 0);
       }
       // refill ring slot i (its old contents are dead now: no register copies, no drain)
       load_step(s + PF < nsteps ? s + PF : nsteps - 1, pc[i], p1[i], p2[i], pp[i]);
@@ -372,177 +377,182 @@ __global__ __launch_bounds__(64) void lbp_fwd_stereo_dpp(FwdArgs a) {
   };
 
 #pragma unroll
-  for (int i = 0; i < PF; ++i) load_step(i < nsteps ? i : (nsteps > 0 ? nsteps - 1 : 0), pc[i], p1[i], p2[i], pp[i]);
+  for (int i = 0; i < PF; ++i) {
+    load_step(i < nsteps ? i : (nsteps > 0 ? nsteps - 1 : 0), pc[i], p1[i], p2[i], pp[i]);
+    // keep the slots in issue order: the loop's wait counts assume slot 0 is the oldest
+    __builtin_amdgcn_sched_barrier(0);
+  }
 
   for (int s0 = 0; s0 < nsteps; s0 += PF) {
 #pragma unroll
     for (int i = 0; i < PF; ++i) {
       const int s = s0 + i;
-      if (s >= nsteps) break;
-      float h[LPL];
-#pragma unroll
-      for (int k = 0; k < LPL; ++k)
-        h[k] = VERT ? ((pc[i][k] + p1[i][k]) + p2[i][k]) + chain[k] : ((pc[i][k] + chain[k]) + p1[i][k]) + p2[i][k];
-      const float pv = pp[i];
-      int y, x;
-      yx_of(pos_of(s), y, x);
-
-      // per-pixel pairwise parameters of this segment
-      const float w = seg_param<SEG>(pv, K, seg);
-      const float o = seg_param<SEG>(pv, 0, seg);
-      const float JK = seg_param<SEG>(pv, K - 1, seg);
-      const float Jmid = seg_param<SEG>(pv, K / 2, seg);
-
-      // global minimum (value), then its label by ballot (tree tie order when tied)
-      float hm = __builtin_inff();
-#pragma unroll
-      for (int k = 0; k < LPL; ++k)
-        if (c0 + k < C) hm = fminf(hm, h[k]);
-      const float Hs = seg_reduce<SEG>(hm, OpMin());
-      int lstar = 0;
-      {
-        uint64_t mk[LPL];
-        bool single = true;
-#pragma unroll
-        for (int k = 0; k < LPL; ++k) mk[k] = __ballot(c0 + k < C && h[k] == Hs);
-#pragma unroll
-        for (int sg = 0; sg < SPW; ++sg) {
-          const uint64_t sm = SEG == 64 ? ~0ull : (sg ? 0xFFFFFFFF00000000ull : 0xFFFFFFFFull);
-          int cnt = 0;
-#pragma unroll
-          for (int k = 0; k < LPL; ++k) cnt += __popcll(mk[k] & sm);
-          single = single && cnt <= 1;
-        }
-        if (single) {
-#pragma unroll
-          for (int k = 0; k < LPL; ++k) {
-            const uint64_t m = mk[k] & segmask;
-            if (m) lstar = ((__ffsll((long long)m) - 1) - seg * SEG) * LPL + k;
-          }
-        } else {
-          float bv = __builtin_inff();
-          int bk = -1;
-#pragma unroll
-          for (int k = 0; k < LPL; ++k)
-            if (c0 + k < C) argmin_step(bv, bk, h[k], tie_key(c0 + k, a.bits));
-          seg_argmin<SEG>(bv, bk);
-          lstar = tie_key(bk, a.bits);
-        }
-      }
-      const float gv = fmaf(JK, w, Hs);
-      float msg[LPL];
-      int arg[LPL];
-#pragma unroll
-      for (int k = 0; k < LPL; ++k) { msg[k] = gv; arg[k] = kFromTrunc; }
-
-      // uniform decision: register window (offset 0 everywhere in the wave) or LDS window
-      const bool regwin = __ballot(o != 0.f) == 0 && delta <= 3;
-      if (regwin) {
-        float L1[LPL], L2[LPL], L3[LPL], R1[LPL], R2[LPL], R3[LPL];
-#pragma unroll
-        for (int k = 0; k < LPL; ++k) { L1[k] = wave_shr1(h[k]); R1[k] = wave_shl1(h[k]); }
-#pragma unroll
-        for (int k = 0; k < LPL; ++k) { L2[k] = wave_shr1(L1[k]); R2[k] = wave_shl1(R1[k]); }
-#pragma unroll
-        for (int k = 0; k < LPL; ++k) { L3[k] = wave_shr1(L2[k]); R3[k] = wave_shl1(R2[k]); }
-#pragma unroll
-        for (int kk = -3; kk <= 3; ++kk) {
-          if (kk < -delta || kk > delta) continue;
-          const float fk = seg_param<SEG>(pv, min(max(K / 2 - kk, 1), K - 1), seg);
-#pragma unroll
-          for (int k = 0; k < LPL; ++k) {
-            const int e = k + kk;  // compile-time slot offset from this lane's first label
-            float hv;
-            if (e >= 0 && e < LPL) {
-              hv = h[e >= 0 && e < LPL ? e : 0];
-            } else if (e < 0) {
-              const int j = -e, d = (j + LPL - 1) / LPL, slot = d * LPL - j;
-              hv = d == 1 ? L1[slot] : d == 2 ? L2[slot] : L3[slot];
-            } else {
-              const int j = e - LPL + 1, d = (j + LPL - 1) / LPL, slot = j - 1 - (d - 1) * LPL;
-              hv = d == 1 ? R1[slot] : d == 2 ? R2[slot] : R3[slot];
-            }
-            const int c = c0 + k, l = c + kk;
-            if (c < C && l >= 0 && l < C) {
-              const float nv = fmaf(fk, w, hv);
-              if (nv <= msg[k]) { msg[k] = nv; arg[k] = l; }
-            }
-          }
-        }
-      } else {
-        const int buf = s & 1;
+      if (s < nsteps) {
+        float h[LPL];
 #pragma unroll
         for (int k = 0; k < LPL; ++k)
-          if (c0 + k < C) lds_h[buf][seg][c0 + k] = h[k];
-        __syncthreads();
-        int k_lo = -delta, k_hi = delta;
-        const bool any_off = __ballot(o != 0.f) != 0;
-        if (any_off) {
-          // widest window over the wave's segments
-          const float omin = -seg_reduce<64>(-o, OpMax()), omax = seg_reduce<64>(o, OpMax());
-          k_lo = (int)floorf(omin - 0.5f) - delta;
-          k_hi = (int)ceilf(omax + 0.5f) + delta;
-        }
-        int st[LPL], sp[LPL];
+          h[k] = VERT ? ((pc[i][k] + p1[i][k]) + p2[i][k]) + chain[k] : ((pc[i][k] + chain[k]) + p1[i][k]) + p2[i][k];
+        const float pv = pp[i];
+        int y, x;
+        yx_of(pos_of(s), y, x);
+
+        // per-pixel pairwise parameters of this segment
+        const float w = seg_param<SEG>(pv, K, seg);
+        const float o = seg_param<SEG>(pv, 0, seg);
+        const float JK = seg_param<SEG>(pv, K - 1, seg);
+        const float Jmid = seg_param<SEG>(pv, K / 2, seg);
+
+        // global minimum (value), then its label by ballot (tree tie order when tied)
+        float hm = __builtin_inff();
 #pragma unroll
-        for (int k = 0; k < LPL; ++k) {
-          const int c = c0 + k;
-          st[k] = sat_u16(roundf((float)(c - delta + 1) + o - 1.0f));
-          sp[k] = min(sat_u16(roundf((float)(c + delta) + o + 1.0f)), C);
+        for (int k = 0; k < LPL; ++k)
+          if (c0 + k < C) hm = fminf(hm, h[k]);
+        const float Hs = seg_reduce<SEG>(hm, OpMin());
+        int lstar = 0;
+        {
+          uint64_t mk[LPL];
+          bool single = true;
+#pragma unroll
+          for (int k = 0; k < LPL; ++k) mk[k] = __ballot(c0 + k < C && h[k] == Hs);
+#pragma unroll
+          for (int sg = 0; sg < SPW; ++sg) {
+            const uint64_t sm = SEG == 64 ? ~0ull : (sg ? 0xFFFFFFFF00000000ull : 0xFFFFFFFFull);
+            int cnt = 0;
+#pragma unroll
+            for (int k = 0; k < LPL; ++k) cnt += __popcll(mk[k] & sm);
+            single = single && cnt <= 1;
+          }
+          if (single) {
+#pragma unroll
+            for (int k = 0; k < LPL; ++k) {
+              const uint64_t m = mk[k] & segmask;
+              if (m) lstar = ((__ffsll((long long)m) - 1) - seg * SEG) * LPL + k;
+            }
+          } else {
+            float bv = __builtin_inff();
+            int bk = -1;
+#pragma unroll
+            for (int k = 0; k < LPL; ++k)
+              if (c0 + k < C) argmin_step(bv, bk, h[k], tie_key(c0 + k, a.bits));
+            seg_argmin<SEG>(bv, bk);
+            lstar = tie_key(bk, a.bits);
+          }
         }
-        for (int kk = k_lo; kk <= k_hi; ++kk) {
-          const float fk = seg_param<SEG>(pv, min(max(K / 2 - kk, 1), K - 1), seg);
+        const float gv = fmaf(JK, w, Hs);
+        float msg[LPL];
+        int arg[LPL];
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) { msg[k] = gv; arg[k] = kFromTrunc; }
+
+        // uniform decision: register window (offset 0 everywhere in the wave) or LDS window
+        const bool regwin = __ballot(o != 0.f) == 0 && delta <= 3;
+        if (regwin) {
+          float L1[LPL], L2[LPL], L3[LPL], R1[LPL], R2[LPL], R3[LPL];
+#pragma unroll
+          for (int k = 0; k < LPL; ++k) { L1[k] = wave_shr1(h[k]); R1[k] = wave_shl1(h[k]); }
+#pragma unroll
+          for (int k = 0; k < LPL; ++k) { L2[k] = wave_shr1(L1[k]); R2[k] = wave_shl1(R1[k]); }
+#pragma unroll
+          for (int k = 0; k < LPL; ++k) { L3[k] = wave_shr1(L2[k]); R3[k] = wave_shl1(R2[k]); }
+#pragma unroll
+          for (int kk = -3; kk <= 3; ++kk) {
+            if (kk < -delta || kk > delta) continue;
+            const float fk = seg_param<SEG>(pv, min(max(K / 2 - kk, 1), K - 1), seg);
+#pragma unroll
+            for (int k = 0; k < LPL; ++k) {
+              const int e = k + kk;  // compile-time slot offset from this lane's first label
+              float hv;
+              if (e >= 0 && e < LPL) {
+                hv = h[e >= 0 && e < LPL ? e : 0];
+              } else if (e < 0) {
+                const int j = -e, d = (j + LPL - 1) / LPL, slot = d * LPL - j;
+                hv = d == 1 ? L1[slot] : d == 2 ? L2[slot] : L3[slot];
+              } else {
+                const int j = e - LPL + 1, d = (j + LPL - 1) / LPL, slot = j - 1 - (d - 1) * LPL;
+                hv = d == 1 ? R1[slot] : d == 2 ? R2[slot] : R3[slot];
+              }
+              const int c = c0 + k, l = c + kk;
+              if (c < C && l >= 0 && l < C) {
+                const float nv = fmaf(fk, w, hv);
+                if (nv <= msg[k]) { msg[k] = nv; arg[k] = l; }
+              }
+            }
+          }
+        } else {
+          const int buf = s & 1;
+#pragma unroll
+          for (int k = 0; k < LPL; ++k)
+            if (c0 + k < C) lds_h[buf][seg][c0 + k] = h[k];
+          __syncthreads();
+          int k_lo = -delta, k_hi = delta;
+          const bool any_off = __ballot(o != 0.f) != 0;
+          if (any_off) {
+            // widest window over the wave's segments
+            const float omin = -seg_reduce<64>(-o, OpMax()), omax = seg_reduce<64>(o, OpMax());
+            k_lo = (int)floorf(omin - 0.5f) - delta;
+            k_hi = (int)ceilf(omax + 0.5f) + delta;
+          }
+          int st[LPL], sp[LPL];
 #pragma unroll
           for (int k = 0; k < LPL; ++k) {
-            const int c = c0 + k, l = c + kk;
-            if (c < C && l >= st[k] && l < sp[k]) {
-              const float nv = fmaf(fk, w, lds_h[buf][seg][l]);
-              if (nv <= msg[k]) { msg[k] = nv; arg[k] = l; }
+            const int c = c0 + k;
+            st[k] = sat_u16(roundf((float)(c - delta + 1) + o - 1.0f));
+            sp[k] = min(sat_u16(roundf((float)(c + delta) + o + 1.0f)), C);
+          }
+          for (int kk = k_lo; kk <= k_hi; ++kk) {
+            const float fk = seg_param<SEG>(pv, min(max(K / 2 - kk, 1), K - 1), seg);
+#pragma unroll
+            for (int k = 0; k < LPL; ++k) {
+              const int c = c0 + k, l = c + kk;
+              if (c < C && l >= st[k] && l < sp[k]) {
+                const float nv = fmaf(fk, w, lds_h[buf][seg][l]);
+                if (nv <= msg[k]) { msg[k] = nv; arg[k] = l; }
+              }
             }
           }
         }
-      }
 
-      // normalisation shifted by gv >= every message; the label l* term keeps the sum away
-      // from underflow unless (JK - J[K/2]) * w is huge, where the max-shift is used instead
-      float shift = gv;
-      if (__ballot(!((JK - Jmid) * w <= 80.f)) != 0) {
-        float mx = -__builtin_inff();
+        // normalisation shifted by gv >= every message; the label l* term keeps the sum away
+        // from underflow unless (JK - J[K/2]) * w is huge, where the max-shift is used instead
+        float shift = gv;
+        if (__ballot(!((JK - Jmid) * w <= 80.f)) != 0) {
+          float mx = -__builtin_inff();
+#pragma unroll
+          for (int k = 0; k < LPL; ++k)
+            if (c0 + k < C) mx = fmaxf(mx, msg[k]);
+          shift = seg_reduce<SEG>(mx, OpMax());
+        }
+        float se = 0.f;
 #pragma unroll
         for (int k = 0; k < LPL; ++k)
-          if (c0 + k < C) mx = fmaxf(mx, msg[k]);
-        shift = seg_reduce<SEG>(mx, OpMax());
-      }
-      float se = 0.f;
-#pragma unroll
-      for (int k = 0; k < LPL; ++k)
-        if (c0 + k < C) se += fast_exp(msg[k] - shift);
-      se = seg_reduce<SEG>(se, OpSum());
-      const float lse = shift + fast_log(se);
+          if (c0 + k < C) se += fast_exp(msg[k] - shift);
+        se = seg_reduce<SEG>(se, OpSum());
+        const float lse = shift + fast_log(se);
 
-      int yq = y, xq = x;
-      if (VERT) yq += fwd ? 1 : -1; else xq += fwd ? 1 : -1;
-      const int64_t oq = sl_px(n, rslot, yq, xq, H, W);
+        int yq = y, xq = x;
+        if (VERT) yq += fwd ? 1 : -1; else xq += fwd ? 1 : -1;
+        const int64_t oq = sl_px(n, rslot, yq, xq, H, W);
 #pragma unroll
-      for (int k = 0; k < LPL; ++k) chain[k] = msg[k] - lse;
-      if (line_ok) {
-        float *mo = a.msg_out + oq * C;
-        if (full) {
+        for (int k = 0; k < LPL; ++k) chain[k] = msg[k] - lse;
+        if (line_ok) {
+          float *mo = a.msg_out + oq * C;
+          if (full) {
 #pragma unroll
-          for (int k = 0; k < LPL; ++k) mo[c0 + k] = chain[k];
-        } else {
+            for (int k = 0; k < LPL; ++k) mo[c0 + k] = chain[k];
+          } else {
 #pragma unroll
-          for (int k = 0; k < LPL; ++k)
-            if (c0 + k < C) mo[c0 + k] = chain[k];
-        }
-        if (a.argmin) {
+            for (int k = 0; k < LPL; ++k)
+              if (c0 + k < C) mo[c0 + k] = chain[k];
+          }
+          if (a.argmin) {
 #pragma unroll
-          for (int k = 0; k < LPL; ++k)
-            if (c0 + k < C) a.argmin[oq * C + c0 + k] = (uint8_t)arg[k];
-        }
-        if (sl == 0) {
-          if (a.scale) a.scale[oq] = fast_exp(lse);
-          if (a.gargmin) a.gargmin[oq] = (uint8_t)lstar;
+            for (int k = 0; k < LPL; ++k)
+              if (c0 + k < C) a.argmin[oq * C + c0 + k] = (uint8_t)arg[k];
+          }
+          if (sl == 0) {
+            if (a.scale) a.scale[oq] = fast_exp(lse);
+            if (a.gargmin) a.gargmin[oq] = (uint8_t)lstar;
+          }
         }
       }
       // refill ring slot i after its last use (see the generic kernel)
@@ -564,15 +574,294 @@ void launch_stereo_dpp(const FwdArgs &a, bool vert, hipStream_t st) {
     hipLaunchKernelGGL((lbp_fwd_stereo_dpp<SEG, LPL, false, PF>), grid, dim3(kWave), 0, st, a);
 }
 
+// ---------------------------------------------------------------------------------------
+// Stereo fast path for 32 < C <= 256: one scanline per wave, label c = lane * LPL + k.
+// Compared with the kernel above it removes per-label control flow from the step:
+//   * labels past C (only when C != 64 * LPL) carry h = +inf, so they never win a minimum,
+//     and the neighbour-lane shifts fill the wave's ends with +inf, so window candidates
+//     outside [0, C) need no range test;
+//   * minima are reduced as order-preserving integer keys and sums as floats, with the DPP
+//     row stages folded into the arithmetic and the half/row exchanges done by permlane
+//     swaps;
+//   * rows are addressed as a uniform scanline pointer plus a per-lane 32-bit offset, and a
+//     full label row is one vector load/store per lane.
+// The message, argmin and normalisation semantics are those of the kernel above.
+// ---------------------------------------------------------------------------------------
+constexpr int kArgTrunc = 64;  // window-offset sentinel: "truncation term won"
+
+template <int LPL>
+__device__ __forceinline__ void ld_lanes(const float *row, uint32_t voff, float *d) {
+  const float *p = reinterpret_cast<const float *>(reinterpret_cast<const char *>(row) + voff);
+#pragma unroll
+  for (int k = 0; k < LPL; ++k) d[k] = p[k];
+}
+template <int LPL>
+__device__ __forceinline__ void st_lanes(float *row, uint32_t voff, const float *d) {
+  float *p = reinterpret_cast<float *>(reinterpret_cast<char *>(row) + voff);
+#pragma unroll
+  for (int k = 0; k < LPL; ++k) p[k] = d[k];
+}
+__device__ __forceinline__ float shr1_inf(float v) {  // lane i <- lane i-1, lane 0 <- +inf
+  return __int_as_float(__builtin_amdgcn_update_dpp(0x7F800000, __float_as_int(v), 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float shl1_inf(float v) {  // lane i <- lane i+1, lane 63 <- +inf
+  return __int_as_float(__builtin_amdgcn_update_dpp(0x7F800000, __float_as_int(v), 0x130, 0xF, 0xF, false));
+}
+
+template <int LPL, bool VERT, bool FULL, int PF>
+__global__ __launch_bounds__(64) void lbp_fwd_stereo_w64(FwdArgs a) {
+  constexpr int ND = (3 + LPL - 1) / LPL;  // neighbour lanes reached by a +-3 window
+  __shared__ float lds_h[2][kWave * LPL];
+
+  const int H = a.H, W = a.W, C = a.C, K = a.K;
+  const int L = VERT ? W : H;
+  const int len = VERT ? H : W;
+  const int nsteps = len - 1;
+  const int g = blockIdx.x / L;
+  const int ln = blockIdx.x - g * L;
+  const int n = g >> 1;
+  const int dir = VERT ? ((g & 1) ? UP : DOWN) : ((g & 1) ? LEFT : RIGHT);
+  const int rslot = dir == RIGHT ? LEFT : dir == LEFT ? RIGHT : dir == DOWN ? UP : DOWN;
+  const bool fwd = (dir == RIGHT || dir == DOWN);
+  const int lane = threadIdx.x;
+  const int c0 = lane * LPL;
+  const int delta = a.delta;
+  const int nj = a.Nj > 1 ? n : 0;
+  const int64_t HW = (int64_t)H * W;
+  // in-plane pixel index of step s: q0 + s * dq
+  const int64_t dq = VERT ? (fwd ? (int64_t)W : -(int64_t)W) : (fwd ? 1 : -1);
+  const int64_t q0 = VERT ? (int64_t)(fwd ? 0 : H - 1) * W + ln : (int64_t)ln * W + (fwd ? 0 : W - 1);
+  const float *cost_p = a.cost + (int64_t)n * HW * C;
+  const float *x1_p = (VERT ? a.msg_out : a.msg_in) + ((int64_t)n * 4 + (VERT ? RIGHT : UP)) * HW * C;
+  const float *x2_p = (VERT ? a.msg_out : a.msg_in) + ((int64_t)n * 4 + (VERT ? LEFT : DOWN)) * HW * C;
+  float *out_p = a.msg_out + ((int64_t)n * 4 + rslot) * HW * C;
+  uint8_t *arg_p = a.argmin ? a.argmin + ((int64_t)n * 4 + rslot) * HW * C : nullptr;
+  const int64_t rs_base = ((int64_t)n * 4 + rslot) * HW;
+  // lane j < K streams jump entry j of (dir, pixel), lane >= K the edge weight
+  const float *par_p = lane < K ? a.jump + (((int64_t)nj * 4 + dir) * K + lane) * HW
+                                : a.edge + ((int64_t)n * 4 + dir) * HW;
+  // per-lane byte offsets of the label row (labels past C read label C-1)
+  uint32_t voff[LPL];
+  bool valid[LPL];
+#pragma unroll
+  for (int k = 0; k < LPL; ++k) {
+    valid[k] = FULL || c0 + k < C;
+    voff[k] = (uint32_t)(FULL ? c0 + k : min(c0 + k, C - 1)) * 4u;
+  }
+  auto ld_row = [&](const float *row, float *d) {
+    if (FULL) {
+      ld_lanes<LPL>(row, voff[0], d);
+    } else {
+#pragma unroll
+      for (int k = 0; k < LPL; ++k) d[k] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(row) + voff[k]);
+    }
+  };
+
+  float chain[LPL];
+  {
+    const float *src = a.msg_in + (rs_base + q0) * C;
+    ld_row(src, chain);
+#pragma unroll
+    for (int k = 0; k < LPL; ++k)
+      if (valid[k]) {
+        out_p[q0 * C + c0 + k] = chain[k];
+        if (arg_p) arg_p[q0 * C + c0 + k] = 0;
+      }
+    if (lane == 0) {
+      if (a.scale) a.scale[rs_base + q0] = 0.f;
+      if (a.gargmin) a.gargmin[rs_base + q0] = 0;
+    }
+  }
+
+  float pc[PF][LPL], p1[PF][LPL], p2[PF][LPL], pp[PF];
+  auto load_step = [&](int s, float *dc, float *d1, float *d2, float &dp) {
+    const int64_t q = q0 + (int64_t)s * dq;
+    ld_row(cost_p + q * C, dc);
+    ld_row(x1_p + q * C, d1);
+    ld_row(x2_p + q * C, d2);
+    dp = par_p[q];
+  };
+#pragma unroll
+  for (int i = 0; i < PF; ++i) {
+    load_step(i < nsteps ? i : (nsteps > 0 ? nsteps - 1 : 0), pc[i], p1[i], p2[i], pp[i]);
+    // keep the slots in issue order: the loop's wait counts assume slot 0 is the oldest
+    __builtin_amdgcn_sched_barrier(0);
+  }
+
+  for (int s0 = 0; s0 < nsteps; s0 += PF) {
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      const int s = s0 + i;
+      if (s < nsteps) {
+        float h[LPL];
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) {
+          const float v = VERT ? ((pc[i][k] + p1[i][k]) + p2[i][k]) + chain[k]
+                               : ((pc[i][k] + chain[k]) + p1[i][k]) + p2[i][k];
+          h[k] = valid[k] ? v : __builtin_inff();
+        }
+        const float pv = pp[i];
+        const float w = readlane_f(pv, K);
+        const float o = readlane_f(pv, 0);
+        const float JK = readlane_f(pv, K - 1);
+        const float Jmid = readlane_f(pv, K / 2);
+
+        // global minimum, then its label: the single minimal lane by ballot, the reference's
+        // tree tie order otherwise
+        int hk = float_key(h[0]);
+#pragma unroll
+        for (int k = 1; k < LPL; ++k) hk = min(hk, float_key(h[k]));
+        const float Hs = key_to_float(wave_min_i(hk));
+        int lstar = 0;
+        {
+          uint64_t mk[LPL];
+          int cnt = 0;
+#pragma unroll
+          for (int k = 0; k < LPL; ++k) {
+            mk[k] = __ballot(valid[k] && h[k] == Hs);
+            cnt += __popcll(mk[k]);
+          }
+          if (cnt == 1) {
+#pragma unroll
+            for (int k = 0; k < LPL; ++k)
+              if (mk[k]) lstar = (__ffsll((long long)mk[k]) - 1) * LPL + k;
+          } else {
+            float bv = __builtin_inff();
+            int bk = -1;
+#pragma unroll
+            for (int k = 0; k < LPL; ++k)
+              if (valid[k]) argmin_step(bv, bk, h[k], tie_key(c0 + k, a.bits));
+            seg_argmin<64>(bv, bk);
+            lstar = tie_key(bk, a.bits);
+          }
+        }
+        const float gv = fmaf(JK, w, Hs);
+        float msg[LPL];
+        int arg[LPL];
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) { msg[k] = gv; arg[k] = kArgTrunc; }
+
+        if (o == 0.f && delta <= 3) {
+          float Lh[ND][LPL], Rh[ND][LPL];
+#pragma unroll
+          for (int k = 0; k < LPL; ++k) { Lh[0][k] = shr1_inf(h[k]); Rh[0][k] = shl1_inf(h[k]); }
+#pragma unroll
+          for (int d = 1; d < ND; ++d)
+#pragma unroll
+            for (int k = 0; k < LPL; ++k) { Lh[d][k] = shr1_inf(Lh[d - 1][k]); Rh[d][k] = shl1_inf(Rh[d - 1][k]); }
+#pragma unroll
+          for (int kk = -3; kk <= 3; ++kk) {
+            if (kk < -delta || kk > delta) continue;
+            const float fk = readlane_f(pv, min(max(K / 2 - kk, 1), K - 1));
+#pragma unroll
+            for (int k = 0; k < LPL; ++k) {
+              const int e = k + kk;  // label offset from this lane's first label
+              float hv;
+              if (e >= 0 && e < LPL) {
+                hv = h[e >= 0 && e < LPL ? e : 0];
+              } else if (e < 0) {
+                const int j = -e, d = (j + LPL - 1) / LPL, slot = d * LPL - j;
+                hv = Lh[d - 1][slot];
+              } else {
+                const int j = e - LPL + 1, d = (j + LPL - 1) / LPL, slot = j - 1 - (d - 1) * LPL;
+                hv = Rh[d - 1][slot];
+              }
+              const float nv = fmaf(fk, w, hv);
+              if (nv <= msg[k]) { msg[k] = nv; arg[k] = kk; }
+            }
+          }
+        } else {
+          // offset window or delta > 3: candidates from an LDS copy of h
+          const int buf = s & 1;
+#pragma unroll
+          for (int k = 0; k < LPL; ++k) lds_h[buf][c0 + k] = h[k];
+          __syncthreads();
+          const int k_lo = (int)floorf(o - 0.5f) - delta;
+          const int k_hi = (int)ceilf(o + 0.5f) + delta;
+          int st[LPL], sp[LPL];
+#pragma unroll
+          for (int k = 0; k < LPL; ++k) {
+            const int c = c0 + k;
+            st[k] = sat_u16(roundf((float)(c - delta + 1) + o - 1.0f));
+            sp[k] = min(sat_u16(roundf((float)(c + delta) + o + 1.0f)), C);
+          }
+          for (int kk = k_lo; kk <= k_hi; ++kk) {
+            const float fk = readlane_f(pv, min(max(K / 2 - kk, 1), K - 1));
+#pragma unroll
+            for (int k = 0; k < LPL; ++k) {
+              const int l = c0 + k + kk;
+              if (l >= st[k] && l < sp[k]) {
+                const float nv = fmaf(fk, w, lds_h[buf][l]);
+                if (nv <= msg[k]) { msg[k] = nv; arg[k] = kk; }
+              }
+            }
+          }
+        }
+
+        // normalisation shifted by gv >= every message; the label l* term keeps the sum away
+        // from underflow unless (JK - J[K/2]) * w is huge, where the max-shift is used instead
+        float shift = gv;
+        if (!((JK - Jmid) * w <= 80.f)) {
+          int mx = float_key(valid[0] ? msg[0] : -__builtin_inff());
+#pragma unroll
+          for (int k = 1; k < LPL; ++k) mx = max(mx, float_key(valid[k] ? msg[k] : -__builtin_inff()));
+          shift = key_to_float(wave_max_i(mx));
+        }
+        float se = 0.f;
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) se += valid[k] ? fast_exp(msg[k] - shift) : 0.f;
+        se = wave_sum_f(se);
+        const float lse = shift + fast_log(se);
+
+        const int64_t qq = q0 + (int64_t)(s + 1) * dq;  // receiver
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) chain[k] = msg[k] - lse;
+        if (FULL) {
+          st_lanes<LPL>(out_p + qq * C, voff[0], chain);
+        } else {
+#pragma unroll
+          for (int k = 0; k < LPL; ++k)
+            if (valid[k]) out_p[qq * C + c0 + k] = chain[k];
+        }
+        if (arg_p) {
+#pragma unroll
+          for (int k = 0; k < LPL; ++k)
+            if (valid[k]) arg_p[qq * C + c0 + k] = (uint8_t)(arg[k] == kArgTrunc ? kFromTrunc : c0 + k + arg[k]);
+        }
+        if (lane == 0) {
+          if (a.scale) a.scale[rs_base + qq] = fast_exp(lse);
+          if (a.gargmin) a.gargmin[rs_base + qq] = (uint8_t)lstar;
+        }
+      }
+      // refill ring slot i after its last use
+      load_step(s + PF < nsteps ? s + PF : nsteps - 1, pc[i], p1[i], p2[i], pp[i]);
+    }
+  }
+}
+
+template <int LPL, bool FULL>
+void launch_stereo_w64(const FwdArgs &a, bool vert, hipStream_t st) {
+  const dim3 grid((unsigned)(a.N * 2 * (vert ? a.W : a.H)));
+  if (vert)
+    hipLaunchKernelGGL((lbp_fwd_stereo_w64<LPL, true, FULL, 4>), grid, dim3(kWave), 0, st, a);
+  else
+    hipLaunchKernelGGL((lbp_fwd_stereo_w64<LPL, false, FULL, 4>), grid, dim3(kWave), 0, st, a);
+}
+template <int LPL>
+void launch_stereo_w64_any(const FwdArgs &a, bool vert, hipStream_t st) {
+  if (a.C == kWave * LPL) launch_stereo_w64<LPL, true>(a, vert, st);
+  else launch_stereo_w64<LPL, false>(a, vert, st);
+}
+
 bool stereo_dpp_ok(const FwdArgs &a) { return a.C > 16 && a.delta < a.C; }
 
 void launch_stereo_dpp_any(const FwdArgs &a, bool vert, hipStream_t st) {
   const int C = a.C;
   if (C <= 32) launch_stereo_dpp<32, 1>(a, vert, st);
-  else if (C <= 64) launch_stereo_dpp<64, 1>(a, vert, st);
-  else if (C <= 128) launch_stereo_dpp<64, 2>(a, vert, st);
-  else if (C <= 192) launch_stereo_dpp<64, 3>(a, vert, st);
-  else launch_stereo_dpp<64, 4>(a, vert, st);
+  else if (C <= 64) launch_stereo_w64_any<1>(a, vert, st);
+  else if (C <= 128) launch_stereo_w64_any<2>(a, vert, st);
+  else if (C <= 192) launch_stereo_w64_any<3>(a, vert, st);
+  else launch_stereo_w64_any<4>(a, vert, st);
 }
 
 using FwdKernel = void (*)(FwdArgs);

@@ -109,45 +109,66 @@ __global__ void sad_stereo_fwd_step_k(const float *__restrict__ f0, const float 
 }
 
 // ----------------------------------------------------------------------------- stereo bwd
-// Block: one row, 64 pixels x 4 channel lanes.  The two gradient volumes needed by a pixel
-// (g[y,x,:] for df0 and the diagonal g[y,x+d,d] for df1) are staged in LDS once and reused by
-// every channel; each (channel, pixel) sums its D terms in disparity order.
-constexpr int SB_TX = 64, SB_CL = 4;
+// Block: one row, 32 pixels x 8 channel lanes.  The gradient rows needed by the tile
+// (g[y,x,:] for df0 and the diagonal g[y,x+d,d] for df1) are staged in LDS once, and for
+// each chunk of 8 channels the f1 segment [x0-dmax, x0+31-dmin] and the f0 segment
+// [x0+dmin, x0+31+dmax] too; each (channel, pixel) then sums its D terms in disparity order
+// from LDS only.
+constexpr int SB_TX = 32, SB_CB = 8;
 
 __global__ __launch_bounds__(256) void sad_stereo_bwd_k(const float *__restrict__ f0, const float *__restrict__ f1,
                                                         const float *__restrict__ g, float *__restrict__ df0,
                                                         float *__restrict__ df1, int C, int H, int W, int D, int mind) {
   extern __shared__ float sm[];
-  float *gx = sm;                       // [SB_TX][D+1]
-  float *gd = sm + SB_TX * (D + 1);     // [SB_TX][D+1]
+  const int DP = D + 1;
+  const int SPAN = SB_TX + D;            // staged f columns per channel
+  float *gx = sm;                        // [SB_TX][D+1]
+  float *gd = gx + SB_TX * DP;           // [SB_TX][D+1]
+  float *f1s = gd + SB_TX * DP;          // [SB_CB][SPAN]: f1 at x0 - dmax + j
+  float *f0s = f1s + SB_CB * SPAN;       // [SB_CB][SPAN]: f0 at x0 + dmin + j
   const int x0 = blockIdx.x * SB_TX;
   const int y = blockIdx.y;
   const int n = blockIdx.z;
   const int t = threadIdx.x;
+  const int maxd = mind + D - 1;
   const int64_t HW = (int64_t)H * W;
   const float *G = g + ((int64_t)n * H + y) * (int64_t)W * D;
   for (int i = t; i < SB_TX * D; i += 256) {
     const int xl = i / D, k = i - xl * D;
     const int x = x0 + xl;
-    const int d = mind + k;
-    gx[xl * (D + 1) + k] = (x < W) ? G[(int64_t)x * D + k] : 0.f;
-    const int xd = x + d;
-    gd[xl * (D + 1) + k] = (x < W && xd >= 0 && xd < W) ? G[(int64_t)xd * D + k] : 0.f;
+    const int xd = x + mind + k;
+    gx[xl * DP + k] = (x < W) ? G[(int64_t)x * D + k] : 0.f;
+    gd[xl * DP + k] = (x < W && xd >= 0 && xd < W) ? G[(int64_t)xd * D + k] : 0.f;
   }
-  __syncthreads();
   const int xl = t % SB_TX;
   const int cl = t / SB_TX;
   const int x = x0 + xl;
-  if (x >= W) return;
-  for (int c = cl; c < C; c += SB_CL) {
+  for (int c0 = 0; c0 < C; c0 += SB_CB) {
+    __syncthreads();
+    for (int i = t; i < SB_CB * SPAN; i += 256) {
+      const int cc = i / SPAN, j = i - cc * SPAN;
+      const int c = c0 + cc;
+      const int64_t b = ((int64_t)n * C + c) * HW + (int64_t)y * W;
+      const int xa = x0 - maxd + j, xb = x0 + mind + j;
+      f1s[i] = (c < C && xa >= 0 && xa < W) ? f1[b + xa] : 0.f;
+      f0s[i] = (c < C && xb >= 0 && xb < W) ? f0[b + xb] : 0.f;
+    }
+    __syncthreads();
+    const int c = c0 + cl;
+    if (c >= C || x >= W) continue;
     const int64_t b = ((int64_t)n * C + c) * HW + (int64_t)y * W;
     const float a0 = f0[b + x], b1 = f1[b + x];
+    const float *F1 = f1s + cl * SPAN;
+    const float *F0 = f0s + cl * SPAN;
+    const float *GX = gx + xl * DP;
+    const float *GD = gd + xl * DP;
+    // x - d = x0 - maxd + j  ->  j = xl + maxd - d ;  x + d = x0 + mind + j -> j = xl + d - mind
     float r0 = 0.f, r1 = 0.f;
     for (int k = 0; k < D; ++k) {
       const int d = mind + k;
       const int xm = x - d, xp = x + d;
-      if (xm >= 0 && xm < W) r0 += sgn_eps(a0 - f1[b + xm]) * gx[xl * (D + 1) + k];
-      if (xp >= 0 && xp < W) r1 -= sgn_eps(f0[b + xp] - b1) * gd[xl * (D + 1) + k];
+      if (xm >= 0 && xm < W) r0 += sgn_eps(a0 - F1[xl + maxd - d]) * GX[k];
+      if (xp >= 0 && xp < W) r1 -= sgn_eps(F0[xl + k] - b1) * GD[k];
     }
     df0[b + x] = r0;
     df1[b + x] = r1;
@@ -401,9 +422,15 @@ extern "C" int bpl_sad_stereo_bwd(const float *f0, const float *f1, const float 
   BPL_REQUIRE(N >= 1 && C >= 1 && H >= 1 && W >= 1, BPL_E_ARG, "bpl_sad_stereo_bwd: sizes must be positive");
   const int D = max_disp - min_disp + 1;
   BPL_REQUIRE(D >= 1, BPL_E_ARG, "bpl_sad_stereo_bwd: empty disparity range");
-  const size_t shm = (size_t)2 * SB_TX * (D + 1) * sizeof(float);
+  const size_t shm = ((size_t)2 * SB_TX * (D + 1) + (size_t)2 * SB_CB * (SB_TX + D)) * sizeof(float);
   BPL_REQUIRE(shm <= 160 * 1024, BPL_E_LIMIT, "bpl_sad_stereo_bwd: D=%d needs %zu bytes of LDS", D, shm);
   BPL_REQUIRE(H < 65536 && N < 65536, BPL_E_LIMIT, "bpl_sad_stereo_bwd: tensor too large");
+  if (shm > 64 * 1024) {
+    const int rc = check_hip(hipFuncSetAttribute((const void *)sad_stereo_bwd_k,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm),
+                             "bpl_sad_stereo_bwd: raising the LDS limit");
+    if (rc != BPL_OK) return rc;
+  }
   dim3 grid((unsigned)((W + SB_TX - 1) / SB_TX), (unsigned)H, (unsigned)N);
   hipLaunchKernelGGL(sad_stereo_bwd_k, grid, dim3(256), shm, as_stream(stream), f0, f1, grad_cost, df0, df1,
                      (int)C, (int)H, (int)W, D, min_disp);
