@@ -6,6 +6,7 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 
 #include "../../include/bpl_amd.h"
 
@@ -30,6 +31,13 @@ int check_hip(hipError_t e, const char *what);
   } while (0)
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// BPL_LBP_GENERIC=1 selects the generic LBP kernels for every variant (used by tests to
+// compare the stereo fast paths with them).
+inline bool generic_only() {
+  const char *e = getenv("BPL_LBP_GENERIC");
+  return e && e[0] == '1';
+}
 
 // ------------------------------------------------------------------ lane exchange
 // DPP controls (GFX9 encoding): quad_perm [1,0,3,2] / [2,3,0,1], row_half_mirror, row_mirror.
@@ -125,6 +133,47 @@ __device__ __forceinline__ float wave_sum_f(float v) {
   v = lo + hi;
   swap_halves(v, lo, hi);
   return lo + hi;
+}
+
+// Reduce-scatter of eight per-lane partial sums v[0..7] over the wave.  The two-operand swaps
+// exchange the upper 32 lanes of the first operand with the lower 32 of the second
+// (v_permlane32_swap) and the odd 16-lane rows of the first with the even rows of the second
+// (v_permlane16_swap), so each swap-and-add halves the lanes a sum spans while keeping two
+// sums in one register.  Result: the total of v[b] is held by every lane of 16-lane row
+// b >> 1 in the register returned for b & 1.
+__device__ __forceinline__ void swap_halves2(float a, float b, float &x, float &y) {
+  const auto r = __builtin_amdgcn_permlane32_swap((unsigned)__float_as_int(a), (unsigned)__float_as_int(b), false, false);
+  x = __int_as_float((int)r[0]);
+  y = __int_as_float((int)r[1]);
+}
+__device__ __forceinline__ void swap_rows2(float a, float b, float &x, float &y) {
+  const auto r = __builtin_amdgcn_permlane16_swap((unsigned)__float_as_int(a), (unsigned)__float_as_int(b), false, false);
+  x = __int_as_float((int)r[0]);
+  y = __int_as_float((int)r[1]);
+}
+__device__ __forceinline__ float row_sum_f(float v) {
+  v = v + dpp_src_f<kDppXor1>(v);
+  v = v + dpp_src_f<kDppXor2>(v);
+  v = v + dpp_src_f<kDppHalfMirror>(v);
+  return v + dpp_src_f<kDppMirror>(v);
+}
+__device__ __forceinline__ void reduce_scatter8(const float *v, float &even, float &odd) {
+  float u[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {  // lanes 0-31: v[j] over both halves, lanes 32-63: v[j + 4]
+    float x, y;
+    swap_halves2(v[j], v[j + 4], x, y);
+    u[j] = x + y;
+  }
+  float z[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {  // rows 0..3: u[j].lo, u[j+2].lo, u[j].hi, u[j+2].hi
+    float x, y;
+    swap_rows2(u[j], u[j + 2], x, y);
+    z[j] = x + y;
+  }
+  even = row_sum_f(z[0]);  // rows 0..3 hold v[0], v[2], v[4], v[6]
+  odd = row_sum_f(z[1]);   // rows 0..3 hold v[1], v[3], v[5], v[7]
 }
 
 // All-reduce over aligned segments of SEG lanes (16, 32 or 64); every lane of a segment

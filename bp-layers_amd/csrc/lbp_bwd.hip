@@ -299,6 +299,307 @@ __global__ __launch_bounds__(64) void lbp_bwd_phase(BwdArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------------------
+// Stereo backward for 32 < C <= 256 with K <= 8: one scanline per wave, label
+// c = lane * LPL + k (the forward's layout).  Same adjoint as lbp_bwd_phase; per step:
+//   * routing: a sender label l collects g(c) of the receiver labels c = l - 3 .. l + 3 whose
+//     argmin is l, reading the neighbour lanes' g and argmin through DPP shifts (an LDS
+//     gather over the widest offset only when some |argmin(c) - c| > 3);
+//   * pairwise gradients: per-lane sums for the K - 1 bins and the edge term
+//     sum_b J[b] v[b] (bin slot 0) reduced together by one reduce-scatter;
+//   * no per-label control flow: labels past C carry g = 0 and argmin -1.
+// ---------------------------------------------------------------------------------------
+template <int LPL>
+__device__ __forceinline__ void ld_lanes_f(const float *row, uint32_t voff, float *d) {
+  const float *p = reinterpret_cast<const float *>(reinterpret_cast<const char *>(row) + voff);
+#pragma unroll
+  for (int k = 0; k < LPL; ++k) d[k] = p[k];
+}
+__device__ __forceinline__ float shr1_f(float v, float fill) {  // lane i <- lane i-1
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(fill), __float_as_int(v), 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float shl1_f(float v, float fill) {  // lane i <- lane i+1
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(fill), __float_as_int(v), 0x130, 0xF, 0xF, false));
+}
+__device__ __forceinline__ int shr1_i(int v, int fill) {
+  return __builtin_amdgcn_update_dpp(fill, v, 0x138, 0xF, 0xF, false);
+}
+__device__ __forceinline__ int shl1_i(int v, int fill) {
+  return __builtin_amdgcn_update_dpp(fill, v, 0x130, 0xF, 0xF, false);
+}
+
+template <int LPL, bool VERT, bool FULL, int PF>
+__global__ __launch_bounds__(64) void lbp_bwd_stereo_w64(BwdArgs a) {
+  constexpr int ND = (3 + LPL - 1) / LPL;  // neighbour lanes reached by offsets up to 3
+  __shared__ float lds_g[2][kWave * LPL];
+  __shared__ int lds_a[2][kWave * LPL];
+
+  const int H = a.H, W = a.W, C = a.C, K = a.K;
+  const int L = VERT ? W : H;
+  const int len = VERT ? H : W;
+  const int nsteps = len - 1;
+  const int g_ = blockIdx.x / L;
+  const int ln = blockIdx.x - g_ * L;
+  const int n = a.n_base + (g_ >> 1);
+  const int dir = VERT ? ((g_ & 1) ? UP : DOWN) : ((g_ & 1) ? LEFT : RIGHT);
+  const int rslot = dir == RIGHT ? LEFT : dir == LEFT ? RIGHT : dir == DOWN ? UP : DOWN;
+  const bool fwd = (dir == RIGHT || dir == DOWN);
+  const int lane = threadIdx.x;
+  const int c0 = lane * LPL;
+  const int nj = a.Nj > 1 ? n : 0;
+  const bool exact = a.mode == BPL_GRAD_EXACT;
+  const int64_t HW = (int64_t)H * W;
+  // sender pixel of forward step s: q0 + s * dq (in-plane index); its receiver is one further
+  const int64_t dq = VERT ? (fwd ? (int64_t)W : -(int64_t)W) : (fwd ? 1 : -1);
+  const int64_t q0 = VERT ? (int64_t)(fwd ? 0 : H - 1) * W + ln : (int64_t)ln * W + (fwd ? 0 : W - 1);
+  const int64_t img = (int64_t)n * HW * C;
+  const int64_t rsl = ((int64_t)n * 4 + rslot) * HW;
+  float *Xo = a.X[dir] + img;
+  const float *XDp = a.X[DOWN] + img;
+  const float *XUp = a.X[UP] + img;
+  const float *ig_p = a.in_grad + rsl * C;
+  const float *m_p = a.msgs + rsl * C;
+  const uint8_t *am_p = a.argmin + rsl * C;
+  const uint8_t *gl_p = a.gargmin + rsl;
+  const float *par_p = lane < K ? a.jump + (((int64_t)nj * 4 + dir) * K + lane) * HW
+                                : a.edge + ((int64_t)n * 4 + dir) * HW;
+  float *gj_p = a.grad_jump + ((int64_t)nj * 4 + dir) * K * HW;
+  float *ge_p = a.grad_edge + ((int64_t)n * 4 + dir) * HW;
+
+  uint32_t voff[LPL];
+  bool valid[LPL];
+#pragma unroll
+  for (int k = 0; k < LPL; ++k) {
+    valid[k] = FULL || c0 + k < C;
+    voff[k] = (uint32_t)(FULL ? c0 + k : min(c0 + k, C - 1));
+  }
+  auto ld_row = [&](const float *row, float *d) {
+    if (FULL) {
+      ld_lanes_f<LPL>(row, voff[0] * 4u, d);
+    } else {
+#pragma unroll
+      for (int k = 0; k < LPL; ++k) d[k] = row[voff[k]];
+    }
+  };
+
+  // ---- the last position of the pass sends nothing: zero its outputs
+  {
+    const int64_t q = q0 + (int64_t)nsteps * dq;
+#pragma unroll
+    for (int k = 0; k < LPL; ++k)
+      if (valid[k]) Xo[q * C + c0 + k] = 0.f;
+    if (lane == 0) ge_p[q] = 0.f;
+    if (lane < K && !a.accum) gj_p[(int64_t)lane * HW + q] = 0.f;
+  }
+
+  // argmin bytes stay in the ring as loaded (a full row of 2 or 3 bytes per lane arrives as
+  // one 16-bit word plus one byte) and are split where they are used: a ring holding split
+  // bytes makes the compiler split every slot at the loop latch, waiting for fresh loads
+  constexpr int NRAW = FULL ? (LPL + 1) / 2 : LPL;
+  auto ld_args = [&](const uint8_t *row, uint32_t *d) {
+    if (FULL && LPL >= 2) {
+      const uint8_t *p = row + voff[0];
+      d[0] = (uint32_t)*reinterpret_cast<const uint16_t *>(p);
+      if (LPL == 3) d[1] = (uint32_t)p[2];
+    } else {
+#pragma unroll
+      for (int k = 0; k < NRAW; ++k) d[k] = (uint32_t)row[voff[k]];
+    }
+  };
+  auto arg_of = [&](const uint32_t *d, int k) -> int {
+    if (FULL && LPL >= 2) return k == 0 ? (int)(d[0] & 0xFFu) : k == 1 ? (int)(d[0] >> 8) : (int)d[1 < NRAW ? 1 : 0];
+    return (int)d[k < NRAW ? k : 0];
+  };
+
+  float pg[PF][LPL], pm[PF][LPL], pd[PF][LPL], pu[PF][LPL], pp[PF];
+  uint32_t pa[PF][NRAW];
+  int pl[PF];
+  auto load_step = [&](int s, float *dg, float *dm, uint32_t *da, float *dd, float *du, float &dp, int &dl) {
+    const int64_t q = q0 + (int64_t)s * dq;
+    const int64_t qr = q + dq;
+    ld_row(ig_p + qr * C, dg);
+    ld_row(m_p + qr * C, dm);
+    ld_args(am_p + qr * C, da);
+    if (!VERT) {
+      ld_row(XDp + qr * C, dd);
+      ld_row(XUp + qr * C, du);
+    } else {
+#pragma unroll
+      for (int k = 0; k < LPL; ++k) { dd[k] = 0.f; du[k] = 0.f; }
+    }
+    dl = (int)gl_p[qr];
+    dp = par_p[q];
+  };
+  if (nsteps > 0) {
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      const int s = nsteps - 1 - i;
+      load_step(s >= 0 ? s : 0, pg[i], pm[i], pa[i], pd[i], pu[i], pp[i], pl[i]);
+      // keep the slots in issue order: the loop's wait counts assume slot 0 is the oldest
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+
+  float acc[LPL];
+#pragma unroll
+  for (int k = 0; k < LPL; ++k) acc[k] = 0.f;
+
+  for (int t0 = 0; t0 < nsteps; t0 += PF) {
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      const int t = t0 + i;
+      const int s = nsteps - 1 - t;  // adjoint order
+      if (t < nsteps) {
+        const int64_t q = q0 + (int64_t)s * dq;
+        int A[LPL];
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) A[k] = arg_of(pa[i], k);
+        const float pv = pp[i];
+        const int gl = pl[i];
+
+        // normalisation Jacobian: g = t - softmax(msg) * sum(t), t = in_grad + chain
+        float tt[LPL], ts = 0.f;
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) {
+          tt[k] = valid[k] ? pg[i][k] + acc[k] : 0.f;
+          ts += tt[k];
+        }
+        ts = wave_sum_f(ts);
+        float gv[LPL], fs = 0.f;
+        bool wide = false;
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) {
+          float gk = tt[k] - fast_exp(pm[i][k]) * ts;
+          if (!VERT) gk += pd[i][k] + pu[i][k];
+          gv[k] = valid[k] ? gk : 0.f;
+          const bool mk = A[k] == kFromTrunc;
+          fs += mk ? gv[k] : 0.f;
+          wide = wide || (valid[k] && !mk && abs(A[k] - (c0 + k)) > 3);
+        }
+        const float fsum = wave_sum_f(fs);
+        // argmin of the receiver labels, -1 where nothing is routed by offset
+        int ar[LPL];
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) ar[k] = (valid[k] && A[k] != kFromTrunc) ? A[k] : -1;
+
+        // ---- route to the sender labels (deterministic gather)
+        float gh[LPL];
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) gh[k] = 0.f;
+        if (__ballot(wide) == 0) {
+          float gL[ND][LPL], gR[ND][LPL];
+          int aL[ND][LPL], aR[ND][LPL];
+#pragma unroll
+          for (int k = 0; k < LPL; ++k) {
+            gL[0][k] = shr1_f(gv[k], 0.f);
+            gR[0][k] = shl1_f(gv[k], 0.f);
+            aL[0][k] = shr1_i(ar[k], -1);
+            aR[0][k] = shl1_i(ar[k], -1);
+          }
+#pragma unroll
+          for (int d = 1; d < ND; ++d)
+#pragma unroll
+            for (int k = 0; k < LPL; ++k) {
+              gL[d][k] = shr1_f(gL[d - 1][k], 0.f);
+              gR[d][k] = shl1_f(gR[d - 1][k], 0.f);
+              aL[d][k] = shr1_i(aL[d - 1][k], -1);
+              aR[d][k] = shl1_i(aR[d - 1][k], -1);
+            }
+#pragma unroll
+          for (int kk = -3; kk <= 3; ++kk) {
+#pragma unroll
+            for (int k = 0; k < LPL; ++k) {
+              const int e = k - kk;  // receiver c = l - kk, as a label offset from c0
+              float gc;
+              int ac;
+              if (e >= 0 && e < LPL) {
+                gc = gv[e >= 0 && e < LPL ? e : 0];
+                ac = ar[e >= 0 && e < LPL ? e : 0];
+              } else if (e < 0) {
+                const int j = -e, d = (j + LPL - 1) / LPL, slot = d * LPL - j;
+                gc = gL[d - 1][slot];
+                ac = aL[d - 1][slot];
+              } else {
+                const int j = e - LPL + 1, d = (j + LPL - 1) / LPL, slot = j - 1 - (d - 1) * LPL;
+                gc = gR[d - 1][slot];
+                ac = aR[d - 1][slot];
+              }
+              gh[k] += ac == c0 + k ? gc : 0.f;
+            }
+          }
+        } else {
+          const int buf = t & 1;
+          int dm = 0;
+#pragma unroll
+          for (int k = 0; k < LPL; ++k) {
+            lds_g[buf][c0 + k] = gv[k];
+            lds_a[buf][c0 + k] = ar[k];
+            if (ar[k] >= 0) dm = max(dm, abs(ar[k] - (c0 + k)));
+          }
+          const int D = wave_max_i(dm);
+          __syncthreads();
+          for (int kk = -D; kk <= D; ++kk) {
+#pragma unroll
+            for (int k = 0; k < LPL; ++k) {
+              const int l = c0 + k;
+              const int c = l - kk;
+              if (c >= 0 && c < C && lds_a[buf][c] == l) gh[k] += lds_g[buf][c];
+            }
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) gh[k] += c0 + k == gl ? fsum : 0.f;
+
+        // ---- pairwise gradients of (dir, sender): bins 1..K-1 and the edge term in slot 0
+        const float w = readlane_f(pv, K);
+        float vb[8];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) vb[b] = 0.f;
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) {
+          const bool mk = A[k] == kFromTrunc;
+          const int src = mk ? gl : A[k];
+          const int bin = (mk && exact) ? K - 1 : min(max(K / 2 + c0 + k - src, 1), K - 1);
+#pragma unroll
+          for (int b = 1; b < 8; ++b) vb[b] += bin == b ? gv[k] : 0.f;
+        }
+#pragma unroll
+        for (int b = 1; b < 8; ++b)
+          if (b < K) vb[0] = fmaf(readlane_f(pv, b), vb[b], vb[0]);
+        float even, odd;
+        reduce_scatter8(vb, even, odd);
+        const int row = lane >> 4;
+        const int bin_here = 2 * row + (lane & 1);
+        const float tot = (lane & 1) ? odd : even;
+        if ((lane & 15) < 2) {
+          if (bin_here == 0) {
+            ge_p[q] = tot;
+            if (!a.accum) gj_p[q] = 0.f;
+          } else if (bin_here < K) {
+            float *dst = gj_p + (int64_t)bin_here * HW + q;
+            *dst = a.accum ? *dst + w * tot : w * tot;
+          }
+        }
+
+        // ---- chain and sender h-gradient
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) acc[k] = gh[k];
+        if (FULL) {
+          float *p = Xo + q * C + c0;
+#pragma unroll
+          for (int k = 0; k < LPL; ++k) p[k] = gh[k];
+        } else {
+#pragma unroll
+          for (int k = 0; k < LPL; ++k)
+            if (valid[k]) Xo[q * C + c0 + k] = gh[k];
+        }
+      }
+      // refill ring slot i after its last use
+      load_step(s - PF >= 0 ? s - PF : 0, pg[i], pm[i], pa[i], pd[i], pu[i], pp[i], pl[i]);
+    }
+  }
+}
+
 // grad_cost and the input-message gradient from the four sender h-gradients.
 __global__ void lbp_bwd_combine(const float *__restrict__ XL, const float *__restrict__ XR,
                                 const float *__restrict__ XU, const float *__restrict__ XD,
@@ -374,6 +675,31 @@ void launch_kind(const BwdArgs &a, bool vert, int ngroups, hipStream_t st) {
   else if (C <= 128) launch_phase<KIND, 64, 2>(a, vert, ngroups, st);
   else if (C <= 192) launch_phase<KIND, 64, 3>(a, vert, ngroups, st);
   else launch_phase<KIND, 64, 4>(a, vert, ngroups, st);
+}
+
+template <int LPL, bool FULL>
+void launch_stereo_w64(const BwdArgs &a, bool vert, int ngroups, hipStream_t st) {
+  constexpr int PF = 4;
+  const dim3 grid((unsigned)(ngroups * 2 * (vert ? a.W : a.H)));
+  if (vert)
+    hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, true, FULL, PF>), grid, dim3(kWave), 0, st, a);
+  else
+    hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, PF>), grid, dim3(kWave), 0, st, a);
+}
+template <int LPL>
+void launch_stereo_w64_any(const BwdArgs &a, bool vert, int ngroups, hipStream_t st) {
+  if (a.C == kWave * LPL) launch_stereo_w64<LPL, true>(a, vert, ngroups, st);
+  else launch_stereo_w64<LPL, false>(a, vert, ngroups, st);
+}
+// the 64-lane stereo kernel covers 32 < C <= 256 and up to 8 pairwise bins
+bool stereo_w64_ok(const BwdArgs &a) { return a.C > 32 && a.K <= 8; }
+void launch_stereo(const BwdArgs &a, bool vert, int ngroups, hipStream_t st) {
+  const int C = a.C;
+  if (!stereo_w64_ok(a) || generic_only()) launch_kind<BPL_LBP_STEREO>(a, vert, ngroups, st);
+  else if (C <= 64) launch_stereo_w64_any<1>(a, vert, ngroups, st);
+  else if (C <= 128) launch_stereo_w64_any<2>(a, vert, ngroups, st);
+  else if (C <= 192) launch_stereo_w64_any<3>(a, vert, ngroups, st);
+  else launch_stereo_w64_any<4>(a, vert, ngroups, st);
 }
 
 int seg_width(int C) { return C <= 16 ? 16 : C <= 32 ? 32 : 64; }
@@ -454,7 +780,7 @@ extern "C" int bpl_lbp_bwd(int kind, int mode, const float *jump, int64_t jump_k
       a.accum = per_n && b > 0;
       const int ng = per_n ? 1 : (int)N;
       switch (kind) {
-        case BPL_LBP_STEREO: launch_kind<BPL_LBP_STEREO>(a, vert, ng, st); break;
+        case BPL_LBP_STEREO: launch_stereo(a, vert, ng, st); break;
         case BPL_LBP_SEMANTIC_GLOBAL: launch_kind<BPL_LBP_SEMANTIC_GLOBAL>(a, vert, ng, st); break;
         default: launch_kind<BPL_LBP_SEMANTIC_PIXEL>(a, vert, ng, st); break;
       }

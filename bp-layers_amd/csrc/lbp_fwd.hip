@@ -12,8 +12,6 @@
 //     from a per-segment LDS copy of h;
 //   * the next PF steps' streamed inputs (cost and the two cross messages) are prefetched
 //     into a register ring.
-#include <stdlib.h>
-
 #include "bpl_common.h"
 
 namespace bpl {
@@ -890,12 +888,6 @@ void launch_kind(const FwdArgs &a, bool vert, hipStream_t st, size_t shm) {
   else launch_phase<KIND, 64, 4>(a, vert, st, shm);
 }
 
-// BPL_LBP_GENERIC=1 selects the generic kernels for every variant (used by tests to compare
-// the two stereo implementations).
-bool generic_only() {
-  const char *e = getenv("BPL_LBP_GENERIC");
-  return e && e[0] == '1';
-}
 
 }  // namespace
 }  // namespace bpl
