@@ -13,7 +13,6 @@ Prints ONE JSON line (see the contract in the repository docs):
 import argparse
 import json
 import os
-import statistics
 import sys
 import time
 
@@ -22,10 +21,21 @@ sys.path.insert(0, os.path.join(ROOT, "bp-layers_amd"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 # canonical algorithmic bytes per pixel*label per LBP iteration (fp32 messages, u8 argmins,
-# passes fused per phase): forward 44, exact backward 52; per-pixel parameters add
-# (P1/P2 per direction: 32 B/px read fwd, 64 B/px read+grad bwd; edge weights 16 / 32 B/px) / C
+# passes fused per phase, chain message on chip): forward 44, exact backward 52; per-pixel
+# parameters add (P1/P2 per direction: 32 B/px read fwd, 64 B/px read+grad bwd; edge weights
+# 16 / 32 B/px) / C.  Split by phase (kernel class of libbpl_amd):
+#   forward horizontal  read cost, M_U, M_D; write M_L, M_R, 2 argmins        22 B/pl
+#   forward vertical    read cost, M_L, M_R; write M_U, M_D, 2 argmins        22 B/pl
+#   backward vertical   read M_U, M_D, 2 argmins, G_U, G_D; write S_V         22 B/pl
+#   backward horizontal read M_L, M_R, 2 argmins, G_L, G_R, S_V;
+#                       write g_cost, g_msg (with the combine kernel)         30 B/pl
 FWD_B_PL = 44.0
 BWD_B_PL = 52.0
+PHASE_B_PL = {"lbp_fwd_horizontal": (22.0, 24.0), "lbp_fwd_vertical": (22.0, 24.0),
+              "lbp_bwd_vertical": (22.0, 48.0), "lbp_bwd_horizontal": (30.0, 48.0)}  # (per pl, per px / C)
+LBP_CLASSES = ("lbp_fwd_horizontal", "lbp_fwd_vertical", "lbp_bwd_vertical", "lbp_bwd_horizontal",
+               "lbp_bwd_combine")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
 def parse():
@@ -38,6 +48,9 @@ def parse():
     ap.add_argument("--rows", type=int, default=0, help="crop the image to this many rows (debug)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--pmc", action="store_true",
+                    help="counter-collection run for rocprofv3 --pmc: one warm-up step, one step and a "
+                         "256 MiB device copy (the calibration kernel); prints what ran")
     return ap.parse_args()
 
 
@@ -119,20 +132,6 @@ class StereoPlan:
                 g_in = g_out
 
 
-def time_events(fn, reps, torch):
-    """Average duration (ms) of fn() measured with HIP events on the current stream."""
-    s = torch.cuda.Event(enable_timing=True)
-    e = torch.cuda.Event(enable_timing=True)
-    ts = []
-    for _ in range(reps):
-        s.record()
-        fn()
-        e.record()
-        e.synchronize()
-        ts.append(s.elapsed_time(e))
-    return statistics.median(ts), ts
-
-
 def cpu_baseline(wl, plan_shape, seconds, iters_per_sample=1):
     """The C++/OpenMP restatement (oracle/c) on a crop of the same workload, fp32, all host
     cores: forward + exact backward of one LBP iteration; returns G pl/s."""
@@ -169,6 +168,37 @@ def cpu_baseline(wl, plan_shape, seconds, iters_per_sample=1):
                       f"{rows}x{W} crop with {C} labels ({pl / 1e6:.1f} M pl, {t:.1f} s)"}
 
 
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated with a copy kernel), if it covers this workload."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("workload") != workload or kernel not in t.get("kernels", {}):
+        return None
+    k = t["kernels"][kernel]
+    return {"bytes_per_launch": k["hbm_bytes_per_launch"], "source": os.path.relpath(TRAFFIC_FILE, ROOT)}
+
+
+def pmc_run(plan, torch):
+    """One warm-up step, one step and a 256 MiB device-to-device copy, so that a
+    rocprofv3 --pmc pass sees every kernel class of the step once per iteration and a
+    copy kernel of known size (256 MiB read + 256 MiB written) to calibrate the counters."""
+    plan.step()
+    torch.cuda.synchronize()
+    plan.step()
+    torch.cuda.synchronize()
+    src = torch.ones(64 * 1024 * 1024, device=plan.dev)  # 256 MiB of fp32
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"pmc_run": True, "steps": 2, "iters_per_step": plan.iters, "pl": plan.pl(),
+                      "calibration_copy_bytes": 2 * src.numel() * 4}), flush=True)
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -187,10 +217,19 @@ def main():
         backward = False
     plan = StereoPlan(wl, wl["iters"], rows=args.rows, backward=backward)
     torch.cuda.synchronize()
+    nat = plan.nat
+
+    if args.pmc:
+        pmc_run(plan, torch)
+        return
 
     for _ in range(args.warmup):
         plan.step()
     torch.cuda.synchronize()
+    # the timed region; the library brackets each of its launches with HIP events on the
+    # launch stream (two event records per launch, no synchronisation)
+    nat.timing_reset()
+    nat.timing_enable(True)
     t0 = time.perf_counter()
     start = torch.cuda.Event(enable_timing=True)
     end = torch.cuda.Event(enable_timing=True)
@@ -200,44 +239,50 @@ def main():
     end.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    nat.timing_enable(False)
+    kt = nat.timing_read()
     ms_step = start.elapsed_time(end) / args.steps
 
     pl = plan.pl()
     iters = plan.iters
     value = pl * iters / (ms_step * 1e-3) / 1e9
-
-    # per-op durations (HIP events on the launch stream) for the roofline
-    t_fwd, _ = time_events(lambda: plan.fwd(0), 5, torch)
-    res = {"fwd_ms": t_fwd}
     C = plan.C
+    avg = {k: v[0] / v[1] for k, v in kt.items()}   # ms per launch
+
+    # roofline of the dominant kernel class and of the whole iteration
+    lbp_k = {k: kt[k][0] for k in kt if k in PHASE_B_PL}
+    dom = max(lbp_k, key=lbp_k.get)
+    per_pl, per_px = PHASE_B_PL[dom]
+    dom_bytes = pl * (per_pl + per_px / C)
+    dom_achieved = dom_bytes / (avg[dom] * 1e-3) / 1e9
     fwd_b = FWD_B_PL + (32.0 + 16.0) / C
     bwd_b = BWD_B_PL + (64.0 + 32.0) / C
-    if backward:
-        t_bwd, _ = time_events(lambda: plan.bwd(0, plan.G, plan.gmsg[0]), 5, torch)
-        res["bwd_ms"] = t_bwd
-        it_ms = t_fwd + t_bwd
-        alg = pl * (fwd_b + bwd_b)
-        unit_name = "lbp_iteration_fwd+bwd (2 forward phase kernels, 2 backward phase kernels, combine)"
-    else:
-        it_ms = t_fwd
-        alg = pl * (FWD_B_PL - 4.0 + 48.0 / C)  # forward-only: no argmin writes needed
-        unit_name = "lbp_iteration_fwd (2 forward phase kernels)"
-    achieved = alg / (it_ms * 1e-3) / 1e9
-    # SAD timing (separately, same workload)
-    from bplayers_amd import pytorch_cuda_stereo_sad_op as sad
+    it_classes = LBP_CLASSES if backward else LBP_CLASSES[:2]
+    it_ms = sum(avg[k] for k in it_classes if k in avg)
+    it_bytes = pl * (fwd_b + (bwd_b if backward else -4.0))  # forward-only: no argmin writes
+    it_achieved = it_bytes / (it_ms * 1e-3) / 1e9
+    traffic = measured_traffic(args.workload, dom)
+    res = {f"{k}_ms": v for k, v in avg.items()}
+
+    # SAD cost volume (same workload), timed the same way
     cost_buf = torch.empty((plan.N, plan.H, plan.W, plan.max_disp - plan.min_disp + 1), device=plan.dev)
-    t_sad, _ = time_events(lambda: plan.nat.call(
-        "bpl_sad_stereo_fwd", plan.sp(plan.f0), plan.sp(plan.f1), plan.sp(cost_buf), plan.N, plan.f0.shape[1],
-        plan.H, plan.W, plan.min_disp, plan.max_disp, 1.0, plan.nat.stream_of(plan.f0)), 5, torch)
-    res["sad_fwd_ms"] = t_sad
-    if backward:
-        d0 = torch.empty_like(plan.f0)
-        d1 = torch.empty_like(plan.f1)
-        t_sadb, _ = time_events(lambda: plan.nat.call(
-            "bpl_sad_stereo_bwd", plan.sp(plan.f0), plan.sp(plan.f1), plan.sp(plan.sad_grad), plan.sp(d0),
-            plan.sp(d1), plan.N, plan.f0.shape[1], plan.H, plan.W, plan.min_disp, plan.max_disp,
-            plan.nat.stream_of(plan.f0)), 3, torch)
-        res["sad_bwd_ms"] = t_sadb
+    d0 = torch.empty_like(plan.f0)
+    d1 = torch.empty_like(plan.f1)
+    for rep in range(6):
+        if rep == 1:
+            nat.timing_reset()
+            nat.timing_enable(True)
+        nat.call("bpl_sad_stereo_fwd", plan.sp(plan.f0), plan.sp(plan.f1), plan.sp(cost_buf), plan.N,
+                 plan.f0.shape[1], plan.H, plan.W, plan.min_disp, plan.max_disp, 1.0, nat.stream_of(plan.f0))
+        if backward:
+            nat.call("bpl_sad_stereo_bwd", plan.sp(plan.f0), plan.sp(plan.f1), plan.sp(plan.sad_grad), plan.sp(d0),
+                     plan.sp(d1), plan.N, plan.f0.shape[1], plan.H, plan.W, plan.min_disp, plan.max_disp,
+                     nat.stream_of(plan.f0))
+    torch.cuda.synchronize()
+    nat.timing_enable(False)
+    for k, v in nat.timing_read().items():
+        res[f"{k}_ms"] = v[0] / v[1]
+    t_sad = res["sad_stereo_fwd_ms"]
 
     line = {
         "metric": "G(pixel*label)/s per LBP iter fwd+bwd" if backward else "G(pixel*label)/s per LBP iter fwd",
@@ -256,13 +301,19 @@ def main():
                    "H": plan.H, "W": plan.W, "labels": plan.C, "iters": iters, "delta": plan.delta,
                    "pairwise": "per-pixel per-direction P1/P2 (K=6 jump)" if backward else "global P1=0.1 P2=0.5",
                    "grad_mode": "exact" if backward else None},
-        "roofline": {"bound": "hbm", "kernel": unit_name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "alg_bytes_per_pl": round(fwd_b + (bwd_b if backward else 0.0), 3),
-                     "iteration_ms": round(it_ms, 4)},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(dom_achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(dom_achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "alg_bytes_per_launch": round(dom_bytes), "alg_bytes_per_pl": round(per_pl + per_px / C, 3),
+                     "avg_ms": round(avg[dom], 4), "launches": kt[dom][1],
+                     "traffic_source": traffic["source"] if traffic else None},
+        "roofline_iteration": {"bound": "hbm", "kernels": list(it_classes), "achieved": round(it_achieved, 2),
+                               "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(it_achieved / HBM_PEAK_GBS, 4),
+                               "alg_bytes_per_pl": round(fwd_b + (bwd_b if backward else -4.0), 3),
+                               "iteration_ms": round(it_ms, 4)},
         "timings_ms": {k: round(v, 4) for k, v in res.items()},
         "sad": {"fwd_G_pd_per_s": round(pl / (t_sad * 1e-3) / 1e9, 3),
-                "bwd_G_pd_per_s": round(pl / (res["sad_bwd_ms"] * 1e-3) / 1e9, 3) if backward else None},
+                "bwd_G_pd_per_s": round(pl / (res["sad_stereo_bwd_ms"] * 1e-3) / 1e9, 3) if backward else None},
         "wall_s": round(wall, 3),
     }
     if not args.no_cpu:

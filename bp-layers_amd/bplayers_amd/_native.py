@@ -37,7 +37,12 @@ _SIGS = {
     "bpl_lbp_beliefs": ([_P, _P, _P, _P, _I64, _I64, _I64, _I64, _F, _P], _I),
     "bpl_lbp_bwd": ([_I, _I, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I64,
                      _I64, _I64, _I64, _I64, _I, _P], _I),
+    "bpl_timing_enable": ([_I], _I),
+    "bpl_timing_reset": ([], _I),
+    "bpl_timing_read": ([_I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I64)], _I),
+    "bpl_timing_name": ([_I], ctypes.c_char_p),
 }
+KERNEL_COUNT = 12  # BPL_KERNEL_COUNT
 
 
 class BplError(RuntimeError):
@@ -95,3 +100,25 @@ def check_input(t, name):
         raise RuntimeError(f"{name} must be contiguous")
     if t.dtype != torch.float32:
         raise RuntimeError(f"{name} must be float32 (got {t.dtype})")
+
+
+# ---------------------------------------------------------------- kernel timing
+def timing_enable(on=True):
+    """Bracket every library kernel launch with HIP events on its stream (or stop doing so)."""
+    call("bpl_timing_enable", 1 if on else 0)
+
+
+def timing_reset():
+    call("bpl_timing_reset")
+
+
+def timing_read():
+    """{kernel class name: (total ms, launches)} for the classes launched since the reset;
+    waits for the recorded events."""
+    out = {}
+    for k in range(KERNEL_COUNT):
+        ms, n = ctypes.c_double(0.0), _I64(0)
+        call("bpl_timing_read", k, ctypes.byref(ms), ctypes.byref(n))
+        if n.value:
+            out[lib().bpl_timing_name(k).decode()] = (ms.value, n.value)
+    return out

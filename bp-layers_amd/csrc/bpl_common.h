@@ -32,6 +32,21 @@ int check_hip(hipError_t e, const char *what);
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Scope guard of the timing instrumentation: while bpl_timing_enable(1) is in effect the
+// launches made inside the scope are bracketed by events on `stream` and booked to `id`.
+class KernelTimer {
+ public:
+  KernelTimer(int id, hipStream_t stream);
+  ~KernelTimer();
+  KernelTimer(const KernelTimer &) = delete;
+  KernelTimer &operator=(const KernelTimer &) = delete;
+
+ private:
+  int id_;
+  hipStream_t stream_;
+  hipEvent_t start_ = nullptr, stop_ = nullptr;
+};
+
 // BPL_LBP_GENERIC=1 selects the generic LBP kernels for every variant (used by tests to
 // compare the stereo fast paths with them).
 inline bool generic_only() {

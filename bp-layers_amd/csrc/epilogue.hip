@@ -113,6 +113,7 @@ extern "C" int bpl_softmax_labels(const float *x, float *y, int64_t rows, int64_
   const unsigned grid = (unsigned)((rows + 3) / 4);
   hipStream_t st = as_stream(stream);
   const int lpl = (int)((C + 63) / 64);
+  KernelTimer kt(BPL_KERNEL_SOFTMAX, st);
   switch (lpl) {
     case 1: hipLaunchKernelGGL(softmax_labels_k<1>, dim3(grid), dim3(256), 0, st, x, y, rows, (int)C, scale, sign); break;
     case 2: hipLaunchKernelGGL(softmax_labels_k<2>, dim3(grid), dim3(256), 0, st, x, y, rows, (int)C, scale, sign); break;
@@ -133,6 +134,7 @@ extern "C" int bpl_lbp_beliefs(const float *cost, const float *messages, float *
   hipStream_t st = as_stream(stream);
   const int lpl = (int)((C + 63) / 64);
   const float it = 1.f / T;
+  KernelTimer kt(BPL_KERNEL_BELIEFS, st);
   switch (lpl) {
     case 1: hipLaunchKernelGGL(beliefs_k<1>, dim3(grid), dim3(256), 0, st, cost, messages, beliefs, label, (int)N, (int)H, (int)W, (int)C, it); break;
     case 2: hipLaunchKernelGGL(beliefs_k<2>, dim3(grid), dim3(256), 0, st, cost, messages, beliefs, label, (int)N, (int)H, (int)W, (int)C, it); break;

@@ -775,6 +775,7 @@ extern "C" int bpl_lbp_bwd(int kind, int mode, const float *jump, int64_t jump_k
     const bool vert = phase == 0;
     if (kind == BPL_LBP_SEMANTIC_GLOBAL) a.partial = ws + planes + (vert ? 0 : pv);
     const int nb = per_n ? (int)N : 1;
+    KernelTimer kt(vert ? BPL_KERNEL_LBP_BWD_V : BPL_KERNEL_LBP_BWD_H, st);
     for (int b = 0; b < nb; ++b) {
       a.n_base = per_n ? b : 0;
       a.accum = per_n && b > 0;
@@ -793,6 +794,7 @@ extern "C" int bpl_lbp_bwd(int kind, int mode, const float *jump, int64_t jump_k
     const int threads = 256;
     int64_t blocks = (total_el + threads - 1) / threads;
     if (blocks > 65536) blocks = 65536;
+    KernelTimer kt(BPL_KERNEL_LBP_BWD_COMBINE, st);
     hipLaunchKernelGGL(lbp_bwd_combine, dim3((unsigned)blocks), dim3(threads), 0, st, a.X[LEFT], a.X[RIGHT],
                        a.X[UP], a.X[DOWN], in_grad, grad_cost, grad_msg, (int)N, (int)H, (int)W, (int)C,
                        mode == BPL_GRAD_EXACT ? 1 : 0);
@@ -802,6 +804,7 @@ extern "C" int bpl_lbp_bwd(int kind, int mode, const float *jump, int64_t jump_k
   if (kind == BPL_LBP_SEMANTIC_GLOBAL) {
     const int spw = kWave / seg_width((int)C);
     const int wpg_v = (int)((W + spw - 1) / spw), wpg_h = (int)((H + spw - 1) / spw);
+    KernelTimer kt(BPL_KERNEL_LBP_BWD_REDUCE, st);
     hipLaunchKernelGGL(lbp_bwd_reduce_partials, dim3((unsigned)((C * C + 255) / 256), 4), dim3(256), 0, st,
                        ws + planes, grad_jump, (int)C, 0, 0, (int64_t)0, pv, (int)N, wpg_v, wpg_h, spw);
     int rc = check_hip(hipGetLastError(), "bpl_lbp_bwd reduce launch");

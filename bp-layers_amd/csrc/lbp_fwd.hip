@@ -927,6 +927,7 @@ extern "C" int bpl_lbp_fwd(int kind, const float *cost, const float *jump, int64
   hipStream_t st = as_stream(stream);
   for (int phase = 0; phase < 2; ++phase) {
     const bool vert = phase == 1;
+    KernelTimer kt(vert ? BPL_KERNEL_LBP_FWD_V : BPL_KERNEL_LBP_FWD_H, st);
     switch (kind) {
       case BPL_LBP_STEREO:
         if (stereo_dpp_ok(a) && !generic_only()) launch_stereo_dpp_any(a, vert, st);

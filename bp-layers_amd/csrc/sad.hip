@@ -402,6 +402,7 @@ extern "C" int bpl_sad_stereo_fwd(const float *f0, const float *f1, float *cost,
   const int D = (int)((float)(max_disp - min_disp + 1) / step);
   BPL_REQUIRE(D >= 1, BPL_E_ARG, "bpl_sad_stereo_fwd: empty disparity range");
   hipStream_t st = as_stream(stream);
+  KernelTimer kt(BPL_KERNEL_SAD_STEREO_FWD, st);
   if (step == 1.0f) {
     const int dpasses = (D + ST_TD - 1) / ST_TD;
     BPL_REQUIRE(N * dpasses < 65536, BPL_E_LIMIT, "bpl_sad_stereo_fwd: N * ceil(D/64) too large");
@@ -432,6 +433,7 @@ extern "C" int bpl_sad_stereo_bwd(const float *f0, const float *f1, const float 
     if (rc != BPL_OK) return rc;
   }
   dim3 grid((unsigned)((W + SB_TX - 1) / SB_TX), (unsigned)H, (unsigned)N);
+  KernelTimer kt(BPL_KERNEL_SAD_STEREO_BWD, as_stream(stream));
   hipLaunchKernelGGL(sad_stereo_bwd_k, grid, dim3(256), shm, as_stream(stream), f0, f1, grad_cost, df0, df1,
                      (int)C, (int)H, (int)W, D, min_disp);
   return check_hip(hipGetLastError(), "bpl_sad_stereo_bwd launch");
@@ -444,6 +446,7 @@ extern "C" int bpl_sad_flow_fwd(const float *f0, const float *f1, float *cv_u, f
   BPL_REQUIRE(N >= 1 && C >= 1 && H >= 1 && W >= 1, BPL_E_ARG, "bpl_sad_flow_fwd: sizes must be positive");
   BPL_REQUIRE(sws >= 0 && sws <= 1024, BPL_E_ARG, "bpl_sad_flow_fwd: sws=%d out of range", sws);
   hipStream_t st = as_stream(stream);
+  KernelTimer kt(BPL_KERNEL_SAD_FLOW_FWD, st);
   if (sws == 24) {
     dim3 grid((unsigned)((W + FL_TX - 1) / FL_TX), (unsigned)((H + FL_TY - 1) / FL_TY), (unsigned)N);
     hipLaunchKernelGGL(sad_flow_fwd_k<25>, grid, dim3(256), 0, st, f0, f1, cv_u, cv_v, u_star, v_star, (int)C,
@@ -463,6 +466,7 @@ extern "C" int bpl_sad_flow_bwd(const float *f0, const float *f1, const float *g
               "bpl_sad_flow_bwd: null pointer");
   BPL_REQUIRE(N >= 1 && C >= 1 && H >= 1 && W >= 1, BPL_E_ARG, "bpl_sad_flow_bwd: sizes must be positive");
   BPL_REQUIRE(sws >= 0 && sws <= 1024, BPL_E_ARG, "bpl_sad_flow_bwd: sws=%d out of range", sws);
+  KernelTimer kt(BPL_KERNEL_SAD_FLOW_BWD, as_stream(stream));
   hipLaunchKernelGGL(sad_flow_bwd_k, dim3(grid_for(N * H * W, 256)), dim3(256), 0, as_stream(stream), f0, f1,
                      grad_u, grad_v, u_star, v_star, df0, df1, (int)N, (int)C, (int)H, (int)W, sws);
   return check_hip(hipGetLastError(), "bpl_sad_flow_bwd launch");

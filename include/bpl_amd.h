@@ -49,6 +49,32 @@ int bpl_version(void);
 const char *bpl_last_error(void);
 
 /*
+ * Kernel timing (instrumentation; no counterpart in the reference, for benchmarks and
+ * profiling tools).  While enabled, every kernel launch of the library is bracketed by a
+ * pair of HIP events recorded on the launch stream.  bpl_timing_read() waits for the
+ * recorded events and returns the accumulated duration and launch count of one kernel
+ * class; bpl_timing_reset() clears the totals.  Disabled by default; when enabled it adds
+ * two event records per launch and no synchronisation.
+ */
+#define BPL_KERNEL_LBP_FWD_H 0      /* forward, horizontal phase (RIGHT and LEFT passes) */
+#define BPL_KERNEL_LBP_FWD_V 1      /* forward, vertical phase (DOWN and UP passes)      */
+#define BPL_KERNEL_LBP_BWD_V 2      /* backward, vertical phase                           */
+#define BPL_KERNEL_LBP_BWD_H 3      /* backward, horizontal phase                         */
+#define BPL_KERNEL_LBP_BWD_COMBINE 4
+#define BPL_KERNEL_LBP_BWD_REDUCE 5 /* semantic-global matrix gradient reduction          */
+#define BPL_KERNEL_SAD_STEREO_FWD 6
+#define BPL_KERNEL_SAD_STEREO_BWD 7
+#define BPL_KERNEL_SAD_FLOW_FWD 8
+#define BPL_KERNEL_SAD_FLOW_BWD 9
+#define BPL_KERNEL_SOFTMAX 10
+#define BPL_KERNEL_BELIEFS 11
+#define BPL_KERNEL_COUNT 12
+int bpl_timing_enable(int on);
+int bpl_timing_reset(void);
+int bpl_timing_read(int kernel, double *total_ms, int64_t *launches);
+const char *bpl_timing_name(int kernel);
+
+/*
  * Stereo SAD cost volume.  Replaces pytorch_cuda_stereo_sad_op.forward
  * (reference ops/sad/src/stereo_sad.cpp:10-15; kernel ops/sad/src/stereo_sad_kernel.cu:16-62).
  * cost[n,y,x,i] = sum_c |f0[n,c,y,x] - f1[n,c,y,x-d]|, d = int(i*step) + min_disp,

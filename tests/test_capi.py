@@ -89,3 +89,19 @@ def test_reference_aliases():
     assert hasattr(pytorch_cuda_lbp_op, "forward_minsum") and hasattr(pytorch_cuda_lbp_op, "backward_minsum")
     for k in ("pytorch_cuda_lbp_op", "pytorch_cuda_stereo_sad_op", "pytorch_cuda_flow_mp_sad_op"):
         sys.modules.pop(k)
+
+
+def test_timing_instrumentation_without_launches(lib):
+    """Enabling, resetting and reading the timing totals needs no GPU and reports nothing
+    before any launch."""
+    from bplayers_amd import _native as nat
+    nat.timing_enable(True)
+    nat.timing_reset()
+    assert nat.timing_read() == {}
+    nat.timing_enable(False)
+    names = [lib.bpl_timing_name(k).decode() for k in range(nat.KERNEL_COUNT)]
+    assert len(set(names)) == nat.KERNEL_COUNT and all(names)
+    assert lib.bpl_timing_name(nat.KERNEL_COUNT) == b""
+    ms = ctypes.c_double(0.0)
+    n = ctypes.c_int64(0)
+    assert lib.bpl_timing_read(99, ctypes.byref(ms), ctypes.byref(n)) == -1
