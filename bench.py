@@ -91,6 +91,7 @@ class StereoPlan:
                     [torch.empty((N, 4, H, W, C), device=dev) for _ in range(iters)]
         self.argmin = [torch.empty((N, 4, H, W, C), dtype=torch.uint8, device=dev) for _ in range(iters)]
         self.garg = [torch.empty((N, 4, H, W), dtype=torch.uint8, device=dev) for _ in range(iters)]
+        self.scale = [torch.empty((N, 4, H, W), device=dev) for _ in range(iters)]  # message_scale
         self.backward = backward
         if backward:
             g = torch.Generator(device="cpu").manual_seed(5)
@@ -111,7 +112,7 @@ class StereoPlan:
     def fwd(self, t):
         nat, p = self.nat, self.sp
         nat.call("bpl_lbp_fwd", nat.LBP_STEREO, p(self.cost), p(self.jump), self.K, self.jump.shape[0], p(self.edge),
-                 p(self.msgs[t]), p(self.msgs[t + 1]), p(self.argmin[t]), p(self.garg[t]), None,
+                 p(self.msgs[t]), p(self.msgs[t + 1]), p(self.argmin[t]), p(self.garg[t]), p(self.scale[t]),
                  self.N, self.H, self.W, self.C, self.delta, nat.stream_of(self.cost))
 
     def bwd(self, t, g_in, g_out):

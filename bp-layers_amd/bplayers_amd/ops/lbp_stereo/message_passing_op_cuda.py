@@ -72,13 +72,14 @@ class LBPMinSumFunction(torch.autograd.Function):
         jump = pack_affinities(affinities, offset.to(affinities.device))
         messages, messages_argmin, message_scale = lbp.forward_minsum(cost, jump, edge, messages, delta)
         ctx.save_for_backward(cost, jump, edge, messages, messages_argmin, message_scale)
+        ctx.delta = int(delta)
         return messages
 
     @staticmethod
     def backward(ctx, in_grad):
         cost, jump, edge, messages, messages_argmin, message_scale = ctx.saved_tensors
         grad_cost, grad_jump, grad_edge, grad_message = lbp.backward_minsum(
-            cost, jump, edge, in_grad.contiguous(), messages, messages_argmin, message_scale)
+            cost, jump, edge, in_grad.contiguous(), messages, messages_argmin, message_scale, delta=ctx.delta)
         grad_aff, grad_offset = unpack_affinity_grad(grad_jump)
         return grad_cost, grad_edge, grad_message, None, grad_aff, grad_offset
 
@@ -93,13 +94,15 @@ class LBPMinSumExactFunction(torch.autograd.Function):
         out, messages_argmin, message_scale = lbp.forward_minsum(cost, jump, edge, messages.contiguous(), delta,
                                                                   out=out)
         ctx.save_for_backward(cost, jump, edge, out, messages_argmin, message_scale)
+        ctx.delta = int(delta)
         return out
 
     @staticmethod
     def backward(ctx, in_grad):
         cost, jump, edge, messages, messages_argmin, message_scale = ctx.saved_tensors
         grad_cost, grad_jump, grad_edge, grad_message = lbp.backward_minsum(
-            cost, jump, edge, in_grad.contiguous(), messages, messages_argmin, message_scale, mode="exact")
+            cost, jump, edge, in_grad.contiguous(), messages, messages_argmin, message_scale, mode="exact",
+            delta=ctx.delta)
         grad_aff, grad_offset = unpack_affinity_grad(grad_jump)
         return grad_cost, grad_edge, grad_message, None, grad_aff, grad_offset
 

@@ -10,6 +10,10 @@ def forward_minsum(cost, jump, edge, messages, delta, *, out=None):
     return _lbp_op.forward_minsum(KIND, cost, jump, edge, messages, delta, out=out)
 
 
-def backward_minsum(cost, jump, edge, in_grad, messages, messages_argmin, message_scale, *, mode="reference"):
-    """Gradients [grad_unary, grad_pairwise, grad_edge, grad_messages] of one iteration."""
-    return _lbp_op.backward_minsum(KIND, cost, jump, edge, in_grad, messages, messages_argmin, message_scale, mode=mode)
+def backward_minsum(cost, jump, edge, in_grad, messages, messages_argmin, message_scale, *, mode="reference",
+                    delta=None):
+    """Gradients [grad_unary, grad_pairwise, grad_edge, grad_messages] of one iteration.
+    ``delta`` (optional, not in the reference's signature) is the forward's delta; when given
+    it lets the library pick its fastest kernels."""
+    return _lbp_op.backward_minsum(KIND, cost, jump, edge, in_grad, messages, messages_argmin, message_scale,
+                                   mode=mode, delta=delta)

@@ -25,6 +25,7 @@ struct BwdArgs {
   float *X[4];        // per-direction sender h-gradients [N,H,W,C], indexed by pass direction
   float *partial;     // semantic global: per-(wave, segment) C x C partial sums
   int N, H, W, C, K, Nj, mode, n_base, accum, mat_global;
+  int delta;  // the forward's window half-width, or -1 when the caller does not know it
 };
 
 __device__ __forceinline__ int64_t px(int n, int y, int x, int H, int W) {
@@ -328,7 +329,17 @@ __device__ __forceinline__ int shl1_i(int v, int fill) {
   return __builtin_amdgcn_update_dpp(fill, v, 0x130, 0xF, 0xF, false);
 }
 
-template <int LPL, bool VERT, bool FULL, int PF>
+// Fast step (FAST launches: delta <= 3, no batch accumulation; used for every scanline whose
+// window offsets are all zero, so every routed argmin lies within 3 labels of its receiver).
+// The step has no branches, so the compiler can overlap its tail (pairwise bins, stores) with
+// the dependent chain of the next step.  The normalisation sum ts = sum_c (G + acc) is carried
+// as a scalar recursion instead of a reduction of the chain: routing moves every receiver
+// gradient to exactly one sender label, so sum_c acc_t = sum_c g_{t-1}
+// = ts_{t-1} (1 - sum_c exp(M_{t-1})) + sum_c X_{t-1}, and
+// ts_t = ts_{t-1} + sum_c (G_t + X_{t-1} - ts_{t-1} exp(M_{t-1})) needs only loaded data and
+// the previous ts.  The scattered outputs (bins, edge) are raw buffer stores whose unused
+// lanes carry an out-of-range offset and are dropped by the hardware.
+template <int LPL, bool VERT, bool FULL, bool FAST, int PF>
 __global__ __launch_bounds__(64) void lbp_bwd_stereo_w64(BwdArgs a) {
   constexpr int ND = (3 + LPL - 1) / LPL;  // neighbour lanes reached by offsets up to 3
   __shared__ float lds_g[2][kWave * LPL];
@@ -443,6 +454,147 @@ __global__ __launch_bounds__(64) void lbp_bwd_stereo_w64(BwdArgs a) {
   float acc[LPL];
 #pragma unroll
   for (int k = 0; k < LPL; ++k) acc[k] = 0.f;
+
+  if constexpr (FAST) {
+    bool off = false;  // any non-zero window offset on this scanline (jump entry 0)
+    const float *o_p = a.jump + ((int64_t)nj * 4 + dir) * K * HW;
+    for (int p = lane; p < len; p += kWave) off = off || o_p[VERT ? (int64_t)p * W + ln : (int64_t)ln * W + p] != 0.f;
+    if (__ballot(off) == 0) {
+      const __amdgpu_buffer_rsrc_t gj_rs = __builtin_amdgcn_make_buffer_rsrc(gj_p, (short)0, (int)(K * HW * 4), 0x00020000);
+      const __amdgpu_buffer_rsrc_t ge_rs = __builtin_amdgcn_make_buffer_rsrc(ge_p, (short)0, (int)(HW * 4), 0x00020000);
+      const int row16 = lane >> 4;
+      const int bin_here = 2 * row16 + (lane & 1);  // bin held by this lane after the reduce-scatter
+      const bool bin_lane = (lane & 15) < 2 && bin_here < K;
+      constexpr int kDrop = 0x7FFFFFFF;              // out-of-range buffer offset: store dropped
+      float tsprev = 0.f, xprev[LPL], smprev[LPL];
+#pragma unroll
+      for (int k = 0; k < LPL; ++k) { xprev[k] = 0.f; smprev[k] = 0.f; }
+      auto step = [&](int i, int t) {
+        const int s = nsteps - 1 - t;
+        const int64_t q = q0 + (int64_t)s * dq;
+        int A[LPL];
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) A[k] = arg_of(pa[i], k);
+        // uses of ring slot i wait for the previous step (see the forward's fast step)
+        float pv = pp[i];
+        asm("" : "+v"(pv) : "v"(acc[0]));
+        int gl = pl[i];
+        asm("" : "+v"(gl) : "v"(acc[0]));
+        // normalisation sum by the scalar recursion
+        float r = 0.f;
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) r += valid[k] ? (pg[i][k] + xprev[k]) - tsprev * smprev[k] : 0.f;
+        const float ts = tsprev + wave_sum_f(r);
+        float sm[LPL], X[LPL], gv[LPL], fs = 0.f;
+        int ar[LPL];
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) {
+          sm[k] = valid[k] ? fast_exp(pm[i][k]) : 0.f;
+          X[k] = (!VERT && valid[k]) ? pd[i][k] + pu[i][k] : 0.f;
+          const float gk = ((pg[i][k] + acc[k]) - sm[k] * ts) + X[k];
+          gv[k] = valid[k] ? gk : 0.f;
+          const bool mk = A[k] == kFromTrunc;
+          fs += mk ? gv[k] : 0.f;
+          ar[k] = (valid[k] && !mk) ? A[k] : -1;
+        }
+        const float fsum = wave_sum_f(fs);
+        // routing over offsets -3..3 from the neighbour lanes
+        float gL[ND][LPL], gR[ND][LPL];
+        int aL[ND][LPL], aR[ND][LPL];
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) {
+          gL[0][k] = shr1_f(gv[k], 0.f);
+          gR[0][k] = shl1_f(gv[k], 0.f);
+          aL[0][k] = shr1_i(ar[k], -1);
+          aR[0][k] = shl1_i(ar[k], -1);
+        }
+#pragma unroll
+        for (int d = 1; d < ND; ++d)
+#pragma unroll
+          for (int k = 0; k < LPL; ++k) {
+            gL[d][k] = shr1_f(gL[d - 1][k], 0.f);
+            gR[d][k] = shl1_f(gR[d - 1][k], 0.f);
+            aL[d][k] = shr1_i(aL[d - 1][k], -1);
+            aR[d][k] = shl1_i(aR[d - 1][k], -1);
+          }
+        float gh[LPL];
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) gh[k] = c0 + k == gl ? fsum : 0.f;
+#pragma unroll
+        for (int kk = -3; kk <= 3; ++kk) {
+#pragma unroll
+          for (int k = 0; k < LPL; ++k) {
+            const int e = k - kk;
+            float gc;
+            int ac;
+            if (e >= 0 && e < LPL) {
+              gc = gv[e >= 0 && e < LPL ? e : 0];
+              ac = ar[e >= 0 && e < LPL ? e : 0];
+            } else if (e < 0) {
+              const int j = -e, d = (j + LPL - 1) / LPL, slot = d * LPL - j;
+              gc = gL[d - 1][slot];
+              ac = aL[d - 1][slot];
+            } else {
+              const int j = e - LPL + 1, d = (j + LPL - 1) / LPL, slot = j - 1 - (d - 1) * LPL;
+              gc = gR[d - 1][slot];
+              ac = aR[d - 1][slot];
+            }
+            gh[k] += ac == c0 + k ? gc : 0.f;
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) acc[k] = gh[k];
+
+        // ---- tail: pairwise bins and edge term, stores
+        const float w = readlane_f(pv, K);
+        float vb[8];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) vb[b] = 0.f;
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) {
+          const bool mk = A[k] == kFromTrunc;
+          const int src = mk ? gl : A[k];
+          const int bin = (mk && exact) ? K - 1 : min(max(K / 2 + c0 + k - src, 1), K - 1);
+#pragma unroll
+          for (int b = 1; b < 8; ++b) vb[b] += bin == b ? gv[k] : 0.f;
+        }
+#pragma unroll
+        for (int b = 1; b < 8; ++b) vb[0] = fmaf(readlane_f(pv, b < K ? b : 0), b < K ? vb[b] : 0.f, vb[0]);
+        float even, odd;
+        reduce_scatter8(vb, even, odd);
+        const float tot = (lane & 1) ? odd : even;
+        const int qi = (int)q;
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(bin_here == 0 ? 0.f : w * tot), gj_rs,
+                                              bin_lane ? (bin_here * (int)HW + qi) * 4 : kDrop, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(tot), ge_rs, lane == 0 ? qi * 4 : kDrop, 0, 0);
+        if (FULL) {
+          float *p = Xo + q * C + c0;
+#pragma unroll
+          for (int k = 0; k < LPL; ++k) p[k] = gh[k];
+        } else {
+#pragma unroll
+          for (int k = 0; k < LPL; ++k)
+            if (valid[k]) Xo[q * C + c0 + k] = gh[k];
+        }
+        tsprev = ts;
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) { xprev[k] = X[k]; smprev[k] = sm[k]; }
+      };
+      int t0 = 0;
+      for (; t0 + PF <= nsteps; t0 += PF) {
+#pragma unroll
+        for (int i = 0; i < PF; ++i) {
+          step(i, t0 + i);
+          const int s = nsteps - 1 - (t0 + i);
+          load_step(s - PF >= 0 ? s - PF : 0, pg[i], pm[i], pa[i], pd[i], pu[i], pp[i], pl[i]);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < PF; ++i)
+        if (t0 + i < nsteps) step(i, t0 + i);
+      return;
+    }
+  }
 
   for (int t0 = 0; t0 < nsteps; t0 += PF) {
 #pragma unroll
@@ -681,10 +833,20 @@ template <int LPL, bool FULL>
 void launch_stereo_w64(const BwdArgs &a, bool vert, int ngroups, hipStream_t st) {
   constexpr int PF = 4;
   const dim3 grid((unsigned)(ngroups * 2 * (vert ? a.W : a.H)));
-  if (vert)
-    hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, true, FULL, PF>), grid, dim3(kWave), 0, st, a);
-  else
-    hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, PF>), grid, dim3(kWave), 0, st, a);
+  // the fast step needs the window half-width (argmins within 3 labels) and 32-bit buffer
+  // offsets into the (direction) planes of the pairwise gradient
+  const bool fast = a.delta >= 0 && a.delta <= 3 && !a.accum && (int64_t)a.K * a.H * a.W * 4 < (1LL << 31);
+  if (fast) {
+    if (vert)
+      hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, true, FULL, true, PF>), grid, dim3(kWave), 0, st, a);
+    else
+      hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, true, PF>), grid, dim3(kWave), 0, st, a);
+  } else {
+    if (vert)
+      hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, true, FULL, false, PF>), grid, dim3(kWave), 0, st, a);
+    else
+      hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, false, PF>), grid, dim3(kWave), 0, st, a);
+  }
 }
 template <int LPL>
 void launch_stereo_w64_any(const BwdArgs &a, bool vert, int ngroups, hipStream_t st) {
@@ -737,7 +899,6 @@ extern "C" int bpl_lbp_bwd(int kind, int mode, const float *jump, int64_t jump_k
                            float *grad_cost, float *grad_jump, float *grad_edge, float *grad_msg,
                            void *workspace, int64_t workspace_bytes,
                            int64_t N, int64_t H, int64_t W, int64_t C, int delta, void *stream) {
-  (void)delta;
   BPL_REQUIRE(kind >= 0 && kind <= 2, BPL_E_ARG, "bpl_lbp_bwd: unknown kind %d", kind);
   BPL_REQUIRE(mode == BPL_GRAD_REFERENCE || mode == BPL_GRAD_EXACT, BPL_E_ARG, "bpl_lbp_bwd: unknown mode %d", mode);
   BPL_REQUIRE(N >= 1 && H >= 1 && W >= 1 && C >= 1, BPL_E_ARG, "bpl_lbp_bwd: sizes must be positive");
@@ -768,6 +929,7 @@ extern "C" int bpl_lbp_bwd(int kind, int mode, const float *jump, int64_t jump_k
   a.K = kind == BPL_LBP_STEREO ? (int)jump_k : 0;
   a.Nj = kind == BPL_LBP_STEREO ? (int)jump_n : 1;
   a.mode = mode; a.mat_global = 1;
+  a.delta = delta;
   hipStream_t st = as_stream(stream);
   // parameter gradients shared across the batch accumulate over n in launch order
   const bool per_n = (kind == BPL_LBP_STEREO && a.Nj == 1 && N > 1) || (kind == BPL_LBP_SEMANTIC_PIXEL && N > 1);

@@ -606,7 +606,23 @@ __device__ __forceinline__ float shl1_inf(float v) {  // lane i <- lane i+1, lan
   return __int_as_float(__builtin_amdgcn_update_dpp(0x7F800000, __float_as_int(v), 0x130, 0xF, 0xF, false));
 }
 
-template <int LPL, bool VERT, bool FULL, int PF>
+// Fast step (FAST launches: delta == 3, argmin, gargmin and scale requested; used for every
+// scanline whose window offsets are all zero).  The step has no branches, so the compiler can
+// overlap the tail of one step (label of the minimum, normalisation, stores) with the
+// dependent chain of the next:
+//   * the chain carried to the next pixel is u = msg - gv instead of the normalised message
+//     msg - lse.  Messages are shift-invariant in their input (a constant added to h moves
+//     every candidate and gv by that constant), so this changes no message and no argmin; the
+//     normalised output is u - L with L = log sum exp(u), and the scale exp(lse) of the true
+//     frame is exp(gv + L - L_prev), L_prev being the previous pixel's L (0 at the start);
+//   * the window minimum (last minimal candidate in offset order) is formed before the global
+//     minimum is known and merged with the truncation value gv afterwards: gv precedes the
+//     window in the reference's candidate order, so a window value equal to gv wins;
+//   * the label of the minimum is the largest tie key among the labels attaining it (an
+//     integer max-reduction), which is the reference's tree order for ties and the label
+//     itself when the minimum is unique;
+//   * the normalisation is shifted by the wave maximum of u.
+template <int LPL, bool VERT, bool FULL, bool FAST, int PF>
 __global__ __launch_bounds__(64) void lbp_fwd_stereo_w64(FwdArgs a) {
   constexpr int ND = (3 + LPL - 1) / LPL;  // neighbour lanes reached by a +-3 window
   __shared__ float lds_h[2][kWave * LPL];
@@ -654,6 +670,147 @@ __global__ __launch_bounds__(64) void lbp_fwd_stereo_w64(FwdArgs a) {
       for (int k = 0; k < LPL; ++k) d[k] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(row) + voff[k]);
     }
   };
+
+  if constexpr (FAST) {
+    bool off = false;  // any non-zero window offset on this scanline (jump entry 0)
+    const float *o_p = a.jump + ((int64_t)nj * 4 + dir) * K * HW;
+    for (int p = lane; p < len; p += kWave) off = off || o_p[VERT ? (int64_t)p * W + ln : (int64_t)ln * W + p] != 0.f;
+    if (__ballot(off) == 0) {
+      float chain[LPL];
+      ld_row(a.msg_in + (rs_base + q0) * C, chain);
+#pragma unroll
+      for (int k = 0; k < LPL; ++k)
+        if (valid[k]) {
+          out_p[q0 * C + c0 + k] = chain[k];
+          arg_p[q0 * C + c0 + k] = 0;
+        }
+      a.scale[rs_base + q0] = 0.f;  // every lane stores the same value
+      a.gargmin[rs_base + q0] = 0;
+
+      float pp[PF];
+      float pc[PF][LPL], q1[PF][LPL], q2[PF][LPL];
+      auto load_ring = [&](int s, float *dc, float *d1, float *d2, float &dpar) {
+        const int64_t q = q0 + (int64_t)s * dq;
+        ld_row(cost_p + q * C, dc);
+        ld_row(x1_p + q * C, d1);
+        ld_row(x2_p + q * C, d2);
+        dpar = par_p[q];
+      };
+#pragma unroll
+      for (int i = 0; i < PF; ++i) {
+        load_ring(i < nsteps ? i : (nsteps > 0 ? nsteps - 1 : 0), pc[i], q1[i], q2[i], pp[i]);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      float Lprev = 0.f;
+      auto step = [&](int i, int s) {
+        // Every use of ring slot i is made to depend on the chain of the previous step: the
+        // scheduler takes loads issued in an earlier loop iteration as complete and would
+        // otherwise hoist their uses (and the waits for them) to the top of the loop body.
+        float h[LPL];
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) {
+          const float v = VERT ? ((pc[i][k] + chain[k]) + q1[i][k]) + q2[i][k]
+                               : ((pc[i][k] + chain[k]) + q1[i][k]) + q2[i][k];
+          h[k] = valid[k] ? v : __builtin_inff();
+        }
+        float pv = pp[i];
+        asm("" : "+v"(pv) : "v"(chain[0]));  // no instruction: orders pv after the chain
+        const float w = readlane_f(pv, K);
+        const float JK = readlane_f(pv, K - 1);
+        // global minimum
+        int hk = float_key(h[0]);
+#pragma unroll
+        for (int k = 1; k < LPL; ++k) hk = min(hk, float_key(h[k]));
+        const float Hs = key_to_float(wave_min_i(hk));
+        // window minimum over offsets -3..3, last minimal candidate in offset order
+        float Lh[ND][LPL], Rh[ND][LPL];
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) { Lh[0][k] = shr1_inf(h[k]); Rh[0][k] = shl1_inf(h[k]); }
+#pragma unroll
+        for (int d = 1; d < ND; ++d)
+#pragma unroll
+          for (int k = 0; k < LPL; ++k) { Lh[d][k] = shr1_inf(Lh[d - 1][k]); Rh[d][k] = shl1_inf(Rh[d - 1][k]); }
+        float mw[LPL];
+        int aw[LPL];
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) { mw[k] = __builtin_inff(); aw[k] = kArgTrunc; }
+#pragma unroll
+        for (int kk = -3; kk <= 3; ++kk) {
+          const float fk = readlane_f(pv, min(max(K / 2 - kk, 1), K - 1));
+#pragma unroll
+          for (int k = 0; k < LPL; ++k) {
+            const int e = k + kk;
+            float hv;
+            if (e >= 0 && e < LPL) {
+              hv = h[e >= 0 && e < LPL ? e : 0];
+            } else if (e < 0) {
+              const int j = -e, d = (j + LPL - 1) / LPL, slot = d * LPL - j;
+              hv = Lh[d - 1][slot];
+            } else {
+              const int j = e - LPL + 1, d = (j + LPL - 1) / LPL, slot = j - 1 - (d - 1) * LPL;
+              hv = Rh[d - 1][slot];
+            }
+            const float nv = fmaf(fk, w, hv);
+            const bool t = nv <= mw[k];
+            mw[k] = t ? nv : mw[k];
+            aw[k] = t ? kk : aw[k];
+          }
+        }
+        const float gv = fmaf(JK, w, Hs);
+        int arg[LPL];
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) {
+          const bool t = mw[k] <= gv;
+          chain[k] = (t ? mw[k] : gv) - gv;  // u: the next pixel's chain term
+          arg[k] = t ? aw[k] : kArgTrunc;
+        }
+        // ---- tail: label of the minimum, normalisation, stores
+        int tk = -1;
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) tk = max(tk, (valid[k] && h[k] == Hs) ? tie_key(c0 + k, a.bits) : -1);
+        const int lstar = tie_key(wave_max_i(tk), a.bits);
+        int mk = float_key(valid[0] ? chain[0] : -__builtin_inff());
+#pragma unroll
+        for (int k = 1; k < LPL; ++k) mk = max(mk, float_key(valid[k] ? chain[k] : -__builtin_inff()));
+        const float umax = key_to_float(wave_max_i(mk));
+        float se = 0.f;
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) se += valid[k] ? fast_exp(chain[k] - umax) : 0.f;
+        se = wave_sum_f(se);
+        const float Lc = umax + fast_log(se);
+        const int64_t qq = q0 + (int64_t)(s + 1) * dq;
+        float outv[LPL];
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) outv[k] = chain[k] - Lc;
+        if (FULL) {
+          st_lanes<LPL>(out_p + qq * C, voff[0], outv);
+        } else {
+#pragma unroll
+          for (int k = 0; k < LPL; ++k)
+            if (valid[k]) out_p[qq * C + c0 + k] = outv[k];
+        }
+#pragma unroll
+        for (int k = 0; k < LPL; ++k)
+          if (valid[k]) arg_p[qq * C + c0 + k] = (uint8_t)(arg[k] == kArgTrunc ? kFromTrunc : c0 + k + arg[k]);
+        a.scale[rs_base + qq] = fast_exp(gv + Lc - Lprev);  // same value from every lane
+        a.gargmin[rs_base + qq] = (uint8_t)lstar;
+        Lprev = Lc;
+      };
+      int s0 = 0;
+      for (; s0 + PF <= nsteps; s0 += PF) {
+#pragma unroll
+        for (int i = 0; i < PF; ++i) {
+          step(i, s0 + i);
+          const int sn = s0 + i + PF;
+          load_ring(sn < nsteps ? sn : nsteps - 1, pc[i], q1[i], q2[i], pp[i]);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < PF; ++i)
+        if (s0 + i < nsteps) step(i, s0 + i);
+      return;
+    }
+  }
 
   float chain[LPL];
   {
@@ -840,10 +997,18 @@ __global__ __launch_bounds__(64) void lbp_fwd_stereo_w64(FwdArgs a) {
 template <int LPL, bool FULL>
 void launch_stereo_w64(const FwdArgs &a, bool vert, hipStream_t st) {
   const dim3 grid((unsigned)(a.N * 2 * (vert ? a.W : a.H)));
-  if (vert)
-    hipLaunchKernelGGL((lbp_fwd_stereo_w64<LPL, true, FULL, 4>), grid, dim3(kWave), 0, st, a);
-  else
-    hipLaunchKernelGGL((lbp_fwd_stereo_w64<LPL, false, FULL, 4>), grid, dim3(kWave), 0, st, a);
+  const bool fast = a.delta == 3 && a.argmin && a.gargmin && a.scale;
+  if (fast) {
+    if (vert)
+      hipLaunchKernelGGL((lbp_fwd_stereo_w64<LPL, true, FULL, true, 4>), grid, dim3(kWave), 0, st, a);
+    else
+      hipLaunchKernelGGL((lbp_fwd_stereo_w64<LPL, false, FULL, true, 4>), grid, dim3(kWave), 0, st, a);
+  } else {
+    if (vert)
+      hipLaunchKernelGGL((lbp_fwd_stereo_w64<LPL, true, FULL, false, 4>), grid, dim3(kWave), 0, st, a);
+    else
+      hipLaunchKernelGGL((lbp_fwd_stereo_w64<LPL, false, FULL, false, 4>), grid, dim3(kWave), 0, st, a);
+  }
 }
 template <int LPL>
 void launch_stereo_w64_any(const FwdArgs &a, bool vert, hipStream_t st) {

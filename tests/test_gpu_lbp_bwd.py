@@ -18,7 +18,7 @@ def _ties_ok(cost, jump, edge, M0, delta, kind):
     return margin
 
 
-def run_case(kind, N, H, W, C, delta, mode, seed, **extra):
+def run_case(kind, N, H, W, C, delta, mode, seed, pass_delta=False, **extra):
     cost, jump, edge, M0 = make_case(kind, N, H, W, C, delta, seed=seed, **extra)
     mod = _mods()[kind]
     dev = torch.device("cuda:0")
@@ -31,7 +31,8 @@ def run_case(kind, N, H, W, C, delta, mode, seed, **extra):
         res = mod.backward_minsum(t(cost), t(edge), t(G), msgs, am, sc, mode=mode,
                                   jump=t(jump) if mode == "exact" else None)
     else:
-        res = mod.backward_minsum(t(cost), t(jump), t(edge), t(G), msgs, am, sc, mode=mode)
+        kw = {"delta": delta} if pass_delta else {}
+        res = mod.backward_minsum(t(cost), t(jump), t(edge), t(G), msgs, am, sc, mode=mode, **kw)
     torch.cuda.synchronize()
     gc, gj, ge, gm = [r.cpu().numpy().astype(np.float64) for r in res]
     pw = O.make_pairwise(kind, jump)
@@ -80,7 +81,12 @@ CASES = [
 
 @pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c[0]}-N{c[1]}H{c[2]}W{c[3]}C{c[4]}d{c[5]}{''.join(c[6])}")
 @pytest.mark.parametrize("mode", ["exact", "reference"])
-def test_lbp_backward_matches_oracle(case, mode):
+@pytest.mark.parametrize("pass_delta", [False, True], ids=["no-delta", "delta"])
+def test_lbp_backward_matches_oracle(case, mode, pass_delta):
+    """'delta' passes the forward's delta, which selects the stereo fast step where it applies
+    (delta <= 3, C > 32); 'no-delta' runs the general kernels."""
     kind, N, H, W, C, delta, extra = case
-    print(f"case {case} mode={mode}", flush=True)
-    run_case(kind, N, H, W, C, delta, mode, seed=100 + C + H, **extra)
+    if pass_delta and kind != "stereo":
+        pytest.skip("only the stereo backward takes the forward's delta")
+    print(f"case {case} mode={mode} pass_delta={pass_delta}", flush=True)
+    run_case(kind, N, H, W, C, delta, mode, seed=100 + C + H, pass_delta=pass_delta, **extra)
