@@ -109,69 +109,98 @@ __global__ void sad_stereo_fwd_step_k(const float *__restrict__ f0, const float 
 }
 
 // ----------------------------------------------------------------------------- stereo bwd
-// Block: one row, 32 pixels x 8 channel lanes.  The gradient rows needed by the tile
-// (g[y,x,:] for df0 and the diagonal g[y,x+d,d] for df1) are staged in LDS once, and for
-// each chunk of 8 channels the f1 segment [x0-dmax, x0+31-dmin] and the f0 segment
-// [x0+dmin, x0+31+dmax] too; each (channel, pixel) then sums its D terms in disparity order
-// from LDS only.
-constexpr int SB_TX = 32, SB_CB = 8;
+// Block: one row, a tile of 32 pixels, 256 threads = 16 pixel pairs x 16 channels.
+//   G0[k][x] = g[y, x, k]      (0 where x - d is outside the image)   -> df0 gather
+//   G1[k][x] = g[y, x + d, k]  (0 where x + d is outside the image)   -> df1 gather
+// are staged once per block (the diagonal G1 row by row of the source pixel, so the global
+// reads stay contiguous); then, per chunk of 16 channels, the f1 segment [x0-dmax, x0+31-dmin]
+// and the f0 segment [x0+dmin, x0+31+dmax].  Each thread sums its D terms in disparity order
+// (the reference's order) from LDS, reading one new f value per disparity per side; the
+// invalid pairs carry g = 0, so the loop has no range tests.
+constexpr int SB_TX = 32, SB_CB = 16, SB_GP = SB_TX + 1;
+
+// sign(t) * g with the reference's dead zone |t| <= 1e-15 -> 0 (a sign-bit flip: exact)
+__device__ __forceinline__ float sgn_mul(float t, float g) {
+  const float v = __int_as_float(__float_as_int(g) ^ (__float_as_int(t) & (int)0x80000000));
+  return fabsf(t) > kSignEps ? v : 0.f;
+}
+
+size_t sad_bwd_lds_bytes(int D) {
+  return ((size_t)2 * D * SB_GP + (size_t)2 * SB_CB * (SB_TX + D - 1)) * sizeof(float);
+}
 
 __global__ __launch_bounds__(256) void sad_stereo_bwd_k(const float *__restrict__ f0, const float *__restrict__ f1,
                                                         const float *__restrict__ g, float *__restrict__ df0,
                                                         float *__restrict__ df1, int C, int H, int W, int D, int mind) {
   extern __shared__ float sm[];
-  const int DP = D + 1;
-  const int SPAN = SB_TX + D;            // staged f columns per channel
-  float *gx = sm;                        // [SB_TX][D+1]
-  float *gd = gx + SB_TX * DP;           // [SB_TX][D+1]
-  float *f1s = gd + SB_TX * DP;          // [SB_CB][SPAN]: f1 at x0 - dmax + j
-  float *f0s = f1s + SB_CB * SPAN;       // [SB_CB][SPAN]: f0 at x0 + dmin + j
+  const int SPAN = SB_TX + D - 1;
+  float *G0 = sm;                   // [D][SB_GP]
+  float *G1 = G0 + D * SB_GP;       // [D][SB_GP]
+  float *F1 = G1 + D * SB_GP;       // [SB_CB][SPAN]: f1 at x0 - dmax + j
+  float *F0 = F1 + SB_CB * SPAN;    // [SB_CB][SPAN]: f0 at x0 + dmin + j
   const int x0 = blockIdx.x * SB_TX;
   const int y = blockIdx.y;
   const int n = blockIdx.z;
   const int t = threadIdx.x;
   const int maxd = mind + D - 1;
   const int64_t HW = (int64_t)H * W;
-  const float *G = g + ((int64_t)n * H + y) * (int64_t)W * D;
+  const float *Gr = g + ((int64_t)n * H + y) * (int64_t)W * D;
   for (int i = t; i < SB_TX * D; i += 256) {
     const int xl = i / D, k = i - xl * D;
-    const int x = x0 + xl;
-    const int xd = x + mind + k;
-    gx[xl * DP + k] = (x < W) ? G[(int64_t)x * D + k] : 0.f;
-    gd[xl * DP + k] = (x < W && xd >= 0 && xd < W) ? G[(int64_t)xd * D + k] : 0.f;
+    const int x = x0 + xl, xm = x - (mind + k);
+    G0[k * SB_GP + xl] = (x < W && xm >= 0 && xm < W) ? Gr[(int64_t)x * D + k] : 0.f;
   }
-  const int xl = t % SB_TX;
-  const int cl = t / SB_TX;
-  const int x = x0 + xl;
+  // source pixel p = x0 + dmin + m carries the entries k = m - xl of the tile's pixels x0 + xl
+  for (int i = t; i < SPAN * SB_TX; i += 256) {
+    const int m = i / SB_TX, xl = i - m * SB_TX;
+    const int k = m - xl;
+    if (k < 0 || k >= D) continue;
+    const int x = x0 + xl, p = x0 + mind + m;
+    G1[k * SB_GP + xl] = (x < W && p >= 0 && p < W) ? Gr[(int64_t)p * D + k] : 0.f;
+  }
+  const int xg = t & 15;
+  const int cl = t >> 4;
+  const int xa = x0 + 2 * xg;
   for (int c0 = 0; c0 < C; c0 += SB_CB) {
-    __syncthreads();
+    __syncthreads();  // staging of G (first chunk) / reads of the previous chunk are done
     for (int i = t; i < SB_CB * SPAN; i += 256) {
       const int cc = i / SPAN, j = i - cc * SPAN;
       const int c = c0 + cc;
       const int64_t b = ((int64_t)n * C + c) * HW + (int64_t)y * W;
-      const int xa = x0 - maxd + j, xb = x0 + mind + j;
-      f1s[i] = (c < C && xa >= 0 && xa < W) ? f1[b + xa] : 0.f;
-      f0s[i] = (c < C && xb >= 0 && xb < W) ? f0[b + xb] : 0.f;
+      const int xs1 = x0 - maxd + j, xs0 = x0 + mind + j;
+      F1[i] = (c < C && xs1 >= 0 && xs1 < W) ? f1[b + xs1] : 0.f;
+      F0[i] = (c < C && xs0 >= 0 && xs0 < W) ? f0[b + xs0] : 0.f;
     }
     __syncthreads();
     const int c = c0 + cl;
-    if (c >= C || x >= W) continue;
-    const int64_t b = ((int64_t)n * C + c) * HW + (int64_t)y * W;
-    const float a0 = f0[b + x], b1 = f1[b + x];
-    const float *F1 = f1s + cl * SPAN;
-    const float *F0 = f0s + cl * SPAN;
-    const float *GX = gx + xl * DP;
-    const float *GD = gd + xl * DP;
-    // x - d = x0 - maxd + j  ->  j = xl + maxd - d ;  x + d = x0 + mind + j -> j = xl + d - mind
-    float r0 = 0.f, r1 = 0.f;
+    const int64_t b = ((int64_t)n * C + (c < C ? c : C - 1)) * HW + (int64_t)y * W;
+    const float a0 = xa < W ? f0[b + xa] : 0.f, a1 = xa + 1 < W ? f0[b + xa + 1] : 0.f;
+    const float b0 = xa < W ? f1[b + xa] : 0.f, b1 = xa + 1 < W ? f1[b + xa + 1] : 0.f;
+    // f1[x - d] = F1c[j - k], f0[x + d] = F0c[j + k] for pixel xa + j
+    const float *F1c = F1 + cl * SPAN + 2 * xg + D - 1;
+    const float *F0c = F0 + cl * SPAN + 2 * xg;
+    const float *G0c = G0 + 2 * xg;
+    const float *G1c = G1 + 2 * xg;
+    float r00 = 0.f, r01 = 0.f, r10 = 0.f, r11 = 0.f;
+    float v1 = F1c[1];  // f1 window (pixel xa + 1 at k = 0 is F1c[1])
+    float v0 = F0c[0];  // f0 window
+#pragma unroll 4
     for (int k = 0; k < D; ++k) {
-      const int d = mind + k;
-      const int xm = x - d, xp = x + d;
-      if (xm >= 0 && xm < W) r0 += sgn_eps(a0 - F1[xl + maxd - d]) * GX[k];
-      if (xp >= 0 && xp < W) r1 -= sgn_eps(F0[xl + k] - b1) * GD[k];
+      const float v1n = F1c[-k];     // f1[xa - d]; f1[xa + 1 - d] is the previous step's
+      const float v0n = F0c[k + 1];  // f0[xa + 1 + d]; f0[xa + d] is the previous step's
+      const float g00 = G0c[k * SB_GP], g01 = G0c[k * SB_GP + 1];
+      const float g10 = G1c[k * SB_GP], g11 = G1c[k * SB_GP + 1];
+      r00 += sgn_mul(a0 - v1n, g00);
+      r01 += sgn_mul(a1 - v1, g01);
+      r10 -= sgn_mul(v0 - b0, g10);
+      r11 -= sgn_mul(v0n - b1, g11);
+      v1 = v1n;
+      v0 = v0n;
     }
-    df0[b + x] = r0;
-    df1[b + x] = r1;
+    if (c < C) {
+      if (xa < W) { df0[b + xa] = r00; df1[b + xa] = r10; }
+      if (xa + 1 < W) { df0[b + xa + 1] = r01; df1[b + xa + 1] = r11; }
+    }
   }
 }
 
@@ -423,7 +452,7 @@ extern "C" int bpl_sad_stereo_bwd(const float *f0, const float *f1, const float 
   BPL_REQUIRE(N >= 1 && C >= 1 && H >= 1 && W >= 1, BPL_E_ARG, "bpl_sad_stereo_bwd: sizes must be positive");
   const int D = max_disp - min_disp + 1;
   BPL_REQUIRE(D >= 1, BPL_E_ARG, "bpl_sad_stereo_bwd: empty disparity range");
-  const size_t shm = ((size_t)2 * SB_TX * (D + 1) + (size_t)2 * SB_CB * (SB_TX + D)) * sizeof(float);
+  const size_t shm = sad_bwd_lds_bytes(D);
   BPL_REQUIRE(shm <= 160 * 1024, BPL_E_LIMIT, "bpl_sad_stereo_bwd: D=%d needs %zu bytes of LDS", D, shm);
   BPL_REQUIRE(H < 65536 && N < 65536, BPL_E_LIMIT, "bpl_sad_stereo_bwd: tensor too large");
   if (shm > 64 * 1024) {
