@@ -709,8 +709,9 @@ __global__ __launch_bounds__(64) void lbp_fwd_stereo_w64(FwdArgs a) {
         float h[LPL];
 #pragma unroll
         for (int k = 0; k < LPL; ++k) {
-          const float v = VERT ? ((pc[i][k] + chain[k]) + q1[i][k]) + q2[i][k]
-                               : ((pc[i][k] + chain[k]) + q1[i][k]) + q2[i][k];
+          float c = pc[i][k];
+          if (VERT) asm("" : "+v"(c) : "v"(chain[k]));  // the vertical order adds the chain last
+          const float v = VERT ? ((c + q1[i][k]) + q2[i][k]) + chain[k] : ((c + chain[k]) + q1[i][k]) + q2[i][k];
           h[k] = valid[k] ? v : __builtin_inff();
         }
         float pv = pp[i];
