@@ -44,7 +44,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="stereo_fb_384x1248_d192",
-                    choices=["stereo_fb_384x1248_d192", "stereo_fwd_128x256_d64"])
+                    choices=["stereo_fb_384x1248_d192", "stereo_fwd_128x256_d64", "semantic_fb_2x512x1024_c19",
+                             "flow_fb_256x512_r12", "serving_c4"])
+    ap.add_argument("--requests", type=int, default=96, help="serving_c4: requests per measurement")
     ap.add_argument("--rows", type=int, default=0, help="crop the image to this many rows (debug)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -169,6 +171,226 @@ def cpu_baseline(wl, plan_shape, seconds, iters_per_sample=1):
                       f"{rows}x{W} crop with {C} labels ({pl / 1e6:.1f} M pl, {t:.1f} s)"}
 
 
+class LBPPlan:
+    """Buffers of an LBP step for any pairwise variant: ``iters`` forward iterations from the
+    given messages, then (``backward``) their exact backward from a fixed synthetic gradient."""
+
+    def __init__(self, kind, cost, jump, K, Nj, edge, delta, iters, backward=True, msgs0=None):
+        import torch
+        from bplayers_amd import _native as nat
+        self.nat, self.sp = nat, nat.ptr
+        self.kind, self.cost, self.jump, self.K, self.Nj, self.edge = kind, cost, jump, K, Nj, edge
+        self.delta, self.iters, self.backward = delta, iters, backward
+        N, H, W, C = cost.shape
+        self.N, self.H, self.W, self.C = N, H, W, C
+        dev = cost.device
+        m0 = msgs0 if msgs0 is not None else torch.zeros((N, 4, H, W, C), device=dev)
+        self.msgs = [m0] + [torch.empty((N, 4, H, W, C), device=dev) for _ in range(iters)]
+        self.argmin = [torch.empty((N, 4, H, W, C), dtype=torch.uint8, device=dev) for _ in range(iters)]
+        self.garg = [torch.empty((N, 4, H, W), dtype=torch.uint8, device=dev) for _ in range(iters)]
+        self.scale = [torch.empty((N, 4, H, W), device=dev) for _ in range(iters)]
+        if backward:
+            g = torch.Generator(device="cpu").manual_seed(5)
+            self.G = torch.randn((N, 4, H, W, C), generator=g).to(dev)
+            self.gmsg = [torch.empty((N, 4, H, W, C), device=dev) for _ in range(2)]
+            self.gcost = torch.empty((N, H, W, C), device=dev)
+            self.gjump = torch.empty_like(jump)
+            self.gedge = torch.empty((N, 4, H, W), device=dev)
+            self.ws_bytes = int(nat.lib().bpl_lbp_bwd_workspace_size(kind, N, H, W, C))
+            self.ws = torch.empty(max(self.ws_bytes, 1), dtype=torch.uint8, device=dev)
+
+    def pl(self):
+        return self.N * self.H * self.W * self.C
+
+    def fwd(self, t):
+        nat, p = self.nat, self.sp
+        nat.call("bpl_lbp_fwd", self.kind, p(self.cost), p(self.jump), self.K, self.Nj, p(self.edge),
+                 p(self.msgs[t]), p(self.msgs[t + 1]), p(self.argmin[t]), p(self.garg[t]), p(self.scale[t]),
+                 self.N, self.H, self.W, self.C, self.delta, nat.stream_of(self.cost))
+
+    def bwd(self, t, g_in, g_out):
+        nat, p = self.nat, self.sp
+        nat.call("bpl_lbp_bwd", self.kind, nat.GRAD_EXACT, p(self.jump), self.K, self.Nj, p(self.edge), p(g_in),
+                 p(self.msgs[t + 1]), p(self.argmin[t]), p(self.garg[t]), p(self.gcost), p(self.gjump),
+                 p(self.gedge), p(g_out), p(self.ws), self.ws_bytes, self.N, self.H, self.W, self.C, self.delta,
+                 nat.stream_of(self.cost))
+
+    def step(self):
+        for t in range(self.iters):
+            self.fwd(t)
+        if self.backward:
+            g_in = self.G
+            for t in reversed(range(self.iters)):
+                g_out = self.gmsg[t % 2]
+                self.bwd(t, g_in, g_out)
+                g_in = g_out
+
+
+def timed_steps(steps, warmup, fn, nat, torch):
+    """Run ``fn`` warmup + steps times; returns (ms per step, wall s, {kernel: (ms, launches)})."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    nat.timing_reset()
+    nat.timing_enable(True)
+    t0 = time.perf_counter()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(steps):
+        fn()
+    end.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    nat.timing_enable(False)
+    return start.elapsed_time(end) / steps, wall, nat.timing_read()
+
+
+def common_line(args, metric, value, unit, ms_step, config, dtype="f32", higher=True):
+    return {"metric": metric, "value": round(value, 4), "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": higher, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config}
+
+
+def bench_semantic(args, torch):
+    """semantic_fb_2x512x1024_c19: N=2, 512x1024, 19 labels, global C x C pairwise (pw_mat used for
+    both directions' pairs, RIGHT/DOWN transposed), per-pixel edge weights, delta = C (full
+    min-plus), 2 iterations forward + exact backward per step."""
+    from bplayers_amd import workloads, _native as nat
+    from bplayers_amd.ops.lbp_semantic_pw.message_passing_op_cuda import pack_pairwise
+    wl = workloads.semantic()
+    dev = torch.device("cuda:0")
+    cost = (-torch.from_numpy(wl["prob"])).to(dev).contiguous()
+    jump = pack_pairwise(torch.from_numpy(wl["pw_mat"]).to(dev), False)
+    edge = torch.from_numpy(wl["edge"]).to(dev)
+    plan = LBPPlan(nat.LBP_SEMANTIC_GLOBAL, cost, jump, 0, 1, edge, wl["delta"], wl["iters"])
+    ms, wall, kt = timed_steps(args.steps, args.warmup, plan.step, nat, torch)
+    value = plan.pl() * plan.iters / (ms * 1e-3) / 1e9
+    avg = {k: v[0] / v[1] for k, v in kt.items()}
+    it_ms = sum(avg[k] for k in LBP_CLASSES if k in avg) + avg.get("lbp_bwd_reduce", 0.0)
+    C = plan.C
+    it_bytes = plan.pl() * (FWD_B_PL + BWD_B_PL + (16.0 + 32.0) / C)  # edge weights per pixel
+    line = common_line(args, "G(pixel*label)/s per LBP iter fwd+bwd", value, "G(pixel*label)/s", ms,
+                       {"workload": args.workload, "N": plan.N, "H": plan.H, "W": plan.W, "labels": C,
+                        "iters": plan.iters, "delta": plan.delta, "pairwise": "global CxC matrix per direction",
+                        "grad_mode": "exact"})
+    line["roofline_iteration"] = {"bound": "hbm", "kernels": [k for k in avg if k.startswith("lbp")],
+                                  "achieved": round(it_bytes / (it_ms * 1e-3) / 1e9, 2), "peak": HBM_PEAK_GBS,
+                                  "unit": "GB/s", "frac": round(it_bytes / (it_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                  "alg_bytes_per_pl": round(it_bytes / plan.pl(), 3), "iteration_ms": round(it_ms, 4)}
+    line["timings_ms"] = {f"{k}_ms": round(v, 4) for k, v in avg.items()}
+    line["wall_s"] = round(wall, 3)
+    print(json.dumps(line), flush=True)
+
+
+def bench_flow(args, torch):
+    """flow_fb_256x512_r12: flow SAD over a 25x25 window (sws=24) with min-projections to u and
+    v volumes [1,256,512,25], cost = -softmax(-SAD), then 2 LBP iterations on u and 2 on v
+    (v starting from u's messages, as the reference's 5-D BP path), forward + exact backward,
+    per-pixel-constant P1=0.1 / P2=0.5."""
+    from bplayers_amd import workloads, _native as nat
+    from bplayers_amd.ops.lbp_stereo.message_passing_op_cuda import pack_affinities
+    wl = workloads.flow()
+    dev = torch.device("cuda:0")
+    f0 = torch.from_numpy(wl["f0"]).to(dev)
+    f1 = torch.from_numpy(wl["f1"]).to(dev)
+    N, Cf, H, W = f0.shape
+    K = wl["sws"] + 1
+    cu, cv, us, vs = (torch.empty((N, H, W, K), device=dev) for _ in range(4))
+    st = nat.stream_of(f0)
+    sad_args = (nat.ptr(f0), nat.ptr(f1), nat.ptr(cu), nat.ptr(cv), nat.ptr(us), nat.ptr(vs), N, Cf, H, W, wl["sws"],
+                0, 0, 0, 0, st)
+    nat.call("bpl_sad_flow_fwd", *sad_args)
+    cost_u = (-torch.softmax(-cu, dim=3)).contiguous()
+    cost_v = (-torch.softmax(-cv, dim=3)).contiguous()
+    aff = torch.zeros((1, 4, 3, H, W), device=dev)
+    aff[:, :, :2] = wl["P1"]
+    aff[:, :, 2] = wl["P2"]
+    jump = pack_affinities(aff, torch.zeros((1, 4, H, W), device=dev))
+    edge = torch.ones((N, 4, H, W), device=dev)
+    pu = LBPPlan(nat.LBP_STEREO, cost_u, jump, jump.shape[2], 1, edge, wl["delta"], wl["iters"])
+    pv = LBPPlan(nat.LBP_STEREO, cost_v, jump, jump.shape[2], 1, edge, wl["delta"], wl["iters"],
+                 msgs0=pu.msgs[-1])
+
+    def step():
+        for t in range(pu.iters):
+            pu.fwd(t)
+        for t in range(pv.iters):
+            pv.fwd(t)
+        for plan in (pv, pu):
+            g_in = plan.G
+            for t in reversed(range(plan.iters)):
+                g_out = plan.gmsg[t % 2]
+                plan.bwd(t, g_in, g_out)
+                g_in = g_out
+
+    ms, wall, kt = timed_steps(args.steps, args.warmup, step, nat, torch)
+    pl_step = (pu.pl() + pv.pl()) * wl["iters"]
+    avg = {k: v[0] / v[1] for k, v in kt.items()}
+    # flow SAD forward and backward, timed the same way
+    df0, df1 = torch.empty_like(f0), torch.empty_like(f1)
+    gu, gv = torch.randn_like(cu), torch.randn_like(cv)
+
+    def sad_step():
+        nat.call("bpl_sad_flow_fwd", *sad_args)
+        nat.call("bpl_sad_flow_bwd", nat.ptr(f0), nat.ptr(f1), nat.ptr(gu), nat.ptr(gv), nat.ptr(us), nat.ptr(vs),
+                 nat.ptr(df0), nat.ptr(df1), N, Cf, H, W, wl["sws"], st)
+
+    _, _, kt2 = timed_steps(5, 1, sad_step, nat, torch)
+    for k, v in kt2.items():
+        avg[k] = v[0] / v[1]
+    line = common_line(args, "G(pixel*label)/s per LBP iter fwd+bwd", pl_step / (ms * 1e-3) / 1e9,
+                       "G(pixel*label)/s", ms,
+                       {"workload": args.workload, "N": N, "H": H, "W": W, "labels": K, "iters": wl["iters"],
+                        "volumes": "u and v", "delta": wl["delta"], "pairwise": "global P1=0.1 P2=0.5 (K=6 jump)",
+                        "grad_mode": "exact"})
+    line["timings_ms"] = {f"{k}_ms": round(v, 4) for k, v in avg.items()}
+    line["sad"] = {"flow_fwd_G_pixel_window_per_s": round(N * H * W * K * K / (avg["sad_flow_fwd"] * 1e-3) / 1e9, 3),
+                   "window": f"{K}x{K}", "feature_channels": Cf}
+    line["wall_s"] = round(wall, 3)
+    print(json.dumps(line), flush=True)
+
+
+def bench_serving(args, torch):
+    """serving_c4: stereo requests (C=64 features) of four shape classes {256x512 D64,
+    384x1248 D192, 544x960 D128, 768x1024 D128}, each solved forward-only (SAD -> softmax ->
+    1 LBP iteration -> beliefs / WTA disparity). Capacity = frames/s with every request queued
+    at t=0 (continuous batching vs one request at a time, same kernels); latency p50/p99 at a
+    Poisson offered load of 70% of the sequential capacity."""
+    from bplayers_amd import workloads
+    from bplayers_amd.serving import StereoServer, make_requests
+    dev = torch.device("cuda:0")
+    pool = {}
+    for i, shp in enumerate(workloads.SERVING_SHAPES):
+        r = workloads.serving_request(40 + i, i)
+        pool[shp] = [(torch.from_numpy(r["f0"]).to(dev), torch.from_numpy(r["f1"]).to(dev))]
+    n = args.requests
+    budget = 96 << 30
+
+    def run(sequential, rate, seed):
+        srv = StereoServer(dev, C=64, hbm_budget=budget, sequential=sequential, max_batch=4, max_slots_per_class=2)
+        srv.warm(sorted(pool))
+        srv.serve(make_requests(pool, 8, rate_hz=1e9, seed=99))  # warm-up (not measured)
+        return srv.serve(make_requests(pool, n, rate_hz=rate, seed=seed))
+
+    seq = run(True, 1e9, 1)
+    mix = run(False, 1e9, 1)
+    rate = 0.7 * seq["frames_per_s"]
+    seq_l = run(True, rate, 2)
+    mix_l = run(False, rate, 2)
+    line = common_line(args, "frames/s (stereo serving, 4 shape classes)", mix["frames_per_s"], "frames/s",
+                       1e3 / mix["frames_per_s"],
+                       {"workload": args.workload, "shapes": [list(s) for s in workloads.SERVING_SHAPES],
+                        "feature_channels": 64, "iters": 1, "requests": n, "max_batch": 4, "slots_per_class": 2})
+    line["sequential_frames_per_s"] = round(seq["frames_per_s"], 3)
+    line["speedup_vs_sequential"] = round(mix["frames_per_s"] / seq["frames_per_s"], 3)
+    line["latency_ms_at_offered_load"] = {"offered_frames_per_s": round(rate, 3),
+                                          "continuous": {"p50": round(mix_l["p50_ms"], 2), "p99": round(mix_l["p99_ms"], 2)},
+                                          "sequential": {"p50": round(seq_l["p50_ms"], 2), "p99": round(seq_l["p99_ms"], 2)}}
+    line["resident_bytes"] = mix["resident_bytes"]
+    print(json.dumps(line), flush=True)
+
+
 def measured_traffic(workload, kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
     (FETCH_SIZE + WRITE_SIZE, calibrated with a copy kernel), if it covers this workload."""
@@ -210,6 +432,11 @@ def main():
     import torch
     from bplayers_amd import workloads
 
+    others = {"semantic_fb_2x512x1024_c19": bench_semantic, "flow_fb_256x512_r12": bench_flow,
+              "serving_c4": bench_serving}
+    if args.workload in others:
+        others[args.workload](args, torch)
+        return
     if args.workload == "stereo_fb_384x1248_d192":
         wl = workloads.stereo_full()
         backward = True

@@ -209,82 +209,93 @@ __global__ __launch_bounds__(256) void sad_stereo_bwd_k(const float *__restrict_
 // the f1 rows y+ov+v-h of the tile (plus the horizontal halo) are staged in LDS channel chunk
 // by chunk; S(u,v) for all u is accumulated in registers, then cv_v[v] (min over u) is
 // written and the running cv_u (min over v) updated -- both with first-index ties.
-constexpr int FL_TY = 8, FL_TX = 64, FL_CB = 8, FL_KMAX = 25;
+constexpr int FL_TY = 4, FL_TX = 32, FL_CB = 8, FL_G = 4, FL_KMAX = 25;
 
+// Block: 4 rows x 32 pixels x 4 groups of vertical displacements (512 threads).  Group g owns
+// the displacements vi = g, g+4, ...; the 4 groups advance together, so one staging of the f1
+// rows for 4 consecutive displacements (and 8 channels) serves all of them.  For each vi a thread
+// forms S(u, vi) for all K horizontal u in registers, writes the v-projection (min over u,
+// first index) and keeps a running u-projection (min over its vi); the groups' partial
+// u-projections are merged through LDS (smaller vi wins ties: the reference's first index).
 template <int K>
-__global__ __launch_bounds__(256) void sad_flow_fwd_k(const float *__restrict__ f0, const float *__restrict__ f1,
+__global__ __launch_bounds__(512) void sad_flow_fwd_k(const float *__restrict__ f0, const float *__restrict__ f1,
                                                       float *__restrict__ cv_u, float *__restrict__ cv_v,
                                                       float *__restrict__ u_star, float *__restrict__ v_star,
                                                       int C, int H, int W, int sws, int ou, int ov, int bu, int bv) {
-  constexpr int SW = FL_TX + K - 1;
-  __shared__ float s1[FL_CB][FL_TY][SW];
-  __shared__ float s0[FL_CB][FL_TY][FL_TX];
+  constexpr int G = FL_G;
+  constexpr int SW = FL_TX + K - 1;  // staged f1 columns
+  constexpr int SR = FL_TY + G - 1;  // staged f1 rows (4 consecutive displacements)
+  constexpr int NI = (K + G - 1) / G;
+  constexpr int NP = FL_TY * FL_TX;
+  // staging tiles, and after the main loop the partial u-projections of groups 1..G-1
+  constexpr size_t kStage = sizeof(float) * (FL_CB * SR * SW + FL_CB * FL_TY * FL_TX);
+  constexpr size_t kMerge = (sizeof(float) + 1) * (G - 1) * NP * K;
+  __shared__ __align__(16) unsigned char lds[kStage > kMerge ? kStage : kMerge];
+  auto s1 = reinterpret_cast<float (*)[SR][SW]>(lds);
+  auto s0 = reinterpret_cast<float (*)[FL_TY][FL_TX]>(lds + sizeof(float) * FL_CB * SR * SW);
+  auto m_v = reinterpret_cast<float (*)[NP][K]>(lds);
+  auto m_i = reinterpret_cast<unsigned char (*)[NP][K]>(lds + sizeof(float) * (G - 1) * NP * K);
   const int hs = sws / 2;
   const int x0 = blockIdx.x * FL_TX;
   const int y0 = blockIdx.y * FL_TY;
   const int n = blockIdx.z;
   const int t = threadIdx.x;
-  const int ty = t / 32, tx = (t % 32) * 2;
-  const int y = y0 + ty;
+  const int g = t / (FL_TY * FL_TX);
+  const int pl = t - g * (FL_TY * FL_TX);
+  const int ty = pl / FL_TX, tx = pl - ty * FL_TX;
+  const int y = y0 + ty, x = x0 + tx;
   const int64_t HW = (int64_t)H * W;
   const float *F0 = f0 + (int64_t)n * C * HW;
   const float *F1 = f1 + (int64_t)n * C * HW;
-  float bu_v[2][K];
-  int bu_i[2][K];
+  float bmin[K];
+  int barg[K];
 #pragma unroll
-  for (int p = 0; p < 2; ++p)
-#pragma unroll
-    for (int u = 0; u < K; ++u) { bu_v[p][u] = 9999999.f; bu_i[p][u] = 0; }
+  for (int u = 0; u < K; ++u) { bmin[u] = 9999999.f; barg[u] = 0; }
 
-  for (int vi = 0; vi < K; ++vi) {
-    const int dv = ov + vi - hs;
-    float S[2][K];
+  for (int i = 0; i < NI; ++i) {
+    const int vi = g + G * i;
+    const int dv0 = ov + G * i - hs;  // displacement of group 0 at this round
+    float S[K];
 #pragma unroll
-    for (int p = 0; p < 2; ++p)
-#pragma unroll
-      for (int u = 0; u < K; ++u) S[p][u] = 0.f;
+    for (int u = 0; u < K; ++u) S[u] = 0.f;
     for (int c0 = 0; c0 < C; c0 += FL_CB) {
       __syncthreads();
-      for (int i = t; i < FL_CB * FL_TY * SW; i += 256) {
-        const int cc = i / (FL_TY * SW);
-        const int r = i - cc * FL_TY * SW;
+      for (int e = t; e < FL_CB * SR * SW; e += G * FL_TY * FL_TX) {
+        const int cc = e / (SR * SW);
+        const int r = e - cc * SR * SW;
         const int ry = r / SW, j = r - ry * SW;
-        const int c = c0 + cc, yy = y0 + ry + dv, xx = x0 + ou - hs + j;
+        const int c = c0 + cc, yy = y0 + ry + dv0, xx = x0 + ou - hs + j;
         s1[cc][ry][j] = (c < C && yy >= 0 && yy < H && xx >= 0 && xx < W) ? F1[(int64_t)c * HW + (int64_t)yy * W + xx] : 0.f;
       }
-      for (int i = t; i < FL_CB * FL_TY * FL_TX; i += 256) {
-        const int cc = i / (FL_TY * FL_TX);
-        const int r = i - cc * FL_TY * FL_TX;
+      for (int e = t; e < FL_CB * FL_TY * FL_TX; e += G * FL_TY * FL_TX) {
+        const int cc = e / (FL_TY * FL_TX);
+        const int r = e - cc * FL_TY * FL_TX;
         const int ry = r / FL_TX, j = r - ry * FL_TX;
         const int c = c0 + cc, yy = y0 + ry, xx = x0 + j;
         s0[cc][ry][j] = (c < C && yy < H && xx < W) ? F0[(int64_t)c * HW + (int64_t)yy * W + xx] : 0.f;
       }
       __syncthreads();
-      const int nc = min(FL_CB, C - c0);
-      for (int cc = 0; cc < nc; ++cc) {
-        const float a0 = s0[cc][ty][tx], a1 = s0[cc][ty][tx + 1];
-        float prev = s1[cc][ty][tx];
+      if (vi < K) {
+        const int nc = min(FL_CB, C - c0);
+        for (int cc = 0; cc < nc; ++cc) {
+          const float a = s0[cc][ty][tx];
+          const float *row = &s1[cc][ty + g][tx];
 #pragma unroll
-        for (int u = 0; u < K; ++u) {
-          const float nx = s1[cc][ty][tx + u + 1];
-          S[0][u] += fabsf(a0 - prev);
-          S[1][u] += fabsf(a1 - nx);
-          prev = nx;
+          for (int u = 0; u < K; ++u) S[u] += fabsf(a - row[u]);
         }
       }
     }
-    const bool rowok = (y + dv >= 0) && (y + dv < H);
-#pragma unroll
-    for (int p = 0; p < 2; ++p) {
-      const int x = x0 + tx + p;
+    if (vi < K) {
+      const int dv = dv0 + g;
+      const bool rowok = (y + dv >= 0) && (y + dv < H);
       float mv = 9999999.f;
       int mi = 0;
 #pragma unroll
       for (int u = 0; u < K; ++u) {
         const int xx = x + ou + u - hs;
-        const float s = (rowok && xx >= 0 && xx < W) ? S[p][u] : kFill;
-        if (s < mv) { mv = s; mi = u; }
-        if (s < bu_v[p][u]) { bu_v[p][u] = s; bu_i[p][u] = vi; }
+        const float sv = (rowok && xx >= 0 && xx < W) ? S[u] : kFill;
+        if (sv < mv) { mv = sv; mi = u; }
+        if (sv < bmin[u]) { bmin[u] = sv; barg[u] = vi; }
       }
       if (y < H && x < W) {
         const int64_t o = (((int64_t)n * H + y) * W + x) * K + vi;
@@ -293,16 +304,30 @@ __global__ __launch_bounds__(256) void sad_flow_fwd_k(const float *__restrict__ 
       }
     }
   }
+  // merge the groups' u-projections in group 0 (smaller vi wins among equal minima)
+  __syncthreads();  // the staging tiles are dead from here on; their space takes the partials
+  if (g > 0) {
 #pragma unroll
-  for (int p = 0; p < 2; ++p) {
-    const int x = x0 + tx + p;
-    if (y < H && x < W) {
-      const int64_t o = (((int64_t)n * H + y) * W + x) * K;
+    for (int u = 0; u < K; ++u) {
+      m_v[g - 1][pl][u] = bmin[u];
+      m_i[g - 1][pl][u] = (unsigned char)barg[u];
+    }
+  }
+  __syncthreads();
+  if (g == 0 && y < H && x < W) {
+    const int64_t o = (((int64_t)n * H + y) * W + x) * K;
 #pragma unroll
-      for (int u = 0; u < K; ++u) {
-        cv_u[o + u] = bu_v[p][u];
-        u_star[o + u] = (float)(bu_i[p][u] + bv * sws);
+    for (int u = 0; u < K; ++u) {
+      float bv_ = bmin[u];
+      int bi = barg[u];
+#pragma unroll
+      for (int gg = 1; gg < G; ++gg) {
+        const float v2 = m_v[gg - 1][pl][u];
+        const int i2 = m_i[gg - 1][pl][u];
+        if (v2 < bv_ || (v2 == bv_ && i2 < bi)) { bv_ = v2; bi = i2; }
       }
+      cv_u[o + u] = bv_;
+      u_star[o + u] = (float)(bi + bv * sws);
     }
   }
 }
@@ -346,66 +371,122 @@ __global__ void sad_flow_fwd_generic_k(const float *__restrict__ f0, const float
 // (y,x,k) whose argmin lands on it: for the u-part x = x'-u and y = y'-v with
 // u_star[y,x,u] = v+h; for the v-part y = y'-v and x = x'-u with v_star[y,x,v] = u+h.
 // Terms are summed in a fixed order, so the result is deterministic.
-__global__ __launch_bounds__(256) void sad_flow_bwd_k(const float *__restrict__ f0, const float *__restrict__ f1,
-                                                      const float *__restrict__ gu, const float *__restrict__ gv,
-                                                      const float *__restrict__ us, const float *__restrict__ vs,
-                                                      float *__restrict__ df0, float *__restrict__ df1,
-                                                      int N, int C, int H, int W, int sws) {
+// One thread per pixel and chunk of FB_CB channels: the argmin reads and the search for the
+// sources landing on the pixel do not depend on the channel, so they run once per chunk while
+// all FB_CB channel sums advance together.  The sources a lane finds (about K per part on
+// average, at different points of the K x K search for different lanes) are first collected in
+// search order in a per-thread LDS list; the channel updates then run over the lists in
+// lockstep, instead of once per search step in which any lane of the wave lands.  Terms keep
+// the order of the scalar definition (u part then v part; label ascending, inner displacement
+// ascending); a list that overflows FB_CAP is finished by a second, divergent pass over the
+// remaining landings.
+constexpr int FB_CB = 32;
+constexpr int FB_CAP = 48;
+constexpr int FB_THREADS = 128;
+
+__global__ __launch_bounds__(FB_THREADS) void sad_flow_bwd_k(const float *__restrict__ f0, const float *__restrict__ f1,
+                                                             const float *__restrict__ gu, const float *__restrict__ gv,
+                                                             const float *__restrict__ us, const float *__restrict__ vs,
+                                                             float *__restrict__ df0, float *__restrict__ df1,
+                                                             int N, int C, int H, int W, int sws) {
+  __shared__ int l_src[FB_CAP][FB_THREADS];
+  __shared__ float l_g[FB_CAP][FB_THREADS];
   const int K = sws + 1, hs = sws / 2;
   const int64_t HW = (int64_t)H * W;
   const int64_t total = (int64_t)N * HW;
-  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < total; p += (int64_t)gridDim.x * blockDim.x) {
+  const int t = threadIdx.x;
+  // every thread of the block runs the same number of grid-stride rounds (barrier-free body)
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + t; p < total; p += (int64_t)gridDim.x * blockDim.x) {
     const int n = (int)(p / HW);
     const int y = (int)((p / W) % H), x = (int)(p % W);
     const int64_t pix = p - (int64_t)n * HW;
-    const float *F0 = f0 + (int64_t)n * C * HW;
-    const float *F1 = f1 + (int64_t)n * C * HW;
-    float *D0 = df0 + (int64_t)n * C * HW;
-    float *D1 = df1 + (int64_t)n * C * HW;
-    for (int c = 0; c < C; ++c) {
-      const float a = F0[(int64_t)c * HW + pix];
-      const float b = F1[(int64_t)c * HW + pix];
-      float r0 = 0.f, r1 = 0.f;
-      // df0: u-part then v-part (kernel order of flow_mp_sad_kernel.cu:153-196)
-      for (int k = 0; k < K; ++k) {
-        const int64_t o = p * K + k;
-        {
-          const int u = k - hs, v = (int)us[o] - hs;
+    const int64_t img = (int64_t)n * HW;
+    for (int c0 = 0; c0 < C; c0 += FB_CB) {
+      const int nc = min(FB_CB, C - c0);
+      const float *F0 = f0 + ((int64_t)n * C + c0) * HW;
+      const float *F1 = f1 + ((int64_t)n * C + c0) * HW;
+      float a[FB_CB], b[FB_CB], r0[FB_CB], r1[FB_CB];
+#pragma unroll
+      for (int j = 0; j < FB_CB; ++j) {
+        const bool ok = j < nc;
+        a[j] = ok ? F0[(int64_t)j * HW + pix] : 0.f;
+        b[j] = ok ? F1[(int64_t)j * HW + pix] : 0.f;
+        r0[j] = 0.f;
+        r1[j] = 0.f;
+      }
+      // df0: along this pixel's own argmins
+      for (int part = 0; part < 2; ++part) {
+        const float *st = part ? vs : us;
+        const float *gg = part ? gv : gu;
+        for (int k = 0; k < K; ++k) {
+          const int64_t o = p * K + k;
+          const int m = (int)st[o] - hs;
+          const int u = part ? m : k - hs, v = part ? k - hs : m;
           const int xx = x + u, yy = y + v;
-          if (xx >= 0 && xx < W && yy >= 0 && yy < H) r0 += sgn_eps(a - F1[(int64_t)c * HW + (int64_t)yy * W + xx]) * gu[o];
+          if (xx < 0 || xx >= W || yy < 0 || yy >= H) continue;
+          const float g = gg[o];
+          const int64_t src = (int64_t)yy * W + xx;
+#pragma unroll
+          for (int j = 0; j < FB_CB; ++j)
+            if (j < nc) r0[j] += sgn_eps(a[j] - F1[(int64_t)j * HW + src]) * g;
         }
       }
-      for (int k = 0; k < K; ++k) {
-        const int64_t o = p * K + k;
-        const int v = k - hs, u = (int)vs[o] - hs;
-        const int xx = x + u, yy = y + v;
-        if (xx >= 0 && xx < W && yy >= 0 && yy < H) r0 += sgn_eps(a - F1[(int64_t)c * HW + (int64_t)yy * W + xx]) * gv[o];
-      }
-      // df1: sources landing on (y, x)
-      for (int k = 0; k < K; ++k) {
-        const int u = k - hs;
-        const int xs = x - u;
-        if (xs < 0 || xs >= W) continue;
-        for (int v = -hs; v <= hs; ++v) {
-          const int ys = y - v;
-          if (ys < 0 || ys >= H) continue;
-          const int64_t o = (((int64_t)n * H + ys) * W + xs) * K + k;
-          if ((int)us[o] - hs == v) r1 -= sgn_eps(F0[(int64_t)c * HW + (int64_t)ys * W + xs] - b) * gu[o];
+      // df1: the sources whose argmin lands on this pixel
+      for (int part = 0; part < 2; ++part) {
+        const float *st = part ? vs : us;
+        const float *gg = part ? gv : gu;
+        int cnt = 0;
+        for (int k = 0; k < K; ++k) {
+          for (int e = -hs; e <= hs; ++e) {
+            // u part: label k is the horizontal displacement, e the vertical one; v part: swapped
+            const int u = part ? e : k - hs, v = part ? k - hs : e;
+            const int xs = x - u, ys = y - v;
+            if (xs < 0 || xs >= W || ys < 0 || ys >= H) continue;
+            const int64_t src = (int64_t)ys * W + xs;
+            const int64_t o = (img + src) * K + k;
+            if ((int)st[o] - hs != e) continue;
+            if (cnt < FB_CAP) {
+              l_src[cnt][t] = (int)src;
+              l_g[cnt][t] = gg[o];
+            }
+            ++cnt;
+          }
+        }
+        const int m = min(cnt, FB_CAP);
+        for (int i = 0; i < m; ++i) {
+          const int64_t src = l_src[i][t];
+          const float g = l_g[i][t];
+#pragma unroll
+          for (int j = 0; j < FB_CB; ++j)
+            if (j < nc) r1[j] -= sgn_eps(F0[(int64_t)j * HW + src] - b[j]) * g;
+        }
+        if (cnt > FB_CAP) {  // rare: finish the landings past the list, in search order
+          int seen = 0;
+          for (int k = 0; k < K; ++k) {
+            for (int e = -hs; e <= hs; ++e) {
+              const int u = part ? e : k - hs, v = part ? k - hs : e;
+              const int xs = x - u, ys = y - v;
+              if (xs < 0 || xs >= W || ys < 0 || ys >= H) continue;
+              const int64_t src = (int64_t)ys * W + xs;
+              const int64_t o = (img + src) * K + k;
+              if ((int)st[o] - hs != e) continue;
+              if (seen++ < FB_CAP) continue;
+              const float g = gg[o];
+#pragma unroll
+              for (int j = 0; j < FB_CB; ++j)
+                if (j < nc) r1[j] -= sgn_eps(F0[(int64_t)j * HW + src] - b[j]) * g;
+            }
+          }
         }
       }
-      for (int k = 0; k < K; ++k) {
-        const int v = k - hs;
-        const int ys = y - v;
-        if (ys < 0 || ys >= H) continue;
-        for (int u = -hs; u <= hs; ++u) {
-          const int xs = x - u;
-          if (xs < 0 || xs >= W) continue;
-          const int64_t o = (((int64_t)n * H + ys) * W + xs) * K + k;
-          if ((int)vs[o] - hs == u) r1 -= sgn_eps(F0[(int64_t)c * HW + (int64_t)ys * W + xs] - b) * gv[o];
+      float *D0 = df0 + ((int64_t)n * C + c0) * HW;
+      float *D1 = df1 + ((int64_t)n * C + c0) * HW;
+#pragma unroll
+      for (int j = 0; j < FB_CB; ++j)
+        if (j < nc) {
+          D0[(int64_t)j * HW + pix] = r0[j];
+          D1[(int64_t)j * HW + pix] = r1[j];
         }
-      }
-      D0[(int64_t)c * HW + pix] = r0;
-      D1[(int64_t)c * HW + pix] = r1;
     }
   }
 }
@@ -478,7 +559,7 @@ extern "C" int bpl_sad_flow_fwd(const float *f0, const float *f1, float *cv_u, f
   KernelTimer kt(BPL_KERNEL_SAD_FLOW_FWD, st);
   if (sws == 24) {
     dim3 grid((unsigned)((W + FL_TX - 1) / FL_TX), (unsigned)((H + FL_TY - 1) / FL_TY), (unsigned)N);
-    hipLaunchKernelGGL(sad_flow_fwd_k<25>, grid, dim3(256), 0, st, f0, f1, cv_u, cv_v, u_star, v_star, (int)C,
+    hipLaunchKernelGGL(sad_flow_fwd_k<25>, grid, dim3(FL_G * FL_TY * FL_TX), 0, st, f0, f1, cv_u, cv_v, u_star, v_star, (int)C,
                        (int)H, (int)W, sws, offset_u, offset_v, block_u, block_v);
   } else {
     hipLaunchKernelGGL(sad_flow_fwd_generic_k, dim3(grid_for(N * H * W, 128)), dim3(128), 0, st, f0, f1, cv_u,
@@ -496,7 +577,7 @@ extern "C" int bpl_sad_flow_bwd(const float *f0, const float *f1, const float *g
   BPL_REQUIRE(N >= 1 && C >= 1 && H >= 1 && W >= 1, BPL_E_ARG, "bpl_sad_flow_bwd: sizes must be positive");
   BPL_REQUIRE(sws >= 0 && sws <= 1024, BPL_E_ARG, "bpl_sad_flow_bwd: sws=%d out of range", sws);
   KernelTimer kt(BPL_KERNEL_SAD_FLOW_BWD, as_stream(stream));
-  hipLaunchKernelGGL(sad_flow_bwd_k, dim3(grid_for(N * H * W, 256)), dim3(256), 0, as_stream(stream), f0, f1,
+  hipLaunchKernelGGL(sad_flow_bwd_k, dim3(grid_for(N * H * W, FB_THREADS)), dim3(FB_THREADS), 0, as_stream(stream), f0, f1,
                      grad_u, grad_v, u_star, v_star, df0, df1, (int)N, (int)C, (int)H, (int)W, sws);
   return check_hip(hipGetLastError(), "bpl_sad_flow_bwd launch");
 }
