@@ -13,6 +13,8 @@
 // Routing a receiver-label gradient g(c) to the sender label argmin(c) is a gather over
 // the few label offsets that occur (|argmin(c) - c| <= D) plus one segment sum for the
 // labels whose value came from the truncation term (all routed to the global argmin).
+#include <type_traits>
+
 #include "bpl_common.h"
 
 namespace bpl {
@@ -329,7 +331,7 @@ __device__ __forceinline__ int shl1_i(int v, int fill) {
   return __builtin_amdgcn_update_dpp(fill, v, 0x130, 0xF, 0xF, false);
 }
 
-// Fast step (FAST launches: delta <= 3, no batch accumulation; used for every scanline whose
+// Fast step (FASTK = K in {6, 8}; delta <= 3, no batch accumulation; used for every scanline whose
 // window offsets are all zero, so every routed argmin lies within 3 labels of its receiver).
 // The step has no branches, so the compiler can overlap its tail (pairwise bins, stores) with
 // the dependent chain of the next step.  The normalisation sum ts = sum_c (G + acc) is carried
@@ -339,7 +341,16 @@ __device__ __forceinline__ int shl1_i(int v, int fill) {
 // ts_t = ts_{t-1} + sum_c (G_t + X_{t-1} - ts_{t-1} exp(M_{t-1})) needs only loaded data and
 // the previous ts.  The scattered outputs (bins, edge) are raw buffer stores whose unused
 // lanes carry an out-of-range offset and are dropped by the hardware.
-template <int LPL, bool VERT, bool FULL, bool FAST, int PF>
+// Uniform (SGPR) copy of a wave-uniform 64-bit pointer: a buffer resource built from it is
+// then known to be uniform (otherwise every buffer access becomes a readfirstlane loop).
+__device__ __forceinline__ void *uniform_ptr(const void *p) {
+  const uint64_t v = (uint64_t)(uintptr_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return (void *)(uintptr_t)(((uint64_t)hi << 32) | lo);
+}
+
+template <int LPL, bool VERT, bool FULL, int FASTK, int PF>
 __global__ __launch_bounds__(64) void lbp_bwd_stereo_w64(BwdArgs a) {
   constexpr int ND = (3 + LPL - 1) / LPL;  // neighbour lanes reached by offsets up to 3
   __shared__ float lds_g[2][kWave * LPL];
@@ -455,16 +466,18 @@ __global__ __launch_bounds__(64) void lbp_bwd_stereo_w64(BwdArgs a) {
 #pragma unroll
   for (int k = 0; k < LPL; ++k) acc[k] = 0.f;
 
-  if constexpr (FAST) {
+  if constexpr (FASTK > 0) {
+    constexpr int KK = FASTK;  // jump length of the fast step (K = a.K at launch)
     bool off = false;  // any non-zero window offset on this scanline (jump entry 0)
     const float *o_p = a.jump + ((int64_t)nj * 4 + dir) * K * HW;
     for (int p = lane; p < len; p += kWave) off = off || o_p[VERT ? (int64_t)p * W + ln : (int64_t)ln * W + p] != 0.f;
     if (__ballot(off) == 0) {
-      const __amdgpu_buffer_rsrc_t gj_rs = __builtin_amdgcn_make_buffer_rsrc(gj_p, (short)0, (int)(K * HW * 4), 0x00020000);
-      const __amdgpu_buffer_rsrc_t ge_rs = __builtin_amdgcn_make_buffer_rsrc(ge_p, (short)0, (int)(HW * 4), 0x00020000);
+      const int hw4 = __builtin_amdgcn_readfirstlane((int)(HW * 4));
+      const __amdgpu_buffer_rsrc_t gj_rs = __builtin_amdgcn_make_buffer_rsrc(uniform_ptr(gj_p), (short)0, KK * hw4, 0x00020000);
+      const __amdgpu_buffer_rsrc_t ge_rs = __builtin_amdgcn_make_buffer_rsrc(uniform_ptr(ge_p), (short)0, hw4, 0x00020000);
       const int row16 = lane >> 4;
       const int bin_here = 2 * row16 + (lane & 1);  // bin held by this lane after the reduce-scatter
-      const bool bin_lane = (lane & 15) < 2 && bin_here < K;
+      const bool bin_lane = (lane & 15) < 2 && bin_here < KK;
       constexpr int kDrop = 0x7FFFFFFF;              // out-of-range buffer offset: store dropped
       float tsprev = 0.f, xprev[LPL], smprev[LPL];
 #pragma unroll
@@ -546,7 +559,7 @@ __global__ __launch_bounds__(64) void lbp_bwd_stereo_w64(BwdArgs a) {
         for (int k = 0; k < LPL; ++k) acc[k] = gh[k];
 
         // ---- tail: pairwise bins and edge term, stores
-        const float w = readlane_f(pv, K);
+        const float w = readlane_f(pv, KK);
         float vb[8];
 #pragma unroll
         for (int b = 0; b < 8; ++b) vb[b] = 0.f;
@@ -554,12 +567,12 @@ __global__ __launch_bounds__(64) void lbp_bwd_stereo_w64(BwdArgs a) {
         for (int k = 0; k < LPL; ++k) {
           const bool mk = A[k] == kFromTrunc;
           const int src = mk ? gl : A[k];
-          const int bin = (mk && exact) ? K - 1 : min(max(K / 2 + c0 + k - src, 1), K - 1);
+          const int bin = (mk && exact) ? KK - 1 : min(max(KK / 2 + c0 + k - src, 1), KK - 1);
 #pragma unroll
-          for (int b = 1; b < 8; ++b) vb[b] += bin == b ? gv[k] : 0.f;
+          for (int b = 1; b < KK; ++b) vb[b] += bin == b ? gv[k] : 0.f;
         }
 #pragma unroll
-        for (int b = 1; b < 8; ++b) vb[0] = fmaf(readlane_f(pv, b < K ? b : 0), b < K ? vb[b] : 0.f, vb[0]);
+        for (int b = 1; b < KK; ++b) vb[0] = fmaf(readlane_f(pv, b), vb[b], vb[0]);
         float even, odd;
         reduce_scatter8(vb, even, odd);
         const float tot = (lane & 1) ? odd : even;
@@ -836,17 +849,16 @@ void launch_stereo_w64(const BwdArgs &a, bool vert, int ngroups, hipStream_t st)
   // the fast step needs the window half-width (argmins within 3 labels) and 32-bit buffer
   // offsets into the (direction) planes of the pairwise gradient
   const bool fast = a.delta >= 0 && a.delta <= 3 && !a.accum && (int64_t)a.K * a.H * a.W * 4 < (1LL << 31);
-  if (fast) {
+  auto go = [&](auto fk) {
+    constexpr int FK = decltype(fk)::value;
     if (vert)
-      hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, true, FULL, true, PF>), grid, dim3(kWave), 0, st, a);
+      hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, true, FULL, FK, PF>), grid, dim3(kWave), 0, st, a);
     else
-      hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, true, PF>), grid, dim3(kWave), 0, st, a);
-  } else {
-    if (vert)
-      hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, true, FULL, false, PF>), grid, dim3(kWave), 0, st, a);
-    else
-      hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, false, PF>), grid, dim3(kWave), 0, st, a);
-  }
+      hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, FK, PF>), grid, dim3(kWave), 0, st, a);
+  };
+  if (fast && a.K == 6) go(std::integral_constant<int, 6>());
+  else if (fast && a.K == 8) go(std::integral_constant<int, 8>());
+  else go(std::integral_constant<int, 0>());
 }
 template <int LPL>
 void launch_stereo_w64_any(const BwdArgs &a, bool vert, int ngroups, hipStream_t st) {

@@ -8,8 +8,9 @@
 //     in registers;
 //   * labels live on lanes (SEG lanes per scanline, LPL labels per lane); small label
 //     counts pack 64/SEG scanlines into one wave;
-//   * reductions over labels are DPP / swizzle all-reduces, the candidate window is read
-//     from a per-segment LDS copy of h;
+//   * reductions over labels are DPP row stages folded into the arithmetic plus the gfx950
+//     row/half permlane swaps; the generic kernel reads the candidate window from a
+//     per-segment LDS copy of h, the stereo kernels from neighbouring lanes (DPP shifts);
 //   * the next PF steps' streamed inputs (cost and the two cross messages) are prefetched
 //     into a register ring.
 #include "bpl_common.h"
@@ -606,23 +607,25 @@ __device__ __forceinline__ float shl1_inf(float v) {  // lane i <- lane i+1, lan
   return __int_as_float(__builtin_amdgcn_update_dpp(0x7F800000, __float_as_int(v), 0x130, 0xF, 0xF, false));
 }
 
-// Fast step (FAST launches: delta == 3, argmin, gargmin and scale requested; used for every
-// scanline whose window offsets are all zero).  The step has no branches, so the compiler can
-// overlap the tail of one step (label of the minimum, normalisation, stores) with the
-// dependent chain of the next:
+// Fast step (FASTK = K in {6, 8}; launched for delta == 3 with argmin, gargmin and scale
+// requested, and used for every scanline whose window offsets are all zero).  The step has
+// no branches except rare uniform ones placed after the next pixel's chain value, so the
+// compiler can overlap the tail of one step (label of the minimum, normalisation, stores)
+// with the dependent chain of the next:
 //   * the chain carried to the next pixel is u = msg - gv instead of the normalised message
 //     msg - lse.  Messages are shift-invariant in their input (a constant added to h moves
 //     every candidate and gv by that constant), so this changes no message and no argmin; the
 //     normalised output is u - L with L = log sum exp(u), and the scale exp(lse) of the true
 //     frame is exp(gv + L - L_prev), L_prev being the previous pixel's L (0 at the start);
-//   * the window minimum (last minimal candidate in offset order) is formed before the global
-//     minimum is known and merged with the truncation value gv afterwards: gv precedes the
-//     window in the reference's candidate order, so a window value equal to gv wins;
-//   * the label of the minimum is the largest tie key among the labels attaining it (an
-//     integer max-reduction), which is the reference's tree order for ties and the label
-//     itself when the minimum is unique;
-//   * the normalisation is shifted by the wave maximum of u.
-template <int LPL, bool VERT, bool FULL, bool FAST, int PF>
+//   * the window candidates are formed before the global minimum is known; their minimum is a
+//     min3 tree and the winner the last offset attaining it (the reference's `<=` scan); gv
+//     precedes the window in the reference's candidate order, so a window value equal to gv
+//     wins the merge;
+//   * the label of the minimum comes from ballots (scalar ffs); ties, which are rare, take the
+//     largest tie key (an integer max-reduction), the reference's tree order;
+//   * the log-sum-exp needs no shift: every u lies in [(Fmin - JK) w, 0], so unless
+//     (JK - Fmin) w > 80 no exp(u) underflows; otherwise (rare) it is shifted by max u.
+template <int LPL, bool VERT, bool FULL, int FASTK, int PF>
 __global__ __launch_bounds__(64) void lbp_fwd_stereo_w64(FwdArgs a) {
   constexpr int ND = (3 + LPL - 1) / LPL;  // neighbour lanes reached by a +-3 window
   __shared__ float lds_h[2][kWave * LPL];
@@ -671,7 +674,8 @@ __global__ __launch_bounds__(64) void lbp_fwd_stereo_w64(FwdArgs a) {
     }
   };
 
-  if constexpr (FAST) {
+  if constexpr (FASTK > 0) {
+    constexpr int KK = FASTK;  // jump length of the fast step (K = a.K at launch)
     bool off = false;  // any non-zero window offset on this scanline (jump entry 0)
     const float *o_p = a.jump + ((int64_t)nj * 4 + dir) * K * HW;
     for (int p = lane; p < len; p += kWave) off = off || o_p[VERT ? (int64_t)p * W + ln : (int64_t)ln * W + p] != 0.f;
@@ -716,8 +720,8 @@ __global__ __launch_bounds__(64) void lbp_fwd_stereo_w64(FwdArgs a) {
         }
         float pv = pp[i];
         asm("" : "+v"(pv) : "v"(chain[0]));  // no instruction: orders pv after the chain
-        const float w = readlane_f(pv, K);
-        const float JK = readlane_f(pv, K - 1);
+        const float w = readlane_f(pv, KK);
+        const float JK = readlane_f(pv, KK - 1);
         // global minimum
         int hk = float_key(h[0]);
 #pragma unroll
@@ -731,13 +735,11 @@ __global__ __launch_bounds__(64) void lbp_fwd_stereo_w64(FwdArgs a) {
         for (int d = 1; d < ND; ++d)
 #pragma unroll
           for (int k = 0; k < LPL; ++k) { Lh[d][k] = shr1_inf(Lh[d - 1][k]); Rh[d][k] = shl1_inf(Rh[d - 1][k]); }
-        float mw[LPL];
-        int aw[LPL];
-#pragma unroll
-        for (int k = 0; k < LPL; ++k) { mw[k] = __builtin_inff(); aw[k] = kArgTrunc; }
+        // candidates, their minimum (min3 tree), then the last offset attaining it
+        float cand[7][LPL];
 #pragma unroll
         for (int kk = -3; kk <= 3; ++kk) {
-          const float fk = readlane_f(pv, min(max(K / 2 - kk, 1), K - 1));
+          const float fk = readlane_f(pv, min(max(KK / 2 - kk, 1), KK - 1));
 #pragma unroll
           for (int k = 0; k < LPL; ++k) {
             const int e = k + kk;
@@ -751,29 +753,57 @@ __global__ __launch_bounds__(64) void lbp_fwd_stereo_w64(FwdArgs a) {
               const int j = e - LPL + 1, d = (j + LPL - 1) / LPL, slot = j - 1 - (d - 1) * LPL;
               hv = Rh[d - 1][slot];
             }
-            const float nv = fmaf(fk, w, hv);
-            const bool t = nv <= mw[k];
-            mw[k] = t ? nv : mw[k];
-            aw[k] = t ? kk : aw[k];
+            cand[kk + 3][k] = fmaf(fk, w, hv);
           }
         }
+        float mw[LPL];
+#pragma unroll
+        for (int k = 0; k < LPL; ++k)
+          mw[k] = fminf(fminf(fminf(cand[0][k], cand[1][k]), fminf(cand[2][k], cand[3][k])),
+                        fminf(fminf(cand[4][k], cand[5][k]), cand[6][k]));
         const float gv = fmaf(JK, w, Hs);
+        // msg = min(window, gv); its winner is the last candidate equal to it, or the
+        // truncation term when none is (a window value equal to gv wins, as in the reference)
         int arg[LPL];
 #pragma unroll
         for (int k = 0; k < LPL; ++k) {
-          const bool t = mw[k] <= gv;
-          chain[k] = (t ? mw[k] : gv) - gv;  // u: the next pixel's chain term
-          arg[k] = t ? aw[k] : kArgTrunc;
+          const float msg = fminf(mw[k], gv);
+          chain[k] = msg - gv;  // u: the next pixel's chain term
+          arg[k] = kArgTrunc;
+#pragma unroll
+          for (int kk = 0; kk < 7; ++kk) arg[k] = cand[kk][k] == msg ? kk - 3 : arg[k];
         }
         // ---- tail: label of the minimum, normalisation, stores
-        int tk = -1;
+        // label of the minimum: the single lane attaining it by ballot; ties (rare) take the
+        // largest tie key, the reference's tree order
+        int lstar = 0;
+        {
+          uint64_t mk[LPL];
+          int cnt = 0;
 #pragma unroll
-        for (int k = 0; k < LPL; ++k) tk = max(tk, (valid[k] && h[k] == Hs) ? tie_key(c0 + k, a.bits) : -1);
-        const int lstar = tie_key(wave_max_i(tk), a.bits);
-        int mk = float_key(valid[0] ? chain[0] : -__builtin_inff());
+          for (int k = 0; k < LPL; ++k) {
+            mk[k] = __ballot(valid[k] && h[k] == Hs);
+            cnt += __popcll(mk[k]);
+          }
 #pragma unroll
-        for (int k = 1; k < LPL; ++k) mk = max(mk, float_key(valid[k] ? chain[k] : -__builtin_inff()));
-        const float umax = key_to_float(wave_max_i(mk));
+          for (int k = 0; k < LPL; ++k)
+            if (mk[k]) lstar = (__ffsll((long long)mk[k]) - 1) * LPL + k;
+          if (cnt > 1) {
+            int tk = -1;
+#pragma unroll
+            for (int k = 0; k < LPL; ++k) tk = max(tk, (valid[k] && h[k] == Hs) ? tie_key(c0 + k, a.bits) : -1);
+            lstar = tie_key(wave_max_i(tk), a.bits);
+          }
+        }
+        // shift of the log-sum-exp: 0 in the u frame (gv) whenever every u >= (Fmin - JK) w
+        // is >= -80, i.e. no term can underflow; the wave maximum of u otherwise (rare)
+        float umax = 0.f;
+        if (__ballot(lane >= 1 && lane < KK && (JK - pv) * w > 80.f) != 0) {
+          int mk = float_key(valid[0] ? chain[0] : -__builtin_inff());
+#pragma unroll
+          for (int k = 1; k < LPL; ++k) mk = max(mk, float_key(valid[k] ? chain[k] : -__builtin_inff()));
+          umax = key_to_float(wave_max_i(mk));
+        }
         float se = 0.f;
 #pragma unroll
         for (int k = 0; k < LPL; ++k) se += valid[k] ? fast_exp(chain[k] - umax) : 0.f;
@@ -995,21 +1025,22 @@ __global__ __launch_bounds__(64) void lbp_fwd_stereo_w64(FwdArgs a) {
   }
 }
 
+template <int LPL, bool FULL, int FASTK>
+void launch_stereo_w64_k(const FwdArgs &a, bool vert, hipStream_t st) {
+  const dim3 grid((unsigned)(a.N * 2 * (vert ? a.W : a.H)));
+  if (vert)
+    hipLaunchKernelGGL((lbp_fwd_stereo_w64<LPL, true, FULL, FASTK, 4>), grid, dim3(kWave), 0, st, a);
+  else
+    hipLaunchKernelGGL((lbp_fwd_stereo_w64<LPL, false, FULL, FASTK, 4>), grid, dim3(kWave), 0, st, a);
+}
 template <int LPL, bool FULL>
 void launch_stereo_w64(const FwdArgs &a, bool vert, hipStream_t st) {
-  const dim3 grid((unsigned)(a.N * 2 * (vert ? a.W : a.H)));
+  // fast step: delta 3, every output requested, and the jump length of the stereo packings
+  // (K_in = 3 -> 6, K_in = 5 -> 8)
   const bool fast = a.delta == 3 && a.argmin && a.gargmin && a.scale;
-  if (fast) {
-    if (vert)
-      hipLaunchKernelGGL((lbp_fwd_stereo_w64<LPL, true, FULL, true, 4>), grid, dim3(kWave), 0, st, a);
-    else
-      hipLaunchKernelGGL((lbp_fwd_stereo_w64<LPL, false, FULL, true, 4>), grid, dim3(kWave), 0, st, a);
-  } else {
-    if (vert)
-      hipLaunchKernelGGL((lbp_fwd_stereo_w64<LPL, true, FULL, false, 4>), grid, dim3(kWave), 0, st, a);
-    else
-      hipLaunchKernelGGL((lbp_fwd_stereo_w64<LPL, false, FULL, false, 4>), grid, dim3(kWave), 0, st, a);
-  }
+  if (fast && a.K == 6) launch_stereo_w64_k<LPL, FULL, 6>(a, vert, st);
+  else if (fast && a.K == 8) launch_stereo_w64_k<LPL, FULL, 8>(a, vert, st);
+  else launch_stereo_w64_k<LPL, FULL, 0>(a, vert, st);
 }
 template <int LPL>
 void launch_stereo_w64_any(const FwdArgs &a, bool vert, hipStream_t st) {
