@@ -267,6 +267,52 @@ inline int tie_bits(int C) {
   return b;
 }
 
+// Workgroup barrier that orders LDS accesses only: the fences are restricted to the local
+// address space, so the barrier waits for outstanding LDS operations (lgkmcnt) but not for
+// global loads in flight (a plain __syncthreads would also drain the prefetch ring).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Workgroups of one launch are dealt to the 8 XCDs round-robin (workgroup b runs on XCD
+// b mod 8).  The remap gives each XCD a contiguous range of logical indices, so that
+// neighbouring scanlines, which share cache lines of per-pixel parameters, share one L2.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+  const int q = nb >> 3, r = nb & 7;
+  const int x = b & 7, i = b >> 3;
+  return x * q + min(x, r) + i;
+}
+
+// Uniform (SGPR) copy of a wave-uniform 64-bit pointer: a buffer resource built from it is
+// then known to be uniform (otherwise every buffer access becomes a readfirstlane loop).
+__device__ __forceinline__ void *uniform_ptr(const void *p) {
+  const uint64_t v = (uint64_t)(uintptr_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return (void *)(uintptr_t)(((uint64_t)hi << 32) | lo);
+}
+
+// Raw buffer access: a resource over `bytes` bytes from a wave-uniform base, a per-lane
+// byte offset `voff` and a wave-uniform byte offset `soff` (a scanline's row).  The
+// address is formed by the memory unit, so the kernel carries no per-lane 64-bit address;
+// an access whose offsets reach past `bytes` reads 0 and a store there is dropped
+// (kBufDrop as `voff` drops a lane).
+constexpr uint32_t kBufDrop = 0x7FFFFFFFu;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(uniform_ptr(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, voff, soff, 0);
+}
+__device__ __forceinline__ void buf_st8(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, uint8_t v) {
+  __builtin_amdgcn_raw_buffer_store_b8(v, r, voff, soff, 0);
+}
+
 // exp / log from the hardware base-2 instructions with a change of base.
 __device__ __forceinline__ float fast_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
 __device__ __forceinline__ float fast_log(float x) { return __builtin_amdgcn_logf(x) * 0.6931471805599453f; }

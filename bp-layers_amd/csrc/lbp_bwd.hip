@@ -341,14 +341,6 @@ __device__ __forceinline__ int shl1_i(int v, int fill) {
 // ts_t = ts_{t-1} + sum_c (G_t + X_{t-1} - ts_{t-1} exp(M_{t-1})) needs only loaded data and
 // the previous ts.  The scattered outputs (bins, edge) are raw buffer stores whose unused
 // lanes carry an out-of-range offset and are dropped by the hardware.
-// Uniform (SGPR) copy of a wave-uniform 64-bit pointer: a buffer resource built from it is
-// then known to be uniform (otherwise every buffer access becomes a readfirstlane loop).
-__device__ __forceinline__ void *uniform_ptr(const void *p) {
-  const uint64_t v = (uint64_t)(uintptr_t)p;
-  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-  return (void *)(uintptr_t)(((uint64_t)hi << 32) | lo);
-}
 
 template <int LPL, bool VERT, bool FULL, int FASTK, int PF>
 __global__ __launch_bounds__(64) void lbp_bwd_stereo_w64(BwdArgs a) {

@@ -608,7 +608,8 @@ __device__ __forceinline__ float shl1_inf(float v) {  // lane i <- lane i+1, lan
 }
 
 // Fast step (FASTK = K in {6, 8}; launched for delta == 3 with argmin, gargmin and scale
-// requested, and used for every scanline whose window offsets are all zero).  The step has
+// all requested (OUTS) or none of them, and used for every scanline whose window offsets
+// are all zero).  The step has
 // no branches except rare uniform ones placed after the next pixel's chain value, so the
 // compiler can overlap the tail of one step (label of the minimum, normalisation, stores)
 // with the dependent chain of the next:
@@ -625,7 +626,7 @@ __device__ __forceinline__ float shl1_inf(float v) {  // lane i <- lane i+1, lan
 //     largest tie key (an integer max-reduction), the reference's tree order;
 //   * the log-sum-exp needs no shift: every u lies in [(Fmin - JK) w, 0], so unless
 //     (JK - Fmin) w > 80 no exp(u) underflows; otherwise (rare) it is shifted by max u.
-template <int LPL, bool VERT, bool FULL, int FASTK, int PF>
+template <int LPL, bool VERT, bool FULL, int FASTK, bool OUTS, int PF>
 __global__ __launch_bounds__(64) void lbp_fwd_stereo_w64(FwdArgs a) {
   constexpr int ND = (3 + LPL - 1) / LPL;  // neighbour lanes reached by a +-3 window
   __shared__ float lds_h[2][kWave * LPL];
@@ -686,10 +687,12 @@ __global__ __launch_bounds__(64) void lbp_fwd_stereo_w64(FwdArgs a) {
       for (int k = 0; k < LPL; ++k)
         if (valid[k]) {
           out_p[q0 * C + c0 + k] = chain[k];
-          arg_p[q0 * C + c0 + k] = 0;
+          if (OUTS) arg_p[q0 * C + c0 + k] = 0;
         }
-      a.scale[rs_base + q0] = 0.f;  // every lane stores the same value
-      a.gargmin[rs_base + q0] = 0;
+      if (OUTS) {
+        a.scale[rs_base + q0] = 0.f;  // every lane stores the same value
+        a.gargmin[rs_base + q0] = 0;
+      }
 
       float pp[PF];
       float pc[PF][LPL], q1[PF][LPL], q2[PF][LPL];
@@ -770,14 +773,16 @@ __global__ __launch_bounds__(64) void lbp_fwd_stereo_w64(FwdArgs a) {
           const float msg = fminf(mw[k], gv);
           chain[k] = msg - gv;  // u: the next pixel's chain term
           arg[k] = kArgTrunc;
+          if (OUTS) {
 #pragma unroll
-          for (int kk = 0; kk < 7; ++kk) arg[k] = cand[kk][k] == msg ? kk - 3 : arg[k];
+            for (int kk = 0; kk < 7; ++kk) arg[k] = cand[kk][k] == msg ? kk - 3 : arg[k];
+          }
         }
         // ---- tail: label of the minimum, normalisation, stores
         // label of the minimum: the single lane attaining it by ballot; ties (rare) take the
         // largest tie key, the reference's tree order
         int lstar = 0;
-        {
+        if (OUTS) {
           uint64_t mk[LPL];
           int cnt = 0;
 #pragma unroll
@@ -820,11 +825,13 @@ __global__ __launch_bounds__(64) void lbp_fwd_stereo_w64(FwdArgs a) {
           for (int k = 0; k < LPL; ++k)
             if (valid[k]) out_p[qq * C + c0 + k] = outv[k];
         }
+        if (OUTS) {
 #pragma unroll
-        for (int k = 0; k < LPL; ++k)
-          if (valid[k]) arg_p[qq * C + c0 + k] = (uint8_t)(arg[k] == kArgTrunc ? kFromTrunc : c0 + k + arg[k]);
-        a.scale[rs_base + qq] = fast_exp(gv + Lc - Lprev);  // same value from every lane
-        a.gargmin[rs_base + qq] = (uint8_t)lstar;
+          for (int k = 0; k < LPL; ++k)
+            if (valid[k]) arg_p[qq * C + c0 + k] = (uint8_t)(arg[k] == kArgTrunc ? kFromTrunc : c0 + k + arg[k]);
+          a.scale[rs_base + qq] = fast_exp(gv + Lc - Lprev);  // same value from every lane
+          a.gargmin[rs_base + qq] = (uint8_t)lstar;
+        }
         Lprev = Lc;
       };
       int s0 = 0;
@@ -1025,22 +1032,29 @@ __global__ __launch_bounds__(64) void lbp_fwd_stereo_w64(FwdArgs a) {
   }
 }
 
-template <int LPL, bool FULL, int FASTK>
+template <int LPL, bool FULL, int FASTK, bool OUTS>
 void launch_stereo_w64_k(const FwdArgs &a, bool vert, hipStream_t st) {
   const dim3 grid((unsigned)(a.N * 2 * (vert ? a.W : a.H)));
   if (vert)
-    hipLaunchKernelGGL((lbp_fwd_stereo_w64<LPL, true, FULL, FASTK, 4>), grid, dim3(kWave), 0, st, a);
+    hipLaunchKernelGGL((lbp_fwd_stereo_w64<LPL, true, FULL, FASTK, OUTS, 4>), grid, dim3(kWave), 0, st, a);
   else
-    hipLaunchKernelGGL((lbp_fwd_stereo_w64<LPL, false, FULL, FASTK, 4>), grid, dim3(kWave), 0, st, a);
+    hipLaunchKernelGGL((lbp_fwd_stereo_w64<LPL, false, FULL, FASTK, OUTS, 4>), grid, dim3(kWave), 0, st, a);
 }
 template <int LPL, bool FULL>
 void launch_stereo_w64(const FwdArgs &a, bool vert, hipStream_t st) {
-  // fast step: delta 3, every output requested, and the jump length of the stereo packings
-  // (K_in = 3 -> 6, K_in = 5 -> 8)
-  const bool fast = a.delta == 3 && a.argmin && a.gargmin && a.scale;
-  if (fast && a.K == 6) launch_stereo_w64_k<LPL, FULL, 6>(a, vert, st);
-  else if (fast && a.K == 8) launch_stereo_w64_k<LPL, FULL, 8>(a, vert, st);
-  else launch_stereo_w64_k<LPL, FULL, 0>(a, vert, st);
+  // fast step: delta 3, the jump lengths of the stereo packings (K_in = 3 -> 6, K_in = 5 ->
+  // 8), and argmins, labels and scales all requested (training) or none (inference)
+  const bool all = a.argmin && a.gargmin && a.scale, none = !a.argmin && !a.gargmin && !a.scale;
+  const bool fast = a.delta == 3 && (all || none);
+  if (fast && a.K == 6) {
+    if (all) launch_stereo_w64_k<LPL, FULL, 6, true>(a, vert, st);
+    else launch_stereo_w64_k<LPL, FULL, 6, false>(a, vert, st);
+  } else if (fast && a.K == 8) {
+    if (all) launch_stereo_w64_k<LPL, FULL, 8, true>(a, vert, st);
+    else launch_stereo_w64_k<LPL, FULL, 8, false>(a, vert, st);
+  } else {
+    launch_stereo_w64_k<LPL, FULL, 0, false>(a, vert, st);
+  }
 }
 template <int LPL>
 void launch_stereo_w64_any(const FwdArgs &a, bool vert, hipStream_t st) {

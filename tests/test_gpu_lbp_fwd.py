@@ -121,9 +121,10 @@ def test_lbp_forward_matches_oracle(case, inplace, impl, monkeypatch):
     print(f"  max rel err {e:.3g}, near-tie fraction {tie:.4f}")
 
 
-def test_lbp_forward_two_iterations_long_rows():
+@pytest.mark.parametrize("C", [64, 192])
+def test_lbp_forward_two_iterations_long_rows(C):
     """Chains long enough for fp32 drift to show (W=300)."""
-    cost, jump, edge, M0 = make_case("stereo", 1, 4, 300, 64, 3, seed=5)
+    cost, jump, edge, M0 = make_case("stereo", 1, 4, 300, C, 3, seed=5)
     from bplayers_amd import pytorch_cuda_lbp_op_stereo as st
     dev = torch.device("cuda:0")
     t = lambda a: torch.from_numpy(a).to(dev)
@@ -136,3 +137,24 @@ def test_lbp_forward_two_iterations_long_rows():
     err = np.abs(m.cpu().numpy() - M) / np.maximum(1.0, np.abs(M))
     print(f"max rel err after 2 iterations, W=300: {err.max():.3g}")
     assert err.max() <= MSG_TOL
+
+
+@pytest.mark.parametrize("C", [40, 64, 100, 192])
+@pytest.mark.parametrize("offset", [False, True])
+def test_lbp_forward_messages_only(C, offset):
+    """Forward without argmins and scales (the serving call): the same messages as the full
+    call, bit for bit."""
+    from bplayers_amd import _native as nat
+    from bplayers_amd import pytorch_cuda_lbp_op_stereo as st
+    print(f"messages-only C={C} offset={offset}", flush=True)
+    cost, jump, edge, M0 = make_case("stereo", 1, 7, 37, C, 3, seed=C, offset=offset)
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.from_numpy(a).to(dev)
+    full, _, _ = st.forward_minsum(t(cost), t(jump), t(edge), t(M0), 3, out=torch.empty_like(t(M0)))
+    c_, j_, e_, m_ = t(cost), t(jump), t(edge), t(M0)
+    out = torch.empty_like(m_)
+    N, H, W = 1, 7, 37
+    nat.call("bpl_lbp_fwd", nat.LBP_STEREO, nat.ptr(c_), nat.ptr(j_), j_.shape[2], 1, nat.ptr(e_), nat.ptr(m_),
+             nat.ptr(out), None, None, None, N, H, W, C, 3, nat.stream_of(c_))
+    torch.cuda.synchronize()
+    assert torch.equal(out, full)
