@@ -276,6 +276,14 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Orders the LDS accesses of one wave: writes by its lanes are visible to its later reads
+// (no other wave is waited for).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
 // Workgroups of one launch are dealt to the 8 XCDs round-robin (workgroup b runs on XCD
 // b mod 8).  The remap gives each XCD a contiguous range of logical indices, so that
 // neighbouring scanlines, which share cache lines of per-pixel parameters, share one L2.

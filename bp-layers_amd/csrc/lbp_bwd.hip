@@ -25,6 +25,7 @@ struct BwdArgs {
   const uint8_t *argmin, *gargmin;
   float *grad_jump, *grad_edge;
   float *X[4];        // per-direction sender h-gradients [N,H,W,C], indexed by pass direction
+  float *grad_cost, *grad_msg;  // outputs written by the paired horizontal phase
   float *partial;     // semantic global: per-(wave, segment) C x C partial sums
   int N, H, W, C, K, Nj, mode, n_base, accum, mat_global;
   int delta;  // the forward's window half-width, or -1 when the caller does not know it
@@ -342,23 +343,38 @@ __device__ __forceinline__ int shl1_i(int v, int fill) {
 // the previous ts.  The scattered outputs (bins, edge) are raw buffer stores whose unused
 // lanes carry an out-of-range offset and are dropped by the hardware.
 
-template <int LPL, bool VERT, bool FULL, int FASTK, int PF>
-__global__ __launch_bounds__(64) void lbp_bwd_stereo_w64(BwdArgs a) {
+// PAIR (horizontal phase, exact mode): the adjoints of the RIGHT and LEFT passes of one row
+// run as the two waves of one workgroup.  They sweep the row in opposite directions, so every
+// pixel is visited first by one wave and later by the other; after a barrier at the middle of
+// the sweep (step Tb), the second visitor reads the first visitor's h-gradient and writes the
+// pixel's grad_cost = ((X_D + X_U) + X_R) + X_L and the U and D slots X_R + X_L of its
+// input-message gradient itself (the first visitor writes the L and R slots, 0), instead of a
+// separate combine pass over four planes.  The
+// pixels whose visits both fall before the barrier (the middle of the row) and the row ends
+// are combined after a final barrier; the y-boundary terms of the U and D slots are added by
+// lbp_bwd_pair_rows.
+template <int LPL, bool VERT, bool FULL, int FASTK, bool PAIR, int PF>
+__global__ __launch_bounds__(PAIR ? 2 * kWave : kWave) void lbp_bwd_stereo_w64(BwdArgs a) {
+  static_assert(!(PAIR && VERT), "pairing is for the horizontal phase");
   constexpr int ND = (3 + LPL - 1) / LPL;  // neighbour lanes reached by offsets up to 3
-  __shared__ float lds_g[2][kWave * LPL];
-  __shared__ int lds_a[2][kWave * LPL];
+  constexpr int NWG = PAIR ? 2 : 1;        // waves per workgroup
+  __shared__ float lds_g_all[NWG][2][kWave * LPL];
+  __shared__ int lds_a_all[NWG][2][kWave * LPL];
 
   const int H = a.H, W = a.W, C = a.C, K = a.K;
   const int L = VERT ? W : H;
   const int len = VERT ? H : W;
   const int nsteps = len - 1;
-  const int g_ = blockIdx.x / L;
-  const int ln = blockIdx.x - g_ * L;
+  const int wv = PAIR ? (int)(threadIdx.x >> 6) : 0;  // wave index in the workgroup
+  const int g_ = PAIR ? 2 * (int)(blockIdx.x / L) + wv : (int)(blockIdx.x / L);
+  const int ln = blockIdx.x - (PAIR ? g_ >> 1 : g_) * L;
   const int n = a.n_base + (g_ >> 1);
   const int dir = VERT ? ((g_ & 1) ? UP : DOWN) : ((g_ & 1) ? LEFT : RIGHT);
   const int rslot = dir == RIGHT ? LEFT : dir == LEFT ? RIGHT : dir == DOWN ? UP : DOWN;
   const bool fwd = (dir == RIGHT || dir == DOWN);
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
+  float(&lds_g)[2][kWave * LPL] = lds_g_all[wv];
+  int(&lds_a)[2][kWave * LPL] = lds_a_all[wv];
   const int c0 = lane * LPL;
   const int nj = a.Nj > 1 ? n : 0;
   const bool exact = a.mode == BPL_GRAD_EXACT;
@@ -458,12 +474,21 @@ __global__ __launch_bounds__(64) void lbp_bwd_stereo_w64(BwdArgs a) {
 #pragma unroll
   for (int k = 0; k < LPL; ++k) acc[k] = 0.f;
 
+  bool fast_done = false;           // PAIR: the fast sweep handled the row
+  int fix_lo = 0, fix_hi = W - 1;   // PAIR: pixels combined after the sweep (plus the row ends)
   if constexpr (FASTK > 0) {
     constexpr int KK = FASTK;  // jump length of the fast step (K = a.K at launch)
     bool off = false;  // any non-zero window offset on this scanline (jump entry 0)
     const float *o_p = a.jump + ((int64_t)nj * 4 + dir) * K * HW;
     for (int p = lane; p < len; p += kWave) off = off || o_p[VERT ? (int64_t)p * W + ln : (int64_t)ln * W + p] != 0.f;
-    if (__ballot(off) == 0) {
+    bool fast = __ballot(off) == 0;
+    if constexpr (PAIR) {  // both passes of the row must take the fast step
+      __shared__ int pair_fast[2];
+      if (lane == 0) pair_fast[wv] = fast ? 1 : 0;
+      __syncthreads();
+      fast = pair_fast[0] != 0 && pair_fast[1] != 0;
+    }
+    if (fast) {
       const int hw4 = __builtin_amdgcn_readfirstlane((int)(HW * 4));
       const __amdgpu_buffer_rsrc_t gj_rs = __builtin_amdgcn_make_buffer_rsrc(uniform_ptr(gj_p), (short)0, KK * hw4, 0x00020000);
       const __amdgpu_buffer_rsrc_t ge_rs = __builtin_amdgcn_make_buffer_rsrc(uniform_ptr(ge_p), (short)0, hw4, 0x00020000);
@@ -474,7 +499,23 @@ __global__ __launch_bounds__(64) void lbp_bwd_stereo_w64(BwdArgs a) {
       float tsprev = 0.f, xprev[LPL], smprev[LPL];
 #pragma unroll
       for (int k = 0; k < LPL; ++k) { xprev[k] = 0.f; smprev[k] = 0.f; }
-      auto step = [&](int i, int t) {
+      // PAIR: the other pass's h-gradient at the sender, read by the second visitor
+      const float *Xt_p = a.X[dir == RIGHT ? LEFT : RIGHT] + img;
+      const bool isR = dir == RIGHT;
+      float xo[PAIR ? PF : 1][LPL];
+      auto st_row = [&](float *row, const float *v) {
+        if (FULL) {
+          float *p = row + c0;
+#pragma unroll
+          for (int k = 0; k < LPL; ++k) p[k] = v[k];
+        } else {
+#pragma unroll
+          for (int k = 0; k < LPL; ++k)
+            if (valid[k]) row[c0 + k] = v[k];
+        }
+      };
+      auto step = [&](int i, int t, auto second_tag) {
+        constexpr bool SECOND = decltype(second_tag)::value;
         const int s = nsteps - 1 - t;
         const int64_t q = q0 + (int64_t)s * dq;
         int A[LPL];
@@ -572,188 +613,326 @@ __global__ __launch_bounds__(64) void lbp_bwd_stereo_w64(BwdArgs a) {
         __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(bin_here == 0 ? 0.f : w * tot), gj_rs,
                                               bin_lane ? (bin_here * (int)HW + qi) * 4 : kDrop, 0, 0);
         __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(tot), ge_rs, lane == 0 ? qi * 4 : kDrop, 0, 0);
-        if (FULL) {
-          float *p = Xo + q * C + c0;
+        const int64_t HWC = HW * C;
+        float *gm = a.grad_msg + (int64_t)n * 4 * HWC + q * C;
+        if constexpr (SECOND) {
+          // second visitor: the pixel's grad_cost and the U and D slots of its input-message
+          // gradient, X_R + X_L (X_D and X_U at the sender are the next step's receiver terms,
+          // already in its ring slot)
+          const int ni = (i + 1) % PF;
+          float cst[LPL], hr[LPL];
 #pragma unroll
-          for (int k = 0; k < LPL; ++k) p[k] = gh[k];
+          for (int k = 0; k < LPL; ++k) {
+            const float xr = isR ? gh[k] : xo[i][k];
+            const float xl = isR ? xo[i][k] : gh[k];
+            cst[k] = ((pd[ni][k] + pu[ni][k]) + xr) + xl;
+            hr[k] = xr + xl;
+          }
+          st_row(a.grad_cost + img + q * C, cst);
+          st_row(gm + UP * HWC, hr);
+          st_row(gm + DOWN * HWC, hr);
         } else {
+          st_row(Xo + q * C, gh);
+          if constexpr (PAIR) {
+            // first visitor: the L and R slots, 0 away from the row ends (which are combined
+            // after the sweep); written here so that both halves store three rows per step
+            float zero[LPL];
 #pragma unroll
-          for (int k = 0; k < LPL; ++k)
-            if (valid[k]) Xo[q * C + c0 + k] = gh[k];
+            for (int k = 0; k < LPL; ++k) zero[k] = 0.f;
+            st_row(gm + LEFT * HWC, zero);
+            st_row(gm + RIGHT * HWC, zero);
+          }
         }
         tsprev = ts;
 #pragma unroll
         for (int k = 0; k < LPL; ++k) { xprev[k] = X[k]; smprev[k] = sm[k]; }
       };
-      int t0 = 0;
-      for (; t0 + PF <= nsteps; t0 += PF) {
+      auto load_xo = [&](int s, float *d) { ld_row(Xt_p + (q0 + (int64_t)s * dq) * C, d); };
+      // a group of PF steps; each ring slot is refilled after its use (with the other pass's
+      // h-gradient too in the second half)
+      auto group = [&](int t0, auto second_tag) {
+        constexpr bool SECOND = decltype(second_tag)::value;
 #pragma unroll
         for (int i = 0; i < PF; ++i) {
-          step(i, t0 + i);
+          step(i, t0 + i, second_tag);
           const int s = nsteps - 1 - (t0 + i);
-          load_step(s - PF >= 0 ? s - PF : 0, pg[i], pm[i], pa[i], pd[i], pu[i], pp[i], pl[i]);
+          const int sn = s - PF >= 0 ? s - PF : 0;
+          load_step(sn, pg[i], pm[i], pa[i], pd[i], pu[i], pp[i], pl[i]);
+          if constexpr (SECOND) load_xo(sn, xo[i]);
+        }
+      };
+      int t0 = 0;
+      bool swept = false;
+      if constexpr (PAIR) {
+        // steps t < Tb (a multiple of PF, at least the middle of the row) write the pass's
+        // h-gradient; steps Tb <= t < nsteps - 1 are second visits and combine.  The last
+        // step (a row end) and the pixels of [W - 1 - Tb, Tb] are combined after the sweep.
+        const int nl = nsteps - 1;
+        const int Tb = PF * ((((W - 1) / 2) + PF - 1) / PF);
+        if (Tb < nl) {
+          for (; t0 < Tb; t0 += PF) group(t0, std::false_type());
+          __syncthreads();  // the first half's h-gradients of both passes are written
+#pragma unroll
+          for (int i = 0; i < PF; ++i) {
+            const int s = nsteps - 1 - (Tb + i);
+            load_xo(s >= 0 ? s : 0, xo[i]);
+          }
+          for (; t0 + PF <= nl; t0 += PF) group(t0, std::true_type());
+#pragma unroll
+          for (int i = 0; i < PF; ++i) {
+            if (t0 + i < nl) step(i, t0 + i, std::true_type());
+            else if (t0 + i == nl) step(i, t0 + i, std::false_type());
+          }
+          fix_lo = W - 1 - Tb;
+          fix_hi = Tb;
+          swept = true;
         }
       }
+      if (!swept) {
+        for (; t0 + PF <= nsteps; t0 += PF) group(t0, std::false_type());
 #pragma unroll
-      for (int i = 0; i < PF; ++i)
-        if (t0 + i < nsteps) step(i, t0 + i);
-      return;
+        for (int i = 0; i < PF; ++i)
+          if (t0 + i < nsteps) step(i, t0 + i, std::false_type());
+      }
+      if constexpr (!PAIR) return;
+      fast_done = true;
     }
   }
 
-  for (int t0 = 0; t0 < nsteps; t0 += PF) {
-#pragma unroll
-    for (int i = 0; i < PF; ++i) {
-      const int t = t0 + i;
-      const int s = nsteps - 1 - t;  // adjoint order
-      if (t < nsteps) {
-        const int64_t q = q0 + (int64_t)s * dq;
-        int A[LPL];
-#pragma unroll
-        for (int k = 0; k < LPL; ++k) A[k] = arg_of(pa[i], k);
-        const float pv = pp[i];
-        const int gl = pl[i];
+  if (!fast_done) {
+    for (int t0 = 0; t0 < nsteps; t0 += PF) {
+  #pragma unroll
+      for (int i = 0; i < PF; ++i) {
+        const int t = t0 + i;
+        const int s = nsteps - 1 - t;  // adjoint order
+        if (t < nsteps) {
+          const int64_t q = q0 + (int64_t)s * dq;
+          int A[LPL];
+  #pragma unroll
+          for (int k = 0; k < LPL; ++k) A[k] = arg_of(pa[i], k);
+          const float pv = pp[i];
+          const int gl = pl[i];
 
-        // normalisation Jacobian: g = t - softmax(msg) * sum(t), t = in_grad + chain
-        float tt[LPL], ts = 0.f;
-#pragma unroll
-        for (int k = 0; k < LPL; ++k) {
-          tt[k] = valid[k] ? pg[i][k] + acc[k] : 0.f;
-          ts += tt[k];
-        }
-        ts = wave_sum_f(ts);
-        float gv[LPL], fs = 0.f;
-        bool wide = false;
-#pragma unroll
-        for (int k = 0; k < LPL; ++k) {
-          float gk = tt[k] - fast_exp(pm[i][k]) * ts;
-          if (!VERT) gk += pd[i][k] + pu[i][k];
-          gv[k] = valid[k] ? gk : 0.f;
-          const bool mk = A[k] == kFromTrunc;
-          fs += mk ? gv[k] : 0.f;
-          wide = wide || (valid[k] && !mk && abs(A[k] - (c0 + k)) > 3);
-        }
-        const float fsum = wave_sum_f(fs);
-        // argmin of the receiver labels, -1 where nothing is routed by offset
-        int ar[LPL];
-#pragma unroll
-        for (int k = 0; k < LPL; ++k) ar[k] = (valid[k] && A[k] != kFromTrunc) ? A[k] : -1;
-
-        // ---- route to the sender labels (deterministic gather)
-        float gh[LPL];
-#pragma unroll
-        for (int k = 0; k < LPL; ++k) gh[k] = 0.f;
-        if (__ballot(wide) == 0) {
-          float gL[ND][LPL], gR[ND][LPL];
-          int aL[ND][LPL], aR[ND][LPL];
-#pragma unroll
+          // normalisation Jacobian: g = t - softmax(msg) * sum(t), t = in_grad + chain
+          float tt[LPL], ts = 0.f;
+  #pragma unroll
           for (int k = 0; k < LPL; ++k) {
-            gL[0][k] = shr1_f(gv[k], 0.f);
-            gR[0][k] = shl1_f(gv[k], 0.f);
-            aL[0][k] = shr1_i(ar[k], -1);
-            aR[0][k] = shl1_i(ar[k], -1);
+            tt[k] = valid[k] ? pg[i][k] + acc[k] : 0.f;
+            ts += tt[k];
           }
-#pragma unroll
-          for (int d = 1; d < ND; ++d)
-#pragma unroll
+          ts = wave_sum_f(ts);
+          float gv[LPL], fs = 0.f;
+          bool wide = false;
+  #pragma unroll
+          for (int k = 0; k < LPL; ++k) {
+            float gk = tt[k] - fast_exp(pm[i][k]) * ts;
+            if (!VERT) gk += pd[i][k] + pu[i][k];
+            gv[k] = valid[k] ? gk : 0.f;
+            const bool mk = A[k] == kFromTrunc;
+            fs += mk ? gv[k] : 0.f;
+            wide = wide || (valid[k] && !mk && abs(A[k] - (c0 + k)) > 3);
+          }
+          const float fsum = wave_sum_f(fs);
+          // argmin of the receiver labels, -1 where nothing is routed by offset
+          int ar[LPL];
+  #pragma unroll
+          for (int k = 0; k < LPL; ++k) ar[k] = (valid[k] && A[k] != kFromTrunc) ? A[k] : -1;
+
+          // ---- route to the sender labels (deterministic gather)
+          float gh[LPL];
+  #pragma unroll
+          for (int k = 0; k < LPL; ++k) gh[k] = 0.f;
+          if (__ballot(wide) == 0) {
+            float gL[ND][LPL], gR[ND][LPL];
+            int aL[ND][LPL], aR[ND][LPL];
+  #pragma unroll
             for (int k = 0; k < LPL; ++k) {
-              gL[d][k] = shr1_f(gL[d - 1][k], 0.f);
-              gR[d][k] = shl1_f(gR[d - 1][k], 0.f);
-              aL[d][k] = shr1_i(aL[d - 1][k], -1);
-              aR[d][k] = shl1_i(aR[d - 1][k], -1);
+              gL[0][k] = shr1_f(gv[k], 0.f);
+              gR[0][k] = shl1_f(gv[k], 0.f);
+              aL[0][k] = shr1_i(ar[k], -1);
+              aR[0][k] = shl1_i(ar[k], -1);
             }
-#pragma unroll
-          for (int kk = -3; kk <= 3; ++kk) {
-#pragma unroll
-            for (int k = 0; k < LPL; ++k) {
-              const int e = k - kk;  // receiver c = l - kk, as a label offset from c0
-              float gc;
-              int ac;
-              if (e >= 0 && e < LPL) {
-                gc = gv[e >= 0 && e < LPL ? e : 0];
-                ac = ar[e >= 0 && e < LPL ? e : 0];
-              } else if (e < 0) {
-                const int j = -e, d = (j + LPL - 1) / LPL, slot = d * LPL - j;
-                gc = gL[d - 1][slot];
-                ac = aL[d - 1][slot];
-              } else {
-                const int j = e - LPL + 1, d = (j + LPL - 1) / LPL, slot = j - 1 - (d - 1) * LPL;
-                gc = gR[d - 1][slot];
-                ac = aR[d - 1][slot];
+  #pragma unroll
+            for (int d = 1; d < ND; ++d)
+  #pragma unroll
+              for (int k = 0; k < LPL; ++k) {
+                gL[d][k] = shr1_f(gL[d - 1][k], 0.f);
+                gR[d][k] = shl1_f(gR[d - 1][k], 0.f);
+                aL[d][k] = shr1_i(aL[d - 1][k], -1);
+                aR[d][k] = shl1_i(aR[d - 1][k], -1);
               }
-              gh[k] += ac == c0 + k ? gc : 0.f;
+  #pragma unroll
+            for (int kk = -3; kk <= 3; ++kk) {
+  #pragma unroll
+              for (int k = 0; k < LPL; ++k) {
+                const int e = k - kk;  // receiver c = l - kk, as a label offset from c0
+                float gc;
+                int ac;
+                if (e >= 0 && e < LPL) {
+                  gc = gv[e >= 0 && e < LPL ? e : 0];
+                  ac = ar[e >= 0 && e < LPL ? e : 0];
+                } else if (e < 0) {
+                  const int j = -e, d = (j + LPL - 1) / LPL, slot = d * LPL - j;
+                  gc = gL[d - 1][slot];
+                  ac = aL[d - 1][slot];
+                } else {
+                  const int j = e - LPL + 1, d = (j + LPL - 1) / LPL, slot = j - 1 - (d - 1) * LPL;
+                  gc = gR[d - 1][slot];
+                  ac = aR[d - 1][slot];
+                }
+                gh[k] += ac == c0 + k ? gc : 0.f;
+              }
             }
-          }
-        } else {
-          const int buf = t & 1;
-          int dm = 0;
-#pragma unroll
-          for (int k = 0; k < LPL; ++k) {
-            lds_g[buf][c0 + k] = gv[k];
-            lds_a[buf][c0 + k] = ar[k];
-            if (ar[k] >= 0) dm = max(dm, abs(ar[k] - (c0 + k)));
-          }
-          const int D = wave_max_i(dm);
-          __syncthreads();
-          for (int kk = -D; kk <= D; ++kk) {
-#pragma unroll
+          } else {
+            const int buf = t & 1;
+            int dm = 0;
+  #pragma unroll
             for (int k = 0; k < LPL; ++k) {
-              const int l = c0 + k;
-              const int c = l - kk;
-              if (c >= 0 && c < C && lds_a[buf][c] == l) gh[k] += lds_g[buf][c];
+              lds_g[buf][c0 + k] = gv[k];
+              lds_a[buf][c0 + k] = ar[k];
+              if (ar[k] >= 0) dm = max(dm, abs(ar[k] - (c0 + k)));
+            }
+            const int D = wave_max_i(dm);
+            wave_lds_sync();  // this wave's LDS rows only (a paired workgroup's other wave may not be here)
+            for (int kk = -D; kk <= D; ++kk) {
+  #pragma unroll
+              for (int k = 0; k < LPL; ++k) {
+                const int l = c0 + k;
+                const int c = l - kk;
+                if (c >= 0 && c < C && lds_a[buf][c] == l) gh[k] += lds_g[buf][c];
+              }
             }
           }
-        }
-#pragma unroll
-        for (int k = 0; k < LPL; ++k) gh[k] += c0 + k == gl ? fsum : 0.f;
+  #pragma unroll
+          for (int k = 0; k < LPL; ++k) gh[k] += c0 + k == gl ? fsum : 0.f;
 
-        // ---- pairwise gradients of (dir, sender): bins 1..K-1 and the edge term in slot 0
-        const float w = readlane_f(pv, K);
-        float vb[8];
-#pragma unroll
-        for (int b = 0; b < 8; ++b) vb[b] = 0.f;
-#pragma unroll
-        for (int k = 0; k < LPL; ++k) {
-          const bool mk = A[k] == kFromTrunc;
-          const int src = mk ? gl : A[k];
-          const int bin = (mk && exact) ? K - 1 : min(max(K / 2 + c0 + k - src, 1), K - 1);
-#pragma unroll
-          for (int b = 1; b < 8; ++b) vb[b] += bin == b ? gv[k] : 0.f;
-        }
-#pragma unroll
-        for (int b = 1; b < 8; ++b)
-          if (b < K) vb[0] = fmaf(readlane_f(pv, b), vb[b], vb[0]);
-        float even, odd;
-        reduce_scatter8(vb, even, odd);
-        const int row = lane >> 4;
-        const int bin_here = 2 * row + (lane & 1);
-        const float tot = (lane & 1) ? odd : even;
-        if ((lane & 15) < 2) {
-          if (bin_here == 0) {
-            ge_p[q] = tot;
-            if (!a.accum) gj_p[q] = 0.f;
-          } else if (bin_here < K) {
-            float *dst = gj_p + (int64_t)bin_here * HW + q;
-            *dst = a.accum ? *dst + w * tot : w * tot;
+          // ---- pairwise gradients of (dir, sender): bins 1..K-1 and the edge term in slot 0
+          const float w = readlane_f(pv, K);
+          float vb[8];
+  #pragma unroll
+          for (int b = 0; b < 8; ++b) vb[b] = 0.f;
+  #pragma unroll
+          for (int k = 0; k < LPL; ++k) {
+            const bool mk = A[k] == kFromTrunc;
+            const int src = mk ? gl : A[k];
+            const int bin = (mk && exact) ? K - 1 : min(max(K / 2 + c0 + k - src, 1), K - 1);
+  #pragma unroll
+            for (int b = 1; b < 8; ++b) vb[b] += bin == b ? gv[k] : 0.f;
+          }
+  #pragma unroll
+          for (int b = 1; b < 8; ++b)
+            if (b < K) vb[0] = fmaf(readlane_f(pv, b), vb[b], vb[0]);
+          float even, odd;
+          reduce_scatter8(vb, even, odd);
+          const int row = lane >> 4;
+          const int bin_here = 2 * row + (lane & 1);
+          const float tot = (lane & 1) ? odd : even;
+          if ((lane & 15) < 2) {
+            if (bin_here == 0) {
+              ge_p[q] = tot;
+              if (!a.accum) gj_p[q] = 0.f;
+            } else if (bin_here < K) {
+              float *dst = gj_p + (int64_t)bin_here * HW + q;
+              *dst = a.accum ? *dst + w * tot : w * tot;
+            }
+          }
+
+          // ---- chain and sender h-gradient
+  #pragma unroll
+          for (int k = 0; k < LPL; ++k) acc[k] = gh[k];
+          if (FULL) {
+            float *p = Xo + q * C + c0;
+  #pragma unroll
+            for (int k = 0; k < LPL; ++k) p[k] = gh[k];
+          } else {
+  #pragma unroll
+            for (int k = 0; k < LPL; ++k)
+              if (valid[k]) Xo[q * C + c0 + k] = gh[k];
           }
         }
-
-        // ---- chain and sender h-gradient
-#pragma unroll
-        for (int k = 0; k < LPL; ++k) acc[k] = gh[k];
-        if (FULL) {
-          float *p = Xo + q * C + c0;
-#pragma unroll
-          for (int k = 0; k < LPL; ++k) p[k] = gh[k];
-        } else {
-#pragma unroll
-          for (int k = 0; k < LPL; ++k)
-            if (valid[k]) Xo[q * C + c0 + k] = gh[k];
-        }
+        // refill ring slot i after its last use
+        load_step(s - PF >= 0 ? s - PF : 0, pg[i], pm[i], pa[i], pd[i], pu[i], pp[i], pl[i]);
       }
-      // refill ring slot i after its last use
-      load_step(s - PF >= 0 ? s - PF : 0, pg[i], pm[i], pa[i], pd[i], pu[i], pp[i], pl[i]);
     }
+  }
+
+  if constexpr (PAIR) {
+    // combine the row ends and the pixels of [fix_lo, fix_hi] (every pixel when the sweep was
+    // not paired); the U and D slots get X_R + X_L here, their y-boundary terms are added by
+    // lbp_bwd_pair_rows
+    __syncthreads();  // every h-gradient of both passes is written
+    const int64_t HWC = HW * C;
+    const float *XRp = a.X[RIGHT] + img, *XLp = a.X[LEFT] + img;
+    const float *igr = a.in_grad + (int64_t)n * 4 * HWC;
+    auto fix = [&](int x) {
+      const int64_t q = (int64_t)ln * W + x;
+      float xd[LPL], xu[LPL], xr[LPL], xl[LPL], ge[LPL], cst[LPL], hr[LPL], sl[LPL], sr[LPL];
+      ld_row(XDp + q * C, xd);
+      ld_row(XUp + q * C, xu);
+      if (x <= W - 2) ld_row(XRp + q * C, xr);
+      else
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) xr[k] = 0.f;
+      if (x >= 1) ld_row(XLp + q * C, xl);
+      else
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) xl[k] = 0.f;
+#pragma unroll
+      for (int k = 0; k < LPL; ++k) {
+        const float sv = xd[k] + xu[k];
+        cst[k] = (sv + xr[k]) + xl[k];
+        hr[k] = xr[k] + xl[k];
+        sl[k] = sr[k] = 0.f;
+      }
+      if (x == 0) {
+        ld_row(igr + LEFT * HWC + q * C, ge);
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) sl[k] = (ge[k] + xr[k]) + (xd[k] + xu[k]);
+      }
+      if (x == W - 1) {
+        ld_row(igr + RIGHT * HWC + q * C, ge);
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) sr[k] = (ge[k] + xl[k]) + (xd[k] + xu[k]);
+      }
+      float *gm = a.grad_msg + (int64_t)n * 4 * HWC + q * C;
+      auto st = [&](float *row, const float *v) {
+#pragma unroll
+        for (int k = 0; k < LPL; ++k)
+          if (valid[k]) row[c0 + k] = v[k];
+      };
+      st(a.grad_cost + img + q * C, cst);
+      st(gm + UP * HWC, hr);
+      st(gm + DOWN * HWC, hr);
+      st(gm + LEFT * HWC, sl);
+      st(gm + RIGHT * HWC, sr);
+    };
+    // the two waves take alternate pixels of the list (row ends first)
+    int idx = 0;
+    auto visit = [&](int x) {
+      if ((idx++ & 1) == wv) fix(x);
+    };
+    visit(0);
+    if (W > 1) visit(W - 1);
+    for (int x = max(fix_lo, 1); x <= min(fix_hi, W - 2); ++x) visit(x);
+  }
+}
+
+// y-boundary terms of the input-message gradient after the paired horizontal phase (which
+// wrote X_R + X_L into the U and D slots): U slot of row 0 = (in_grad_U + X_D) + (X_R + X_L),
+// D slot of row H - 1 = (in_grad_D + X_U) + (X_R + X_L), the combine kernel's expressions.
+__global__ void lbp_bwd_pair_rows(const float *__restrict__ in_grad, const float *__restrict__ XD,
+                                  const float *__restrict__ XU, float *__restrict__ grad_msg, int N, int H, int W,
+                                  int C) {
+  const int64_t WC = (int64_t)W * C, HWC = (int64_t)H * WC;
+  const int64_t total = (int64_t)N * 2 * WC;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t n = i / (2 * WC), r = i - n * 2 * WC;
+    const int side = (int)(r / WC);
+    const int64_t pix = (side ? (int64_t)(H - 1) * WC : 0) + (r - side * WC);
+    const int slot = side ? DOWN : UP;
+    const float xv = side ? XU[n * HWC + pix] : XD[n * HWC + pix];
+    float *g = grad_msg + (n * 4 + slot) * HWC + pix;
+    *g = (in_grad[(n * 4 + slot) * HWC + pix] + xv) + *g;
   }
 }
 
@@ -834,38 +1013,52 @@ void launch_kind(const BwdArgs &a, bool vert, int ngroups, hipStream_t st) {
   else launch_phase<KIND, 64, 4>(a, vert, ngroups, st);
 }
 
+// the fast step needs the window half-width (argmins within 3 labels), a stereo packing's
+// jump length and 32-bit buffer offsets into the (direction) planes of the pairwise gradient
+bool stereo_fast_ok(const BwdArgs &a) {
+  return a.delta >= 0 && a.delta <= 3 && !a.accum && (a.K == 6 || a.K == 8) &&
+         (int64_t)a.K * a.H * a.W * 4 < (1LL << 31);
+}
 template <int LPL, bool FULL>
-void launch_stereo_w64(const BwdArgs &a, bool vert, int ngroups, hipStream_t st) {
+void launch_stereo_w64(const BwdArgs &a, bool vert, bool pair, int ngroups, hipStream_t st) {
   constexpr int PF = 4;
-  const dim3 grid((unsigned)(ngroups * 2 * (vert ? a.W : a.H)));
-  // the fast step needs the window half-width (argmins within 3 labels) and 32-bit buffer
-  // offsets into the (direction) planes of the pairwise gradient
-  const bool fast = a.delta >= 0 && a.delta <= 3 && !a.accum && (int64_t)a.K * a.H * a.W * 4 < (1LL << 31);
+  const bool fast = stereo_fast_ok(a);
   auto go = [&](auto fk) {
     constexpr int FK = decltype(fk)::value;
-    if (vert)
-      hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, true, FULL, FK, PF>), grid, dim3(kWave), 0, st, a);
-    else
-      hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, FK, PF>), grid, dim3(kWave), 0, st, a);
+    if (vert) {
+      const dim3 grid((unsigned)(ngroups * 2 * a.W));
+      hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, true, FULL, FK, false, PF>), grid, dim3(kWave), 0, st, a);
+    } else if (pair) {
+      const dim3 grid((unsigned)(ngroups * a.H));
+      hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, FK, true, PF>), grid, dim3(2 * kWave), 0, st, a);
+    } else {
+      const dim3 grid((unsigned)(ngroups * 2 * a.H));
+      hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, FK, false, PF>), grid, dim3(kWave), 0, st, a);
+    }
   };
   if (fast && a.K == 6) go(std::integral_constant<int, 6>());
   else if (fast && a.K == 8) go(std::integral_constant<int, 8>());
   else go(std::integral_constant<int, 0>());
 }
 template <int LPL>
-void launch_stereo_w64_any(const BwdArgs &a, bool vert, int ngroups, hipStream_t st) {
-  if (a.C == kWave * LPL) launch_stereo_w64<LPL, true>(a, vert, ngroups, st);
-  else launch_stereo_w64<LPL, false>(a, vert, ngroups, st);
+void launch_stereo_w64_any(const BwdArgs &a, bool vert, bool pair, int ngroups, hipStream_t st) {
+  if (a.C == kWave * LPL) launch_stereo_w64<LPL, true>(a, vert, pair, ngroups, st);
+  else launch_stereo_w64<LPL, false>(a, vert, pair, ngroups, st);
 }
 // the 64-lane stereo kernel covers 32 < C <= 256 and up to 8 pairwise bins
 bool stereo_w64_ok(const BwdArgs &a) { return a.C > 32 && a.K <= 8; }
-void launch_stereo(const BwdArgs &a, bool vert, int ngroups, hipStream_t st) {
+// exact-mode stereo backward whose horizontal phase pairs the two passes of a row and writes
+// grad_cost and the input-message gradient itself (no combine pass)
+bool stereo_pair_ok(const BwdArgs &a) {
+  return stereo_w64_ok(a) && !generic_only() && a.mode == BPL_GRAD_EXACT && stereo_fast_ok(a) && a.W >= 2;
+}
+void launch_stereo(const BwdArgs &a, bool vert, bool pair, int ngroups, hipStream_t st) {
   const int C = a.C;
   if (!stereo_w64_ok(a) || generic_only()) launch_kind<BPL_LBP_STEREO>(a, vert, ngroups, st);
-  else if (C <= 64) launch_stereo_w64_any<1>(a, vert, ngroups, st);
-  else if (C <= 128) launch_stereo_w64_any<2>(a, vert, ngroups, st);
-  else if (C <= 192) launch_stereo_w64_any<3>(a, vert, ngroups, st);
-  else launch_stereo_w64_any<4>(a, vert, ngroups, st);
+  else if (C <= 64) launch_stereo_w64_any<1>(a, vert, pair, ngroups, st);
+  else if (C <= 128) launch_stereo_w64_any<2>(a, vert, pair, ngroups, st);
+  else if (C <= 192) launch_stereo_w64_any<3>(a, vert, pair, ngroups, st);
+  else launch_stereo_w64_any<4>(a, vert, pair, ngroups, st);
 }
 
 int seg_width(int C) { return C <= 16 ? 16 : C <= 32 ? 32 : 64; }
@@ -934,9 +1127,12 @@ extern "C" int bpl_lbp_bwd(int kind, int mode, const float *jump, int64_t jump_k
   a.Nj = kind == BPL_LBP_STEREO ? (int)jump_n : 1;
   a.mode = mode; a.mat_global = 1;
   a.delta = delta;
+  a.grad_cost = grad_cost; a.grad_msg = grad_msg;
   hipStream_t st = as_stream(stream);
   // parameter gradients shared across the batch accumulate over n in launch order
   const bool per_n = (kind == BPL_LBP_STEREO && a.Nj == 1 && N > 1) || (kind == BPL_LBP_SEMANTIC_PIXEL && N > 1);
+  a.accum = 0;
+  const bool pair = kind == BPL_LBP_STEREO && !per_n && stereo_pair_ok(a);
   for (int phase = 0; phase < 2; ++phase) {
     const bool vert = phase == 0;
     if (kind == BPL_LBP_SEMANTIC_GLOBAL) a.partial = ws + planes + (vert ? 0 : pv);
@@ -947,7 +1143,7 @@ extern "C" int bpl_lbp_bwd(int kind, int mode, const float *jump, int64_t jump_k
       a.accum = per_n && b > 0;
       const int ng = per_n ? 1 : (int)N;
       switch (kind) {
-        case BPL_LBP_STEREO: launch_stereo(a, vert, ng, st); break;
+        case BPL_LBP_STEREO: launch_stereo(a, vert, pair, ng, st); break;
         case BPL_LBP_SEMANTIC_GLOBAL: launch_kind<BPL_LBP_SEMANTIC_GLOBAL>(a, vert, ng, st); break;
         default: launch_kind<BPL_LBP_SEMANTIC_PIXEL>(a, vert, ng, st); break;
       }
@@ -955,7 +1151,18 @@ extern "C" int bpl_lbp_bwd(int kind, int mode, const float *jump, int64_t jump_k
       if (rc) return rc;
     }
   }
-  {
+  if (pair) {
+    // the paired horizontal phase wrote grad_cost and grad_msg; add the y-boundary terms
+    const int64_t total_el = N * 2 * W * C;
+    const int threads = 256;
+    int64_t blocks = (total_el + threads - 1) / threads;
+    if (blocks > 65536) blocks = 65536;
+    KernelTimer kt(BPL_KERNEL_LBP_BWD_COMBINE, st);
+    hipLaunchKernelGGL(lbp_bwd_pair_rows, dim3((unsigned)blocks), dim3(threads), 0, st, in_grad, a.X[DOWN], a.X[UP],
+                       grad_msg, (int)N, (int)H, (int)W, (int)C);
+    int rc = check_hip(hipGetLastError(), "bpl_lbp_bwd boundary-row launch");
+    if (rc) return rc;
+  } else {
     const int64_t total_el = plane;
     const int threads = 256;
     int64_t blocks = (total_el + threads - 1) / threads;

@@ -72,6 +72,12 @@ CASES = [
     ("stereo", 1, 5, 7, 30, 2, {"ordered": True}),
     ("stereo", 1, 1, 8, 16, 3, {}),
     ("stereo", 1, 8, 1, 16, 3, {}),
+    # longer rows: the paired horizontal adjoint (exact mode) with several groups in each half
+    ("stereo", 1, 3, 37, 64, 3, {}),
+    ("stereo", 2, 2, 23, 100, 3, {}),
+    ("stereo", 1, 2, 41, 192, 3, {}),
+    ("stereo", 1, 4, 13, 64, 3, {"offset": True}),
+    ("stereo", 1, 3, 2, 64, 3, {}),
     ("semantic_global", 2, 5, 6, 19, 19, {}),
     ("semantic_global", 1, 4, 5, 19, 3, {}),
     ("semantic_pixel", 1, 4, 5, 7, 7, {}),
