@@ -109,15 +109,19 @@ __global__ void sad_stereo_fwd_step_k(const float *__restrict__ f0, const float 
 }
 
 // ----------------------------------------------------------------------------- stereo bwd
-// Block: one row, a tile of 32 pixels, 256 threads = 16 pixel pairs x 16 channels.
+// Block: one row, a tile of 32 pixels, 256 threads = 16 pixel pairs x 16 channel lanes.  A
+// thread accumulates its 2 pixels for the channels cl, cl + 16, cl + 32, cl + 48 of a group of
+// 64 channels.  Disparities are taken in chunks of SB_DC; per chunk the gradient tiles
 //   G0[k][x] = g[y, x, k]      (0 where x - d is outside the image)   -> df0 gather
 //   G1[k][x] = g[y, x + d, k]  (0 where x + d is outside the image)   -> df1 gather
-// are staged once per block (the diagonal G1 row by row of the source pixel, so the global
-// reads stay contiguous); then, per chunk of 16 channels, the f1 segment [x0-dmax, x0+31-dmin]
-// and the f0 segment [x0+dmin, x0+31+dmax].  Each thread sums its D terms in disparity order
-// (the reference's order) from LDS, reading one new f value per disparity per side; the
-// invalid pairs carry g = 0, so the loop has no range tests.
-constexpr int SB_TX = 32, SB_CB = 16, SB_GP = SB_TX + 1;
+// are staged once (the diagonal G1 row by row of its source pixel, so the global reads stay
+// contiguous), then, for each 16-channel part of the group, the f1 segment
+// [x0 - d_hi, x0 + 31 - d_lo] and the f0 segment [x0 + d_lo, x0 + 31 + d_hi] of the chunk.  Each
+// thread sums its terms in disparity order (the reference's order) from LDS, reading one new f
+// value per disparity per side; the invalid pairs carry g = 0, so the loop has no range tests.
+// Chunking keeps the block's LDS at 29 KB, so five blocks (20 waves) share a CU.
+constexpr int SB_TX = 32, SB_CB = 16, SB_GP = SB_TX + 1, SB_DC = 64, SB_CG = 4;
+constexpr int SB_SPAN = SB_TX + SB_DC - 1;
 
 // sign(t) * g with the reference's dead zone |t| <= 1e-15 -> 0 (a sign-bit flip: exact)
 __device__ __forceinline__ float sgn_mul(float t, float g) {
@@ -125,81 +129,97 @@ __device__ __forceinline__ float sgn_mul(float t, float g) {
   return fabsf(t) > kSignEps ? v : 0.f;
 }
 
-size_t sad_bwd_lds_bytes(int D) {
-  return ((size_t)2 * D * SB_GP + (size_t)2 * SB_CB * (SB_TX + D - 1)) * sizeof(float);
-}
-
 __global__ __launch_bounds__(256) void sad_stereo_bwd_k(const float *__restrict__ f0, const float *__restrict__ f1,
                                                         const float *__restrict__ g, float *__restrict__ df0,
                                                         float *__restrict__ df1, int C, int H, int W, int D, int mind) {
-  extern __shared__ float sm[];
-  const int SPAN = SB_TX + D - 1;
-  float *G0 = sm;                   // [D][SB_GP]
-  float *G1 = G0 + D * SB_GP;       // [D][SB_GP]
-  float *F1 = G1 + D * SB_GP;       // [SB_CB][SPAN]: f1 at x0 - dmax + j
-  float *F0 = F1 + SB_CB * SPAN;    // [SB_CB][SPAN]: f0 at x0 + dmin + j
+  __shared__ float G0[SB_DC * SB_GP];      // [k - d0][x]
+  __shared__ float G1[SB_DC * SB_GP];      // [k - d0][x]
+  __shared__ float F1[SB_CB * SB_SPAN];    // [channel][j]: f1 at x0 - (mind + d0 + SB_DC - 1) + j
+  __shared__ float F0[SB_CB * SB_SPAN];    // [channel][j]: f0 at x0 + mind + d0 + j
   const int x0 = blockIdx.x * SB_TX;
   const int y = blockIdx.y;
   const int n = blockIdx.z;
   const int t = threadIdx.x;
-  const int maxd = mind + D - 1;
   const int64_t HW = (int64_t)H * W;
   const float *Gr = g + ((int64_t)n * H + y) * (int64_t)W * D;
-  for (int i = t; i < SB_TX * D; i += 256) {
-    const int xl = i / D, k = i - xl * D;
-    const int x = x0 + xl, xm = x - (mind + k);
-    G0[k * SB_GP + xl] = (x < W && xm >= 0 && xm < W) ? Gr[(int64_t)x * D + k] : 0.f;
-  }
-  // source pixel p = x0 + dmin + m carries the entries k = m - xl of the tile's pixels x0 + xl
-  for (int i = t; i < SPAN * SB_TX; i += 256) {
-    const int m = i / SB_TX, xl = i - m * SB_TX;
-    const int k = m - xl;
-    if (k < 0 || k >= D) continue;
-    const int x = x0 + xl, p = x0 + mind + m;
-    G1[k * SB_GP + xl] = (x < W && p >= 0 && p < W) ? Gr[(int64_t)p * D + k] : 0.f;
-  }
   const int xg = t & 15;
   const int cl = t >> 4;
   const int xa = x0 + 2 * xg;
-  for (int c0 = 0; c0 < C; c0 += SB_CB) {
-    __syncthreads();  // staging of G (first chunk) / reads of the previous chunk are done
-    for (int i = t; i < SB_CB * SPAN; i += 256) {
-      const int cc = i / SPAN, j = i - cc * SPAN;
-      const int c = c0 + cc;
-      const int64_t b = ((int64_t)n * C + c) * HW + (int64_t)y * W;
-      const int xs1 = x0 - maxd + j, xs0 = x0 + mind + j;
-      F1[i] = (c < C && xs1 >= 0 && xs1 < W) ? f1[b + xs1] : 0.f;
-      F0[i] = (c < C && xs0 >= 0 && xs0 < W) ? f0[b + xs0] : 0.f;
+  for (int cg0 = 0; cg0 < C; cg0 += SB_CB * SB_CG) {
+    float r00[SB_CG], r01[SB_CG], r10[SB_CG], r11[SB_CG], a0[SB_CG], a1[SB_CG], b0[SB_CG], b1[SB_CG];
+#pragma unroll
+    for (int p = 0; p < SB_CG; ++p) {
+      const int c = cg0 + p * SB_CB + cl;
+      const int64_t b = ((int64_t)n * C + (c < C ? c : C - 1)) * HW + (int64_t)y * W;
+      a0[p] = xa < W ? f0[b + xa] : 0.f;
+      a1[p] = xa + 1 < W ? f0[b + xa + 1] : 0.f;
+      b0[p] = xa < W ? f1[b + xa] : 0.f;
+      b1[p] = xa + 1 < W ? f1[b + xa + 1] : 0.f;
+      r00[p] = r01[p] = r10[p] = r11[p] = 0.f;
     }
-    __syncthreads();
-    const int c = c0 + cl;
-    const int64_t b = ((int64_t)n * C + (c < C ? c : C - 1)) * HW + (int64_t)y * W;
-    const float a0 = xa < W ? f0[b + xa] : 0.f, a1 = xa + 1 < W ? f0[b + xa + 1] : 0.f;
-    const float b0 = xa < W ? f1[b + xa] : 0.f, b1 = xa + 1 < W ? f1[b + xa + 1] : 0.f;
-    // f1[x - d] = F1c[j - k], f0[x + d] = F0c[j + k] for pixel xa + j
-    const float *F1c = F1 + cl * SPAN + 2 * xg + D - 1;
-    const float *F0c = F0 + cl * SPAN + 2 * xg;
-    const float *G0c = G0 + 2 * xg;
-    const float *G1c = G1 + 2 * xg;
-    float r00 = 0.f, r01 = 0.f, r10 = 0.f, r11 = 0.f;
-    float v1 = F1c[1];  // f1 window (pixel xa + 1 at k = 0 is F1c[1])
-    float v0 = F0c[0];  // f0 window
+    for (int d0 = 0; d0 < D; d0 += SB_DC) {
+      const int dn = min(SB_DC, D - d0);
+      __syncthreads();  // the previous chunk's readers are done
+      for (int i = t; i < SB_TX * dn; i += 256) {
+        const int xl = i / dn, kk = i - xl * dn;
+        const int x = x0 + xl, xm = x - (mind + d0 + kk);
+        G0[kk * SB_GP + xl] = (x < W && xm >= 0 && xm < W) ? Gr[(int64_t)x * D + d0 + kk] : 0.f;
+      }
+      // source pixel p = x0 + mind + m carries the entries k = m - xl of the tile's pixels x0 + xl
+      for (int i = t; i < (dn + SB_TX - 1) * SB_TX; i += 256) {
+        const int mm = i / SB_TX, xl = i - mm * SB_TX;
+        const int kk = mm - xl;  // k - d0
+        if (kk < 0 || kk >= dn) continue;
+        const int x = x0 + xl, pp = x0 + mind + d0 + mm;
+        G1[kk * SB_GP + xl] = (x < W && pp >= 0 && pp < W) ? Gr[(int64_t)pp * D + d0 + kk] : 0.f;
+      }
+#pragma unroll
+      for (int p = 0; p < SB_CG; ++p) {
+        const int cb = cg0 + p * SB_CB;
+        if (cb >= C) break;
+        if (p > 0) __syncthreads();  // the previous part's readers of F are done
+        for (int i = t; i < SB_CB * SB_SPAN; i += 256) {
+          const int cc = i / SB_SPAN, j = i - cc * SB_SPAN;
+          const int c = cb + cc;
+          const int64_t b = ((int64_t)n * C + c) * HW + (int64_t)y * W;
+          const int xs1 = x0 - (mind + d0 + SB_DC - 1) + j, xs0 = x0 + mind + d0 + j;
+          F1[i] = (c < C && xs1 >= 0 && xs1 < W) ? f1[b + xs1] : 0.f;
+          F0[i] = (c < C && xs0 >= 0 && xs0 < W) ? f0[b + xs0] : 0.f;
+        }
+        __syncthreads();
+        // f1[x - d] = F1c[-k], f0[x + d] = F0c[k] for pixel xa (k = d - mind, absolute)
+        const float *F1c = F1 + cl * SB_SPAN + 2 * xg + d0 + SB_DC - 1;
+        const float *F0c = F0 + cl * SB_SPAN + 2 * xg - d0;
+        const float *G0c = G0 + 2 * xg - d0 * SB_GP;
+        const float *G1c = G1 + 2 * xg - d0 * SB_GP;
+        float s00 = r00[p], s01 = r01[p], s10 = r10[p], s11 = r11[p];
+        const float fa0 = a0[p], fa1 = a1[p], fb0 = b0[p], fb1 = b1[p];
+        float v1 = F1c[1 - d0];  // f1[xa + 1 - d] at k = d0
+        float v0 = F0c[d0];      // f0[xa + d] at k = d0
 #pragma unroll 4
-    for (int k = 0; k < D; ++k) {
-      const float v1n = F1c[-k];     // f1[xa - d]; f1[xa + 1 - d] is the previous step's
-      const float v0n = F0c[k + 1];  // f0[xa + 1 + d]; f0[xa + d] is the previous step's
-      const float g00 = G0c[k * SB_GP], g01 = G0c[k * SB_GP + 1];
-      const float g10 = G1c[k * SB_GP], g11 = G1c[k * SB_GP + 1];
-      r00 += sgn_mul(a0 - v1n, g00);
-      r01 += sgn_mul(a1 - v1, g01);
-      r10 -= sgn_mul(v0 - b0, g10);
-      r11 -= sgn_mul(v0n - b1, g11);
-      v1 = v1n;
-      v0 = v0n;
+        for (int k = d0; k < d0 + dn; ++k) {
+          const float v1n = F1c[-k];     // f1[xa - d]; f1[xa + 1 - d] is the previous step's
+          const float v0n = F0c[k + 1];  // f0[xa + 1 + d]; f0[xa + d] is the previous step's
+          const float g00 = G0c[k * SB_GP], g01 = G0c[k * SB_GP + 1];
+          const float g10 = G1c[k * SB_GP], g11 = G1c[k * SB_GP + 1];
+          s00 += sgn_mul(fa0 - v1n, g00);
+          s01 += sgn_mul(fa1 - v1, g01);
+          s10 -= sgn_mul(v0 - fb0, g10);
+          s11 -= sgn_mul(v0n - fb1, g11);
+          v1 = v1n;
+          v0 = v0n;
+        }
+        r00[p] = s00; r01[p] = s01; r10[p] = s10; r11[p] = s11;
+      }
     }
-    if (c < C) {
-      if (xa < W) { df0[b + xa] = r00; df1[b + xa] = r10; }
-      if (xa + 1 < W) { df0[b + xa + 1] = r01; df1[b + xa + 1] = r11; }
+#pragma unroll
+    for (int p = 0; p < SB_CG; ++p) {
+      const int c = cg0 + p * SB_CB + cl;
+      if (c < C) {
+        const int64_t b = ((int64_t)n * C + c) * HW + (int64_t)y * W;
+        if (xa < W) { df0[b + xa] = r00[p]; df1[b + xa] = r10[p]; }
+        if (xa + 1 < W) { df0[b + xa + 1] = r01[p]; df1[b + xa + 1] = r11[p]; }
+      }
     }
   }
 }
@@ -533,18 +553,10 @@ extern "C" int bpl_sad_stereo_bwd(const float *f0, const float *f1, const float 
   BPL_REQUIRE(N >= 1 && C >= 1 && H >= 1 && W >= 1, BPL_E_ARG, "bpl_sad_stereo_bwd: sizes must be positive");
   const int D = max_disp - min_disp + 1;
   BPL_REQUIRE(D >= 1, BPL_E_ARG, "bpl_sad_stereo_bwd: empty disparity range");
-  const size_t shm = sad_bwd_lds_bytes(D);
-  BPL_REQUIRE(shm <= 160 * 1024, BPL_E_LIMIT, "bpl_sad_stereo_bwd: D=%d needs %zu bytes of LDS", D, shm);
   BPL_REQUIRE(H < 65536 && N < 65536, BPL_E_LIMIT, "bpl_sad_stereo_bwd: tensor too large");
-  if (shm > 64 * 1024) {
-    const int rc = check_hip(hipFuncSetAttribute((const void *)sad_stereo_bwd_k,
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm),
-                             "bpl_sad_stereo_bwd: raising the LDS limit");
-    if (rc != BPL_OK) return rc;
-  }
   dim3 grid((unsigned)((W + SB_TX - 1) / SB_TX), (unsigned)H, (unsigned)N);
   KernelTimer kt(BPL_KERNEL_SAD_STEREO_BWD, as_stream(stream));
-  hipLaunchKernelGGL(sad_stereo_bwd_k, grid, dim3(256), shm, as_stream(stream), f0, f1, grad_cost, df0, df1,
+  hipLaunchKernelGGL(sad_stereo_bwd_k, grid, dim3(256), 0, as_stream(stream), f0, f1, grad_cost, df0, df1,
                      (int)C, (int)H, (int)W, D, min_disp);
   return check_hip(hipGetLastError(), "bpl_sad_stereo_bwd launch");
 }

@@ -35,7 +35,8 @@ def test_stereo_sad_forward_fractional_step():
     np.testing.assert_allclose(got, S.stereo_sad_forward(f0, f1, 0, 9, 0.5), rtol=SAD_RTOL, atol=1e-5)
 
 
-@pytest.mark.parametrize("N,C,H,W,mn,mx", [(1, 16, 4, 80, 0, 31), (2, 8, 3, 50, -2, 9), (1, 64, 2, 200, 0, 191)])
+@pytest.mark.parametrize("N,C,H,W,mn,mx", [(1, 16, 4, 80, 0, 31), (2, 8, 3, 50, -2, 9), (1, 64, 2, 200, 0, 191),
+                                           (1, 100, 2, 77, 5, 104), (2, 70, 2, 45, -10, 80)])
 def test_stereo_sad_backward(N, C, H, W, mn, mx):
     from bplayers_amd import pytorch_cuda_stereo_sad_op as op
     rng = np.random.default_rng(C)
