@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--rows", type=int, default=0, help="crop the image to this many rows (debug)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--pmc-summary", nargs="+", metavar="DIR",
+                    help="summarise rocprofv3 --pmc runs of `bench.py --pmc` (directories holding "
+                         "*counter_collection.csv) into profiles/traffic.json and print a table")
     ap.add_argument("--pmc", action="store_true",
                     help="counter-collection run for rocprofv3 --pmc: one warm-up step, one step and a "
                          "256 MiB device copy (the calibration kernel); prints what ran")
@@ -405,6 +408,97 @@ def measured_traffic(workload, kernel):
     return {"bytes_per_launch": k["hbm_bytes_per_launch"], "source": os.path.relpath(TRAFFIC_FILE, ROOT)}
 
 
+def kernel_class(name):
+    """Kernel class (as booked by libbpl_amd's timing) of a rocprofv3 kernel name."""
+    import re
+    base = name.split("(")[0] if "(anonymous" not in name else name.replace("(anonymous namespace)::", "").split("(")[0]
+    if "copyBuffer" in name:
+        return "calibration_copy"
+    if "lbp_bwd_combine" in base or "lbp_bwd_pair_rows" in base:
+        return "lbp_bwd_combine"
+    if "lbp_bwd_reduce" in base:
+        return "lbp_bwd_reduce"
+    for stem, cls in (("sad_stereo_fwd", "sad_stereo_fwd"), ("sad_stereo_bwd", "sad_stereo_bwd"),
+                      ("sad_flow_fwd", "sad_flow_fwd"), ("sad_flow_bwd", "sad_flow_bwd")):
+        if stem in base:
+            return cls
+    m = re.search(r"lbp_(fwd|bwd)_(stereo_w64|stereo_dpp|phase)<([^>]*)>", base)
+    if not m:
+        return None
+    targs = [t.strip() for t in m.group(3).split(",")]
+    vert = targs[3] if m.group(2) == "phase" else targs[1]
+    return f"lbp_{m.group(1)}_{'vertical' if vert == 'true' else 'horizontal'}"
+
+
+def pmc_summary(dirs):
+    """profiles/traffic.json from rocprofv3 --pmc runs of `bench.py --pmc`: per kernel class the
+    HBM bytes per launch (FETCH_SIZE and WRITE_SIZE, in KiB per dispatch, each scaled by what it
+    reads for the 256 MiB calibration copy) and the SQ counters collected."""
+    import csv
+    import glob
+    import collections
+    vals = collections.defaultdict(lambda: collections.defaultdict(list))
+    for d in dirs:
+        for fn in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(fn) as f:
+                for r in csv.DictReader(f):
+                    cls = kernel_class(r["Kernel_Name"])
+                    if cls:
+                        vals[cls][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    mean = {c: {k: sum(v) / len(v) for k, v in d.items()} for c, d in vals.items()}
+    cal = mean.get("calibration_copy", {})
+    copy_bytes = 256 << 20
+    fs = copy_bytes / (cal["FETCH_SIZE"] * 1024) if "FETCH_SIZE" in cal else None
+    ws = copy_bytes / (cal["WRITE_SIZE"] * 1024) if "WRITE_SIZE" in cal else None
+    pl = 384 * 1248 * 192
+    out = {"workload": "stereo_fb_384x1248_d192",
+           "command": "cd /tmp && export TMPDIR=/tmp && rocprofv3 --pmc <counters> --output-format csv "
+                      "-d $REPO/gpurun_out/pmc_<name> -o run -- python3 $REPO/bench.py --pmc  (one run per "
+                      "counter set: FETCH_SIZE; WRITE_SIZE; SQ counters), then "
+                      "python3 bench.py --pmc-summary gpurun_out/pmc_*",
+           "calibration": {"kernel": "__amd_rocclr_copyBuffer (torch copy_ of 256 MiB fp32)",
+                           "bytes_read": copy_bytes, "bytes_written": copy_bytes,
+                           "FETCH_SIZE_KiB": cal.get("FETCH_SIZE"), "WRITE_SIZE_KiB": cal.get("WRITE_SIZE"),
+                           "fetch_scale": round(fs, 4) if fs else None, "write_scale": round(ws, 4) if ws else None},
+           "kernels": {}}
+    canon = {k: v[0] + v[1] / 192 for k, v in PHASE_B_PL.items()}
+    rows = []
+    for cls in sorted(mean):
+        if cls == "calibration_copy":
+            continue
+        m = mean[cls]
+        k = {"launches_sampled": len(next(iter(vals[cls].values())))}
+        if fs and ws and "FETCH_SIZE" in m and "WRITE_SIZE" in m:
+            fb, wb = m["FETCH_SIZE"] * 1024 * fs, m["WRITE_SIZE"] * 1024 * ws
+            k.update({"fetch_bytes_per_launch": int(fb), "write_bytes_per_launch": int(wb),
+                      "hbm_bytes_per_launch": int(fb + wb)})
+            if cls.startswith("lbp"):
+                k["hbm_bytes_per_pl"] = round((fb + wb) / pl, 3)
+                if cls in canon:
+                    k["canonical_bytes_per_pl"] = round(canon[cls], 3)
+        k["sq"] = {c: int(v) for c, v in m.items() if c.startswith("SQ_")}
+        out["kernels"][cls] = k
+        rows.append((cls, k))
+    with open(TRAFFIC_FILE, "w") as f:
+        json.dump(out, f, indent=1)
+    print(f"calibration: fetch x{out['calibration']['fetch_scale']}, write x{out['calibration']['write_scale']}")
+    print("| kernel class | HBM bytes / launch | fetch | write | B/pl measured | B/pl canonical |")
+    print("|---|---:|---:|---:|---:|---:|")
+    for cls, k in rows:
+        if "hbm_bytes_per_launch" in k:
+            print(f"| {cls} | {k['hbm_bytes_per_launch'] / 1e9:.3f} GB | {k['fetch_bytes_per_launch'] / 1e9:.3f} GB | "
+                  f"{k['write_bytes_per_launch'] / 1e9:.3f} GB | {k.get('hbm_bytes_per_pl', '')} | "
+                  f"{k.get('canonical_bytes_per_pl', '')} |")
+    print("| kernel class | waves | VALU / wave | SALU / wave | cycles / wave |")
+    print("|---|---:|---:|---:|---:|")
+    for cls, k in rows:
+        q = k["sq"]
+        if "SQ_WAVES" in q and q["SQ_WAVES"]:
+            w = q["SQ_WAVES"]
+            print(f"| {cls} | {w} | {q.get('SQ_INSTS_VALU', 0) / w:.0f} | {q.get('SQ_INSTS_SALU', 0) / w:.0f} | "
+                  f"{q.get('SQ_WAVE_CYCLES', 0) / w:.0f} |")
+
+
 def pmc_run(plan, torch):
     """One warm-up step, one step and a 256 MiB device-to-device copy, so that a
     rocprofv3 --pmc pass sees every kernel class of the step once per iteration and a
@@ -424,6 +518,9 @@ def pmc_run(plan, torch):
 
 def main():
     args = parse()
+    if args.pmc_summary:
+        pmc_summary(args.pmc_summary)
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py: this build runs on exactly one GPU; the multi-GPU path is not built "
