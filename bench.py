@@ -167,11 +167,16 @@ def cpu_baseline(wl, plan_shape, seconds, iters_per_sample=1):
     # scale the crop so the timed sample takes about `seconds`
     rows = int(max(4, min(H, rows * seconds / max(t, 1e-3))))
     t = run(rows)
+    # the whole image takes less than the budget: repeat it (fresh inputs each time) and average
+    reps = max(1, int(seconds / max(t, 1e-3))) if rows == H else 1
+    if reps > 1:
+        t = sum(run(rows) for _ in range(reps)) / reps
     pl = N * rows * W * C
     return {"value": pl / t / 1e9, "unit": "G(pixel*label)/s per LBP iter fwd+bwd", "cores": threads,
             "kind": "port",
             "sample": f"C++/OpenMP fp32 restatement (oracle/c), 1 iteration fwd+exact bwd on a "
-                      f"{rows}x{W} crop with {C} labels ({pl / 1e6:.1f} M pl, {t:.1f} s)"}
+                      f"{rows}x{W} image with {C} labels ({pl / 1e6:.1f} M pl), mean of {reps} runs "
+                      f"of {t:.2f} s ({reps * t:.0f} s of CPU time)"}
 
 
 class LBPPlan:
