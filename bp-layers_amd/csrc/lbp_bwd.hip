@@ -967,26 +967,53 @@ __global__ void lbp_bwd_combine(const float *__restrict__ XL, const float *__res
   }
 }
 
-// Sum of the per-(wave, segment) C x C partials of each direction (fixed order).
-__global__ void lbp_bwd_reduce_partials(const float *__restrict__ partial, float *__restrict__ out,
-                                        int C, int nparts_per_dir_v, int nparts_per_dir_h,
-                                        int64_t v_base, int64_t h_base, int N, int wpg_v, int wpg_h, int spw) {
-  // partials are laid out in launch order: phase V then phase H; inside a phase block b
-  // handles (group g = b / wpg, with direction bit g & 1), segments contiguous.
-  const int d = blockIdx.y;
+// Sum of the per-(wave, segment) C x C partials of each direction, in two deterministic stages.
+// The partials are laid out in launch order: phase V then phase H; inside a phase, block b
+// handles (group g = b / wpg, with direction bit g & 1), segments contiguous.  The list of a
+// direction (its groups n = 0..N-1, their blocks and segments in order) is cut into
+// kRedChunks chunks; stage 1 sums each chunk with 4 lanes per matrix entry (lane j takes the
+// chunk's entries j, j + 4, ...; the 4 sums are combined as (s0 + s1) + (s2 + s3)); stage 2 adds
+// the chunk sums in chunk order.
+constexpr int kRedChunks = 16;
+__global__ __launch_bounds__(256) void lbp_bwd_reduce_partials(const float *__restrict__ partial,
+                                                               float *__restrict__ chunk_sums, int C,
+                                                               int64_t v_base, int64_t h_base, int N, int wpg_v,
+                                                               int wpg_h, int spw) {
+  __shared__ float part[4][64];
+  const int d = blockIdx.y, k = blockIdx.z;
   const bool vert = d == UP || d == DOWN;
   const int dsel = vert ? (d == UP) : (d == LEFT);
   const int wpg = vert ? wpg_v : wpg_h;
   const int64_t base = vert ? v_base : h_base;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < C * C; i += gridDim.x * blockDim.x) {
-    float s = 0.f;
-    for (int nn = 0; nn < N; ++nn) {
-      const int g = nn * 2 + dsel;
-      for (int b = g * wpg; b < (g + 1) * wpg; ++b)
-        for (int sg = 0; sg < spw; ++sg) s += partial[base + ((int64_t)b * spw + sg) * C * C + i];
+  const int CC = C * C;
+  const int e = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = threadIdx.x >> 6;
+  const int per_n = wpg * spw;             // entries of one group in the list
+  const int T = N * per_n;                 // entries of the direction
+  const int i0 = (int)((int64_t)T * k / kRedChunks), i1 = (int)((int64_t)T * (k + 1) / kRedChunks);
+  float sum = 0.f;
+  if (e < CC) {
+    for (int i = i0 + j; i < i1; i += 4) {
+      const int nn = i / per_n, r = i - nn * per_n;  // group n, then (block, segment) inside it
+      const int64_t slot = (int64_t)(nn * 2 + dsel) * per_n + r;
+      sum += partial[base + slot * CC + e];
     }
-    out[(int64_t)d * C * C + i] = s;
   }
+  part[j][threadIdx.x & 63] = sum;
+  __syncthreads();
+  if (j == 0 && e < CC) {
+    const int l = threadIdx.x & 63;
+    chunk_sums[((int64_t)d * kRedChunks + k) * CC + e] = (part[0][l] + part[1][l]) + (part[2][l] + part[3][l]);
+  }
+}
+__global__ void lbp_bwd_reduce_chunks(const float *__restrict__ chunk_sums, float *__restrict__ out, int C) {
+  const int CC = C * C;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 4 * CC) return;
+  const int d = i / CC, e = i - d * CC;
+  float s = 0.f;
+  for (int k = 0; k < kRedChunks; ++k) s += chunk_sums[((int64_t)d * kRedChunks + k) * CC + e];
+  out[i] = s;
 }
 
 template <int KIND, int SEG, int LPL>
@@ -1072,7 +1099,7 @@ void ws_layout(int kind, int64_t N, int64_t H, int64_t W, int64_t C, int64_t *pl
     const int spw = kWave / seg_width((int)C);
     const int64_t wv = N * 2 * ((W + spw - 1) / spw), wh = N * 2 * ((H + spw - 1) / spw);
     pv = wv * spw * C * C;
-    ph = wh * spw * C * C;
+    ph = wh * spw * C * C + 4 * kRedChunks * C * C;  // then the reduction's chunk sums
   }
   *part_v = pv;
   *part_h = ph;
@@ -1178,9 +1205,14 @@ extern "C" int bpl_lbp_bwd(int kind, int mode, const float *jump, int64_t jump_k
     const int spw = kWave / seg_width((int)C);
     const int wpg_v = (int)((W + spw - 1) / spw), wpg_h = (int)((H + spw - 1) / spw);
     KernelTimer kt(BPL_KERNEL_LBP_BWD_REDUCE, st);
-    hipLaunchKernelGGL(lbp_bwd_reduce_partials, dim3((unsigned)((C * C + 255) / 256), 4), dim3(256), 0, st,
-                       ws + planes, grad_jump, (int)C, 0, 0, (int64_t)0, pv, (int)N, wpg_v, wpg_h, spw);
+    float *chunk_sums = ws + planes + pv + (ph - 4 * kRedChunks * C * C);
+    hipLaunchKernelGGL(lbp_bwd_reduce_partials, dim3((unsigned)((C * C + 63) / 64), 4, kRedChunks), dim3(256), 0, st,
+                       ws + planes, chunk_sums, (int)C, (int64_t)0, pv, (int)N, wpg_v, wpg_h, spw);
     int rc = check_hip(hipGetLastError(), "bpl_lbp_bwd reduce launch");
+    if (rc) return rc;
+    hipLaunchKernelGGL(lbp_bwd_reduce_chunks, dim3((unsigned)((4 * C * C + 255) / 256)), dim3(256), 0, st,
+                       chunk_sums, grad_jump, (int)C);
+    rc = check_hip(hipGetLastError(), "bpl_lbp_bwd reduce launch");
     if (rc) return rc;
   }
   return BPL_OK;

@@ -269,6 +269,147 @@ __global__ __launch_bounds__(64) void lbp_fwd_phase(FwdArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------------------
+// Semantic global pairwise, full min-plus branch (delta >= C), C <= CM (20 or 32): two
+// scanlines per wave (32 lanes each, label c = lane & 31).  Each lane keeps the column
+// M[dir][.][c] of its label in registers (CM values, 0 past C); per step the segment's h row
+// goes through LDS (+inf past C) and comes back as wave-uniform vector reads, the CM
+// candidates h[l] + w M[l][c] form a min3 tree, and the winner is the last l equal to the
+// minimum (the reference's `<=` scan from msg = 1e15 in label order).  Normalisation and
+// outputs as in lbp_fwd_phase (max-shifted log-sum-exp, scale = exp(lse), label of the
+// minimum 0 in this branch).
+// ---------------------------------------------------------------------------------------
+template <int CM, bool VERT, int PF>
+__global__ __launch_bounds__(64) void lbp_fwd_semg(FwdArgs a) {
+  constexpr int SEG = 32, SPW = 2;
+  __shared__ float lds_h[SPW][CM];
+  const int H = a.H, W = a.W, C = a.C;
+  const int L = VERT ? W : H;
+  const int len = VERT ? H : W;
+  const int nsteps = len - 1;
+  const int wpg = (L + SPW - 1) / SPW;
+  const int g = blockIdx.x / wpg;
+  const int wl = blockIdx.x - g * wpg;
+  const int n = g >> 1;
+  const int dir = VERT ? ((g & 1) ? UP : DOWN) : ((g & 1) ? LEFT : RIGHT);
+  const int rslot = dir == RIGHT ? LEFT : dir == LEFT ? RIGHT : dir == DOWN ? UP : DOWN;
+  const bool fwd = (dir == RIGHT || dir == DOWN);
+  const int lane = threadIdx.x;
+  const int seg = lane / SEG;
+  const int c = lane - seg * SEG;
+  const int line = wl * SPW + seg;
+  const bool line_ok = line < L;
+  const int ln = line_ok ? line : 0;
+  const bool lab = c < C;
+  const bool st_ok = line_ok && lab;
+  const int cl = lab ? c : C - 1;  // label loaded by this lane
+  const int64_t HW = (int64_t)H * W;
+  const int64_t dq = VERT ? (fwd ? (int64_t)W : -(int64_t)W) : (fwd ? 1 : -1);
+  const int64_t q0 = VERT ? (int64_t)(fwd ? 0 : H - 1) * W + ln : (int64_t)ln * W + (fwd ? 0 : W - 1);
+  const float *cost_p = a.cost + (int64_t)n * HW * C;
+  const float *x1_p = (VERT ? a.msg_out : a.msg_in) + ((int64_t)n * 4 + (VERT ? RIGHT : UP)) * HW * C;
+  const float *x2_p = (VERT ? a.msg_out : a.msg_in) + ((int64_t)n * 4 + (VERT ? LEFT : DOWN)) * HW * C;
+  float *out_p = a.msg_out + ((int64_t)n * 4 + rslot) * HW * C;
+  uint8_t *arg_p = a.argmin ? a.argmin + ((int64_t)n * 4 + rslot) * HW * C : nullptr;
+  const int64_t rs_base = ((int64_t)n * 4 + rslot) * HW;
+  const float *edge_p = a.edge + ((int64_t)n * 4 + dir) * HW;
+
+  // this label's column of the direction's matrix
+  float Mc[CM];
+#pragma unroll
+  for (int l = 0; l < CM; ++l) Mc[l] = (l < C && lab) ? a.jump[((int64_t)dir * C + l) * C + c] : 0.f;
+
+  // the first sender's chain slot is never written by this pass
+  float chain = a.msg_in[(rs_base + q0) * C + cl];
+  if (st_ok) {
+    out_p[q0 * C + c] = chain;
+    if (arg_p) arg_p[q0 * C + c] = 0;
+  }
+  if (line_ok && c == 0) {
+    if (a.scale) a.scale[rs_base + q0] = 0.f;
+    if (a.gargmin) a.gargmin[rs_base + q0] = 0;
+  }
+
+  float pc[PF], p1[PF], p2[PF], pw[PF];
+  auto load_step = [&](int s, int i) {
+    const int64_t q = q0 + (int64_t)s * dq;
+    pc[i] = cost_p[q * C + cl];
+    p1[i] = x1_p[q * C + cl];
+    p2[i] = x2_p[q * C + cl];
+    pw[i] = edge_p[q];
+  };
+#pragma unroll
+  for (int i = 0; i < PF; ++i) {
+    load_step(i < nsteps ? i : (nsteps > 0 ? nsteps - 1 : 0), i);
+    __builtin_amdgcn_sched_barrier(0);  // keep the ring slots in issue order
+  }
+  for (int s0 = 0; s0 < nsteps; s0 += PF) {
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      const int s = s0 + i;
+      if (s < nsteps) {
+        float h = VERT ? ((pc[i] + p1[i]) + p2[i]) + chain : ((pc[i] + chain) + p1[i]) + p2[i];
+        if (c < CM) lds_h[seg][c] = lab ? h : __builtin_inff();
+        wave_lds_sync();
+        const float w = pw[i];
+        float nv[CM];
+#pragma unroll
+        for (int l = 0; l < CM; ++l) nv[l] = fmaf(Mc[l], w, lds_h[seg][l]);
+        wave_lds_sync();  // the reads are done before the next step's write
+        // minimum: a min3 tree over the candidates and the initial 1e15
+        float m[CM + 1];
+#pragma unroll
+        for (int l = 0; l < CM; ++l) m[l] = nv[l];
+        m[CM] = 1e15f;
+        int cnt = CM + 1;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+          if (cnt > 1) {
+            int o = 0;
+#pragma unroll
+            for (int t = 0; t < CM + 1; t += 3) {
+              if (t < cnt) {
+                float v = m[t];
+                if (t + 1 < cnt) v = fminf(v, m[t + 1]);
+                if (t + 2 < cnt) v = fminf(v, m[t + 2]);
+                m[o++] = v;
+              }
+            }
+            cnt = o;
+          }
+        }
+        const float msg = m[0];
+        int arg = 0;
+#pragma unroll
+        for (int l = 0; l < CM; ++l) arg = nv[l] == msg ? l : arg;
+        // normalisation: m - log sum_c exp(m), max-shifted
+        float mx = seg_reduce<SEG>(lab ? msg : -__builtin_inff(), OpMax());
+        float se = seg_reduce<SEG>(lab ? fast_exp(msg - mx) : 0.f, OpSum());
+        const float lse = mx + fast_log(se);
+        chain = msg - lse;
+        const int64_t qq = q0 + (int64_t)(s + 1) * dq;
+        if (st_ok) {
+          out_p[qq * C + c] = chain;
+          if (arg_p) arg_p[qq * C + c] = (uint8_t)arg;
+        }
+        if (line_ok && c == 0) {
+          if (a.scale) a.scale[rs_base + qq] = fast_exp(lse);
+          if (a.gargmin) a.gargmin[rs_base + qq] = 0;
+        }
+      }
+      load_step(s + PF < nsteps ? s + PF : nsteps - 1, i);
+    }
+  }
+}
+
+template <int CM>
+void launch_semg(const FwdArgs &a, bool vert, hipStream_t st) {
+  const int L = vert ? a.W : a.H;
+  const dim3 grid((unsigned)(a.N * 2 * ((L + 1) / 2)));
+  if (vert) hipLaunchKernelGGL((lbp_fwd_semg<CM, true, 4>), grid, dim3(kWave), 0, st, a);
+  else hipLaunchKernelGGL((lbp_fwd_semg<CM, false, 4>), grid, dim3(kWave), 0, st, a);
+}
+
 
 // ---------------------------------------------------------------------------------------
 // Stereo fast path (label-difference pairwise, delta <= 3): labels are contiguous per lane
@@ -1144,7 +1285,11 @@ extern "C" int bpl_lbp_fwd(int kind, const float *cost, const float *jump, int64
         if (stereo_dpp_ok(a) && !generic_only()) launch_stereo_dpp_any(a, vert, st);
         else launch_kind<BPL_LBP_STEREO>(a, vert, st, shm);
         break;
-      case BPL_LBP_SEMANTIC_GLOBAL: launch_kind<BPL_LBP_SEMANTIC_GLOBAL>(a, vert, st, shm); break;
+      case BPL_LBP_SEMANTIC_GLOBAL:
+        if (a.delta >= a.C && a.C <= 20 && !generic_only()) launch_semg<20>(a, vert, st);
+        else if (a.delta >= a.C && a.C <= 32 && !generic_only()) launch_semg<32>(a, vert, st);
+        else launch_kind<BPL_LBP_SEMANTIC_GLOBAL>(a, vert, st, shm);
+        break;
       default: launch_kind<BPL_LBP_SEMANTIC_PIXEL>(a, vert, st, shm); break;
     }
     int rc = check_hip(hipGetLastError(), "bpl_lbp_fwd launch");

@@ -80,6 +80,8 @@ CASES = [
     ("stereo", 1, 3, 2, 64, 3, {}),
     ("semantic_global", 2, 5, 6, 19, 19, {}),
     ("semantic_global", 1, 4, 5, 19, 3, {}),
+    ("semantic_global", 1, 3, 30, 25, 25, {}),
+    ("semantic_global", 2, 7, 5, 32, 32, {}),
     ("semantic_pixel", 1, 4, 5, 7, 7, {}),
     ("semantic_pixel", 2, 4, 5, 19, 19, {}),
 ]

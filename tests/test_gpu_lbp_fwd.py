@@ -99,6 +99,9 @@ CASES = [
     ("semantic_global", 2, 6, 7, 19, 19, {}),
     ("semantic_global", 1, 5, 6, 19, 3, {}),
     ("semantic_global", 1, 4, 5, 70, 70, {}),
+    ("semantic_global", 1, 3, 40, 25, 25, {}),
+    ("semantic_global", 2, 9, 6, 32, 32, {}),
+    ("semantic_global", 1, 5, 7, 7, 7, {}),
     ("semantic_pixel", 1, 5, 6, 7, 7, {}),
     ("semantic_pixel", 1, 4, 5, 19, 19, {}),
 ]
