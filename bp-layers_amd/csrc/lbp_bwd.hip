@@ -917,6 +917,164 @@ __global__ __launch_bounds__(PAIR ? 2 * kWave : kWave) void lbp_bwd_stereo_w64(B
   }
 }
 
+// ---------------------------------------------------------------------------------------
+// Semantic global pairwise, 16 < C <= CM (20 or 32): the adjoint of one pass with
+// lbp_bwd_phase's semantics, for either forward branch.  Two scanlines per wave, label
+// c = lane & 31.  Per step the receiver gradients g(c) and their routing targets (the argmin,
+// or the label of the minimum for a truncation-term winner) go through LDS; sender label l
+// gathers g(c) over the receivers routed to it, in label order.  The matrix gradient entry
+// [r][c] += w g(c) (r the argmin; a truncation winner credits [0][C-1] in the exact mode, as a
+// segment sum added by the lane owning column C-1, and [label of the minimum][c] in the
+// reference mode) accumulates in an LDS copy of the segment's partial matrix; column c is
+// touched only by lane c, so the sums are ordered and deterministic.  The partial is written
+// once at the end for lbp_bwd_reduce_partials.
+// ---------------------------------------------------------------------------------------
+template <int CM, bool VERT, int PF>
+__global__ __launch_bounds__(64) void lbp_bwd_semg(BwdArgs a) {
+  constexpr int SEG = 32, SPW = 2;
+  __shared__ float lds_g[SPW][CM];
+  __shared__ int lds_a[SPW][CM];
+  __shared__ float lds_mat[CM * CM];       // the direction's matrix [r][c], stride CM
+  __shared__ float lds_part[SPW][CM * CM];  // the segment's matrix-gradient partial [r][c]
+  const int H = a.H, W = a.W, C = a.C;
+  const int L = VERT ? W : H;
+  const int len = VERT ? H : W;
+  const int nsteps = len - 1;
+  const int wpg = (L + SPW - 1) / SPW;
+  const int g_ = blockIdx.x / wpg;
+  const int wl = blockIdx.x - g_ * wpg;
+  const int n = a.n_base + (g_ >> 1);
+  const int dir = VERT ? ((g_ & 1) ? UP : DOWN) : ((g_ & 1) ? LEFT : RIGHT);
+  const int rslot = dir == RIGHT ? LEFT : dir == LEFT ? RIGHT : dir == DOWN ? UP : DOWN;
+  const bool fwd = (dir == RIGHT || dir == DOWN);
+  const int lane = threadIdx.x;
+  const int seg = lane / SEG;
+  const int c = lane - seg * SEG;
+  const int line = wl * SPW + seg;
+  const bool line_ok = line < L;
+  const int ln = line_ok ? line : 0;
+  const bool lab = c < C;
+  const bool st_ok = line_ok && lab;
+  const int cl = lab ? c : C - 1;
+  const int64_t HW = (int64_t)H * W;
+  const int64_t dq = VERT ? (fwd ? (int64_t)W : -(int64_t)W) : (fwd ? 1 : -1);
+  const int64_t q0 = VERT ? (int64_t)(fwd ? 0 : H - 1) * W + ln : (int64_t)ln * W + (fwd ? 0 : W - 1);
+  const int64_t img = (int64_t)n * HW * C;
+  const int64_t rsl = ((int64_t)n * 4 + rslot) * HW;
+  float *Xo = a.X[dir] + img;
+  const float *XDp = a.X[DOWN] + img;
+  const float *XUp = a.X[UP] + img;
+  const float *ig_p = a.in_grad + rsl * C;
+  const float *m_p = a.msgs + rsl * C;
+  const uint8_t *am_p = a.argmin + rsl * C;
+  const uint8_t *gl_p = a.gargmin + rsl;
+  const float *edge_p = a.edge + ((int64_t)n * 4 + dir) * HW;
+  float *ge_p = a.grad_edge + ((int64_t)n * 4 + dir) * HW;
+  const bool exact = a.mode == BPL_GRAD_EXACT;
+
+  // the last position of the pass sends nothing: zero its outputs
+  {
+    const int64_t q = q0 + (int64_t)nsteps * dq;
+    if (st_ok) Xo[q * C + c] = 0.f;
+    if (line_ok && c == 0) ge_p[q] = 0.f;
+  }
+  const float *Md = a.jump + (int64_t)dir * C * C;
+  for (int i = lane; i < CM * CM; i += kWave) {
+    const int r = i / CM, cc = i - r * CM;
+    lds_mat[i] = (r < C && cc < C) ? Md[r * C + cc] : 0.f;
+    lds_part[0][i] = 0.f;
+    lds_part[1][i] = 0.f;
+  }
+  wave_lds_sync();
+
+  float pg[PF], pm[PF], pd[PF], pu[PF], pw[PF];
+  int pa[PF], pl[PF];
+  auto load_step = [&](int s, int i) {
+    const int64_t q = q0 + (int64_t)s * dq;
+    const int64_t qr = q + dq;
+    pg[i] = ig_p[qr * C + cl];
+    pm[i] = m_p[qr * C + cl];
+    pa[i] = (int)am_p[qr * C + cl];
+    pl[i] = (int)gl_p[qr];
+    if (!VERT) {
+      pd[i] = XDp[qr * C + cl];
+      pu[i] = XUp[qr * C + cl];
+    } else {
+      pd[i] = pu[i] = 0.f;
+    }
+    pw[i] = edge_p[q];
+  };
+  if (nsteps > 0) {
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      const int s = nsteps - 1 - i;
+      load_step(s >= 0 ? s : 0, i);
+      __builtin_amdgcn_sched_barrier(0);  // keep the ring slots in issue order
+    }
+  }
+  float acc = 0.f;
+  for (int t0 = 0; t0 < nsteps; t0 += PF) {
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      const int t = t0 + i;
+      const int s = nsteps - 1 - t;  // adjoint order
+      if (t < nsteps) {
+        const int64_t q = q0 + (int64_t)s * dq;
+        // normalisation Jacobian: g = t - softmax(msg) * sum(t), t = in_grad + chain
+        const float tt = pg[i] + acc;
+        const float ts = seg_reduce<SEG>(lab ? tt : 0.f, OpSum());
+        float gk = tt - fast_exp(pm[i]) * ts;
+        if (!VERT) gk += pd[i] + pu[i];
+        gk = lab ? gk : 0.f;
+        const int A = pa[i];
+        const int gl = pl[i];
+        const bool tr = A == kFromTrunc;   // the value came from the truncation term
+        const int r = tr ? gl : A;         // routing target and reference-mode matrix row
+        if (c < CM) {
+          lds_g[seg][c] = gk;
+          lds_a[seg][c] = lab ? r : -1;
+        }
+        wave_lds_sync();
+        // route: sender label c gathers the receivers routed to it, in label order
+        float gh = 0.f;
+#pragma unroll
+        for (int rr = 0; rr < CM; ++rr) gh += lds_a[seg][rr] == c ? lds_g[seg][rr] : 0.f;
+        // parameter gradients of (dir, sender): edge sum_c M[r][c] g(c) and matrix entry
+        // [r][c] (a truncation winner uses the truncation entry [0][C-1] in the exact mode)
+        const float w = pw[i];
+        const bool to_tr = tr && exact;
+        const float mv = to_tr ? lds_mat[C - 1] : lds_mat[r * CM + cl];
+        const float ge = seg_reduce<SEG>(lab ? mv * gk : 0.f, OpSum());
+        if (line_ok && c == 0) ge_p[q] = ge;
+        if (st_ok && !to_tr) lds_part[seg][r * CM + c] += w * gk;
+        if (exact) {
+          const float fsum = seg_reduce<SEG>(lab && tr ? gk : 0.f, OpSum());
+          if (line_ok && c == C - 1) lds_part[seg][C - 1] += w * fsum;
+        }
+        wave_lds_sync();  // the gathers are done before the next step's writes
+        acc = gh;
+        if (st_ok) Xo[q * C + c] = gh;
+      }
+      load_step(s - PF >= 0 ? s - PF : 0, i);
+    }
+  }
+  // the segment's partial matrix, in lbp_bwd_reduce_partials' layout
+  wave_lds_sync();
+  float *part = a.partial + ((int64_t)blockIdx.x * SPW + seg) * C * C;
+  for (int i = c; i < C * C; i += SEG) {
+    const int r = i / C, cc = i - r * C;
+    part[i] = lds_part[seg][r * CM + cc];
+  }
+}
+
+template <int CM>
+void launch_semg(const BwdArgs &a, bool vert, int ngroups, hipStream_t st) {
+  const int L = vert ? a.W : a.H;
+  const dim3 grid((unsigned)(ngroups * 2 * ((L + 1) / 2)));
+  if (vert) hipLaunchKernelGGL((lbp_bwd_semg<CM, true, 4>), grid, dim3(kWave), 0, st, a);
+  else hipLaunchKernelGGL((lbp_bwd_semg<CM, false, 4>), grid, dim3(kWave), 0, st, a);
+}
+
 // y-boundary terms of the input-message gradient after the paired horizontal phase (which
 // wrote X_R + X_L into the U and D slots): U slot of row 0 = (in_grad_U + X_D) + (X_R + X_L),
 // D slot of row H - 1 = (in_grad_D + X_U) + (X_R + X_L), the combine kernel's expressions.
@@ -1089,6 +1247,9 @@ void launch_stereo(const BwdArgs &a, bool vert, bool pair, int ngroups, hipStrea
 }
 
 int seg_width(int C) { return C <= 16 ? 16 : C <= 32 ? 32 : 64; }
+// the semantic-global kernel for 16 < C <= 32: its partials use the 32-lane segments of the
+// generic layout
+bool semg_ok(const BwdArgs &a) { return a.C > 16 && a.C <= 32 && !generic_only(); }
 
 void ws_layout(int kind, int64_t N, int64_t H, int64_t W, int64_t C, int64_t *planes, int64_t *part_v,
                int64_t *part_h, int64_t *total) {
@@ -1171,7 +1332,11 @@ extern "C" int bpl_lbp_bwd(int kind, int mode, const float *jump, int64_t jump_k
       const int ng = per_n ? 1 : (int)N;
       switch (kind) {
         case BPL_LBP_STEREO: launch_stereo(a, vert, pair, ng, st); break;
-        case BPL_LBP_SEMANTIC_GLOBAL: launch_kind<BPL_LBP_SEMANTIC_GLOBAL>(a, vert, ng, st); break;
+        case BPL_LBP_SEMANTIC_GLOBAL:
+          if (semg_ok(a) && C <= 20) launch_semg<20>(a, vert, ng, st);
+          else if (semg_ok(a)) launch_semg<32>(a, vert, ng, st);
+          else launch_kind<BPL_LBP_SEMANTIC_GLOBAL>(a, vert, ng, st);
+          break;
         default: launch_kind<BPL_LBP_SEMANTIC_PIXEL>(a, vert, ng, st); break;
       }
       int rc = check_hip(hipGetLastError(), "bpl_lbp_bwd phase launch");
