@@ -304,7 +304,7 @@ __global__ __launch_bounds__(64) void lbp_bwd_phase(BwdArgs a) {
 }
 
 // ---------------------------------------------------------------------------------------
-// Stereo backward for 32 < C <= 256 with K <= 8: one scanline per wave, label
+// Stereo backward for 16 < C <= 256 with K <= 8: one scanline per wave, label
 // c = lane * LPL + k (the forward's layout).  Same adjoint as lbp_bwd_phase; per step:
 //   * routing: a sender label l collects g(c) of the receiver labels c = l - 3 .. l + 3 whose
 //     argmin is l, reading the neighbour lanes' g and argmin through DPP shifts (an LDS
@@ -1230,8 +1230,8 @@ void launch_stereo_w64_any(const BwdArgs &a, bool vert, bool pair, int ngroups, 
   if (a.C == kWave * LPL) launch_stereo_w64<LPL, true>(a, vert, pair, ngroups, st);
   else launch_stereo_w64<LPL, false>(a, vert, pair, ngroups, st);
 }
-// the 64-lane stereo kernel covers 32 < C <= 256 and up to 8 pairwise bins
-bool stereo_w64_ok(const BwdArgs &a) { return a.C > 32 && a.K <= 8; }
+// the 64-lane stereo kernel covers 16 < C <= 256 and up to 8 pairwise bins
+bool stereo_w64_ok(const BwdArgs &a) { return a.C > 16 && a.K <= 8; }
 // exact-mode stereo backward whose horizontal phase pairs the two passes of a row and writes
 // grad_cost and the input-message gradient itself (no combine pass)
 bool stereo_pair_ok(const BwdArgs &a) {

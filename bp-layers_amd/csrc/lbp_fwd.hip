@@ -412,311 +412,9 @@ void launch_semg(const FwdArgs &a, bool vert, hipStream_t st) {
 
 
 // ---------------------------------------------------------------------------------------
-// Stereo fast path (label-difference pairwise, delta <= 3): labels are contiguous per lane
-// (c = lane * LPL + k, vector loads), the candidate window comes from neighbouring lanes by
-// wave_shr/wave_shl DPP shifts (no LDS round trip), the global minimum is a value-only
-// all-reduce with a ballot for its label, and the log-sum-exp is shifted by the truncation
-// value, which bounds every message from above.  Steps with a non-zero window offset use an
-// LDS window like the generic kernel.
-// ---------------------------------------------------------------------------------------
-__device__ __forceinline__ float wave_shr1(float v) {  // lane i <- lane i-1
-  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x138, 0xF, 0xF, false));
-}
-__device__ __forceinline__ float wave_shl1(float v) {  // lane i <- lane i+1
-  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x130, 0xF, 0xF, false));
-}
-
-template <int SEG>
-__device__ __forceinline__ float seg_param(float pv, int idx, int seg) {
-  // value held by lane idx of this lane's segment (idx uniform)
-  if constexpr (SEG == 64) {
-    return readlane_f(pv, idx);
-  } else {
-    const float r0 = readlane_f(pv, idx), r1 = readlane_f(pv, 32 + idx);
-    return seg ? r1 : r0;
-  }
-}
-
-template <int SEG, int LPL, bool VERT, int PF>
-__global__ __launch_bounds__(64) void lbp_fwd_stereo_dpp(FwdArgs a) {
-  static_assert(SEG == 64 || (SEG == 32 && LPL == 1), "layout");
-  constexpr int SPW = kWave / SEG;
-  __shared__ float lds_h[2][SPW][SEG * LPL];
-
-  const int H = a.H, W = a.W, C = a.C, K = a.K;
-  const int L = VERT ? W : H;
-  const int len = VERT ? H : W;
-  const int nsteps = len - 1;
-  const int wpg = (L + SPW - 1) / SPW;
-  const int g = blockIdx.x / wpg;
-  const int wl = blockIdx.x - g * wpg;
-  const int n = g >> 1;
-  const int dsel = g & 1;
-  const int dir = VERT ? (dsel ? UP : DOWN) : (dsel ? LEFT : RIGHT);
-  const int rslot = dir == RIGHT ? LEFT : dir == LEFT ? RIGHT : dir == DOWN ? UP : DOWN;
-  const bool fwd = (dir == RIGHT || dir == DOWN);
-  const int lane = threadIdx.x;
-  const int seg = lane / SEG;
-  const int sl = lane - seg * SEG;
-  const int line = wl * SPW + seg;
-  const bool line_ok = line < L;
-  const int ln = line_ok ? line : 0;
-  const int delta = a.delta;
-  const int nj = a.Nj > 1 ? n : 0;
-  const int s1 = VERT ? RIGHT : UP;
-  const int s2 = VERT ? LEFT : DOWN;
-  const float *cross = VERT ? a.msg_out : a.msg_in;
-  const int c0 = sl * LPL;
-  const bool full = c0 + LPL <= C;
-  const int64_t HW = (int64_t)H * W;
-  const uint64_t segmask = SEG == 64 ? ~0ull : (seg ? 0xFFFFFFFF00000000ull : 0xFFFFFFFFull);
-
-  auto pos_of = [&](int s) { return fwd ? s : len - 1 - s; };
-  auto yx_of = [&](int p, int &y, int &x) {
-    if (VERT) { y = p; x = ln; } else { y = ln; x = p; }
-  };
-  // labels past C are clamped to C-1 so every load is unconditional
-  const int cl0 = full ? c0 : min(c0, C - 1);
-  auto load_row = [&](const float *base, float *d) {
-#pragma unroll
-    for (int k = 0; k < LPL; ++k) d[k] = base[full ? c0 + k : min(cl0 + k, C - 1)];
-  };
-
-  float chain[LPL];
-  {
-    int y, x;
-    yx_of(pos_of(0), y, x);
-    const int64_t o = sl_px(n, rslot, y, x, H, W);
-    load_row(a.msg_in + o * C, chain);
-    if (line_ok) {
-#pragma unroll
-      for (int k = 0; k < LPL; ++k) {
-        const int c = c0 + k;
-        if (c < C) {
-          a.msg_out[o * C + c] = chain[k];
-          if (a.argmin) a.argmin[o * C + c] = 0;
-        }
-      }
-      if (sl == 0) {
-        if (a.scale) a.scale[o] = 0.f;
-        if (a.gargmin) a.gargmin[o] = 0;
-      }
-    }
-  }
-
-  float pc[PF][LPL], p1[PF][LPL], p2[PF][LPL], pp[PF];
-  auto load_step = [&](int s, float *dc, float *d1, float *d2, float &dp) {
-    int y, x;
-    yx_of(pos_of(s), y, x);
-    load_row(a.cost + px(n, y, x, H, W) * C, dc);
-    load_row(cross + sl_px(n, s1, y, x, H, W) * C, d1);
-    load_row(cross + sl_px(n, s2, y, x, H, W) * C, d2);
-    const int j = sl < K ? sl : -1;
-    dp = j >= 0 ? a.jump[(((int64_t)nj * 4 + dir) * K + j) * HW + (int64_t)y * W + x]
-                : a.edge[sl_px(n, dir, y, x, H, W)];
-  };
-
-#pragma unroll
-  for (int i = 0; i < PF; ++i) {
-    load_step(i < nsteps ? i : (nsteps > 0 ? nsteps - 1 : 0), pc[i], p1[i], p2[i], pp[i]);
-    // keep the slots in issue order: the loop's wait counts assume slot 0 is the oldest
-    __builtin_amdgcn_sched_barrier(0);
-  }
-
-  for (int s0 = 0; s0 < nsteps; s0 += PF) {
-#pragma unroll
-    for (int i = 0; i < PF; ++i) {
-      const int s = s0 + i;
-      if (s < nsteps) {
-        float h[LPL];
-#pragma unroll
-        for (int k = 0; k < LPL; ++k)
-          h[k] = VERT ? ((pc[i][k] + p1[i][k]) + p2[i][k]) + chain[k] : ((pc[i][k] + chain[k]) + p1[i][k]) + p2[i][k];
-        const float pv = pp[i];
-        int y, x;
-        yx_of(pos_of(s), y, x);
-
-        // per-pixel pairwise parameters of this segment
-        const float w = seg_param<SEG>(pv, K, seg);
-        const float o = seg_param<SEG>(pv, 0, seg);
-        const float JK = seg_param<SEG>(pv, K - 1, seg);
-        const float Jmid = seg_param<SEG>(pv, K / 2, seg);
-
-        // global minimum (value), then its label by ballot (tree tie order when tied)
-        float hm = __builtin_inff();
-#pragma unroll
-        for (int k = 0; k < LPL; ++k)
-          if (c0 + k < C) hm = fminf(hm, h[k]);
-        const float Hs = seg_reduce<SEG>(hm, OpMin());
-        int lstar = 0;
-        {
-          uint64_t mk[LPL];
-          bool single = true;
-#pragma unroll
-          for (int k = 0; k < LPL; ++k) mk[k] = __ballot(c0 + k < C && h[k] == Hs);
-#pragma unroll
-          for (int sg = 0; sg < SPW; ++sg) {
-            const uint64_t sm = SEG == 64 ? ~0ull : (sg ? 0xFFFFFFFF00000000ull : 0xFFFFFFFFull);
-            int cnt = 0;
-#pragma unroll
-            for (int k = 0; k < LPL; ++k) cnt += __popcll(mk[k] & sm);
-            single = single && cnt <= 1;
-          }
-          if (single) {
-#pragma unroll
-            for (int k = 0; k < LPL; ++k) {
-              const uint64_t m = mk[k] & segmask;
-              if (m) lstar = ((__ffsll((long long)m) - 1) - seg * SEG) * LPL + k;
-            }
-          } else {
-            float bv = __builtin_inff();
-            int bk = -1;
-#pragma unroll
-            for (int k = 0; k < LPL; ++k)
-              if (c0 + k < C) argmin_step(bv, bk, h[k], tie_key(c0 + k, a.bits));
-            seg_argmin<SEG>(bv, bk);
-            lstar = tie_key(bk, a.bits);
-          }
-        }
-        const float gv = fmaf(JK, w, Hs);
-        float msg[LPL];
-        int arg[LPL];
-#pragma unroll
-        for (int k = 0; k < LPL; ++k) { msg[k] = gv; arg[k] = kFromTrunc; }
-
-        // uniform decision: register window (offset 0 everywhere in the wave) or LDS window
-        const bool regwin = __ballot(o != 0.f) == 0 && delta <= 3;
-        if (regwin) {
-          float L1[LPL], L2[LPL], L3[LPL], R1[LPL], R2[LPL], R3[LPL];
-#pragma unroll
-          for (int k = 0; k < LPL; ++k) { L1[k] = wave_shr1(h[k]); R1[k] = wave_shl1(h[k]); }
-#pragma unroll
-          for (int k = 0; k < LPL; ++k) { L2[k] = wave_shr1(L1[k]); R2[k] = wave_shl1(R1[k]); }
-#pragma unroll
-          for (int k = 0; k < LPL; ++k) { L3[k] = wave_shr1(L2[k]); R3[k] = wave_shl1(R2[k]); }
-#pragma unroll
-          for (int kk = -3; kk <= 3; ++kk) {
-            if (kk < -delta || kk > delta) continue;
-            const float fk = seg_param<SEG>(pv, min(max(K / 2 - kk, 1), K - 1), seg);
-#pragma unroll
-            for (int k = 0; k < LPL; ++k) {
-              const int e = k + kk;  // compile-time slot offset from this lane's first label
-              float hv;
-              if (e >= 0 && e < LPL) {
-                hv = h[e >= 0 && e < LPL ? e : 0];
-              } else if (e < 0) {
-                const int j = -e, d = (j + LPL - 1) / LPL, slot = d * LPL - j;
-                hv = d == 1 ? L1[slot] : d == 2 ? L2[slot] : L3[slot];
-              } else {
-                const int j = e - LPL + 1, d = (j + LPL - 1) / LPL, slot = j - 1 - (d - 1) * LPL;
-                hv = d == 1 ? R1[slot] : d == 2 ? R2[slot] : R3[slot];
-              }
-              const int c = c0 + k, l = c + kk;
-              if (c < C && l >= 0 && l < C) {
-                const float nv = fmaf(fk, w, hv);
-                if (nv <= msg[k]) { msg[k] = nv; arg[k] = l; }
-              }
-            }
-          }
-        } else {
-          const int buf = s & 1;
-#pragma unroll
-          for (int k = 0; k < LPL; ++k)
-            if (c0 + k < C) lds_h[buf][seg][c0 + k] = h[k];
-          __syncthreads();
-          int k_lo = -delta, k_hi = delta;
-          const bool any_off = __ballot(o != 0.f) != 0;
-          if (any_off) {
-            // widest window over the wave's segments
-            const float omin = -seg_reduce<64>(-o, OpMax()), omax = seg_reduce<64>(o, OpMax());
-            k_lo = (int)floorf(omin - 0.5f) - delta;
-            k_hi = (int)ceilf(omax + 0.5f) + delta;
-          }
-          int st[LPL], sp[LPL];
-#pragma unroll
-          for (int k = 0; k < LPL; ++k) {
-            const int c = c0 + k;
-            st[k] = sat_u16(roundf((float)(c - delta + 1) + o - 1.0f));
-            sp[k] = min(sat_u16(roundf((float)(c + delta) + o + 1.0f)), C);
-          }
-          for (int kk = k_lo; kk <= k_hi; ++kk) {
-            const float fk = seg_param<SEG>(pv, min(max(K / 2 - kk, 1), K - 1), seg);
-#pragma unroll
-            for (int k = 0; k < LPL; ++k) {
-              const int c = c0 + k, l = c + kk;
-              if (c < C && l >= st[k] && l < sp[k]) {
-                const float nv = fmaf(fk, w, lds_h[buf][seg][l]);
-                if (nv <= msg[k]) { msg[k] = nv; arg[k] = l; }
-              }
-            }
-          }
-        }
-
-        // normalisation shifted by gv >= every message; the label l* term keeps the sum away
-        // from underflow unless (JK - J[K/2]) * w is huge, where the max-shift is used instead
-        float shift = gv;
-        if (__ballot(!((JK - Jmid) * w <= 80.f)) != 0) {
-          float mx = -__builtin_inff();
-#pragma unroll
-          for (int k = 0; k < LPL; ++k)
-            if (c0 + k < C) mx = fmaxf(mx, msg[k]);
-          shift = seg_reduce<SEG>(mx, OpMax());
-        }
-        float se = 0.f;
-#pragma unroll
-        for (int k = 0; k < LPL; ++k)
-          if (c0 + k < C) se += fast_exp(msg[k] - shift);
-        se = seg_reduce<SEG>(se, OpSum());
-        const float lse = shift + fast_log(se);
-
-        int yq = y, xq = x;
-        if (VERT) yq += fwd ? 1 : -1; else xq += fwd ? 1 : -1;
-        const int64_t oq = sl_px(n, rslot, yq, xq, H, W);
-#pragma unroll
-        for (int k = 0; k < LPL; ++k) chain[k] = msg[k] - lse;
-        if (line_ok) {
-          float *mo = a.msg_out + oq * C;
-          if (full) {
-#pragma unroll
-            for (int k = 0; k < LPL; ++k) mo[c0 + k] = chain[k];
-          } else {
-#pragma unroll
-            for (int k = 0; k < LPL; ++k)
-              if (c0 + k < C) mo[c0 + k] = chain[k];
-          }
-          if (a.argmin) {
-#pragma unroll
-            for (int k = 0; k < LPL; ++k)
-              if (c0 + k < C) a.argmin[oq * C + c0 + k] = (uint8_t)arg[k];
-          }
-          if (sl == 0) {
-            if (a.scale) a.scale[oq] = fast_exp(lse);
-            if (a.gargmin) a.gargmin[oq] = (uint8_t)lstar;
-          }
-        }
-      }
-      // refill ring slot i after its last use (see the generic kernel)
-      load_step(s + PF < nsteps ? s + PF : nsteps - 1, pc[i], p1[i], p2[i], pp[i]);
-    }
-  }
-}
-
-template <int SEG, int LPL>
-void launch_stereo_dpp(const FwdArgs &a, bool vert, hipStream_t st) {
-  constexpr int PF = 4;
-  constexpr int SPW = kWave / SEG;
-  const int L = vert ? a.W : a.H;
-  const int wpg = (L + SPW - 1) / SPW;
-  const dim3 grid((unsigned)(a.N * 2 * wpg));
-  if (vert)
-    hipLaunchKernelGGL((lbp_fwd_stereo_dpp<SEG, LPL, true, PF>), grid, dim3(kWave), 0, st, a);
-  else
-    hipLaunchKernelGGL((lbp_fwd_stereo_dpp<SEG, LPL, false, PF>), grid, dim3(kWave), 0, st, a);
-}
-
-// ---------------------------------------------------------------------------------------
-// Stereo fast path for 32 < C <= 256: one scanline per wave, label c = lane * LPL + k.
-// Compared with the kernel above it removes per-label control flow from the step:
+// Stereo fast path for 16 < C <= 256: one scanline per wave, label c = lane * LPL + k, the
+// candidate window from neighbouring lanes by DPP shifts (no LDS round trip), and no
+// per-label control flow in the step:
 //   * labels past C (only when C != 64 * LPL) carry h = +inf, so they never win a minimum,
 //     and the neighbour-lane shifts fill the wave's ends with +inf, so window candidates
 //     outside [0, C) need no range test;
@@ -725,7 +423,7 @@ void launch_stereo_dpp(const FwdArgs &a, bool vert, hipStream_t st) {
 //     swaps;
 //   * rows are addressed as a uniform scanline pointer plus a per-lane 32-bit offset, and a
 //     full label row is one vector load/store per lane.
-// The message, argmin and normalisation semantics are those of the kernel above.
+// The message, argmin and normalisation semantics are those of lbp_fwd_phase.
 // ---------------------------------------------------------------------------------------
 constexpr int kArgTrunc = 64;  // window-offset sentinel: "truncation term won"
 
@@ -1203,12 +901,11 @@ void launch_stereo_w64_any(const FwdArgs &a, bool vert, hipStream_t st) {
   else launch_stereo_w64<LPL, false>(a, vert, st);
 }
 
-bool stereo_dpp_ok(const FwdArgs &a) { return a.C > 16 && a.delta < a.C; }
+bool stereo_w64_ok(const FwdArgs &a) { return a.C > 16 && a.delta < a.C; }
 
-void launch_stereo_dpp_any(const FwdArgs &a, bool vert, hipStream_t st) {
+void launch_stereo_w64_all(const FwdArgs &a, bool vert, hipStream_t st) {
   const int C = a.C;
-  if (C <= 32) launch_stereo_dpp<32, 1>(a, vert, st);
-  else if (C <= 64) launch_stereo_w64_any<1>(a, vert, st);
+  if (C <= 64) launch_stereo_w64_any<1>(a, vert, st);
   else if (C <= 128) launch_stereo_w64_any<2>(a, vert, st);
   else if (C <= 192) launch_stereo_w64_any<3>(a, vert, st);
   else launch_stereo_w64_any<4>(a, vert, st);
@@ -1282,7 +979,7 @@ extern "C" int bpl_lbp_fwd(int kind, const float *cost, const float *jump, int64
     KernelTimer kt(vert ? BPL_KERNEL_LBP_FWD_V : BPL_KERNEL_LBP_FWD_H, st);
     switch (kind) {
       case BPL_LBP_STEREO:
-        if (stereo_dpp_ok(a) && !generic_only()) launch_stereo_dpp_any(a, vert, st);
+        if (stereo_w64_ok(a) && !generic_only()) launch_stereo_w64_all(a, vert, st);
         else launch_kind<BPL_LBP_STEREO>(a, vert, st, shm);
         break;
       case BPL_LBP_SEMANTIC_GLOBAL:
