@@ -352,7 +352,9 @@ __global__ __launch_bounds__(512) void sad_flow_fwd_k(const float *__restrict__ 
   }
 }
 
-// Generic window size: one thread per (pixel), S(u,v) evaluated on the fly, O(K^2 C).
+// Generic window size: one thread per (pixel), S(u,v) evaluated on the fly, O(K^2 C).  The
+// vertical displacements are v in [-h, h] (h = sws / 2): for odd sws the last row of the
+// K x K window (v_idx = K - 1) keeps the fill value, as in the reference's v loop.
 __global__ void sad_flow_fwd_generic_k(const float *__restrict__ f0, const float *__restrict__ f1, float *__restrict__ cv_u,
                                        float *__restrict__ cv_v, float *__restrict__ u_star, float *__restrict__ v_star,
                                        int N, int C, int H, int W, int sws, int ou, int ov, int bu, int bv) {
@@ -373,7 +375,7 @@ __global__ void sad_flow_fwd_generic_k(const float *__restrict__ f0, const float
       for (int ui = 0; ui < K; ++ui) {
         const int xx = x + ou + ui - hs;
         float s = kFill;
-        if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
+        if (vi <= 2 * hs && yy >= 0 && yy < H && xx >= 0 && xx < W) {
           s = 0.f;
           for (int c = 0; c < C; ++c) s += fabsf(F0[(int64_t)c * HW] - F1[(int64_t)c * HW + (int64_t)yy * W + xx]);
         }
@@ -387,35 +389,304 @@ __global__ void sad_flow_fwd_generic_k(const float *__restrict__ f0, const float
 }
 
 // ----------------------------------------------------------------------------- flow bwd
-// df0 at (y,x): 2K terms along the stored argmins.  df1 at (y',x') gathers every source
-// (y,x,k) whose argmin lands on it: for the u-part x = x'-u and y = y'-v with
-// u_star[y,x,u] = v+h; for the v-part y = y'-v and x = x'-u with v_star[y,x,v] = u+h.
-// Terms are summed in a fixed order, so the result is deterministic.
-// One thread per pixel and chunk of FB_CB channels: the argmin reads and the search for the
-// sources landing on the pixel do not depend on the channel, so they run once per chunk while
-// all FB_CB channel sums advance together.  The sources a lane finds (about K per part on
-// average, at different points of the K x K search for different lanes) are first collected in
+// Gradient of the min-projected volumes with the argmins fixed (flow_mp_sad_kernel.cu:126-200).
+// Labels k in [0, 2h] carry gradient (h = sws / 2; for odd sws the last label has none, as
+// the reference's u/v loops run over [-h, h]).  For label k of pixel p the stored argmin s
+// gives the other displacement e = s - h; with (u, v) = (k - h, e) for the u-volume and
+// (e, k - h) for the v-volume, a match q = p + (u, v) inside the image contributes
+//   df0[p] += sign(f0[p] - f1[q]) * g,   df1[q] -= sign(f0[p] - f1[q]) * g.
+// df0 is a gather along the pixel's own argmins.  df1 is a gather too (deterministic, unlike
+// the reference's atomics): for stored argmins inside the window (0 <= s < K) the sources
+// landing on a pixel lie within K - 1 pixels of it, and are found as below; argmins outside
+// the window (only from block offsets) are added afterwards by sad_flow_bwd_far_k.  Terms
+// are summed per pixel in a fixed order: u-volume then v-volume, label ascending, e ascending.
+
+// K <= 32.  Block = 8 x 32 pixels of one image.  Every source a pixel's terms read lies within
+// E = K - 1 - h pixels of it (in-window argmins), so the block works on a region of the tile
+// plus an E halo:
+//   1. df0, per volume: each thread keeps its own terms (region offset of the match and
+//      gradient, per label) in registers; channel chunk by chunk, f1 of the region is staged
+//      in LDS and the own terms are summed from it;
+//   2. df1, per volume: the argmins of the region's sources are read coalesced (lane =
+//      label, 8 pixel columns per step) and OR-ed into per-target landing masks in LDS --
+//      bit s of mask[k][target] says "the source at target - (u, v) with label k and argmin
+//      s lands here"; LDS OR is order-free, so the masks are deterministic.  The first
+//      landing of each label (usually the only one) is resolved into registers, the others
+//      (displacement, label and gradient) into a per-thread LDS list of FM_XCAP entries; then
+//      channel chunk by chunk with f0 of the region staged in LDS, the first landings are
+//      summed label by label, then the listed ones in list order (label, then argmin
+//      ascending), then -- rarely -- the landings past the list, found again from the masks.
+// The u-volume's partial sums pass through df0 / df1 (same thread).  Own terms whose argmin
+// lies outside the window read f1 from global memory in place.
+constexpr int FM_TX = 32, FM_TY = 8, FM_NT = FM_TX * FM_TY, FM_KMAX = 32, FM_CH = 4, FM_XCAP = 12;
+constexpr int FM_NONE = -1;  // own term without a match inside the image
+
+size_t flow_bwd_lds_bytes(int sws, int ch) {
+  const int K = sws + 1, hs = sws / 2, KL = 2 * hs + 1, E = K - 1 - hs;
+  return sizeof(uint32_t) * KL * FM_NT + (sizeof(float) + sizeof(uint32_t)) * FM_XCAP * FM_NT +
+         sizeof(float) * (size_t)ch * (FM_TY + 2 * E) * (FM_TX + 2 * E);
+}
+
+// region rows of nc channel planes (buffer resource over them) -> reg[CH][RR][RC]; a wave
+// takes one row (RC <= 64) per step, 8 steps' loads in flight before their stores; zeros
+// outside the image and past nc channels (dropped loads read 0)
+__device__ __forceinline__ void stage_region(float *reg, __amdgpu_buffer_rsrc_t F, int nc, int nrows, int RR, int RC,
+                                             int ry0, int rx0, int H, int W, int t) {
+  constexpr int NW = FM_NT / 64, B = 8;
+  const int c = t & 63, w = t >> 6;
+  const int xs = rx0 + c;
+  const bool cok = c < RC && xs >= 0 && xs < W;
+  for (int b0 = 0; b0 < nrows; b0 += NW * B) {
+    float v[B];
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      const int jr = b0 + NW * q + w;
+      const int j = jr / RR, r = jr - j * RR;  // wave-uniform
+      const int ys = ry0 + r;
+      const bool ok = cok && jr < nrows && j < nc && ys >= 0 && ys < H;
+      v[q] = buf_ld(F, ok ? (uint32_t)(((j * H + ys) * W + xs) * 4) : kBufDrop, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      const int jr = b0 + NW * q + w;
+      if (c < RC && jr < nrows) reg[jr * RC + c] = v[q];
+    }
+  }
+}
+
+template <int CH>
+__global__ __launch_bounds__(FM_NT) void sad_flow_bwd_lds_k(const float *__restrict__ f0, const float *__restrict__ f1,
+                                                             const float *__restrict__ gu, const float *__restrict__ gv,
+                                                             const float *__restrict__ us, const float *__restrict__ vs,
+                                                             float *__restrict__ df0, float *__restrict__ df1, int C,
+                                                             int H, int W, int sws) {
+  extern __shared__ __align__(16) uint32_t fm_lds[];
+  const int K = sws + 1, hs = sws / 2, KL = 2 * hs + 1, E = K - 1 - hs;
+  const int RR = FM_TY + 2 * E, RC = FM_TX + 2 * E, RN = RR * RC;
+  uint32_t *mask = fm_lds;                                                       // [KL][FM_NT]
+  float *xg = reinterpret_cast<float *>(fm_lds + KL * FM_NT);                    // [FM_XCAP][FM_NT]
+  float *reg = reinterpret_cast<float *>(fm_lds + (KL + FM_XCAP) * FM_NT);       // [CH][RR][RC]
+  uint32_t *xo = reinterpret_cast<uint32_t *>(reg + CH * RN);  // [FM_XCAP][FM_NT]: v+32 | (u+32) << 6 | k << 12
+  const int t = threadIdx.x;
+  const int ty = t / FM_TX, tx = t - ty * FM_TX;
+  const int x0 = blockIdx.x * FM_TX, y0 = blockIdx.y * FM_TY, n = blockIdx.z;
+  const int y = y0 + ty, x = x0 + tx;
+  const int ry0 = y0 - E, rx0 = x0 - E;  // region origin
+  const bool inside = y < H && x < W;
+  const int64_t HW = (int64_t)H * W;
+  const int64_t img = (int64_t)n * HW;
+  const int64_t pix = inside ? (int64_t)y * W + x : 0;
+  const int64_t p = img + pix;
+  const int own = (ty + E) * RC + (tx + E);  // own pixel in the region
+  const int ipix = (int)pix;                 // own pixel in the image
+  // the argmin and gradient volumes of this image as buffers: 32-bit offsets, dropped
+  // (zero-reading) lanes instead of branches around loads
+  const uint32_t vb = (uint32_t)(HW * K * 4);
+
+  // 1. df0: own terms (region offset >= 0, FM_NONE, or -2 - image offset for a match outside
+  // the window)
+  for (int part = 0; part < 2; ++part) {
+    const __amdgpu_buffer_rsrc_t rst = make_rsrc((part ? vs : us) + img * K, vb);
+    const __amdgpu_buffer_rsrc_t rgr = make_rsrc((part ? gv : gu) + img * K, vb);
+    int toff[FM_KMAX];
+    float tg[FM_KMAX];
+    bool far = false;
+#pragma unroll
+    for (int k = 0; k < FM_KMAX; ++k) {
+      const uint32_t o = inside && k < KL ? (uint32_t)((ipix * K + k) * 4) : kBufDrop;
+      const int sv = (int)buf_ld(rst, o, 0);
+      const float g = buf_ld(rgr, o, 0);
+      const int m = sv - hs;
+      const int u = part ? m : k - hs, v = part ? k - hs : m;
+      const int xx = x + u, yy = y + v;
+      const bool ok = inside && k < KL && xx >= 0 && xx < W && yy >= 0 && yy < H;
+      const bool win = (unsigned)sv < (unsigned)K;
+      toff[k] = !ok ? FM_NONE : win ? own + v * RC + u : -2 - (yy * W + xx);
+      tg[k] = ok ? g : 0.f;
+      far |= ok && !win;
+    }
+    for (int c0 = 0; c0 < C; c0 += CH) {
+      const int nc = min(CH, C - c0);
+      __syncthreads();  // the previous chunk's region is read
+      stage_region(reg, make_rsrc(f1 + ((int64_t)n * C + c0) * HW, (uint32_t)(nc * HW * 4)), nc, CH * RR, RR, RC, ry0,
+                   rx0, H, W, t);
+      __syncthreads();
+      if (inside) {
+        float a[CH], r0[CH];
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+          const int64_t o = ((int64_t)n * C + c0 + min(j, nc - 1)) * HW + pix;
+          a[j] = f0[o];
+          r0[j] = part ? df0[o] : 0.f;
+        }
+#pragma unroll
+        for (int k = 0; k < FM_KMAX; ++k) {
+          const int off = toff[k];
+          const float g = tg[k];
+          if (off >= 0) {
+#pragma unroll
+            for (int j = 0; j < CH; ++j) r0[j] += sgn_eps(a[j] - reg[j * RN + off]) * g;
+          } else if (far && off != FM_NONE) {
+            const int64_t q = -2 - (int64_t)off;
+#pragma unroll
+            for (int j = 0; j < CH; ++j)
+              if (j < nc) r0[j] += sgn_eps(a[j] - f1[((int64_t)n * C + c0 + j) * HW + q]) * g;
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < CH; ++j)
+          if (j < nc) df0[((int64_t)n * C + c0 + j) * HW + pix] = r0[j];
+      }
+    }
+  }
+
+  // 2. df1
+  const int lk = t & 31, cgp = t >> 5;  // scatter: lane label, pixel column group
+  for (int part = 0; part < 2; ++part) {
+    const float *gg = part ? gv : gu;
+    const __amdgpu_buffer_rsrc_t rst = make_rsrc((part ? vs : us) + img * K, vb);
+    const __amdgpu_buffer_rsrc_t rgr = make_rsrc(gg + img * K, vb);
+    __syncthreads();  // the previous masks and region are read
+    for (int k = 0; k < KL; ++k) mask[k * FM_NT + t] = 0u;
+    __syncthreads();
+    if (lk < KL) {
+      for (int r = 0; r < RR; r += 2) {
+        int sr[2][8];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int ys = ry0 + r + h;
+          const bool rok = r + h < RR && ys >= 0 && ys < H;
+#pragma unroll
+          for (int ci = 0; ci < 8; ++ci) {
+            const int c = cgp + 8 * ci, xs = rx0 + c;
+            const bool ok = rok && c < RC && xs >= 0 && xs < W;
+            const int sv = (int)buf_ld(rst, ok ? (uint32_t)(((ys * W + xs) * K + lk) * 4) : kBufDrop, 0);
+            sr[h][ci] = ok ? sv : -1;
+          }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma unroll
+          for (int ci = 0; ci < 8; ++ci) {
+            const int sv = sr[h][ci];
+            if ((unsigned)sv < (unsigned)K) {
+              const int ys = ry0 + r + h, xs = rx0 + cgp + 8 * ci;
+              const int e = sv - hs;
+              const int yt = ys + (part ? lk - hs : e) - y0, xt = xs + (part ? e : lk - hs) - x0;
+              if ((unsigned)yt < (unsigned)FM_TY && (unsigned)xt < (unsigned)FM_TX)
+                atomicOr(&mask[lk * FM_NT + yt * FM_TX + xt], 1u << sv);
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // first landing per label into registers, the others into the list
+    int foff[FM_KMAX];
+    float fg[FM_KMAX];
+    int nx = 0;  // landings past the first of their label (the list holds FM_XCAP of them)
+#pragma unroll
+    for (int k = 0; k < FM_KMAX; ++k) {
+      uint32_t m = inside && k < KL ? mask[k * FM_NT + t] : 0u;
+      const int e = (m != 0u ? __builtin_ctz(m) : hs) - hs;
+      const int u = part ? e : k - hs, v = part ? k - hs : e;
+      const float g = buf_ld(rgr, m != 0u ? (uint32_t)(((ipix - v * W - u) * K + k) * 4) : kBufDrop, 0);
+      foff[k] = m != 0u ? own - v * RC - u : FM_NONE;
+      fg[k] = m != 0u ? g : 0.f;
+      m &= m - 1u;
+      while (m != 0u) {
+        const int e2 = __builtin_ctz(m) - hs;
+        m &= m - 1u;
+        const int u2 = part ? e2 : k - hs, v2 = part ? k - hs : e2;
+        if (nx < FM_XCAP) xo[nx * FM_NT + t] = (uint32_t)(v2 + 32) | (uint32_t)(u2 + 32) << 6 | (uint32_t)k << 12;
+        ++nx;
+      }
+    }
+    // the listed landings' gradients, loads in flight together
+#pragma unroll
+    for (int i = 0; i < FM_XCAP; ++i) {
+      const bool ok = i < nx;
+      const uint32_t w = ok ? xo[i * FM_NT + t] : 0u;
+      const int v = (int)(w & 63u) - 32, u = (int)((w >> 6) & 63u) - 32, k = (int)(w >> 12);
+      const float g = buf_ld(rgr, ok ? (uint32_t)(((ipix - v * W - u) * K + k) * 4) : kBufDrop, 0);
+      if (ok) xg[i * FM_NT + t] = g;
+    }
+    for (int c0 = 0; c0 < C; c0 += CH) {
+      const int nc = min(CH, C - c0);
+      __syncthreads();
+      stage_region(reg, make_rsrc(f0 + ((int64_t)n * C + c0) * HW, (uint32_t)(nc * HW * 4)), nc, CH * RR, RR, RC, ry0,
+                   rx0, H, W, t);
+      __syncthreads();
+      if (inside) {
+        float b[CH], r1[CH];
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+          const int64_t o = ((int64_t)n * C + c0 + min(j, nc - 1)) * HW + pix;
+          b[j] = f1[o];
+          r1[j] = part ? df1[o] : 0.f;
+        }
+#pragma unroll
+        for (int k = 0; k < FM_KMAX; ++k) {
+          const int off = foff[k];
+          if (off >= 0) {
+            const float g = fg[k];
+#pragma unroll
+            for (int j = 0; j < CH; ++j) r1[j] -= sgn_eps(reg[j * RN + off] - b[j]) * g;
+          }
+        }
+        const int nl = min(nx, FM_XCAP);
+        for (int i = 0; i < nl; ++i) {
+          const uint32_t w = xo[i * FM_NT + t];
+          const int off = own - ((int)(w & 63u) - 32) * RC - ((int)((w >> 6) & 63u) - 32);
+          const float g = xg[i * FM_NT + t];
+#pragma unroll
+          for (int j = 0; j < CH; ++j) r1[j] -= sgn_eps(reg[j * RN + off] - b[j]) * g;
+        }
+        if (nx > FM_XCAP) {  // rare: the landings past the list, in the same order
+          int ix = 0;
+          for (int k = 0; k < KL; ++k) {
+            uint32_t m = mask[k * FM_NT + t];
+            m &= m - 1u;
+            while (m != 0u) {
+              const int e = __builtin_ctz(m) - hs;
+              m &= m - 1u;
+              if (ix++ < FM_XCAP) continue;
+              const int u = part ? e : k - hs, v = part ? k - hs : e;
+              const float g = gg[(p - (int64_t)v * W - u) * K + k];
+              const int o2 = own - v * RC - u;
+#pragma unroll
+              for (int j = 0; j < CH; ++j) r1[j] -= sgn_eps(reg[j * RN + o2] - b[j]) * g;
+            }
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < CH; ++j)
+          if (j < nc) df1[((int64_t)n * C + c0 + j) * HW + pix] = r1[j];
+      }
+    }
+  }
+}
+
+// K > 32: one thread per pixel and chunk of FB_CB channels, the sources landing on the pixel
+// found by a K x K search of the argmins.  The sources a lane finds are first collected in
 // search order in a per-thread LDS list; the channel updates then run over the lists in
-// lockstep, instead of once per search step in which any lane of the wave lands.  Terms keep
-// the order of the scalar definition (u part then v part; label ascending, inner displacement
-// ascending); a list that overflows FB_CAP is finished by a second, divergent pass over the
-// remaining landings.
+// lockstep.  A list that overflows FB_CAP is finished by a second pass over the remaining
+// landings.
 constexpr int FB_CB = 32;
 constexpr int FB_CAP = 48;
 constexpr int FB_THREADS = 128;
 
-__global__ __launch_bounds__(FB_THREADS) void sad_flow_bwd_k(const float *__restrict__ f0, const float *__restrict__ f1,
-                                                             const float *__restrict__ gu, const float *__restrict__ gv,
-                                                             const float *__restrict__ us, const float *__restrict__ vs,
-                                                             float *__restrict__ df0, float *__restrict__ df1,
-                                                             int N, int C, int H, int W, int sws) {
+__global__ __launch_bounds__(FB_THREADS) void sad_flow_bwd_search_k(const float *__restrict__ f0, const float *__restrict__ f1,
+                                                                    const float *__restrict__ gu, const float *__restrict__ gv,
+                                                                    const float *__restrict__ us, const float *__restrict__ vs,
+                                                                    float *__restrict__ df0, float *__restrict__ df1,
+                                                                    int N, int C, int H, int W, int sws) {
   __shared__ int l_src[FB_CAP][FB_THREADS];
   __shared__ float l_g[FB_CAP][FB_THREADS];
-  const int K = sws + 1, hs = sws / 2;
+  const int K = sws + 1, hs = sws / 2, KL = 2 * hs + 1, E = K - 1 - hs;
   const int64_t HW = (int64_t)H * W;
   const int64_t total = (int64_t)N * HW;
   const int t = threadIdx.x;
-  // every thread of the block runs the same number of grid-stride rounds (barrier-free body)
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + t; p < total; p += (int64_t)gridDim.x * blockDim.x) {
     const int n = (int)(p / HW);
     const int y = (int)((p / W) % H), x = (int)(p % W);
@@ -434,11 +705,10 @@ __global__ __launch_bounds__(FB_THREADS) void sad_flow_bwd_k(const float *__rest
         r0[j] = 0.f;
         r1[j] = 0.f;
       }
-      // df0: along this pixel's own argmins
       for (int part = 0; part < 2; ++part) {
         const float *st = part ? vs : us;
         const float *gg = part ? gv : gu;
-        for (int k = 0; k < K; ++k) {
+        for (int k = 0; k < KL; ++k) {
           const int64_t o = p * K + k;
           const int m = (int)st[o] - hs;
           const int u = part ? m : k - hs, v = part ? k - hs : m;
@@ -451,14 +721,13 @@ __global__ __launch_bounds__(FB_THREADS) void sad_flow_bwd_k(const float *__rest
             if (j < nc) r0[j] += sgn_eps(a[j] - F1[(int64_t)j * HW + src]) * g;
         }
       }
-      // df1: the sources whose argmin lands on this pixel
       for (int part = 0; part < 2; ++part) {
         const float *st = part ? vs : us;
         const float *gg = part ? gv : gu;
         int cnt = 0;
-        for (int k = 0; k < K; ++k) {
-          for (int e = -hs; e <= hs; ++e) {
-            // u part: label k is the horizontal displacement, e the vertical one; v part: swapped
+        for (int k = 0; k < KL; ++k) {
+          for (int e = -hs; e <= E; ++e) {
+            // u-volume: label k is the horizontal displacement, e the vertical one; v: swapped
             const int u = part ? e : k - hs, v = part ? k - hs : e;
             const int xs = x - u, ys = y - v;
             if (xs < 0 || xs >= W || ys < 0 || ys >= H) continue;
@@ -480,10 +749,10 @@ __global__ __launch_bounds__(FB_THREADS) void sad_flow_bwd_k(const float *__rest
           for (int j = 0; j < FB_CB; ++j)
             if (j < nc) r1[j] -= sgn_eps(F0[(int64_t)j * HW + src] - b[j]) * g;
         }
-        if (cnt > FB_CAP) {  // rare: finish the landings past the list, in search order
+        if (cnt > FB_CAP) {
           int seen = 0;
-          for (int k = 0; k < K; ++k) {
-            for (int e = -hs; e <= hs; ++e) {
+          for (int k = 0; k < KL; ++k) {
+            for (int e = -hs; e <= E; ++e) {
               const int u = part ? e : k - hs, v = part ? k - hs : e;
               const int xs = x - u, ys = y - v;
               if (xs < 0 || xs >= W || ys < 0 || ys >= H) continue;
@@ -507,6 +776,40 @@ __global__ __launch_bounds__(FB_THREADS) void sad_flow_bwd_k(const float *__rest
           D0[(int64_t)j * HW + pix] = r0[j];
           D1[(int64_t)j * HW + pix] = r1[j];
         }
+    }
+  }
+}
+
+// Argmins outside the window (s < 0 or s >= K; only block-offset argmins have them): their
+// df1 terms, added after the gather kernels with atomics (the reference's own scheme).  One
+// thread per pixel reads the pixel's argmins (in-window ones are skipped).
+__global__ __launch_bounds__(256) void sad_flow_bwd_far_k(const float *__restrict__ f0, const float *__restrict__ f1,
+                                                          const float *__restrict__ gu, const float *__restrict__ gv,
+                                                          const float *__restrict__ us, const float *__restrict__ vs,
+                                                          float *__restrict__ df1, int N, int C, int H, int W, int sws) {
+  const int K = sws + 1, hs = sws / 2, KL = 2 * hs + 1;
+  const int64_t HW = (int64_t)H * W;
+  const int64_t total = (int64_t)N * HW;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < total; p += (int64_t)gridDim.x * blockDim.x) {
+    for (int part = 0; part < 2; ++part) {
+      const float *st = part ? vs : us;
+      for (int k = 0; k < KL; ++k) {
+        const int sv = (int)st[p * K + k];
+        if ((unsigned)sv < (unsigned)K) continue;
+        const int n = (int)(p / HW);
+        const int64_t pix = p - (int64_t)n * HW;
+        const int y = (int)(pix / W), x = (int)(pix - (int64_t)y * W);
+        const int e = sv - hs;
+        const int u = part ? e : k - hs, v = part ? k - hs : e;
+        const int xx = x + u, yy = y + v;
+        if (xx < 0 || xx >= W || yy < 0 || yy >= H) continue;
+        const float g = (part ? gv : gu)[p * K + k];
+        const int64_t q = (int64_t)yy * W + xx;
+        for (int c = 0; c < C; ++c) {
+          const int64_t plane = ((int64_t)n * C + c) * HW;
+          atomicAdd(&df1[plane + q], -(sgn_eps(f0[plane + pix] - f1[plane + q]) * g));
+        }
+      }
     }
   }
 }
@@ -588,8 +891,29 @@ extern "C" int bpl_sad_flow_bwd(const float *f0, const float *f1, const float *g
               "bpl_sad_flow_bwd: null pointer");
   BPL_REQUIRE(N >= 1 && C >= 1 && H >= 1 && W >= 1, BPL_E_ARG, "bpl_sad_flow_bwd: sizes must be positive");
   BPL_REQUIRE(sws >= 0 && sws <= 1024, BPL_E_ARG, "bpl_sad_flow_bwd: sws=%d out of range", sws);
-  KernelTimer kt(BPL_KERNEL_SAD_FLOW_BWD, as_stream(stream));
-  hipLaunchKernelGGL(sad_flow_bwd_k, dim3(grid_for(N * H * W, FB_THREADS)), dim3(FB_THREADS), 0, as_stream(stream), f0, f1,
-                     grad_u, grad_v, u_star, v_star, df0, df1, (int)N, (int)C, (int)H, (int)W, sws);
-  return check_hip(hipGetLastError(), "bpl_sad_flow_bwd launch");
+  hipStream_t st = as_stream(stream);
+  KernelTimer kt(BPL_KERNEL_SAD_FLOW_BWD, st);
+  if (sws + 1 <= FM_KMAX) {
+    BPL_REQUIRE(N < 65536 && (H + FM_TY - 1) / FM_TY < 65536, BPL_E_LIMIT, "bpl_sad_flow_bwd: N or H too large");
+    BPL_REQUIRE(H * W * (sws + 1 > FM_CH ? sws + 1 : FM_CH) * 4 < (1LL << 31), BPL_E_LIMIT,
+                "bpl_sad_flow_bwd: H * W * K too large");
+    const size_t shm = flow_bwd_lds_bytes(sws, FM_CH);
+    if (shm > 64 * 1024) {
+      const int rc = check_hip(hipFuncSetAttribute((const void *)sad_flow_bwd_lds_k<FM_CH>,
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm),
+                               "bpl_sad_flow_bwd: raising the LDS limit");
+      if (rc != BPL_OK) return rc;
+    }
+    dim3 grid((unsigned)((W + FM_TX - 1) / FM_TX), (unsigned)((H + FM_TY - 1) / FM_TY), (unsigned)N);
+    hipLaunchKernelGGL(sad_flow_bwd_lds_k<FM_CH>, grid, dim3(FM_NT), shm, st, f0, f1, grad_u, grad_v, u_star, v_star,
+                       df0, df1, (int)C, (int)H, (int)W, sws);
+  } else {
+    hipLaunchKernelGGL(sad_flow_bwd_search_k, dim3(grid_for(N * H * W, FB_THREADS)), dim3(FB_THREADS), 0, st, f0, f1,
+                       grad_u, grad_v, u_star, v_star, df0, df1, (int)N, (int)C, (int)H, (int)W, sws);
+  }
+  const int rc = check_hip(hipGetLastError(), "bpl_sad_flow_bwd launch");
+  if (rc != BPL_OK) return rc;
+  hipLaunchKernelGGL(sad_flow_bwd_far_k, dim3(grid_for(N * H * W, 256)), dim3(256), 0, st, f0,
+                     f1, grad_u, grad_v, u_star, v_star, df1, (int)N, (int)C, (int)H, (int)W, sws);
+  return check_hip(hipGetLastError(), "bpl_sad_flow_bwd far launch");
 }

@@ -76,6 +76,8 @@ def flow_sad_forward(f0, f1, sws, offset_u=0, offset_v=0, block_u=0, block_v=0):
     """Fused K x K window SAD with min-projections (flow_mp_sad_kernel.cu:10-124).
 
     K = sws + 1, h = sws // 2.  S(u,v) = sum_c |f0[c,y,x] - f1[c,y+ov+v,x+ou+u]| or FILL.
+    u_idx runs over [0, K) but the v loop over [-h, h] (:62-80), so for odd sws the row
+    v_idx = K - 1 keeps FILL.
     cv_u[.., u+h] = min_v S, u_star = argmin_v (first) + block_v * sws;
     cv_v[.., v+h] = min_u S, v_star = argmin_u (first) + block_u * sws.
     Argmins are returned as float64 label ids (the reference stores them as fp32).
@@ -86,7 +88,7 @@ def flow_sad_forward(f0, f1, sws, offset_u=0, offset_v=0, block_u=0, block_v=0):
     K = sws + 1
     h = sws // 2
     S = np.full((N, H, W, K, K), FILL)  # [.., v_idx, u_idx]
-    for vi in range(K):
+    for vi in range(2 * h + 1):
         dv = offset_v + vi - h
         ylo, yhi = max(0, -dv), min(H, H - dv)
         if ylo >= yhi:
@@ -109,6 +111,8 @@ def flow_sad_forward(f0, f1, sws, offset_u=0, offset_v=0, block_u=0, block_v=0):
 def flow_sad_backward(f0, f1, sws, grad_u, grad_v, u_star, v_star):
     """(df0, df1) routed through the stored argmins (flow_mp_sad_kernel.cu:126-200).
 
+    Labels run over [-h, h] (:152, :175): for odd sws the last label (K - 1) has no gradient.
+
     For each u: v = u_star[u] - h; if (x+u, y+v) is inside, upd = sign(f0 - f1[y+v,x+u]) *
     grad_u[u]; df0 += upd; df1[y+v, x+u] -= upd.  The same for v with v_star.  Offsets
     and block indices are not applied (as in the reference)."""
@@ -121,7 +125,7 @@ def flow_sad_backward(f0, f1, sws, grad_u, grad_v, u_star, v_star):
     df1 = np.zeros_like(f1)
     ys, xs = np.meshgrid(np.arange(H), np.arange(W), indexing="ij")
     for n in range(N):
-        for k in range(K):
+        for k in range(2 * h + 1):
             for gsrc, star, is_u in ((grad_u, u_star, True), (grad_v, v_star, False)):
                 other = np.asarray(star[n, :, :, k]).astype(np.int64) - h  # float -> short
                 if is_u:

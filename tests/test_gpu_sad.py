@@ -49,7 +49,8 @@ def test_stereo_sad_backward(N, C, H, W, mn, mx):
     np.testing.assert_allclose(d1.cpu().numpy(), e1, rtol=1e-4, atol=1e-4)
 
 
-@pytest.mark.parametrize("sws,ou,ov,bu,bv", [(24, 0, 0, 0, 0), (24, 3, -2, 1, 1), (8, 0, 0, 0, 0), (10, -5, 5, 0, 1)])
+@pytest.mark.parametrize("sws,ou,ov,bu,bv", [(24, 0, 0, 0, 0), (24, 3, -2, 1, 1), (8, 0, 0, 0, 0), (10, -5, 5, 0, 1),
+                                            (5, 0, 0, 0, 0), (7, 2, -1, 1, 0)])
 def test_flow_sad_forward(sws, ou, ov, bu, bv):
     from bplayers_amd import pytorch_cuda_flow_mp_sad_op as op
     rng = np.random.default_rng(sws)
@@ -66,19 +67,52 @@ def test_flow_sad_forward(sws, ou, ov, bu, bv):
         assert mism.mean() < 1e-3, f"argmin mismatches {mism.sum()}"
 
 
-def test_flow_sad_backward():
+@pytest.mark.parametrize("N,C,H,W,sws,stars", [
+    (1, 6, 30, 40, 24, "fwd"),
+    (1, 37, 45, 77, 8, "fwd"),      # ragged channel groups, partial tiles
+    (2, 16, 19, 70, 5, "fwd"),      # odd window: the last label has no gradient
+    (1, 4, 20, 33, 31, "fwd"),      # K = 32, the largest masked window
+    (1, 3, 18, 25, 34, "fwd"),      # K = 35: searched landings
+    (1, 5, 17, 29, 6, "offset"),    # block-offset argmins land outside the window
+    (2, 3, 13, 21, 5, "random"),    # arbitrary argmins, negatives included
+    (1, 2, 12, 40, 33, "random"),
+])
+def test_flow_sad_backward(N, C, H, W, sws, stars):
     from bplayers_amd import pytorch_cuda_flow_mp_sad_op as op
-    rng = np.random.default_rng(7)
-    sws = 24
-    f0 = rng.normal(size=(1, 6, 30, 40)).astype(np.float32)
-    f1 = rng.normal(size=(1, 6, 30, 40)).astype(np.float32)
-    cu, cv, us, vs = S.flow_sad_forward(f0, f1, sws)
-    gu = rng.normal(size=cu.shape).astype(np.float32)
-    gv = rng.normal(size=cv.shape).astype(np.float32)
+    rng = np.random.default_rng(7 + sws)
+    K = sws + 1
+    f0 = rng.normal(size=(N, C, H, W)).astype(np.float32)
+    f1 = rng.normal(size=(N, C, H, W)).astype(np.float32)
+    if stars == "fwd":
+        _, _, us, vs = S.flow_sad_forward(f0, f1, sws)
+    elif stars == "offset":
+        _, _, us, vs = S.flow_sad_forward(f0, f1, sws, 2, -1, 1, 1)
+    else:
+        us = rng.integers(-3, 2 * K + 2, size=(N, H, W, K)).astype(np.float64)
+        vs = rng.integers(-3, 2 * K + 2, size=(N, H, W, K)).astype(np.float64)
+    us = us.astype(np.float32)
+    vs = vs.astype(np.float32)
+    gu = rng.normal(size=(N, H, W, K)).astype(np.float32)
+    gv = rng.normal(size=(N, H, W, K)).astype(np.float32)
     d0, d1 = op.backward(_t(f0), _t(f1), sws, _t(gu), _t(gv), _t(us), _t(vs))
     e0, e1 = S.flow_sad_backward(f0, f1, sws, gu, gv, us, vs)
     np.testing.assert_allclose(d0.cpu().numpy(), e0, rtol=1e-4, atol=1e-4)
     np.testing.assert_allclose(d1.cpu().numpy(), e1, rtol=1e-4, atol=1e-4)
+
+
+def test_flow_sad_backward_deterministic():
+    """In-window argmins: df1 is a fixed-order gather, bitwise repeatable."""
+    from bplayers_amd import pytorch_cuda_flow_mp_sad_op as op
+    rng = np.random.default_rng(5)
+    sws = 24
+    f0 = rng.normal(size=(1, 32, 40, 70)).astype(np.float32)
+    f1 = rng.normal(size=(1, 32, 40, 70)).astype(np.float32)
+    cu, cv, us, vs = op.forward(_t(f0), _t(f1), sws, 0, 0, 0, 0)
+    gu = torch.randn_like(cu)
+    gv = torch.randn_like(cv)
+    a = op.backward(_t(f0), _t(f1), sws, gu, gv, us, vs)
+    b = op.backward(_t(f0), _t(f1), sws, gu, gv, us, vs)
+    assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
 
 
 def test_flow_sad_function_sws216_matches_tiled_merge():
