@@ -424,10 +424,12 @@ def kernel_class(name):
     if "lbp_bwd_reduce" in base:
         return "lbp_bwd_reduce"
     for stem, cls in (("sad_stereo_fwd", "sad_stereo_fwd"), ("sad_stereo_bwd", "sad_stereo_bwd"),
-                      ("sad_flow_fwd", "sad_flow_fwd"), ("sad_flow_bwd", "sad_flow_bwd")):
+                      ("sad_stereo_softmax", "sad_stereo_softmax"), ("sad_flow_fwd", "sad_flow_fwd"),
+                      ("sad_flow_bwd", "sad_flow_bwd"), ("softmax_labels", "softmax_labels"),
+                      ("beliefs_k", "lbp_beliefs")):
         if stem in base:
             return cls
-    m = re.search(r"lbp_(fwd|bwd)_(stereo_w64|stereo_dpp|phase)<([^>]*)>", base)
+    m = re.search(r"lbp_(fwd|bwd)_(stereo_w64|semg|phase)<([^>]*)>", base)
     if not m:
         return None
     targs = [t.strip() for t in m.group(3).split(",")]

@@ -28,6 +28,7 @@ _SIGS = {
     "bpl_version": ([], _I),
     "bpl_last_error": ([], ctypes.c_char_p),
     "bpl_sad_stereo_fwd": ([_P, _P, _P, _I64, _I64, _I64, _I64, _I, _I, _F, _P], _I),
+    "bpl_sad_stereo_softmax_fwd": ([_P, _P, _P, _I64, _I64, _I64, _I64, _I, _I, _F, _F, _P], _I),
     "bpl_sad_stereo_bwd": ([_P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I, _I, _P], _I),
     "bpl_sad_flow_fwd": ([_P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I, _I, _I, _I, _I, _P], _I),
     "bpl_sad_flow_bwd": ([_P, _P, _P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I, _P], _I),
@@ -42,7 +43,7 @@ _SIGS = {
     "bpl_timing_read": ([_I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I64)], _I),
     "bpl_timing_name": ([_I], ctypes.c_char_p),
 }
-KERNEL_COUNT = 12  # BPL_KERNEL_COUNT
+KERNEL_COUNT = 13  # BPL_KERNEL_COUNT
 
 
 class BplError(RuntimeError):

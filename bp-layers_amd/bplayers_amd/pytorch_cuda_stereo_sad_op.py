@@ -31,6 +31,22 @@ def forward(f0, f1, min_disp, max_disp, step=1.0):
     return cost
 
 
+def forward_softmax(f0, f1, min_disp, max_disp, scale, sign=1.0):
+    """sign * softmax(scale * cost) over the disparities, cost = forward(f0, f1, min_disp,
+    max_disp) at step 1, in one kernel: the matching epilogue of matching.py:22-25 / :45-47
+    with corenet.py:97-104 (scale = -1/T).  An addition to the reference's extension API; the
+    result equals forward() followed by the label softmax bit for bit.  D <= 256."""
+    _check(f0, f1)
+    N, C, H, W = f0.shape
+    D = int(max_disp) - int(min_disp) + 1
+    if D < 1:
+        raise RuntimeError("empty disparity range")
+    y = torch.empty((N, H, W, D), dtype=torch.float32, device=f0.device)
+    nat.call("bpl_sad_stereo_softmax_fwd", nat.ptr(f0), nat.ptr(f1), nat.ptr(y), N, C, H, W, int(min_disp),
+             int(max_disp), float(scale), float(sign), nat.stream_of(f0))
+    return y
+
+
 def backward(f0, f1, min_disp, max_disp, in_grad):
     """[df0, df1] of forward at step 1 (stereo_sad_kernel.cu:64-108)."""
     _check(f0, f1)

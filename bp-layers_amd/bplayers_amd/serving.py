@@ -1,13 +1,13 @@
 """Continuous-batching stereo serving on one GPU.
 
 Requests (a stereo feature pair of one of several shape classes H x W x D) arrive over
-time.  Each request is solved forward-only: SAD cost volume -> cost = -softmax(-SAD/T) ->
-``iters`` LBP iterations with a global P1/P2 pairwise -> winner-take-all disparity of the
-beliefs -- all on libbpl_amd kernels, with no allocation on the request path.
+time.  Each request is solved forward-only: cost = -softmax(-SAD/T) (SAD and softmax in one
+kernel) -> ``iters`` LBP iterations with a global P1/P2 pairwise -> winner-take-all disparity
+of the beliefs -- all on libbpl_amd kernels, with no allocation on the request path.
 
 Design:
   * a *slot* owns the resident device buffers for a batch of up to ``max_batch``
-    requests of one shape class (features, SAD volume, cost volume, messages, output) and
+    requests of one shape class (features, cost volume, messages, output) and
     its own HIP stream; slots are created lazily while the total resident bytes stay
     within the HBM budget (admission), and reused;
   * the scheduler loop admits requests as they arrive, groups waiting requests of one
@@ -48,7 +48,7 @@ class Request:
 def slot_bytes(shape, C, batch, iters=1):
     H, W, D = shape
     px = H * W
-    return batch * (2 * C * px * 4 + 2 * px * D * 4 + 4 * px * D * 4 + 4 * px * 4 + px * 4) + 4 * 6 * px * 4
+    return batch * (2 * C * px * 4 + px * D * 4 + 4 * px * D * 4 + 4 * px * 4 + px * 4) + 4 * 6 * px * 4
 
 
 class Slot:
@@ -61,7 +61,6 @@ class Slot:
         self.stream = torch.cuda.Stream(device=dev)  # default priority
         self.f0 = torch.empty((B, C, H, W), device=dev)
         self.f1 = torch.empty((B, C, H, W), device=dev)
-        self.sad = torch.empty((B, H, W, D), device=dev)
         self.cost = torch.empty((B, H, W, D), device=dev)
         self.msgs = torch.empty((B, 4, H, W, D), device=dev)
         self.edge = torch.ones((B, 4, H, W), device=dev)
@@ -85,8 +84,8 @@ class Slot:
             for i, r in enumerate(reqs):
                 self.f0[i].copy_(r.f0[0], non_blocking=True)
                 self.f1[i].copy_(r.f1[0], non_blocking=True)
-            nat.call("bpl_sad_stereo_fwd", p(self.f0), p(self.f1), p(self.sad), n, C, H, W, 0, D - 1, 1.0, st)
-            nat.call("bpl_softmax_labels", p(self.sad), p(self.cost), n * H * W, D, -1.0 / self.T_match, -1.0, st)
+            nat.call("bpl_sad_stereo_softmax_fwd", p(self.f0), p(self.f1), p(self.cost), n, C, H, W, 0, D - 1,
+                     -1.0 / self.T_match, -1.0, st)
             self.msgs[:n].zero_()
             for _ in range(self.iters):
                 nat.call("bpl_lbp_fwd", nat.LBP_STEREO, p(self.cost), p(self.jump), 6, 1, p(self.edge), p(self.msgs),

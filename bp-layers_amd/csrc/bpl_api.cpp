@@ -131,6 +131,7 @@ extern "C" const char *bpl_timing_name(int kernel) {
   static const char *names[BPL_KERNEL_COUNT] = {
       "lbp_fwd_horizontal", "lbp_fwd_vertical", "lbp_bwd_vertical", "lbp_bwd_horizontal",
       "lbp_bwd_combine",    "lbp_bwd_reduce",   "sad_stereo_fwd",   "sad_stereo_bwd",
-      "sad_flow_fwd",       "sad_flow_bwd",     "softmax_labels",   "lbp_beliefs"};
+      "sad_flow_fwd",       "sad_flow_bwd",     "softmax_labels",   "lbp_beliefs",
+      "sad_stereo_softmax"};
   return kernel >= 0 && kernel < BPL_KERNEL_COUNT ? names[kernel] : "";
 }

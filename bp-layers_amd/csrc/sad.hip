@@ -84,6 +84,114 @@ __global__ __launch_bounds__(256) void sad_stereo_fwd_k(const float *__restrict_
   }
 }
 
+// Stereo SAD with the matching softmax fused in: y = sign * softmax(scale * SAD) over the D
+// disparities of each pixel (matching.py:22-25 and :45-47 with corenet.py:97-104: scale =
+// -1/T; sign = -1 gives the message-passing cost -prob, message_passing_op_cuda.py:162).
+// Block: 64 pixels of one row x all D disparities.  The disparity passes of sad_stereo_fwd_k
+// run one after another in the block with the same per-thread arithmetic and leave their SAD
+// tile in the block's own [64][D] slice of y, which stays in L2 (a tile in LDS instead costs
+// the SAD loop its occupancy: 1.18 vs 0.60 ms at 384x1248x192x64); then each wave normalises
+// 16 of the pixels with the arithmetic of bpl_softmax_labels (label c on lane c mod 64, LPL
+// labels per lane, the same max and sum trees) and overwrites the slice, so y equals
+// bpl_sad_stereo_fwd followed by bpl_softmax_labels bit for bit.
+template <int LPL>
+__global__ __launch_bounds__(256) void sad_stereo_softmax_k(const float *__restrict__ f0, const float *__restrict__ f1,
+                                                            float *y, int C, int H, int W, int D, int mind,
+                                                            float scale, float sign) {
+  __shared__ float s0[ST_CB][ST_TX];
+  __shared__ float s1[ST_CB][ST_TX + ST_TD];
+  const int x0 = blockIdx.x * ST_TX;
+  const int row = blockIdx.y;
+  const int n = blockIdx.z;
+  const int t = threadIdx.x;
+  const int dg = t & 15, xg = t >> 4;
+  const int64_t HW = (int64_t)H * W;
+  const float *F0 = f0 + (int64_t)n * C * HW + (int64_t)row * W;
+  const float *F1 = f1 + (int64_t)n * C * HW + (int64_t)row * W;
+  float *sv = y + (((int64_t)n * H + row) * W + x0) * D;  // the block's [64][D] slice of y
+  const int dpasses = (D + ST_TD - 1) / ST_TD;
+  for (int dp = 0; dp < dpasses; ++dp) {
+    const int d0 = mind + dp * ST_TD;
+    const int xs = x0 - d0 - (ST_TD - 1);  // f1 column of s1[.][0]
+    float acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] = 0.f;
+    for (int c0 = 0; c0 < C; c0 += ST_CB) {
+      for (int i = t; i < ST_CB * ST_TX; i += 256) {
+        const int cc = i / ST_TX, xx = i - cc * ST_TX;
+        const int c = c0 + cc, xc = x0 + xx;
+        s0[cc][xx] = (c < C && xc < W) ? F0[(int64_t)c * HW + xc] : 0.f;
+      }
+      for (int i = t; i < ST_CB * (ST_TX + ST_TD); i += 256) {
+        const int cc = i / (ST_TX + ST_TD), j = i - cc * (ST_TX + ST_TD);
+        const int c = c0 + cc, xc = xs + j;
+        s1[cc][j] = (c < C && xc >= 0 && xc < W) ? F1[(int64_t)c * HW + xc] : 0.f;
+      }
+      __syncthreads();
+      const int nc = min(ST_CB, C - c0);
+      const int jb = xg * 4 - dg * 4 + ST_TD - 4;
+      for (int cc = 0; cc < nc; ++cc) {
+        float a4[4], v[7];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) a4[a] = s0[cc][xg * 4 + a];
+#pragma unroll
+        for (int j = 0; j < 7; ++j) v[j] = s1[cc][jb + j];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) acc[a][b] += fabsf(a4[a] - v[a - b + 3]);
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int px = xg * 4 + a, x = x0 + px;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int i = dp * ST_TD + dg * 4 + b;
+        const int d = mind + i;
+        if (i < D && x < W) sv[px * D + i] = (x - d >= 0 && x - d < W) ? acc[a][b] : kFill;
+      }
+    }
+  }
+  // the tile written by all waves of the block is read back by other waves
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  const int lane = t & 63, w = t >> 6;
+  for (int q = 0; q < ST_TX / 4; ++q) {
+    const int px = q * 4 + w, x = x0 + px;
+    if (x >= W) break;  // wave-uniform; later q only go further right
+    const float *xr = sv + px * D;
+    float v[LPL];
+    float mx = -__builtin_inff();
+#pragma unroll
+    for (int k = 0; k < LPL; ++k) {
+      const int c = lane + 64 * k;
+      v[k] = c < D ? xr[c] * scale : -__builtin_inff();
+      mx = fmaxf(mx, v[k]);
+    }
+    mx = seg_reduce<64>(mx, OpMax());
+    float sum = 0.f;
+#pragma unroll
+    for (int k = 0; k < LPL; ++k) {
+      const int c = lane + 64 * k;
+      v[k] = c < D ? fast_exp(v[k] - mx) : 0.f;
+      sum += v[k];
+    }
+    sum = seg_reduce<64>(sum, OpSum());
+    const float r = sign / sum;
+    float *yr = sv + px * D;
+#pragma unroll
+    for (int k = 0; k < LPL; ++k) {
+      const int c = lane + 64 * k;
+      if (c < D) yr[c] = v[k] * r;
+    }
+  }
+}
+
 // general step: one thread per output, d = int(i * step + min_disp) (stereo_sad_kernel.cu:35)
 __global__ void sad_stereo_fwd_step_k(const float *__restrict__ f0, const float *__restrict__ f1, float *__restrict__ cost,
                                       int N, int C, int H, int W, int D, int mind, float step) {
@@ -847,6 +955,28 @@ extern "C" int bpl_sad_stereo_fwd(const float *f0, const float *f1, float *cost,
                        (int)N, (int)C, (int)H, (int)W, D, min_disp, step);
   }
   return check_hip(hipGetLastError(), "bpl_sad_stereo_fwd launch");
+}
+
+extern "C" int bpl_sad_stereo_softmax_fwd(const float *f0, const float *f1, float *y, int64_t N, int64_t C, int64_t H,
+                                          int64_t W, int min_disp, int max_disp, float scale, float sign, void *stream) {
+  BPL_REQUIRE(f0 && f1 && y, BPL_E_ARG, "bpl_sad_stereo_softmax_fwd: null pointer");
+  BPL_REQUIRE(N >= 1 && C >= 1 && H >= 1 && W >= 1, BPL_E_ARG, "bpl_sad_stereo_softmax_fwd: sizes must be positive");
+  const int D = max_disp - min_disp + 1;
+  BPL_REQUIRE(D >= 1, BPL_E_ARG, "bpl_sad_stereo_softmax_fwd: empty disparity range");
+  BPL_REQUIRE(D <= 256, BPL_E_LIMIT, "bpl_sad_stereo_softmax_fwd: D=%d exceeds 256", D);
+  BPL_REQUIRE(H < 65536 && N < 65536 && N * C * H * W < (1LL << 40), BPL_E_LIMIT,
+              "bpl_sad_stereo_softmax_fwd: tensor too large");
+  hipStream_t st = as_stream(stream);
+  const dim3 grid((unsigned)((W + ST_TX - 1) / ST_TX), (unsigned)H, (unsigned)N);
+  const int lpl = (D + 63) / 64;
+  KernelTimer kt(BPL_KERNEL_SAD_STEREO_SOFTMAX, st);
+  switch (lpl) {
+    case 1: hipLaunchKernelGGL(sad_stereo_softmax_k<1>, grid, dim3(256), 0, st, f0, f1, y, (int)C, (int)H, (int)W, D, min_disp, scale, sign); break;
+    case 2: hipLaunchKernelGGL(sad_stereo_softmax_k<2>, grid, dim3(256), 0, st, f0, f1, y, (int)C, (int)H, (int)W, D, min_disp, scale, sign); break;
+    case 3: hipLaunchKernelGGL(sad_stereo_softmax_k<3>, grid, dim3(256), 0, st, f0, f1, y, (int)C, (int)H, (int)W, D, min_disp, scale, sign); break;
+    default: hipLaunchKernelGGL(sad_stereo_softmax_k<4>, grid, dim3(256), 0, st, f0, f1, y, (int)C, (int)H, (int)W, D, min_disp, scale, sign); break;
+  }
+  return check_hip(hipGetLastError(), "bpl_sad_stereo_softmax_fwd launch");
 }
 
 extern "C" int bpl_sad_stereo_bwd(const float *f0, const float *f1, const float *grad_cost, float *df0, float *df1,

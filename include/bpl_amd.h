@@ -68,7 +68,8 @@ const char *bpl_last_error(void);
 #define BPL_KERNEL_SAD_FLOW_BWD 9
 #define BPL_KERNEL_SOFTMAX 10
 #define BPL_KERNEL_BELIEFS 11
-#define BPL_KERNEL_COUNT 12
+#define BPL_KERNEL_SAD_STEREO_SOFTMAX 12
+#define BPL_KERNEL_COUNT 13
 int bpl_timing_enable(int on);
 int bpl_timing_reset(void);
 int bpl_timing_read(int kernel, double *total_ms, int64_t *launches);
@@ -83,6 +84,19 @@ const char *bpl_timing_name(int kernel);
 int bpl_sad_stereo_fwd(const float *f0, const float *f1, float *cost,
                        int64_t N, int64_t C, int64_t H, int64_t W,
                        int min_disp, int max_disp, float step, void *stream);
+
+/*
+ * Stereo SAD (step 1) with the matching softmax fused in: y = sign * softmax(scale * SAD) over
+ * the D = max_disp - min_disp + 1 disparities of each pixel, y [N,H,W,D], D <= 256.  Replaces
+ * the composition StereoMatchingSadFunction -> -cost -> TemperatureSoftmax of the reference's
+ * StereoMatchingSad.forward (matching.py:22-25, :45-47; corenet.py:97-104) with scale = -1/T
+ * and sign = 1 (sign = -1 gives the message-passing cost -prob,
+ * ops/lbp_stereo/message_passing_op_cuda.py:162).  The SAD volume stays on chip; y equals
+ * bpl_sad_stereo_fwd followed by bpl_softmax_labels bit for bit.
+ */
+int bpl_sad_stereo_softmax_fwd(const float *f0, const float *f1, float *y,
+                               int64_t N, int64_t C, int64_t H, int64_t W,
+                               int min_disp, int max_disp, float scale, float sign, void *stream);
 
 /*
  * Gradient of the stereo SAD (step 1).  Replaces pytorch_cuda_stereo_sad_op.backward

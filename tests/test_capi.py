@@ -67,6 +67,13 @@ def test_argument_validation_returns_codes(lib):
     assert rc == -1 and b"empty" in lib.bpl_last_error()
     rc = lib.bpl_sad_flow_fwd(P, P, P, P, P, P, 1, 3, 4, 4, -2, 0, 0, 0, 0, None)
     assert rc == -1
+    f = ctypes.c_float
+    rc = lib.bpl_sad_stereo_softmax_fwd(P, P, P, 1, 3, 4, 4, 0, 256, f(-1.0), f(1.0), None)
+    assert rc == -2 and b"exceeds 256" in lib.bpl_last_error()
+    rc = lib.bpl_sad_stereo_softmax_fwd(P, None, P, 1, 3, 4, 4, 0, 63, f(-1.0), f(1.0), None)
+    assert rc == -1 and b"null" in lib.bpl_last_error()
+    rc = lib.bpl_sad_stereo_softmax_fwd(P, P, P, 1, 3, 4, 4, 5, 2, f(-1.0), f(1.0), None)
+    assert rc == -1 and b"empty" in lib.bpl_last_error()
 
 
 def test_python_ops_require_cuda_tensors(lib):

@@ -138,3 +138,53 @@ def test_flow_sad_function_sws216_matches_tiled_merge():
     a0[idx] = a1[idx]
     assert torch.equal(cu, u0)
     assert torch.equal(us, a0)
+
+
+@pytest.mark.parametrize("N,C,H,W,mn,mx", [(1, 16, 5, 100, 0, 63), (2, 24, 3, 77, -3, 96), (1, 64, 2, 200, 0, 191),
+                                          (1, 8, 4, 130, 2, 257), (1, 3, 2, 40, 0, 9)])
+def test_stereo_sad_softmax_fused_equals_two_kernels(N, C, H, W, mn, mx):
+    """The fused SAD + softmax kernel equals the SAD kernel followed by the label softmax bit
+    for bit (same per-thread SAD arithmetic, same softmax trees), for both signs."""
+    from bplayers_amd import pytorch_cuda_stereo_sad_op as op
+    from bplayers_amd import _native as nat
+    rng = np.random.default_rng(C + W)
+    f0 = _t(rng.normal(size=(N, C, H, W)))
+    f1 = _t(rng.normal(size=(N, C, H, W)))
+    D = mx - mn + 1
+    sad = op.forward(f0, f1, mn, mx)
+    for scale, sign in ((-1.0 / 3.0, 1.0), (-0.25, -1.0)):
+        got = op.forward_softmax(f0, f1, mn, mx, scale, sign)
+        exp = torch.empty_like(sad)
+        nat.call("bpl_softmax_labels", nat.ptr(sad), nat.ptr(exp), N * H * W, D, scale, sign, nat.stream_of(sad))
+        assert torch.equal(got, exp)
+        # and the FP64 oracle
+        s64 = S.stereo_sad_forward(f0.cpu().numpy(), f1.cpu().numpy(), mn, mx) * scale
+        p64 = np.exp(s64 - s64.max(axis=3, keepdims=True))
+        p64 = sign * p64 / p64.sum(axis=3, keepdims=True)
+        np.testing.assert_allclose(got.cpu().numpy(), p64, rtol=1e-4, atol=1e-6)
+
+
+def test_stereo_matching_layer_fused_gradients_match_composition():
+    """StereoMatchingSad (fused forward, softmax Jacobian + SAD backward) against the
+    reference's composition SAD op -> negation -> softmax(x / T) under autograd: the
+    probabilities and the gradients to f0, f1 and T."""
+    from bplayers_amd.matching import StereoMatchingSad
+    rng = np.random.default_rng(3)
+    N, C, H, W, mn, mx = 1, 8, 4, 70, 0, 47
+    a0 = rng.normal(size=(N, C, H, W))
+    a1 = rng.normal(size=(N, C, H, W))
+    G = _t(rng.normal(size=(N, H, W, mx - mn + 1)))
+    layer = StereoMatchingSad(mn, mx, init_temp=2.5, device="cuda:0")
+    f0 = _t(a0).requires_grad_()
+    f1 = _t(a1).requires_grad_()
+    p = layer(f0, f1)
+    (p * G).sum().backward()
+    T2 = torch.full((1,), 2.5, device="cuda:0", requires_grad=True)
+    g0 = _t(a0).requires_grad_()
+    g1 = _t(a1).requires_grad_()
+    q = torch.softmax(layer.compute_score_volume(g0, g1) / T2, dim=3)
+    (q * G).sum().backward()
+    torch.testing.assert_close(p, q, rtol=1e-5, atol=1e-7)
+    torch.testing.assert_close(f0.grad, g0.grad, rtol=1e-4, atol=1e-5)
+    torch.testing.assert_close(f1.grad, g1.grad, rtol=1e-4, atol=1e-5)
+    torch.testing.assert_close(layer._softmax.T.grad, T2.grad, rtol=1e-4, atol=1e-5)
