@@ -55,9 +55,9 @@ def test_legacy_bp_checkpoint_is_adjusted(tmp_path):
     p = tmp_path / "crf.cpt"
     torch.save({"message_passing.L1": torch.tensor(0.3), "message_passing.L2": torch.tensor(1.7)}, p)
     formats.load_checkpoint(bp, p, "cpu")
-    assert float(bp.message_passing._L1) == pytest.approx(0.3)
-    assert float(bp.message_passing._L2) == pytest.approx(1.7)
-    assert float(bp.message_passing.softmin.T) == 1.0
+    assert float(bp.message_passing._L1.detach()) == pytest.approx(0.3)
+    assert float(bp.message_passing._L2.detach()) == pytest.approx(1.7)
+    assert float(bp.message_passing.softmin.T.detach()) == 1.0
 
 
 CRF = sorted(glob.glob(os.path.join(REF_DATA, "params", "*", "*", "crf0_lvl*_best.cpt")))
