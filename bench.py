@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="stereo_fb_384x1248_d192",
                     choices=["stereo_fb_384x1248_d192", "stereo_fwd_128x256_d64", "semantic_fb_2x512x1024_c19",
-                             "flow_fb_256x512_r12", "serving_c4"])
+                             "semantic_pixel_fb_2x512x1024_c19", "flow_fb_256x512_r12", "serving_c4"])
     ap.add_argument("--requests", type=int, default=96, help="serving_c4: requests per measurement")
     ap.add_argument("--rows", type=int, default=0, help="crop the image to this many rows (debug)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -282,6 +282,40 @@ def bench_semantic(args, torch):
                        {"workload": args.workload, "N": plan.N, "H": plan.H, "W": plan.W, "labels": C,
                         "iters": plan.iters, "delta": plan.delta, "pairwise": "global CxC matrix per direction",
                         "grad_mode": "exact"})
+    line["roofline_iteration"] = {"bound": "hbm", "kernels": [k for k in avg if k.startswith("lbp")],
+                                  "achieved": round(it_bytes / (it_ms * 1e-3) / 1e9, 2), "peak": HBM_PEAK_GBS,
+                                  "unit": "GB/s", "frac": round(it_bytes / (it_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                  "alg_bytes_per_pl": round(it_bytes / plan.pl(), 3), "iteration_ms": round(it_ms, 4)}
+    line["timings_ms"] = {f"{k}_ms": round(v, 4) for k, v in avg.items()}
+    line["wall_s"] = round(wall, 3)
+    print(json.dumps(line), flush=True)
+
+
+def bench_semantic_pixel(args, torch):
+    """semantic_pixel_fb_2x512x1024_c19: the pixel-wise variant of the semantic workload -- per
+    pixel and direction the 19x19 matrix scaled by softplus(N(0,1)) weights (of image 0; the
+    op's matrices [4,H,W,C,C] are shared by the batch, 3 GB), unit edge weights, delta = C,
+    2 iterations forward + exact backward per step."""
+    from bplayers_amd import workloads, _native as nat
+    wl = workloads.semantic()
+    dev = torch.device("cuda:0")
+    cost = (-torch.from_numpy(wl["prob"])).to(dev).contiguous()
+    N, H, W, C = cost.shape
+    M = torch.from_numpy(wl["pw_mat"][0, 0]).to(dev)
+    w = torch.from_numpy(wl["edge"][0]).to(dev)  # [4,H,W]
+    jump = (w[:, :, :, None, None] * M).contiguous()
+    edge = torch.ones((N, 4, H, W), device=dev)
+    plan = LBPPlan(nat.LBP_SEMANTIC_PIXEL, cost, jump, 0, 1, edge, wl["delta"], wl["iters"])
+    ms, wall, kt = timed_steps(args.steps, args.warmup, plan.step, nat, torch)
+    value = plan.pl() * plan.iters / (ms * 1e-3) / 1e9
+    avg = {k: v[0] / v[1] for k, v in kt.items()}
+    it_ms = sum(avg[k] for k in LBP_CLASSES if k in avg) + avg.get("lbp_bwd_reduce", 0.0)
+    # per pixel and direction the sender's C x C matrix is read in the forward and read and
+    # written (its gradient) in the backward, once per image of the batch
+    it_bytes = plan.pl() * (FWD_B_PL + BWD_B_PL) + 4 * H * W * C * C * 4 * (N + 2 * N)
+    line = common_line(args, "G(pixel*label)/s per LBP iter fwd+bwd", value, "G(pixel*label)/s", ms,
+                       {"workload": args.workload, "N": N, "H": H, "W": W, "labels": C, "iters": plan.iters,
+                        "delta": plan.delta, "pairwise": "per-pixel CxC matrix per direction", "grad_mode": "exact"})
     line["roofline_iteration"] = {"bound": "hbm", "kernels": [k for k in avg if k.startswith("lbp")],
                                   "achieved": round(it_bytes / (it_ms * 1e-3) / 1e9, 2), "peak": HBM_PEAK_GBS,
                                   "unit": "GB/s", "frac": round(it_bytes / (it_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
@@ -536,7 +570,8 @@ def main():
     import torch
     from bplayers_amd import workloads
 
-    others = {"semantic_fb_2x512x1024_c19": bench_semantic, "flow_fb_256x512_r12": bench_flow,
+    others = {"semantic_fb_2x512x1024_c19": bench_semantic, "semantic_pixel_fb_2x512x1024_c19": bench_semantic_pixel,
+              "flow_fb_256x512_r12": bench_flow,
               "serving_c4": bench_serving}
     if args.workload in others:
         others[args.workload](args, torch)

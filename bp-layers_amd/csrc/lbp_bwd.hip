@@ -1075,6 +1075,202 @@ void launch_semg(const BwdArgs &a, bool vert, int ngroups, hipStream_t st) {
   else hipLaunchKernelGGL((lbp_bwd_semg<CM, false, 4>), grid, dim3(kWave), 0, st, a);
 }
 
+// Semantic per-pixel pairwise, 16 < C <= 32 or C <= 20 (both forward branches, both modes):
+// the adjoint step of lbp_bwd_semg, with the per-pixel parameter gradients written per step:
+// the sender's C x C matrix gradient (lane c writes column c: w g(c) at row argmin(c) -- row
+// gargmin for a truncation winner in the reference mode; in the exact mode a truncation winner
+// credits the truncation entry [0][C-1], which the lane owning column C-1 adds) and the edge
+// gradient (sum_c M[r][c] g(c) in the exact mode from the matrix column read per step; zero in
+// the reference mode, as the reference build).  IMGS (N == 2): the half-waves are the two images
+// of the same scanline and the batch-shared matrix gradient is the sum image 0 + image 1 formed
+// in registers (permlane32 swap), each half-wave writing half of the rows; otherwise (N == 1)
+// two scanlines of one image, each writing its full slices.
+template <int CM, bool VERT, int PF, bool IMGS, bool EXACT>
+__global__ __launch_bounds__(64) void lbp_bwd_semp(BwdArgs a) {
+  constexpr int SEG = 32, SPW = 2;
+  __shared__ float lds_g[SPW][CM];
+  __shared__ int lds_a[SPW][CM];
+  const int H = a.H, W = a.W, C = a.C;
+  const int L = VERT ? W : H;
+  const int len = VERT ? H : W;
+  const int nsteps = len - 1;
+  const int lane = threadIdx.x;
+  const int seg = lane / SEG;
+  const int c = lane - seg * SEG;
+  const int wpg = IMGS ? L : (L + SPW - 1) / SPW;
+  const int g_ = blockIdx.x / wpg;
+  const int wl = blockIdx.x - g_ * wpg;
+  const int n = IMGS ? seg : (g_ >> 1);
+  const int dir = VERT ? ((g_ & 1) ? UP : DOWN) : ((g_ & 1) ? LEFT : RIGHT);
+  const int rslot = dir == RIGHT ? LEFT : dir == LEFT ? RIGHT : dir == DOWN ? UP : DOWN;
+  const bool fwd = (dir == RIGHT || dir == DOWN);
+  const int line = IMGS ? wl : wl * SPW + seg;
+  const bool line_ok = line < L;
+  const int ln = line_ok ? line : 0;
+  const bool lab = c < C;
+  const bool st_ok = line_ok && lab;
+  const int cl = lab ? c : C - 1;
+  const int64_t HW = (int64_t)H * W;
+  const int64_t dq = VERT ? (fwd ? (int64_t)W : -(int64_t)W) : (fwd ? 1 : -1);
+  const int64_t q0 = VERT ? (int64_t)(fwd ? 0 : H - 1) * W + ln : (int64_t)ln * W + (fwd ? 0 : W - 1);
+  const int64_t img = (int64_t)n * HW * C;
+  const int64_t rsl = ((int64_t)n * 4 + rslot) * HW;
+  float *Xo = a.X[dir] + img;
+  const float *XDp = a.X[DOWN] + img;
+  const float *XUp = a.X[UP] + img;
+  const float *ig_p = a.in_grad + rsl * C;
+  const float *m_p = a.msgs + rsl * C;
+  const uint8_t *am_p = a.argmin + rsl * C;
+  const uint8_t *gl_p = a.gargmin + rsl;
+  const float *edge_p = a.edge + ((int64_t)n * 4 + dir) * HW;
+  float *ge_p = a.grad_edge + ((int64_t)n * 4 + dir) * HW;
+  const float *mat_p = EXACT ? a.jump + (int64_t)dir * HW * C * C + cl : nullptr;  // + (q * C + r) * C
+  float *gm_p = a.grad_jump + (int64_t)dir * HW * C * C + cl;
+
+  // the last position of the pass sends nothing: zero its outputs (and its matrix gradient,
+  // once per scanline)
+  {
+    const int64_t q = q0 + (int64_t)nsteps * dq;
+    if (st_ok) Xo[q * C + c] = 0.f;
+    if (line_ok && c == 0) ge_p[q] = 0.f;
+    if (line_ok && lab && (!IMGS || seg == 0))
+      for (int r = 0; r < C; ++r) gm_p[(q * C + r) * C] = 0.f;
+  }
+
+  constexpr int PJ = EXACT ? PF : 1;
+  float pg[PF], pm[PF], pd[PF], pu[PF], pw[PF], pj[PJ][CM];
+  int pa[PF], pl[PF];
+  auto load_step = [&](int s, int i) {
+    const int64_t q = q0 + (int64_t)s * dq;
+    const int64_t qr = q + dq;
+    pg[i] = ig_p[qr * C + cl];
+    pm[i] = m_p[qr * C + cl];
+    pa[i] = (int)am_p[qr * C + cl];
+    pl[i] = (int)gl_p[qr];
+    if (!VERT) {
+      pd[i] = XDp[qr * C + cl];
+      pu[i] = XUp[qr * C + cl];
+    } else {
+      pd[i] = pu[i] = 0.f;
+    }
+    pw[i] = edge_p[q];
+    if (EXACT) {
+#pragma unroll
+      for (int l = 0; l < CM; ++l) pj[EXACT ? i : 0][l] = mat_p[(q * C + (l < C ? l : C - 1)) * C];
+    }
+  };
+  if (nsteps > 0) {
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      const int s = nsteps - 1 - i;
+      load_step(s >= 0 ? s : 0, i);
+      __builtin_amdgcn_sched_barrier(0);  // keep the ring slots in issue order
+    }
+  }
+  float acc = 0.f;
+  for (int t0 = 0; t0 < nsteps; t0 += PF) {
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      const int t = t0 + i;
+      const int s = nsteps - 1 - t;  // adjoint order
+      if (t < nsteps) {
+        const int64_t q = q0 + (int64_t)s * dq;
+        const float tt = pg[i] + acc;
+        const float ts = seg_reduce<SEG>(lab ? tt : 0.f, OpSum());
+        float gk = tt - fast_exp(pm[i]) * ts;
+        if (!VERT) gk += pd[i] + pu[i];
+        gk = lab ? gk : 0.f;
+        const int A = pa[i];
+        const int gl = pl[i];
+        const bool tr = A == kFromTrunc;
+        const int r = tr ? gl : A;
+        if (c < CM) {
+          lds_g[seg][c] = gk;
+          lds_a[seg][c] = lab ? r : -1;
+        }
+        wave_lds_sync();
+        float gh = 0.f;
+#pragma unroll
+        for (int rr = 0; rr < CM; ++rr) gh += lds_a[seg][rr] == c ? lds_g[seg][rr] : 0.f;
+        wave_lds_sync();  // the gathers are done before the next step's writes
+        const float w = pw[i];
+        const bool to_tr = tr && EXACT;
+        // edge gradient
+        float ge = 0.f;
+        if (EXACT) {
+          // the truncation entry M[0][C-1] is the row-0 slot of the segment's lane C-1
+          const int j0 = __float_as_int(pj[EXACT ? i : 0][0]);
+          const float jt0 = __int_as_float(__builtin_amdgcn_readlane(j0, C - 1));
+          const float jt1 = __int_as_float(__builtin_amdgcn_readlane(j0, SEG + C - 1));
+          float mv = 0.f;
+#pragma unroll
+          for (int l = 0; l < CM; ++l) mv = l == A ? pj[EXACT ? i : 0][l] : mv;
+          mv = to_tr ? (seg ? jt1 : jt0) : mv;
+          ge = seg_reduce<SEG>(lab ? mv * gk : 0.f, OpSum());
+        }
+        if (line_ok && c == 0) ge_p[q] = ge;
+        // matrix gradient column c of the sender q
+        const float val = to_tr ? 0.f : w * gk;
+        float ex = 0.f;
+        if (EXACT) {
+          const float fsum = seg_reduce<SEG>(lab && tr ? gk : 0.f, OpSum());
+          ex = w * fsum;
+        }
+        const bool own_ex = EXACT && c == C - 1;
+        if (IMGS) {
+          float v0, v1, x0, x1;
+          int r0, r1;
+          swap_halves(val, v0, v1);
+          swap_halves(ex, x0, x1);
+          swap_halves(r, r0, r1);
+          const int rlo = seg ? C / 2 : 0, rhi = seg ? C : C / 2;
+          if (line_ok && lab) {
+            for (int rr = rlo; rr < rhi; ++rr) {
+              float e0 = rr == r0 ? v0 : 0.f, e1 = rr == r1 ? v1 : 0.f;
+              if (own_ex && rr == 0) {
+                e0 += x0;
+                e1 += x1;
+              }
+              gm_p[(q * C + rr) * C] = e0 + e1;
+            }
+          }
+        } else if (line_ok && lab) {
+          for (int rr = 0; rr < C; ++rr) {
+            float e = rr == r ? val : 0.f;
+            if (own_ex && rr == 0) e += ex;
+            gm_p[(q * C + rr) * C] = e;
+          }
+        }
+        acc = gh;
+        if (st_ok) Xo[q * C + c] = gh;
+      }
+      load_step(s - PF >= 0 ? s - PF : 0, i);
+    }
+  }
+}
+
+template <int CM, int PF>
+void launch_semp(const BwdArgs &a, bool vert, hipStream_t st) {
+  const int L = vert ? a.W : a.H;
+  const bool imgs = a.N == 2;
+  const bool exact = a.mode == BPL_GRAD_EXACT;
+  const dim3 grid((unsigned)(imgs ? 2 * L : a.N * 2 * ((L + 1) / 2)));
+#define BPL_SEMP_LAUNCH(IM, EX)                                                                        \
+  do {                                                                                                 \
+    if (vert) hipLaunchKernelGGL((lbp_bwd_semp<CM, true, PF, IM, EX>), grid, dim3(kWave), 0, st, a);  \
+    else hipLaunchKernelGGL((lbp_bwd_semp<CM, false, PF, IM, EX>), grid, dim3(kWave), 0, st, a);      \
+  } while (0)
+  if (imgs) {
+    if (exact) BPL_SEMP_LAUNCH(true, true); else BPL_SEMP_LAUNCH(true, false);
+  } else {
+    if (exact) BPL_SEMP_LAUNCH(false, true); else BPL_SEMP_LAUNCH(false, false);
+  }
+#undef BPL_SEMP_LAUNCH
+}
+
+// the per-pixel kernels cover C <= 32 for batches of one or two images
+bool semp_ok(const BwdArgs &a) { return a.C <= 32 && a.N <= 2 && !generic_only(); }
+
 // y-boundary terms of the input-message gradient after the paired horizontal phase (which
 // wrote X_R + X_L into the U and D slots): U slot of row 0 = (in_grad_U + X_D) + (X_R + X_L),
 // D slot of row H - 1 = (in_grad_D + X_U) + (X_R + X_L), the combine kernel's expressions.
@@ -1318,7 +1514,8 @@ extern "C" int bpl_lbp_bwd(int kind, int mode, const float *jump, int64_t jump_k
   a.grad_cost = grad_cost; a.grad_msg = grad_msg;
   hipStream_t st = as_stream(stream);
   // parameter gradients shared across the batch accumulate over n in launch order
-  const bool per_n = (kind == BPL_LBP_STEREO && a.Nj == 1 && N > 1) || (kind == BPL_LBP_SEMANTIC_PIXEL && N > 1);
+  const bool semp = kind == BPL_LBP_SEMANTIC_PIXEL && semp_ok(a);
+  const bool per_n = (kind == BPL_LBP_STEREO && a.Nj == 1 && N > 1) || (kind == BPL_LBP_SEMANTIC_PIXEL && N > 1 && !semp);
   a.accum = 0;
   const bool pair = kind == BPL_LBP_STEREO && !per_n && stereo_pair_ok(a);
   for (int phase = 0; phase < 2; ++phase) {
@@ -1337,7 +1534,11 @@ extern "C" int bpl_lbp_bwd(int kind, int mode, const float *jump, int64_t jump_k
           else if (semg_ok(a)) launch_semg<32>(a, vert, ng, st);
           else launch_kind<BPL_LBP_SEMANTIC_GLOBAL>(a, vert, ng, st);
           break;
-        default: launch_kind<BPL_LBP_SEMANTIC_PIXEL>(a, vert, ng, st); break;
+        default:
+          if (semp && C <= 20) launch_semp<20, 4>(a, vert, st);
+          else if (semp) launch_semp<32, 2>(a, vert, st);
+          else launch_kind<BPL_LBP_SEMANTIC_PIXEL>(a, vert, ng, st);
+          break;
       }
       int rc = check_hip(hipGetLastError(), "bpl_lbp_bwd phase launch");
       if (rc) return rc;

@@ -412,6 +412,145 @@ void launch_semg(const FwdArgs &a, bool vert, hipStream_t st) {
 
 
 // ---------------------------------------------------------------------------------------
+// Semantic per-pixel pairwise, full min-plus branch (delta >= C), C <= CM (20 or 32): the
+// lane layout and step of lbp_fwd_semg, with the sender's matrix column M_q[dir][.][c] read
+// per step into the prefetch ring (one coalesced row of C floats per l).  IMGS: the two
+// half-waves are the two images of the same scanline (N == 2), so both read the same matrix
+// addresses in one instruction; otherwise two scanlines of one image.
+// ---------------------------------------------------------------------------------------
+template <int CM, bool VERT, int PF, bool IMGS>
+__global__ __launch_bounds__(64) void lbp_fwd_semp(FwdArgs a) {
+  constexpr int SEG = 32, SPW = 2;
+  __shared__ float lds_h[SPW][CM];
+  const int H = a.H, W = a.W, C = a.C;
+  const int L = VERT ? W : H;
+  const int len = VERT ? H : W;
+  const int nsteps = len - 1;
+  const int lane = threadIdx.x;
+  const int seg = lane / SEG;
+  const int c = lane - seg * SEG;
+  const int wpg = IMGS ? L : (L + SPW - 1) / SPW;
+  const int g = blockIdx.x / wpg;
+  const int wl = blockIdx.x - g * wpg;
+  const int n = IMGS ? seg : g >> 1;
+  const int dir = VERT ? ((g & 1) ? UP : DOWN) : ((g & 1) ? LEFT : RIGHT);
+  const int rslot = dir == RIGHT ? LEFT : dir == LEFT ? RIGHT : dir == DOWN ? UP : DOWN;
+  const bool fwd = (dir == RIGHT || dir == DOWN);
+  const int line = IMGS ? wl : wl * SPW + seg;
+  const bool line_ok = line < L;
+  const int ln = line_ok ? line : 0;
+  const bool lab = c < C;
+  const bool st_ok = line_ok && lab;
+  const int cl = lab ? c : C - 1;  // label loaded by this lane
+  const int64_t HW = (int64_t)H * W;
+  const int64_t dq = VERT ? (fwd ? (int64_t)W : -(int64_t)W) : (fwd ? 1 : -1);
+  const int64_t q0 = VERT ? (int64_t)(fwd ? 0 : H - 1) * W + ln : (int64_t)ln * W + (fwd ? 0 : W - 1);
+  const float *cost_p = a.cost + (int64_t)n * HW * C;
+  const float *x1_p = (VERT ? a.msg_out : a.msg_in) + ((int64_t)n * 4 + (VERT ? RIGHT : UP)) * HW * C;
+  const float *x2_p = (VERT ? a.msg_out : a.msg_in) + ((int64_t)n * 4 + (VERT ? LEFT : DOWN)) * HW * C;
+  float *out_p = a.msg_out + ((int64_t)n * 4 + rslot) * HW * C;
+  uint8_t *arg_p = a.argmin ? a.argmin + ((int64_t)n * 4 + rslot) * HW * C : nullptr;
+  const int64_t rs_base = ((int64_t)n * 4 + rslot) * HW;
+  const float *edge_p = a.edge + ((int64_t)n * 4 + dir) * HW;
+  const float *mat_p = a.jump + (int64_t)dir * HW * C * C + cl;  // + (q * C + l) * C
+
+  float chain = a.msg_in[(rs_base + q0) * C + cl];
+  if (st_ok) {
+    out_p[q0 * C + c] = chain;
+    if (arg_p) arg_p[q0 * C + c] = 0;
+  }
+  if (line_ok && c == 0) {
+    if (a.scale) a.scale[rs_base + q0] = 0.f;
+    if (a.gargmin) a.gargmin[rs_base + q0] = 0;
+  }
+
+  float pc[PF], p1[PF], p2[PF], pw[PF], pm[PF][CM];
+  auto load_step = [&](int s, int i) {
+    const int64_t q = q0 + (int64_t)s * dq;
+    pc[i] = cost_p[q * C + cl];
+    p1[i] = x1_p[q * C + cl];
+    p2[i] = x2_p[q * C + cl];
+    pw[i] = edge_p[q];
+#pragma unroll
+    for (int l = 0; l < CM; ++l) pm[i][l] = mat_p[(q * C + (l < C ? l : C - 1)) * C];
+  };
+#pragma unroll
+  for (int i = 0; i < PF; ++i) {
+    load_step(i < nsteps ? i : (nsteps > 0 ? nsteps - 1 : 0), i);
+    __builtin_amdgcn_sched_barrier(0);  // keep the ring slots in issue order
+  }
+  for (int s0 = 0; s0 < nsteps; s0 += PF) {
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      const int s = s0 + i;
+      if (s < nsteps) {
+        float h = VERT ? ((pc[i] + p1[i]) + p2[i]) + chain : ((pc[i] + chain) + p1[i]) + p2[i];
+        if (c < CM) lds_h[seg][c] = lab ? h : __builtin_inff();
+        wave_lds_sync();
+        const float w = pw[i];
+        float nv[CM];
+#pragma unroll
+        for (int l = 0; l < CM; ++l) nv[l] = fmaf(pm[i][l], w, lds_h[seg][l]);
+        wave_lds_sync();  // the reads are done before the next step's write
+        float m[CM + 1];
+#pragma unroll
+        for (int l = 0; l < CM; ++l) m[l] = nv[l];
+        m[CM] = 1e15f;
+        int cnt = CM + 1;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+          if (cnt > 1) {
+            int o = 0;
+#pragma unroll
+            for (int t = 0; t < CM + 1; t += 3) {
+              if (t < cnt) {
+                float v = m[t];
+                if (t + 1 < cnt) v = fminf(v, m[t + 1]);
+                if (t + 2 < cnt) v = fminf(v, m[t + 2]);
+                m[o++] = v;
+              }
+            }
+            cnt = o;
+          }
+        }
+        const float msg = m[0];
+        int arg = 0;
+#pragma unroll
+        for (int l = 0; l < CM; ++l) arg = nv[l] == msg ? l : arg;
+        float mx = seg_reduce<SEG>(lab ? msg : -__builtin_inff(), OpMax());
+        float se = seg_reduce<SEG>(lab ? fast_exp(msg - mx) : 0.f, OpSum());
+        const float lse = mx + fast_log(se);
+        chain = msg - lse;
+        const int64_t qq = q0 + (int64_t)(s + 1) * dq;
+        if (st_ok) {
+          out_p[qq * C + c] = chain;
+          if (arg_p) arg_p[qq * C + c] = (uint8_t)arg;
+        }
+        if (line_ok && c == 0) {
+          if (a.scale) a.scale[rs_base + qq] = fast_exp(lse);
+          if (a.gargmin) a.gargmin[rs_base + qq] = 0;
+        }
+      }
+      load_step(s + PF < nsteps ? s + PF : nsteps - 1, i);
+    }
+  }
+}
+
+template <int CM, int PF>
+void launch_semp(const FwdArgs &a, bool vert, hipStream_t st) {
+  const int L = vert ? a.W : a.H;
+  const bool imgs = a.N == 2;
+  const dim3 grid((unsigned)(imgs ? 2 * L : a.N * 2 * ((L + 1) / 2)));
+  if (imgs) {
+    if (vert) hipLaunchKernelGGL((lbp_fwd_semp<CM, true, PF, true>), grid, dim3(kWave), 0, st, a);
+    else hipLaunchKernelGGL((lbp_fwd_semp<CM, false, PF, true>), grid, dim3(kWave), 0, st, a);
+  } else {
+    if (vert) hipLaunchKernelGGL((lbp_fwd_semp<CM, true, PF, false>), grid, dim3(kWave), 0, st, a);
+    else hipLaunchKernelGGL((lbp_fwd_semp<CM, false, PF, false>), grid, dim3(kWave), 0, st, a);
+  }
+}
+
+// ---------------------------------------------------------------------------------------
 // Stereo fast path for 16 < C <= 256: one scanline per wave, label c = lane * LPL + k, the
 // candidate window from neighbouring lanes by DPP shifts (no LDS round trip), and no
 // per-label control flow in the step:
@@ -987,7 +1126,11 @@ extern "C" int bpl_lbp_fwd(int kind, const float *cost, const float *jump, int64
         else if (a.delta >= a.C && a.C <= 32 && !generic_only()) launch_semg<32>(a, vert, st);
         else launch_kind<BPL_LBP_SEMANTIC_GLOBAL>(a, vert, st, shm);
         break;
-      default: launch_kind<BPL_LBP_SEMANTIC_PIXEL>(a, vert, st, shm); break;
+      default:
+        if (a.delta >= a.C && a.C <= 20 && !generic_only()) launch_semp<20, 4>(a, vert, st);
+        else if (a.delta >= a.C && a.C <= 32 && !generic_only()) launch_semp<32, 2>(a, vert, st);
+        else launch_kind<BPL_LBP_SEMANTIC_PIXEL>(a, vert, st, shm);
+        break;
     }
     int rc = check_hip(hipGetLastError(), "bpl_lbp_fwd launch");
     if (rc) return rc;

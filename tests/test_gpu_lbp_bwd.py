@@ -84,6 +84,12 @@ CASES = [
     ("semantic_global", 2, 7, 5, 32, 32, {}),
     ("semantic_pixel", 1, 4, 5, 7, 7, {}),
     ("semantic_pixel", 2, 4, 5, 19, 19, {}),
+    ("semantic_pixel", 2, 5, 7, 19, 19, {}),
+    ("semantic_pixel", 2, 3, 9, 25, 25, {}),
+    ("semantic_pixel", 1, 4, 5, 32, 32, {}),
+    ("semantic_pixel", 1, 5, 6, 21, 21, {}),
+    ("semantic_pixel", 2, 4, 6, 19, 3, {}),
+    ("semantic_pixel", 3, 3, 4, 7, 7, {}),
 ]
 
 

@@ -104,6 +104,11 @@ CASES = [
     ("semantic_global", 1, 5, 7, 7, 7, {}),
     ("semantic_pixel", 1, 5, 6, 7, 7, {}),
     ("semantic_pixel", 1, 4, 5, 19, 19, {}),
+    ("semantic_pixel", 2, 5, 7, 19, 19, {}),
+    ("semantic_pixel", 2, 3, 9, 25, 25, {}),
+    ("semantic_pixel", 1, 4, 5, 32, 32, {}),
+    ("semantic_pixel", 1, 5, 6, 19, 3, {}),
+    ("semantic_pixel", 3, 3, 4, 7, 7, {}),
 ]
 
 
@@ -111,12 +116,10 @@ CASES = [
 @pytest.mark.parametrize("inplace", [True, False])
 @pytest.mark.parametrize("impl", ["default", "generic"])
 def test_lbp_forward_matches_oracle(case, inplace, impl, monkeypatch):
-    """'default' runs the stereo fast path (DPP window) where it applies; 'generic' forces the
-    generic kernels (BPL_LBP_GENERIC=1)."""
+    """'default' runs the specialised kernels where they apply (stereo 64-lane kernels, semantic
+    full min-plus kernels); 'generic' forces the generic kernels (BPL_LBP_GENERIC=1)."""
     kind, N, H, W, C, delta, extra = case
     if impl == "generic":
-        if kind != "stereo":
-            pytest.skip("the non-stereo variants only have the generic kernels")
         monkeypatch.setenv("BPL_LBP_GENERIC", "1")
     print(f"case {case} inplace={inplace} impl={impl}", flush=True)
     cost, jump, edge, M0 = make_case(kind, N, H, W, C, delta, seed=hash((N, H, W, C)) % 1000, **extra)
