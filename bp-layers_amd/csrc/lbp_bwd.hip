@@ -932,8 +932,7 @@ __global__ __launch_bounds__(PAIR ? 2 * kWave : kWave) void lbp_bwd_stereo_w64(B
 template <int CM, bool VERT, int PF>
 __global__ __launch_bounds__(64) void lbp_bwd_semg(BwdArgs a) {
   constexpr int SEG = 32, SPW = 2;
-  __shared__ float lds_g[SPW][CM];
-  __shared__ int lds_a[SPW][CM];
+  __shared__ __align__(16) int2 lds_ag[SPW][CM];  // {routing target, gradient bits}
   __shared__ float lds_mat[CM * CM];       // the direction's matrix [r][c], stride CM
   __shared__ float lds_part[SPW][CM * CM];  // the segment's matrix-gradient partial [r][c]
   const int H = a.H, W = a.W, C = a.C;
@@ -1031,14 +1030,17 @@ __global__ __launch_bounds__(64) void lbp_bwd_semg(BwdArgs a) {
         const bool tr = A == kFromTrunc;   // the value came from the truncation term
         const int r = tr ? gl : A;         // routing target and reference-mode matrix row
         if (c < CM) {
-          lds_g[seg][c] = gk;
-          lds_a[seg][c] = lab ? r : -1;
+          lds_ag[seg][c] = make_int2(lab ? r : -1, __float_as_int(gk));
         }
         wave_lds_sync();
         // route: sender label c gathers the receivers routed to it, in label order
         float gh = 0.f;
 #pragma unroll
-        for (int rr = 0; rr < CM; ++rr) gh += lds_a[seg][rr] == c ? lds_g[seg][rr] : 0.f;
+        for (int rr = 0; rr < CM; rr += 2) {  // two entries per 16-byte read, in label order
+          const int4 v = *reinterpret_cast<const int4 *>(&lds_ag[seg][rr]);
+          gh += v.x == c ? __int_as_float(v.y) : 0.f;
+          gh += v.z == c ? __int_as_float(v.w) : 0.f;
+        }
         // parameter gradients of (dir, sender): edge sum_c M[r][c] g(c) and matrix entry
         // [r][c] (a truncation winner uses the truncation entry [0][C-1] in the exact mode)
         const float w = pw[i];
@@ -1088,8 +1090,7 @@ void launch_semg(const BwdArgs &a, bool vert, int ngroups, hipStream_t st) {
 template <int CM, bool VERT, int PF, bool IMGS, bool EXACT>
 __global__ __launch_bounds__(64) void lbp_bwd_semp(BwdArgs a) {
   constexpr int SEG = 32, SPW = 2;
-  __shared__ float lds_g[SPW][CM];
-  __shared__ int lds_a[SPW][CM];
+  __shared__ __align__(16) int2 lds_ag[SPW][CM];  // {routing target, gradient bits}
   const int H = a.H, W = a.W, C = a.C;
   const int L = VERT ? W : H;
   const int len = VERT ? H : W;
@@ -1185,13 +1186,16 @@ __global__ __launch_bounds__(64) void lbp_bwd_semp(BwdArgs a) {
         const bool tr = A == kFromTrunc;
         const int r = tr ? gl : A;
         if (c < CM) {
-          lds_g[seg][c] = gk;
-          lds_a[seg][c] = lab ? r : -1;
+          lds_ag[seg][c] = make_int2(lab ? r : -1, __float_as_int(gk));
         }
         wave_lds_sync();
         float gh = 0.f;
 #pragma unroll
-        for (int rr = 0; rr < CM; ++rr) gh += lds_a[seg][rr] == c ? lds_g[seg][rr] : 0.f;
+        for (int rr = 0; rr < CM; rr += 2) {  // two entries per 16-byte read, in label order
+          const int4 v = *reinterpret_cast<const int4 *>(&lds_ag[seg][rr]);
+          gh += v.x == c ? __int_as_float(v.y) : 0.f;
+          gh += v.z == c ? __int_as_float(v.w) : 0.f;
+        }
         wave_lds_sync();  // the gathers are done before the next step's writes
         const float w = pw[i];
         const bool to_tr = tr && EXACT;
