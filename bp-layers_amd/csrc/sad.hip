@@ -460,38 +460,133 @@ __global__ __launch_bounds__(512) void sad_flow_fwd_k(const float *__restrict__ 
   }
 }
 
-// Generic window size: one thread per (pixel), S(u,v) evaluated on the fly, O(K^2 C).  The
-// vertical displacements are v in [-h, h] (h = sws / 2): for odd sws the last row of the
-// K x K window (v_idx = K - 1) keeps the fill value, as in the reference's v loop.
-__global__ void sad_flow_fwd_generic_k(const float *__restrict__ f0, const float *__restrict__ f1, float *__restrict__ cv_u,
-                                       float *__restrict__ cv_v, float *__restrict__ u_star, float *__restrict__ v_star,
-                                       int N, int C, int H, int W, int sws, int ou, int ov, int bu, int bv) {
+// Any window size: the block layout and staging of sad_flow_fwd_k with the horizontal
+// displacements taken in chunks of FT_UC (S, and the running u-projection, for one chunk in
+// registers).  The v-projection of a chunk is merged into cv_v / v_star by the thread that owns
+// the (pixel, v) entry: a later chunk replaces it only when strictly smaller, so the first u
+// index of the minimum wins as in the reference's scan.  The vertical displacements run over
+// [-h, h] (h = sws / 2): for odd sws the row v_idx = K - 1 keeps the fill value.
+constexpr int FT_UC = 32;
+
+__global__ __launch_bounds__(512) void sad_flow_fwd_tiled_k(const float *__restrict__ f0, const float *__restrict__ f1,
+                                                            float *__restrict__ cv_u, float *__restrict__ cv_v,
+                                                            float *__restrict__ u_star, float *__restrict__ v_star, int C,
+                                                            int H, int W, int sws, int ou, int ov, int bu, int bv) {
+  constexpr int G = FL_G, UC = FT_UC;
+  constexpr int SW = FL_TX + UC - 1;  // staged f1 columns
+  constexpr int SR = FL_TY + G - 1;   // staged f1 rows (4 consecutive displacements)
+  constexpr int NP = FL_TY * FL_TX;
+  constexpr size_t kStage = sizeof(float) * (FL_CB * SR * SW + FL_CB * FL_TY * FL_TX);
+  constexpr size_t kMerge = (sizeof(float) + sizeof(int)) * (G - 1) * NP * UC;
+  __shared__ __align__(16) unsigned char lds[kStage > kMerge ? kStage : kMerge];
+  auto s1 = reinterpret_cast<float (*)[SR][SW]>(lds);
+  auto s0 = reinterpret_cast<float (*)[FL_TY][FL_TX]>(lds + sizeof(float) * FL_CB * SR * SW);
+  auto m_v = reinterpret_cast<float (*)[NP][UC]>(lds);
+  auto m_i = reinterpret_cast<int (*)[NP][UC]>(lds + sizeof(float) * (G - 1) * NP * UC);
   const int K = sws + 1, hs = sws / 2;
+  const int NI = (K + G - 1) / G;
+  const int x0 = blockIdx.x * FL_TX;
+  const int y0 = blockIdx.y * FL_TY;
+  const int n = blockIdx.z;
+  const int t = threadIdx.x;
+  const int g = t / NP;
+  const int pl = t - g * NP;
+  const int ty = pl / FL_TX, tx = pl - ty * FL_TX;
+  const int y = y0 + ty, x = x0 + tx;
+  const bool inside = y < H && x < W;
   const int64_t HW = (int64_t)H * W;
-  const int64_t total = (int64_t)N * HW;
-  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < total; p += (int64_t)gridDim.x * blockDim.x) {
-    const int n = (int)(p / HW);
-    const int y = (int)((p / W) % H), x = (int)(p % W);
-    const float *F0 = f0 + (int64_t)n * C * HW + (int64_t)y * W + x;
-    const float *F1 = f1 + (int64_t)n * C * HW;
-    float *cu = cv_u + p * K, *cvv = cv_v + p * K, *us = u_star + p * K, *vs = v_star + p * K;
-    for (int k = 0; k < K; ++k) { cu[k] = 9999999.f; us[k] = (float)(bv * sws); }
-    for (int vi = 0; vi < K; ++vi) {
-      const int yy = y + ov + vi - hs;
-      float mv = 9999999.f;
-      int mi = 0;
-      for (int ui = 0; ui < K; ++ui) {
-        const int xx = x + ou + ui - hs;
-        float s = kFill;
-        if (vi <= 2 * hs && yy >= 0 && yy < H && xx >= 0 && xx < W) {
-          s = 0.f;
-          for (int c = 0; c < C; ++c) s += fabsf(F0[(int64_t)c * HW] - F1[(int64_t)c * HW + (int64_t)yy * W + xx]);
+  const float *F0 = f0 + (int64_t)n * C * HW;
+  const float *F1 = f1 + (int64_t)n * C * HW;
+  const int64_t obase = (((int64_t)n * H + y) * W + x) * K;
+
+  for (int u0 = 0; u0 < K; u0 += UC) {
+    const int nu = min(UC, K - u0);
+    float bmin[UC];
+    int barg[UC];
+#pragma unroll
+    for (int u = 0; u < UC; ++u) { bmin[u] = 9999999.f; barg[u] = 0; }
+    for (int i = 0; i < NI; ++i) {
+      const int vi = g + G * i;
+      const int dv0 = ov + G * i - hs;  // displacement of group 0 at this round
+      float S[UC];
+#pragma unroll
+      for (int u = 0; u < UC; ++u) S[u] = 0.f;
+      for (int c0 = 0; c0 < C; c0 += FL_CB) {
+        __syncthreads();
+        for (int e = t; e < FL_CB * SR * SW; e += G * NP) {
+          const int cc = e / (SR * SW);
+          const int r = e - cc * SR * SW;
+          const int ry = r / SW, j = r - ry * SW;
+          const int c = c0 + cc, yy = y0 + ry + dv0, xx = x0 + ou - hs + u0 + j;
+          s1[cc][ry][j] = (c < C && yy >= 0 && yy < H && xx >= 0 && xx < W) ? F1[(int64_t)c * HW + (int64_t)yy * W + xx] : 0.f;
         }
-        if (s < mv) { mv = s; mi = ui; }
-        if (s < cu[ui]) { cu[ui] = s; us[ui] = (float)(vi + bv * sws); }
+        for (int e = t; e < FL_CB * NP; e += G * NP) {
+          const int cc = e / NP;
+          const int r = e - cc * NP;
+          const int ry = r / FL_TX, j = r - ry * FL_TX;
+          const int c = c0 + cc, yy = y0 + ry, xx = x0 + j;
+          s0[cc][ry][j] = (c < C && yy < H && xx < W) ? F0[(int64_t)c * HW + (int64_t)yy * W + xx] : 0.f;
+        }
+        __syncthreads();
+        if (vi < K) {
+          const int nc = min(FL_CB, C - c0);
+          for (int cc = 0; cc < nc; ++cc) {
+            const float a = s0[cc][ty][tx];
+            const float *row = &s1[cc][ty + g][tx];
+#pragma unroll
+            for (int u = 0; u < UC; ++u) S[u] += fabsf(a - row[u]);
+          }
+        }
       }
-      cvv[vi] = mv;
-      vs[vi] = (float)(mi + bu * sws);
+      if (vi < K) {
+        const int dv = dv0 + g;
+        const bool rowok = vi <= 2 * hs && y + dv >= 0 && y + dv < H;
+        float mv = 9999999.f;
+        int mi = 0;
+#pragma unroll
+        for (int u = 0; u < UC; ++u) {
+          if (u < nu) {
+            const int xx = x + ou + u0 + u - hs;
+            const float sv = (rowok && xx >= 0 && xx < W) ? S[u] : kFill;
+            if (sv < mv) { mv = sv; mi = u0 + u; }
+            if (sv < bmin[u]) { bmin[u] = sv; barg[u] = vi; }
+          }
+        }
+        if (inside) {
+          const int64_t o = obase + vi;
+          if (u0 == 0 || mv < cv_v[o]) {
+            cv_v[o] = mv;
+            v_star[o] = (float)(mi + bu * sws);
+          }
+        }
+      }
+    }
+    // merge the groups' u-projections of this chunk in group 0 (smaller vi wins ties)
+    __syncthreads();  // the staging tiles are dead from here on; their space takes the partials
+    if (g > 0) {
+#pragma unroll
+      for (int u = 0; u < UC; ++u) {
+        m_v[g - 1][pl][u] = bmin[u];
+        m_i[g - 1][pl][u] = barg[u];
+      }
+    }
+    __syncthreads();
+    if (g == 0 && inside) {
+#pragma unroll
+      for (int u = 0; u < UC; ++u) {
+        if (u < nu) {
+          float bv_ = bmin[u];
+          int bi = barg[u];
+#pragma unroll
+          for (int gg = 1; gg < G; ++gg) {
+            const float v2 = m_v[gg - 1][pl][u];
+            const int i2 = m_i[gg - 1][pl][u];
+            if (v2 < bv_ || (v2 == bv_ && i2 < bi)) { bv_ = v2; bi = i2; }
+          }
+          cv_u[obase + u0 + u] = bv_;
+          u_star[obase + u0 + u] = (float)(bi + bv * sws);
+        }
+      }
     }
   }
 }
@@ -1007,9 +1102,10 @@ extern "C" int bpl_sad_flow_fwd(const float *f0, const float *f1, float *cv_u, f
     hipLaunchKernelGGL(sad_flow_fwd_k<25>, grid, dim3(FL_G * FL_TY * FL_TX), 0, st, f0, f1, cv_u, cv_v, u_star, v_star, (int)C,
                        (int)H, (int)W, sws, offset_u, offset_v, block_u, block_v);
   } else {
-    hipLaunchKernelGGL(sad_flow_fwd_generic_k, dim3(grid_for(N * H * W, 128)), dim3(128), 0, st, f0, f1, cv_u,
-                       cv_v, u_star, v_star, (int)N, (int)C, (int)H, (int)W, sws, offset_u, offset_v, block_u,
-                       block_v);
+    BPL_REQUIRE(N < 65536 && (H + FL_TY - 1) / FL_TY < 65536, BPL_E_LIMIT, "bpl_sad_flow_fwd: N or H too large");
+    dim3 grid((unsigned)((W + FL_TX - 1) / FL_TX), (unsigned)((H + FL_TY - 1) / FL_TY), (unsigned)N);
+    hipLaunchKernelGGL(sad_flow_fwd_tiled_k, grid, dim3(FL_G * FL_TY * FL_TX), 0, st, f0, f1, cv_u, cv_v, u_star,
+                       v_star, (int)C, (int)H, (int)W, sws, offset_u, offset_v, block_u, block_v);
   }
   return check_hip(hipGetLastError(), "bpl_sad_flow_fwd launch");
 }

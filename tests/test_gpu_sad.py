@@ -50,7 +50,8 @@ def test_stereo_sad_backward(N, C, H, W, mn, mx):
 
 
 @pytest.mark.parametrize("sws,ou,ov,bu,bv", [(24, 0, 0, 0, 0), (24, 3, -2, 1, 1), (8, 0, 0, 0, 0), (10, -5, 5, 0, 1),
-                                            (5, 0, 0, 0, 0), (7, 2, -1, 1, 0)])
+                                            (5, 0, 0, 0, 0), (7, 2, -1, 1, 0), (40, 0, 0, 0, 0), (33, 2, -3, 1, 0),
+                                            (64, -4, 4, 0, 1)])
 def test_flow_sad_forward(sws, ou, ov, bu, bv):
     from bplayers_amd import pytorch_cuda_flow_mp_sad_op as op
     rng = np.random.default_rng(sws)
