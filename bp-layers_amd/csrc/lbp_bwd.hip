@@ -544,25 +544,16 @@ __global__ __launch_bounds__(PAIR ? 2 * kWave : kWave) void lbp_bwd_stereo_w64(B
           ar[k] = (valid[k] && !mk) ? A[k] : -1;
         }
         const float fsum = wave_sum_f(fs);
-        // routing over offsets -3..3 from the neighbour lanes
-        float gL[ND][LPL], gR[ND][LPL];
-        int aL[ND][LPL], aR[ND][LPL];
+        // routing over offsets -3..3: gd[kk][k] = g of receiver c0 + k if its argmin is
+        // c0 + k + kk, else 0; sender label c0 + j collects gd[kk] of receiver c0 + j - kk
+        // (this lane or, through DPP shifts, the neighbour lanes), kk ascending
+        float gd[7][LPL];
 #pragma unroll
         for (int k = 0; k < LPL; ++k) {
-          gL[0][k] = shr1_f(gv[k], 0.f);
-          gR[0][k] = shl1_f(gv[k], 0.f);
-          aL[0][k] = shr1_i(ar[k], -1);
-          aR[0][k] = shl1_i(ar[k], -1);
+          const int o = ar[k] >= 0 ? ar[k] - (c0 + k) : 99;
+#pragma unroll
+          for (int kk = -3; kk <= 3; ++kk) gd[kk + 3][k] = o == kk ? gv[k] : 0.f;
         }
-#pragma unroll
-        for (int d = 1; d < ND; ++d)
-#pragma unroll
-          for (int k = 0; k < LPL; ++k) {
-            gL[d][k] = shr1_f(gL[d - 1][k], 0.f);
-            gR[d][k] = shl1_f(gR[d - 1][k], 0.f);
-            aL[d][k] = shr1_i(aL[d - 1][k], -1);
-            aR[d][k] = shl1_i(aR[d - 1][k], -1);
-          }
         float gh[LPL];
 #pragma unroll
         for (int k = 0; k < LPL; ++k) gh[k] = c0 + k == gl ? fsum : 0.f;
@@ -572,20 +563,20 @@ __global__ __launch_bounds__(PAIR ? 2 * kWave : kWave) void lbp_bwd_stereo_w64(B
           for (int k = 0; k < LPL; ++k) {
             const int e = k - kk;
             float gc;
-            int ac;
             if (e >= 0 && e < LPL) {
-              gc = gv[e >= 0 && e < LPL ? e : 0];
-              ac = ar[e >= 0 && e < LPL ? e : 0];
+              gc = gd[kk + 3][e >= 0 && e < LPL ? e : 0];
             } else if (e < 0) {
               const int j = -e, d = (j + LPL - 1) / LPL, slot = d * LPL - j;
-              gc = gL[d - 1][slot];
-              ac = aL[d - 1][slot];
+              gc = gd[kk + 3][slot];
+#pragma unroll
+              for (int m = 0; m < d; ++m) gc = shr1_f(gc, 0.f);
             } else {
               const int j = e - LPL + 1, d = (j + LPL - 1) / LPL, slot = j - 1 - (d - 1) * LPL;
-              gc = gR[d - 1][slot];
-              ac = aR[d - 1][slot];
+              gc = gd[kk + 3][slot];
+#pragma unroll
+              for (int m = 0; m < d; ++m) gc = shl1_f(gc, 0.f);
             }
-            gh[k] += ac == c0 + k ? gc : 0.f;
+            gh[k] += gc;
           }
         }
 #pragma unroll
@@ -596,13 +587,26 @@ __global__ __launch_bounds__(PAIR ? 2 * kWave : kWave) void lbp_bwd_stereo_w64(B
         float vb[8];
 #pragma unroll
         for (int b = 0; b < 8; ++b) vb[b] = 0.f;
+        // bin of a receiver routed by offset kk: clamp(K/2 - kk, 1, K-1); the truncation
+        // winners go to bin K-1 (exact mode) or to the bin of their distance to gargmin
 #pragma unroll
-        for (int k = 0; k < LPL; ++k) {
-          const bool mk = A[k] == kFromTrunc;
-          const int src = mk ? gl : A[k];
-          const int bin = (mk && exact) ? KK - 1 : min(max(KK / 2 + c0 + k - src, 1), KK - 1);
+        for (int kk = -3; kk <= 3; ++kk) {
+          const int b = min(max(KK / 2 - kk, 1), KK - 1);
+          float sk = 0.f;
 #pragma unroll
-          for (int b = 1; b < KK; ++b) vb[b] += bin == b ? gv[k] : 0.f;
+          for (int k = 0; k < LPL; ++k) sk += gd[kk + 3][k];
+          vb[b] += sk;
+        }
+        if (exact) {
+          vb[KK - 1] += fs;
+        } else {
+#pragma unroll
+          for (int k = 0; k < LPL; ++k) {
+            const bool mk = A[k] == kFromTrunc;
+            const int bin = mk ? min(max(KK / 2 + c0 + k - gl, 1), KK - 1) : 0;
+#pragma unroll
+            for (int b = 1; b < KK; ++b) vb[b] += bin == b ? gv[k] : 0.f;
+          }
         }
 #pragma unroll
         for (int b = 1; b < KK; ++b) vb[0] = fmaf(readlane_f(pv, b), vb[b], vb[0]);
