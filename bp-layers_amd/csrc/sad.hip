@@ -1119,10 +1119,11 @@ extern "C" int bpl_sad_flow_bwd(const float *f0, const float *f1, const float *g
   BPL_REQUIRE(sws >= 0 && sws <= 1024, BPL_E_ARG, "bpl_sad_flow_bwd: sws=%d out of range", sws);
   hipStream_t st = as_stream(stream);
   KernelTimer kt(BPL_KERNEL_SAD_FLOW_BWD, st);
-  if (sws + 1 <= FM_KMAX) {
-    BPL_REQUIRE(N < 65536 && (H + FM_TY - 1) / FM_TY < 65536, BPL_E_LIMIT, "bpl_sad_flow_bwd: N or H too large");
-    BPL_REQUIRE(H * W * (sws + 1 > FM_CH ? sws + 1 : FM_CH) * 4 < (1LL << 31), BPL_E_LIMIT,
-                "bpl_sad_flow_bwd: H * W * K too large");
+  // the landing-mask kernel: K <= 32, 32-bit buffer offsets into one image's volumes and
+  // feature planes, grid limits; anything else takes the per-pixel search kernel
+  const bool masks = sws + 1 <= FM_KMAX && N < 65536 && (H + FM_TY - 1) / FM_TY < 65536 &&
+                     H * W * (sws + 1 > FM_CH ? sws + 1 : FM_CH) * 4 < (1LL << 31);
+  if (masks) {
     const size_t shm = flow_bwd_lds_bytes(sws, FM_CH);
     if (shm > 64 * 1024) {
       const int rc = check_hip(hipFuncSetAttribute((const void *)sad_flow_bwd_lds_k<FM_CH>,
