@@ -90,6 +90,11 @@ CASES = [
     ("semantic_pixel", 1, 5, 6, 21, 21, {}),
     ("semantic_pixel", 2, 4, 6, 19, 3, {}),
     ("semantic_pixel", 3, 3, 4, 7, 7, {}),
+    # single rows / columns: passes with no step
+    ("semantic_global", 1, 1, 8, 19, 19, {}),
+    ("semantic_global", 2, 6, 1, 25, 25, {}),
+    ("semantic_pixel", 2, 7, 1, 19, 19, {}),
+    ("semantic_pixel", 1, 1, 6, 21, 21, {}),
 ]
 
 
