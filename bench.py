@@ -122,7 +122,12 @@ class StereoPlan:
 
     def bwd(self, t, g_in, g_out):
         nat, p = self.nat, self.sp
-        nat.call("bpl_lbp_bwd", nat.LBP_STEREO, nat.GRAD_EXACT, p(self.jump), self.K, self.jump.shape[0],
+        # chained iterations: the message gradients between two backward calls carry their
+        # LEFT/RIGHT slots at the row ends only (zero elsewhere in the exact adjoint); the top
+        # call reads the full loss gradient, the bottom one writes the full initial-message gradient
+        mode = nat.GRAD_EXACT | (nat.GRAD_IN_H_ROW_ENDS if t < self.iters - 1 else 0) | \
+            (nat.GRAD_OUT_H_ROW_ENDS if t > 0 else 0)
+        nat.call("bpl_lbp_bwd", nat.LBP_STEREO, mode, p(self.jump), self.K, self.jump.shape[0],
                  p(self.edge), p(g_in), p(self.msgs[t + 1]), p(self.argmin[t]), p(self.garg[t]), p(self.gcost),
                  p(self.gjump), p(self.gedge), p(g_out), p(self.ws), self.ws_bytes, self.N, self.H, self.W, self.C,
                  self.delta, nat.stream_of(self.cost))
@@ -218,7 +223,10 @@ class LBPPlan:
 
     def bwd(self, t, g_in, g_out):
         nat, p = self.nat, self.sp
-        nat.call("bpl_lbp_bwd", self.kind, nat.GRAD_EXACT, p(self.jump), self.K, self.Nj, p(self.edge), p(g_in),
+        # chained iterations (see StereoPlan.bwd): the bottom call writes a full gradient
+        mode = nat.GRAD_EXACT | (nat.GRAD_IN_H_ROW_ENDS if t < self.iters - 1 else 0) | \
+            (nat.GRAD_OUT_H_ROW_ENDS if t > 0 else 0)
+        nat.call("bpl_lbp_bwd", self.kind, mode, p(self.jump), self.K, self.Nj, p(self.edge), p(g_in),
                  p(self.msgs[t + 1]), p(self.argmin[t]), p(self.garg[t]), p(self.gcost), p(self.gjump),
                  p(self.gedge), p(g_out), p(self.ws), self.ws_bytes, self.N, self.H, self.W, self.C, self.delta,
                  nat.stream_of(self.cost))

@@ -18,7 +18,13 @@ Deliberate differences (all internal to the autograd Functions that carry these 
     how backward_minsum receives them through the reference's argument list;
   * every batch element is processed; batch-shared pairwise gradients are summed;
   * ``mode="exact"`` (keyword) returns the exact adjoint w.r.t. the input messages;
-  * ``out=`` (keyword) writes the messages out of place, leaving ``messages`` untouched.
+  * ``out=`` (keyword) writes the messages out of place, leaving ``messages`` untouched;
+  * ``in_h_row_ends`` / ``out_h_row_ends`` (keywords, exact mode) chain the backward calls of
+    consecutive iterations: in the exact adjoint the LEFT/RIGHT slots of the input-message
+    gradient are zero away from the row ends, so a call may skip reading them in in_grad
+    (``in_h_row_ends``) or writing them in its grad_messages (``out_h_row_ends``; those
+    entries are then left uninitialised and the tensor is only fit to be the in_grad of a
+    call with ``in_h_row_ends``).
 """
 import torch
 
@@ -91,7 +97,7 @@ def forward_minsum(kind, cost, jump, edge, messages, delta, out=None):
 
 
 def backward_minsum(kind, cost, jump, edge, in_grad, messages, messages_argmin, message_scale,
-                    mode="reference", delta=None):
+                    mode="reference", delta=None, in_h_row_ends=False, out_h_row_ends=False):
     N, H, W, C, K, Nj = _check_common(kind, cost, jump, edge, messages)
     nat.check_input(in_grad, "in_grad")
     if in_grad.shape != messages.shape:
@@ -100,6 +106,9 @@ def backward_minsum(kind, cost, jump, edge, in_grad, messages, messages_argmin, 
         raise RuntimeError("messages_argmin was not produced by this library's forward_minsum")
     garg = _gargmin_of(message_scale)
     m = {"reference": nat.GRAD_REFERENCE, "exact": nat.GRAD_EXACT}[mode]
+    if (in_h_row_ends or out_h_row_ends) and mode != "exact":
+        raise RuntimeError("in_h_row_ends / out_h_row_ends apply to mode='exact' only")
+    m |= (nat.GRAD_IN_H_ROW_ENDS if in_h_row_ends else 0) | (nat.GRAD_OUT_H_ROW_ENDS if out_h_row_ends else 0)
     dev = cost.device
     grad_cost = torch.empty_like(cost)
     if kind == nat.LBP_STEREO:

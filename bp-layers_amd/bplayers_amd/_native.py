@@ -18,6 +18,9 @@ LBP_SEMANTIC_GLOBAL = 1
 LBP_SEMANTIC_PIXEL = 2
 GRAD_REFERENCE = 0
 GRAD_EXACT = 1
+# chained exact iterations: in_grad's / grad_msg's LEFT and RIGHT slots at the row ends only
+GRAD_IN_H_ROW_ENDS = 0x10
+GRAD_OUT_H_ROW_ENDS = 0x20
 
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64

@@ -29,6 +29,9 @@ struct BwdArgs {
   float *partial;     // semantic global: per-(wave, segment) C x C partial sums
   int N, H, W, C, K, Nj, mode, n_base, accum, mat_global;
   int delta;  // the forward's window half-width, or -1 when the caller does not know it
+  // chained exact iterations: in_grad's LEFT/RIGHT slots are zero away from the row ends
+  // (igz), grad_msg's LEFT/RIGHT slots need writing only at the row ends (osp)
+  int igz, osp;
 };
 
 __device__ __forceinline__ int64_t px(int n, int y, int x, int H, int W) {
@@ -114,7 +117,7 @@ __global__ __launch_bounds__(64) void lbp_bwd_phase(BwdArgs a) {
 #pragma unroll
     for (int k = 0; k < LPL; ++k) {
       const int c = min(sl + SEG * k, C - 1);  // clamped: loads stay unconditional
-      dg[k] = a.in_grad[oq * C + c];
+      dg[k] = (!VERT && a.igz) ? 0.f : a.in_grad[oq * C + c];
       dm[k] = a.msgs[oq * C + c];
       da[k] = (int)a.argmin[oq * C + c];
       if (!VERT) {
@@ -353,7 +356,10 @@ __device__ __forceinline__ int shl1_i(int v, int fill) {
 // pixels whose visits both fall before the barrier (the middle of the row) and the row ends
 // are combined after a final barrier; the y-boundary terms of the U and D slots are added by
 // lbp_bwd_pair_rows.
-template <int LPL, bool VERT, bool FULL, int FASTK, bool PAIR, int PF>
+// HS (PAIR only, the chained-iteration flags as compile-time bits): bit 0 = in_grad's LEFT and
+// RIGHT slots are zero inside the sweep (not loaded), bit 1 = grad_msg's LEFT and RIGHT slots
+// are written at the row ends only.  The other kernels read the same flags from a.igz / a.osp.
+template <int LPL, bool VERT, bool FULL, int FASTK, bool PAIR, int PF, int HS = 0>
 __global__ __launch_bounds__(PAIR ? 2 * kWave : kWave) void lbp_bwd_stereo_w64(BwdArgs a) {
   static_assert(!(PAIR && VERT), "pairing is for the horizontal phase");
   constexpr int ND = (3 + LPL - 1) / LPL;  // neighbour lanes reached by offsets up to 3
@@ -378,6 +384,10 @@ __global__ __launch_bounds__(PAIR ? 2 * kWave : kWave) void lbp_bwd_stereo_w64(B
   const int c0 = lane * LPL;
   const int nj = a.Nj > 1 ? n : 0;
   const bool exact = a.mode == BPL_GRAD_EXACT;
+  // in_grad's slot of a horizontal pass is zero at every receiver (which is never the pass's
+  // first pixel), and grad_msg's LEFT/RIGHT slots are wanted at the row ends only
+  const bool igz = !VERT && (PAIR ? (HS & 1) != 0 : a.igz != 0);
+  constexpr bool kOsp = PAIR && (HS & 2) != 0;
   const int64_t HW = (int64_t)H * W;
   // sender pixel of forward step s: q0 + s * dq (in-plane index); its receiver is one further
   const int64_t dq = VERT ? (fwd ? (int64_t)W : -(int64_t)W) : (fwd ? 1 : -1);
@@ -447,7 +457,12 @@ __global__ __launch_bounds__(PAIR ? 2 * kWave : kWave) void lbp_bwd_stereo_w64(B
   auto load_step = [&](int s, float *dg, float *dm, uint32_t *da, float *dd, float *du, float &dp, int &dl) {
     const int64_t q = q0 + (int64_t)s * dq;
     const int64_t qr = q + dq;
-    ld_row(ig_p + qr * C, dg);
+    if (igz) {
+#pragma unroll
+      for (int k = 0; k < LPL; ++k) dg[k] = 0.f;
+    } else {
+      ld_row(ig_p + qr * C, dg);
+    }
     ld_row(m_p + qr * C, dm);
     ld_args(am_p + qr * C, da);
     if (!VERT) {
@@ -637,7 +652,7 @@ __global__ __launch_bounds__(PAIR ? 2 * kWave : kWave) void lbp_bwd_stereo_w64(B
           st_row(gm + DOWN * HWC, hr);
         } else {
           st_row(Xo + q * C, gh);
-          if constexpr (PAIR) {
+          if constexpr (PAIR && !kOsp) {
             // first visitor: the L and R slots, 0 away from the row ends (which are combined
             // after the sweep); written here so that both halves store three rows per step
             float zero[LPL];
@@ -907,8 +922,8 @@ __global__ __launch_bounds__(PAIR ? 2 * kWave : kWave) void lbp_bwd_stereo_w64(B
       st(a.grad_cost + img + q * C, cst);
       st(gm + UP * HWC, hr);
       st(gm + DOWN * HWC, hr);
-      st(gm + LEFT * HWC, sl);
-      st(gm + RIGHT * HWC, sr);
+      if (!kOsp || x == 0) st(gm + LEFT * HWC, sl);
+      if (!kOsp || x == W - 1) st(gm + RIGHT * HWC, sr);
     };
     // the two waves take alternate pixels of the list (row ends first)
     int idx = 0;
@@ -995,7 +1010,7 @@ __global__ __launch_bounds__(64) void lbp_bwd_semg(BwdArgs a) {
   auto load_step = [&](int s, int i) {
     const int64_t q = q0 + (int64_t)s * dq;
     const int64_t qr = q + dq;
-    pg[i] = ig_p[qr * C + cl];
+    pg[i] = (!VERT && a.igz) ? 0.f : ig_p[qr * C + cl];
     pm[i] = m_p[qr * C + cl];
     pa[i] = (int)am_p[qr * C + cl];
     pl[i] = (int)gl_p[qr];
@@ -1148,7 +1163,7 @@ __global__ __launch_bounds__(64) void lbp_bwd_semp(BwdArgs a) {
   auto load_step = [&](int s, int i) {
     const int64_t q = q0 + (int64_t)s * dq;
     const int64_t qr = q + dq;
-    pg[i] = ig_p[qr * C + cl];
+    pg[i] = (!VERT && a.igz) ? 0.f : ig_p[qr * C + cl];
     pm[i] = m_p[qr * C + cl];
     pa[i] = (int)am_p[qr * C + cl];
     pl[i] = (int)gl_p[qr];
@@ -1302,7 +1317,8 @@ __global__ void lbp_bwd_pair_rows(const float *__restrict__ in_grad, const float
 __global__ void lbp_bwd_combine(const float *__restrict__ XL, const float *__restrict__ XR,
                                 const float *__restrict__ XU, const float *__restrict__ XD,
                                 const float *__restrict__ in_grad, float *__restrict__ grad_cost,
-                                float *__restrict__ grad_msg, int N, int H, int W, int C, int exact) {
+                                float *__restrict__ grad_msg, int N, int H, int W, int C, int exact,
+                                int osp) {
   const int64_t total = (int64_t)N * H * W * C;
   const int64_t HWC = (int64_t)H * W * C;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
@@ -1322,8 +1338,9 @@ __global__ void lbp_bwd_combine(const float *__restrict__ XL, const float *__res
     const int y = (int)(yx / W), x = (int)(yx - (int64_t)y * W);
     const float hr = xr + xl;   // read of U/D inputs by the horizontal passes
     const float sv = xd + xu;   // read of L/R boundary inputs by the vertical passes
-    grad_msg[base + LEFT * HWC] = x == 0 ? (in_grad[base + LEFT * HWC] + xr) + sv : 0.f;
-    grad_msg[base + RIGHT * HWC] = x == W - 1 ? (in_grad[base + RIGHT * HWC] + xl) + sv : 0.f;
+    // osp: the LEFT/RIGHT slots (zero away from the row ends) are written at the row ends only
+    if (!osp || x == 0) grad_msg[base + LEFT * HWC] = x == 0 ? (in_grad[base + LEFT * HWC] + xr) + sv : 0.f;
+    if (!osp || x == W - 1) grad_msg[base + RIGHT * HWC] = x == W - 1 ? (in_grad[base + RIGHT * HWC] + xl) + sv : 0.f;
     grad_msg[base + UP * HWC] = y == 0 ? (in_grad[base + UP * HWC] + xd) + hr : hr;
     grad_msg[base + DOWN * HWC] = y == H - 1 ? (in_grad[base + DOWN * HWC] + xu) + hr : hr;
   }
@@ -1418,8 +1435,19 @@ void launch_stereo_w64(const BwdArgs &a, bool vert, bool pair, int ngroups, hipS
       const dim3 grid((unsigned)(ngroups * 2 * a.W));
       hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, true, FULL, FK, false, PF>), grid, dim3(kWave), 0, st, a);
     } else if (pair) {
-      const dim3 grid((unsigned)(ngroups * a.H));
-      hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, FK, true, PF>), grid, dim3(2 * kWave), 0, st, a);
+      // pairing needs the fast step (stereo_pair_ok), so only FK > 0 is instantiated
+      if constexpr (FK > 0) {
+        const dim3 grid((unsigned)(ngroups * a.H));
+        const int hs = (a.igz ? 1 : 0) | (a.osp ? 2 : 0);
+        if (hs == 0)
+          hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, FK, true, PF, 0>), grid, dim3(2 * kWave), 0, st, a);
+        else if (hs == 1)
+          hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, FK, true, PF, 1>), grid, dim3(2 * kWave), 0, st, a);
+        else if (hs == 2)
+          hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, FK, true, PF, 2>), grid, dim3(2 * kWave), 0, st, a);
+        else
+          hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, FK, true, PF, 3>), grid, dim3(2 * kWave), 0, st, a);
+      }
     } else {
       const dim3 grid((unsigned)(ngroups * 2 * a.H));
       hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, FK, false, PF>), grid, dim3(kWave), 0, st, a);
@@ -1489,7 +1517,11 @@ extern "C" int bpl_lbp_bwd(int kind, int mode, const float *jump, int64_t jump_k
                            void *workspace, int64_t workspace_bytes,
                            int64_t N, int64_t H, int64_t W, int64_t C, int delta, void *stream) {
   BPL_REQUIRE(kind >= 0 && kind <= 2, BPL_E_ARG, "bpl_lbp_bwd: unknown kind %d", kind);
+  const int hflags = mode & (BPL_GRAD_IN_H_ROW_ENDS | BPL_GRAD_OUT_H_ROW_ENDS);
+  mode &= ~(BPL_GRAD_IN_H_ROW_ENDS | BPL_GRAD_OUT_H_ROW_ENDS);
   BPL_REQUIRE(mode == BPL_GRAD_REFERENCE || mode == BPL_GRAD_EXACT, BPL_E_ARG, "bpl_lbp_bwd: unknown mode %d", mode);
+  BPL_REQUIRE(hflags == 0 || mode == BPL_GRAD_EXACT, BPL_E_ARG,
+              "bpl_lbp_bwd: the row-end flags apply to the exact mode only");
   BPL_REQUIRE(N >= 1 && H >= 1 && W >= 1 && C >= 1, BPL_E_ARG, "bpl_lbp_bwd: sizes must be positive");
   BPL_REQUIRE(C <= 255, BPL_E_LIMIT, "bpl_lbp_bwd: C=%lld labels exceeds the supported 255", (long long)C);
   BPL_REQUIRE(edge && in_grad && messages && argmin && gargmin && grad_cost && grad_jump && grad_edge && grad_msg,
@@ -1519,6 +1551,8 @@ extern "C" int bpl_lbp_bwd(int kind, int mode, const float *jump, int64_t jump_k
   a.Nj = kind == BPL_LBP_STEREO ? (int)jump_n : 1;
   a.mode = mode; a.mat_global = 1;
   a.delta = delta;
+  a.igz = (hflags & BPL_GRAD_IN_H_ROW_ENDS) != 0;
+  a.osp = (hflags & BPL_GRAD_OUT_H_ROW_ENDS) != 0;
   a.grad_cost = grad_cost; a.grad_msg = grad_msg;
   hipStream_t st = as_stream(stream);
   // parameter gradients shared across the batch accumulate over n in launch order
@@ -1571,7 +1605,7 @@ extern "C" int bpl_lbp_bwd(int kind, int mode, const float *jump, int64_t jump_k
     KernelTimer kt(BPL_KERNEL_LBP_BWD_COMBINE, st);
     hipLaunchKernelGGL(lbp_bwd_combine, dim3((unsigned)blocks), dim3(threads), 0, st, a.X[LEFT], a.X[RIGHT],
                        a.X[UP], a.X[DOWN], in_grad, grad_cost, grad_msg, (int)N, (int)H, (int)W, (int)C,
-                       mode == BPL_GRAD_EXACT ? 1 : 0);
+                       mode == BPL_GRAD_EXACT ? 1 : 0, a.osp);
     int rc = check_hip(hipGetLastError(), "bpl_lbp_bwd combine launch");
     if (rc) return rc;
   }

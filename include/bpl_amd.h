@@ -43,6 +43,22 @@ extern "C" {
 /* backward modes */
 #define BPL_GRAD_REFERENCE 0  /* the reference's backward_minsum semantics              */
 #define BPL_GRAD_EXACT 1      /* exact adjoint of one iteration (input-message gradient) */
+/*
+ * Flags OR-ed into BPL_GRAD_EXACT for a chain of iterations whose backward calls feed each
+ * other (iteration t's grad_msg is iteration t-1's in_grad).  In the exact adjoint the
+ * LEFT and RIGHT slots of grad_msg are zero away from the row ends (the forward reads those
+ * input slots only at x = 0 and x = W-1), so the chain need not write or read them:
+ *   BPL_GRAD_IN_H_ROW_ENDS   in_grad's LEFT slot is read only at x = 0 and its RIGHT slot
+ *                            only at x = W-1; everywhere else they are taken as zero (their
+ *                            memory is not read);
+ *   BPL_GRAD_OUT_H_ROW_ENDS  grad_msg's LEFT slot is written only at x = 0 and its RIGHT
+ *                            slot only at x = W-1; everywhere else their memory is left as
+ *                            it was (the exact value there is zero).
+ * A grad_msg written with OUT_H_ROW_ENDS is a valid in_grad for IN_H_ROW_ENDS.  No
+ * counterpart in the reference (whose grad_messages is a copy of in_grad).
+ */
+#define BPL_GRAD_IN_H_ROW_ENDS 0x10
+#define BPL_GRAD_OUT_H_ROW_ENDS 0x20
 
 /* Library identification and diagnostics. */
 int bpl_version(void);

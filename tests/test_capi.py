@@ -59,6 +59,14 @@ def test_argument_validation_returns_codes(lib):
     assert rc == -1 and b"null" in lib.bpl_last_error()
     rc = lib.bpl_lbp_bwd(nat.LBP_STEREO, 5, P, 6, 1, P, P, P, P, P, P, P, P, P, P, 1 << 40, 1, 4, 4, 8, 3, None)
     assert rc == -1 and b"mode" in lib.bpl_last_error()
+    # the chained-iteration row-end flags belong to the exact mode only
+    for fl in (nat.GRAD_IN_H_ROW_ENDS, nat.GRAD_OUT_H_ROW_ENDS):
+        rc = lib.bpl_lbp_bwd(nat.LBP_STEREO, nat.GRAD_REFERENCE | fl, P, 6, 1, P, P, P, P, P, P, P, P, P, P,
+                             1 << 40, 1, 4, 4, 8, 3, None)
+        assert rc == -1 and b"exact mode only" in lib.bpl_last_error()
+    rc = lib.bpl_lbp_bwd(nat.LBP_STEREO, nat.GRAD_EXACT | 0x40, P, 6, 1, P, P, P, P, P, P, P, P, P, P,
+                         1 << 40, 1, 4, 4, 8, 3, None)
+    assert rc == -1 and b"unknown mode" in lib.bpl_last_error()
     need = lib.bpl_lbp_bwd_workspace_size(nat.LBP_STEREO, 1, 4, 4, 8)
     assert need == 4 * 1 * 4 * 4 * 8 * 4
     rc = lib.bpl_lbp_bwd(nat.LBP_STEREO, 1, P, 6, 1, P, P, P, P, P, P, P, P, P, P, need - 1, 1, 4, 4, 8, 3, None)

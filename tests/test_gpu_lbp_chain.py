@@ -1,0 +1,89 @@
+"""GPU checks of the chained exact backward (in_h_row_ends / out_h_row_ends): a backward pass
+through several iterations whose calls skip the LEFT/RIGHT slots of the message gradients away
+from the row ends gives bit-identical gradients to the same chain without the flags.
+
+The skipped entries of every intermediate gradient are overwritten with NaN before the next
+call reads it, so a kernel that still read one would put NaN into its outputs."""
+import numpy as np
+import pytest
+import torch
+
+from test_gpu_lbp_fwd import make_case, _mods
+
+pytestmark = pytest.mark.gpu
+
+LEFT, RIGHT = 0, 1
+ITERS = 3
+
+
+def _backward(mod, kind, cost, jump, edge, g, msgs, am, sc, delta, **flags):
+    if kind == "semantic_pixel":
+        return mod.backward_minsum(cost, edge, g, msgs, am, sc, mode="exact", jump=jump, **flags)
+    if kind == "stereo":
+        return mod.backward_minsum(cost, jump, edge, g, msgs, am, sc, mode="exact", delta=delta, **flags)
+    return mod.backward_minsum(cost, jump, edge, g, msgs, am, sc, mode="exact", **flags)
+
+
+def _poison_interior(gm):
+    """NaN into the LEFT/RIGHT entries a row-end-only gradient leaves unwritten."""
+    gm = gm.clone()
+    gm[:, LEFT, :, 1:, :] = float("nan")
+    gm[:, RIGHT, :, :-1, :] = float("nan")
+    return gm
+
+
+CASES = [
+    # paired fast kernel (stereo, delta <= 3, K = 6), full and partial label rows
+    ("stereo", 1, 6, 40, 192, 3, {}, True),
+    ("stereo", 2, 5, 23, 100, 3, {}, True),
+    ("stereo", 1, 4, 13, 64, 3, {"offset": True}, True),   # rows with offsets: general step
+    ("stereo", 1, 5, 9, 64, 3, {}, False),                  # no delta: unpaired kernels + combine
+    ("stereo", 1, 5, 7, 40, 3, {"K_in": 5}, True),          # K = 8
+    ("stereo", 1, 5, 6, 12, 3, {}, True),                   # C <= 16: generic kernels
+    ("stereo", 1, 3, 2, 64, 3, {}, True),                   # two-pixel rows
+    ("semantic_global", 2, 5, 6, 19, 19, {}, False),
+    ("semantic_global", 1, 4, 5, 12, 12, {}, False),
+    ("semantic_pixel", 2, 4, 5, 19, 19, {}, False),
+    ("semantic_pixel", 1, 4, 5, 7, 7, {}, False),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c[0]}-N{c[1]}H{c[2]}W{c[3]}C{c[4]}d{c[5]}"
+                         f"{''.join(c[6])}{'-delta' if c[7] else ''}")
+def test_chained_backward_equals_unchained(case):
+    kind, N, H, W, C, delta, extra, pass_delta = case
+    print(f"chain case {case}", flush=True)
+    cost, jump, edge, M0 = make_case(kind, N, H, W, C, delta, seed=7 + C + W, **extra)
+    mod = _mods()[kind]
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    cost, jump, edge = t(cost), t(jump), t(edge)
+    msgs, fwd = [t(M0)], []
+    for _ in range(ITERS):
+        out = torch.empty_like(msgs[-1])
+        m, am, sc = mod.forward_minsum(cost, jump, edge, msgs[-1], delta, out=out)
+        msgs.append(m)
+        fwd.append((m, am, sc))
+    G = torch.from_numpy(np.random.default_rng(11).normal(size=M0.shape).astype(np.float32)).to(dev)
+    d = delta if pass_delta else None
+
+    g_ref, g_chain = G, G
+    for it in reversed(range(ITERS)):
+        m, am, sc = fwd[it]
+        ref = _backward(mod, kind, cost, jump, edge, g_ref, m, am, sc, d)
+        chain = _backward(mod, kind, cost, jump, edge, g_chain, m, am, sc, d,
+                          in_h_row_ends=it < ITERS - 1, out_h_row_ends=it > 0)
+        torch.cuda.synchronize()
+        for name, a, b in zip(("cost", "pairwise", "edge"), ref[:3], chain[:3]):
+            assert torch.equal(a, b), f"iteration {it}: grad {name} differs"
+        gr, gc = ref[3], chain[3]
+        if it > 0:
+            assert torch.equal(gr[:, 2:], gc[:, 2:]), f"iteration {it}: U/D message gradient differs"
+            assert torch.equal(gr[:, LEFT, :, 0], gc[:, LEFT, :, 0])
+            assert torch.equal(gr[:, RIGHT, :, -1], gc[:, RIGHT, :, -1])
+            # the exact adjoint's L/R slots really are zero away from the row ends
+            assert torch.all(gr[:, LEFT, :, 1:] == 0) and torch.all(gr[:, RIGHT, :, :-1] == 0)
+            g_chain = _poison_interior(gc)
+        else:
+            assert torch.equal(gr, gc), "bottom iteration: message gradient differs"
+        g_ref = gr
