@@ -471,7 +471,7 @@ def kernel_class(name):
                       ("beliefs_k", "lbp_beliefs")):
         if stem in base:
             return cls
-    m = re.search(r"lbp_(fwd|bwd)_(stereo_w64|semg|phase)<([^>]*)>", base)
+    m = re.search(r"lbp_(fwd|bwd)_(stereo_w64|semg|semp|phase)<([^>]*)>", base)
     if not m:
         return None
     targs = [t.strip() for t in m.group(3).split(",")]

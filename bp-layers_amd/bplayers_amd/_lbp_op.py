@@ -133,3 +133,40 @@ def backward_minsum(kind, cost, jump, edge, in_grad, messages, messages_argmin, 
              nat.ptr(grad_edge), nat.ptr(grad_msg), nat.ptr(ws), ws_bytes, N, H, W, C,
              -1 if delta is None else int(delta), nat.stream_of(cost))
     return [grad_cost, gj_ret, grad_edge, grad_msg]
+
+
+def exact_chain_forward(kind, cost, jump, edge, messages, delta, iters):
+    """``iters`` iterations from ``messages`` (left untouched), each into fresh messages.
+    Returns the final messages and the per-iteration records (messages, argmin, scale) that
+    exact_chain_backward replays."""
+    recs = []
+    m = messages
+    for _ in range(iters):
+        m, am, sc = forward_minsum(kind, cost, jump, edge, m, delta, out=torch.empty_like(m))
+        recs.append((m, am, sc))
+    return m, recs
+
+
+def exact_chain_backward(kind, cost, jump, edge, in_grad, recs, delta=None):
+    """Exact adjoint of exact_chain_forward: the backward calls of consecutive iterations pass
+    each other message gradients whose LEFT/RIGHT slots are written and read at the row ends
+    only (zero elsewhere in the exact adjoint); the top call reads the full ``in_grad`` and the
+    bottom call writes the full gradient w.r.t. the initial messages.  Returns
+    (grad_cost, grad_pairwise, grad_edge, grad_messages) with the first three summed over the
+    iterations (top iteration first)."""
+    T = len(recs)
+    if T == 0:
+        raise RuntimeError("exact_chain_backward needs at least one iteration")
+    g = in_grad.contiguous()
+    tot = None
+    for t in reversed(range(T)):
+        m, am, sc = recs[t]
+        gc, gj, ge, g = backward_minsum(kind, cost, jump, edge, g, m, am, sc, mode="exact", delta=delta,
+                                        in_h_row_ends=t < T - 1, out_h_row_ends=t > 0)
+        if tot is None:
+            tot = [gc, gj, ge]
+        else:
+            tot[0] += gc
+            tot[1] += gj
+            tot[2] += ge
+    return tot[0], tot[1], tot[2], g

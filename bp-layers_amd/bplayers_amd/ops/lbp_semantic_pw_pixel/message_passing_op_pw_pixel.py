@@ -3,6 +3,8 @@
 import torch
 import torch.nn as nn
 
+from ... import _lbp_op
+from ... import _native as nat
 from ... import pytorch_cuda_lbp_op_semantic_pw_pixel as lbp
 from ...corenet import TemperatureSoftmin
 from ..lbp_stereo.message_passing_op_cuda import construct_pw_tensor  # noqa: F401
@@ -49,6 +51,29 @@ class LBPMinSumFunction(torch.autograd.Function):
         return grad_cost, None, None, grad_edge, grad_message, None, fold_pairwise_grad(grad_jump), None
 
 
+class LBPMinSumExactChainFunction(torch.autograd.Function):
+    """``max_iter`` (>= 1) iterations with the exact adjoint as one autograd node (the values of
+    ``max_iter`` exact-mode LBPMinSumFunction applications; the iterations' backward calls are
+    chained through bpl_lbp_bwd's row-end flags)."""
+
+    @staticmethod
+    def forward(ctx, cost, edge, messages, delta, jump, max_iter):
+        j = shift_pairwise(jump)
+        out, recs = _lbp_op.exact_chain_forward(nat.LBP_SEMANTIC_PIXEL, cost, j, edge, messages.contiguous(), delta,
+                                                int(max_iter))
+        ctx.save_for_backward(cost, j, edge, *[x for r in recs for x in r])
+        ctx.iters = int(max_iter)
+        return out
+
+    @staticmethod
+    def backward(ctx, in_grad):
+        cost, j, edge, *flat = ctx.saved_tensors
+        recs = [tuple(flat[3 * i:3 * i + 3]) for i in range(ctx.iters)]
+        grad_cost, grad_jump, grad_edge, grad_message = _lbp_op.exact_chain_backward(
+            nat.LBP_SEMANTIC_PIXEL, cost, j, edge, in_grad, recs)
+        return grad_cost, grad_edge, grad_message, None, fold_pairwise_grad(grad_jump), None
+
+
 class MessagePassing(nn.Module):
     def __init__(self, device, max_iter, num_labels, delta, mode='min-sum', grad_mode='reference'):
         super(MessagePassing, self).__init__()
@@ -74,8 +99,13 @@ class MessagePassing(nn.Module):
         if self.mode != 'min-sum':
             raise NotImplementedError("message parsing mode " + self.mode + " is currently not implemented!")
         cost = -prob_vol
-        for it in range(self.max_iter):
-            messages = LBPMinSumFunction.apply(cost, self.L1, self.L2, edge_weights, messages, self.delta, jump,
-                                               self.grad_mode)
+        if self.grad_mode == 'exact':
+            if self.max_iter > 0:
+                messages = LBPMinSumExactChainFunction.apply(cost, edge_weights, messages, self.delta, jump,
+                                                             self.max_iter)
+        else:
+            for it in range(self.max_iter):
+                messages = LBPMinSumFunction.apply(cost, self.L1, self.L2, edge_weights, messages, self.delta, jump,
+                                                   self.grad_mode)
         beliefs = messages.sum(dim=1) + cost
         return self.softmin.forward(beliefs)

@@ -5,12 +5,18 @@
 updated in place, the saved messages therefore alias across iterations, and the gradient
 w.r.t. the input messages is ``in_grad``.  ``LBPMinSumExactFunction`` is the exact
 adjoint: each iteration writes fresh messages and returns the true input-message gradient,
-so gradients through several iterations match finite differences.  ``MessagePassing``
-selects between them with ``grad_mode`` ('reference' by default, or 'exact').
+so gradients through several iterations match finite differences.
+``LBPMinSumExactChainFunction`` runs all iterations of an exact solve as one autograd node,
+so its backward can chain the iterations' backward calls (message gradients whose
+LEFT/RIGHT slots are zero away from the row ends are neither written nor read there).
+``MessagePassing`` selects with ``grad_mode`` ('reference' by default, or 'exact', which uses
+the chained Function).
 """
 import torch
 import torch.nn as nn
 
+from ... import _lbp_op
+from ... import _native as nat
 from ... import pytorch_cuda_lbp_op_stereo as lbp
 from ...corenet import TemperatureSoftmin
 
@@ -107,6 +113,31 @@ class LBPMinSumExactFunction(torch.autograd.Function):
         return grad_cost, grad_edge, grad_message, None, grad_aff, grad_offset
 
 
+class LBPMinSumExactChainFunction(torch.autograd.Function):
+    """``max_iter`` (>= 1) iterations with the exact adjoint, as one autograd node: the same
+    values as ``max_iter`` applications of LBPMinSumExactFunction, with the iterations'
+    backward calls chained (bpl_lbp_bwd's row-end flags)."""
+
+    @staticmethod
+    def forward(ctx, cost, edge, messages, delta, affinities, offset, max_iter):
+        jump = pack_affinities(affinities, offset.to(affinities.device))
+        out, recs = _lbp_op.exact_chain_forward(nat.LBP_STEREO, cost, jump, edge, messages.contiguous(), delta,
+                                                int(max_iter))
+        ctx.save_for_backward(cost, jump, edge, *[x for r in recs for x in r])
+        ctx.delta = int(delta)
+        ctx.iters = int(max_iter)
+        return out
+
+    @staticmethod
+    def backward(ctx, in_grad):
+        cost, jump, edge, *flat = ctx.saved_tensors
+        recs = [tuple(flat[3 * i:3 * i + 3]) for i in range(ctx.iters)]
+        grad_cost, grad_jump, grad_edge, grad_message = _lbp_op.exact_chain_backward(
+            nat.LBP_STEREO, cost, jump, edge, in_grad, recs, delta=ctx.delta)
+        grad_aff, grad_offset = unpack_affinity_grad(grad_jump)
+        return grad_cost, grad_edge, grad_message, None, grad_aff, grad_offset, None
+
+
 class MessagePassing(nn.Module):
     """Min-sum message passing layer (message_passing_op_cuda.py:95-178)."""
 
@@ -164,10 +195,14 @@ class MessagePassing(nn.Module):
             offset = torch.zeros((N, NUM_DIR, H, W), device=dev)
         if self.mode != 'min-sum':
             raise NotImplementedError("message parsing mode " + self.mode + " is currently not implemented!")
-        fn = LBPMinSumExactFunction if self.grad_mode == 'exact' else LBPMinSumFunction
         cost = -prob_vol
-        for it in range(self.max_iter):
-            messages = fn.apply(cost, edge_weights, messages, self.delta, affinities, offset)
+        if self.grad_mode == 'exact':
+            if self.max_iter > 0:
+                messages = LBPMinSumExactChainFunction.apply(cost, edge_weights, messages, self.delta, affinities,
+                                                             offset, self.max_iter)
+        else:
+            for it in range(self.max_iter):
+                messages = LBPMinSumFunction.apply(cost, edge_weights, messages, self.delta, affinities, offset)
         beliefs = messages.sum(dim=1) + cost
         beliefs = self.softmin.forward(beliefs)
         return beliefs, messages

@@ -104,6 +104,39 @@ def test_stereo_reference_mode_multi_iteration_matches_restatement():
     np.testing.assert_allclose(m.grad.cpu().numpy(), G, rtol=1e-6)
 
 
+@pytest.mark.parametrize("shape", [(1, 6, 37, 64), (2, 5, 23, 100), (1, 5, 9, 12)])
+def test_stereo_exact_chain_equals_per_iteration_functions(shape):
+    """One chained autograd node over 3 iterations == 3 per-iteration exact Functions: same
+    messages, and the same gradients up to the order in which the iterations' terms are summed
+    (paired fast kernel for C > 16 with delta 3; generic kernels for C <= 16)."""
+    from bplayers_amd.ops.lbp_stereo.message_passing_op_cuda import (LBPMinSumExactChainFunction,
+                                                                      LBPMinSumExactFunction)
+    N, H, W, C = shape
+    rng = np.random.default_rng(C + W)
+    cost = rng.normal(size=(N, H, W, C))
+    aff = np.log1p(np.exp(rng.normal(size=(N, 4, 3, H, W))))
+    edge = rng.uniform(0.5, 1.5, size=(N, 4, H, W))
+    m0 = rng.normal(size=(N, 4, H, W, C)) * 0.3
+    Wm = torch.tensor(rng.normal(size=(N, 4, H, W, C)), dtype=torch.float32, device=DEV)
+    offset = torch.zeros((N, 4, H, W), device=DEV)
+    outs, grads = [], []
+    for chained in (True, False):
+        tc, ta, te, tm = _t(cost, True), _t(aff, True), _t(edge, True), _t(m0, True)
+        if chained:
+            m = LBPMinSumExactChainFunction.apply(tc, te, tm, 3, ta, offset, 3)
+        else:
+            m = tm
+            for _ in range(3):
+                m = LBPMinSumExactFunction.apply(tc, te, m, 3, ta, offset)
+        (m * Wm).sum().backward()
+        torch.cuda.synchronize()
+        outs.append(m.detach())
+        grads.append([x.grad for x in (tc, ta, te, tm)])
+    assert torch.equal(outs[0], outs[1])
+    for name, a, b in zip(("cost", "affinities", "edge", "messages"), *grads):
+        torch.testing.assert_close(a, b, rtol=1e-5, atol=1e-5, msg=lambda m_: f"grad {name}: {m_}")
+
+
 @pytest.mark.parametrize("variant", ["global", "pixel"])
 def test_semantic_message_passing_exact_gradients(variant):
     rng = np.random.default_rng(5)

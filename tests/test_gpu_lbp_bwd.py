@@ -98,14 +98,17 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c[0]}-N{c[1]}H{c[2]}W{c[3]}C{c[4]}d{c[5]}{''.join(c[6])}")
+# only the stereo backward takes the forward's delta, so the semantic cases run without it
+RUNS = [(c, pd) for c in CASES for pd in ((False, True) if c[0] == "stereo" else (False,))]
+
+
+@pytest.mark.parametrize("case,pass_delta", RUNS,
+                         ids=lambda v: f"{v[0]}-N{v[1]}H{v[2]}W{v[3]}C{v[4]}d{v[5]}{''.join(v[6])}"
+                         if isinstance(v, tuple) else ("delta" if v else "no-delta"))
 @pytest.mark.parametrize("mode", ["exact", "reference"])
-@pytest.mark.parametrize("pass_delta", [False, True], ids=["no-delta", "delta"])
 def test_lbp_backward_matches_oracle(case, mode, pass_delta):
     """'delta' passes the forward's delta, which selects the stereo fast step where it applies
     (delta <= 3, C > 32); 'no-delta' runs the general kernels."""
     kind, N, H, W, C, delta, extra = case
-    if pass_delta and kind != "stereo":
-        pytest.skip("only the stereo backward takes the forward's delta")
     print(f"case {case} mode={mode} pass_delta={pass_delta}", flush=True)
     run_case(kind, N, H, W, C, delta, mode, seed=100 + C + H, pass_delta=pass_delta, **extra)
