@@ -371,7 +371,9 @@ __global__ __launch_bounds__(PAIR ? 2 * kWave : kWave) void lbp_bwd_stereo_w64(B
   const int L = VERT ? W : H;
   const int len = VERT ? H : W;
   const int nsteps = len - 1;
-  const int wv = PAIR ? (int)(threadIdx.x >> 6) : 0;  // wave index in the workgroup
+  // wave index in the workgroup, wave-uniform: everything derived from it (direction, start
+  // pixel, stream pointers) then stays in scalar registers
+  const int wv = PAIR ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) : 0;
   const int g_ = PAIR ? 2 * (int)(blockIdx.x / L) + wv : (int)(blockIdx.x / L);
   const int ln = blockIdx.x - (PAIR ? g_ >> 1 : g_) * L;
   const int n = a.n_base + (g_ >> 1);
