@@ -320,6 +320,45 @@ __device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, uint32_t voff, 
 __device__ __forceinline__ void buf_st8(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, uint8_t v) {
   __builtin_amdgcn_raw_buffer_store_b8(v, r, voff, soff, 0);
 }
+// LPL consecutive floats per lane (one dword / dwordx2 / dwordx3 / dwordx4 access) at
+// base + voff + soff; voff must be a multiple of 4.
+template <int LPL>
+__device__ __forceinline__ void buf_ld_row(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, float *d) {
+  static_assert(LPL >= 1 && LPL <= 4, "1 to 4 floats per lane");
+  if constexpr (LPL == 1) {
+    d[0] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+  } else if constexpr (LPL == 2) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+    d[0] = __uint_as_float(v[0]); d[1] = __uint_as_float(v[1]);
+  } else if constexpr (LPL == 3) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, voff, soff, 0);
+    d[0] = __uint_as_float(v[0]); d[1] = __uint_as_float(v[1]); d[2] = __uint_as_float(v[2]);
+  } else {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+    d[0] = __uint_as_float(v[0]); d[1] = __uint_as_float(v[1]);
+    d[2] = __uint_as_float(v[2]); d[3] = __uint_as_float(v[3]);
+  }
+}
+template <int LPL>
+__device__ __forceinline__ void buf_st_row(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, const float *s) {
+  static_assert(LPL >= 1 && LPL <= 4, "1 to 4 floats per lane");
+  if constexpr (LPL == 1) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(s[0]), r, voff, soff, 0);
+  } else if constexpr (LPL == 2) {
+    decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)) v;
+    v[0] = __float_as_uint(s[0]); v[1] = __float_as_uint(s[1]);
+    __builtin_amdgcn_raw_buffer_store_b64(v, r, voff, soff, 0);
+  } else if constexpr (LPL == 3) {
+    decltype(__builtin_amdgcn_raw_buffer_load_b96(r, 0, 0, 0)) v;
+    v[0] = __float_as_uint(s[0]); v[1] = __float_as_uint(s[1]); v[2] = __float_as_uint(s[2]);
+    __builtin_amdgcn_raw_buffer_store_b96(v, r, voff, soff, 0);
+  } else {
+    decltype(__builtin_amdgcn_raw_buffer_load_b128(r, 0, 0, 0)) v;
+    v[0] = __float_as_uint(s[0]); v[1] = __float_as_uint(s[1]);
+    v[2] = __float_as_uint(s[2]); v[3] = __float_as_uint(s[3]);
+    __builtin_amdgcn_raw_buffer_store_b128(v, r, voff, soff, 0);
+  }
+}
 
 // exp / log from the hardware base-2 instructions with a change of base.
 __device__ __forceinline__ float fast_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }

@@ -385,7 +385,7 @@ __global__ __launch_bounds__(PAIR ? 2 * kWave : kWave) void lbp_bwd_stereo_w64(B
   int(&lds_a)[2][kWave * LPL] = lds_a_all[wv];
   const int c0 = lane * LPL;
   const int nj = a.Nj > 1 ? n : 0;
-  const bool exact = a.mode == BPL_GRAD_EXACT;
+  const bool exact = PAIR || a.mode == BPL_GRAD_EXACT;  // the paired kernel runs in exact mode only
   // in_grad's slot of a horizontal pass is zero at every receiver (which is never the pass's
   // first pixel), and grad_msg's LEFT/RIGHT slots are wanted at the row ends only
   const bool igz = !VERT && (PAIR ? (HS & 1) != 0 : a.igz != 0);
@@ -423,6 +423,35 @@ __global__ __launch_bounds__(PAIR ? 2 * kWave : kWave) void lbp_bwd_stereo_w64(B
       for (int k = 0; k < LPL; ++k) d[k] = row[voff[k]];
     }
   };
+  // Kernels with the fast step (the host checks that every [H,W,C] plane is below 2^31 bytes)
+  // address the float row streams through buffer resources: one wave-uniform byte offset per
+  // row (q * C * 4, 32-bit) and the lane's constant byte offset, so no step forms per-lane
+  // 64-bit addresses.
+  constexpr bool BUF = FASTK > 0;
+  const uint32_t plane_bytes = (uint32_t)(HW * C * 4);
+  const __amdgpu_buffer_rsrc_t ig_rs = make_rsrc(ig_p, plane_bytes);
+  const __amdgpu_buffer_rsrc_t m_rs = make_rsrc(m_p, plane_bytes);
+  const __amdgpu_buffer_rsrc_t xd_rs = make_rsrc(XDp, plane_bytes);
+  const __amdgpu_buffer_rsrc_t xu_rs = make_rsrc(XUp, plane_bytes);
+  const __amdgpu_buffer_rsrc_t am_rs = make_rsrc(am_p, plane_bytes / 4u);
+  const __amdgpu_buffer_rsrc_t gl_rs_in = make_rsrc(gl_p, (uint32_t)HW);
+  auto row_off = [&](int64_t q) -> uint32_t { return (uint32_t)q * (uint32_t)C * 4u; };
+  auto ld_row_b = [&](__amdgpu_buffer_rsrc_t r, uint32_t soff, float *d) {
+    if (FULL) {
+      buf_ld_row<LPL>(r, voff[0] * 4u, soff, d);
+    } else {
+#pragma unroll
+      for (int k = 0; k < LPL; ++k) d[k] = buf_ld(r, voff[k] * 4u, soff);
+    }
+  };
+  auto st_row_b = [&](__amdgpu_buffer_rsrc_t r, uint32_t soff, const float *v) {
+    if (FULL) {
+      buf_st_row<LPL>(r, (uint32_t)c0 * 4u, soff, v);
+    } else {
+#pragma unroll
+      for (int k = 0; k < LPL; ++k) buf_st(r, valid[k] ? (uint32_t)(c0 + k) * 4u : kBufDrop, soff, v[k]);
+    }
+  };
 
   // ---- the last position of the pass sends nothing: zero its outputs
   {
@@ -448,6 +477,17 @@ __global__ __launch_bounds__(PAIR ? 2 * kWave : kWave) void lbp_bwd_stereo_w64(B
       for (int k = 0; k < NRAW; ++k) d[k] = (uint32_t)row[voff[k]];
     }
   };
+  // the same through the buffer resource (byte offset soff of the row; the 16-bit access of
+  // LPL = 3 is unaligned on odd lanes, as the global form above)
+  auto ld_args_b = [&](uint32_t soff, uint32_t *d) {
+    if (FULL && LPL >= 2) {
+      d[0] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b16(am_rs, voff[0], soff, 0);
+      if (LPL == 3) d[1] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(am_rs, voff[0] + 2u, soff, 0);
+    } else {
+#pragma unroll
+      for (int k = 0; k < NRAW; ++k) d[k] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(am_rs, voff[k], soff, 0);
+    }
+  };
   auto arg_of = [&](const uint32_t *d, int k) -> int {
     if (FULL && LPL >= 2) return k == 0 ? (int)(d[0] & 0xFFu) : k == 1 ? (int)(d[0] >> 8) : (int)d[1 < NRAW ? 1 : 0];
     return (int)d[k < NRAW ? k : 0];
@@ -457,24 +497,35 @@ __global__ __launch_bounds__(PAIR ? 2 * kWave : kWave) void lbp_bwd_stereo_w64(B
   uint32_t pa[PF][NRAW];
   int pl[PF];
   auto load_step = [&](int s, float *dg, float *dm, uint32_t *da, float *dd, float *du, float &dp, int &dl) {
-    const int64_t q = q0 + (int64_t)s * dq;
-    const int64_t qr = q + dq;
+    // BUF: in-plane indices fit in 32 bits (the host's plane-size check)
+    const int64_t q = BUF ? (int64_t)((int)q0 + s * (int)dq) : q0 + (int64_t)s * dq;
+    const int64_t qr = BUF ? (int64_t)((int)q + (int)dq) : q + dq;
+    const uint32_t so = row_off(qr);
     if (igz) {
 #pragma unroll
       for (int k = 0; k < LPL; ++k) dg[k] = 0.f;
+    } else if (BUF) {
+      ld_row_b(ig_rs, so, dg);
     } else {
       ld_row(ig_p + qr * C, dg);
     }
-    ld_row(m_p + qr * C, dm);
-    ld_args(am_p + qr * C, da);
+    if (BUF) ld_row_b(m_rs, so, dm);
+    else ld_row(m_p + qr * C, dm);
+    if (BUF) ld_args_b(so / 4u, da);
+    else ld_args(am_p + qr * C, da);
     if (!VERT) {
-      ld_row(XDp + qr * C, dd);
-      ld_row(XUp + qr * C, du);
+      if (BUF) {
+        ld_row_b(xd_rs, so, dd);
+        ld_row_b(xu_rs, so, du);
+      } else {
+        ld_row(XDp + qr * C, dd);
+        ld_row(XUp + qr * C, du);
+      }
     } else {
 #pragma unroll
       for (int k = 0; k < LPL; ++k) { dd[k] = 0.f; du[k] = 0.f; }
     }
-    dl = (int)gl_p[qr];
+    dl = BUF ? (int)__builtin_amdgcn_raw_buffer_load_b8(gl_rs_in, 0u, (uint32_t)qr, 0) : (int)gl_p[qr];
     dp = par_p[q];
   };
   if (nsteps > 0) {
@@ -520,17 +571,16 @@ __global__ __launch_bounds__(PAIR ? 2 * kWave : kWave) void lbp_bwd_stereo_w64(B
       const float *Xt_p = a.X[dir == RIGHT ? LEFT : RIGHT] + img;
       const bool isR = dir == RIGHT;
       float xo[PAIR ? PF : 1][LPL];
-      auto st_row = [&](float *row, const float *v) {
-        if (FULL) {
-          float *p = row + c0;
-#pragma unroll
-          for (int k = 0; k < LPL; ++k) p[k] = v[k];
-        } else {
-#pragma unroll
-          for (int k = 0; k < LPL; ++k)
-            if (valid[k]) row[c0 + k] = v[k];
-        }
-      };
+      // row streams of the fast step, buffer-addressed (see BUF)
+      const int64_t HWC = HW * C;
+      float *gm_p = a.grad_msg + (int64_t)n * 4 * HWC;
+      const __amdgpu_buffer_rsrc_t xt_rs = make_rsrc(Xt_p, plane_bytes);
+      const __amdgpu_buffer_rsrc_t xo_rs = make_rsrc(Xo, plane_bytes);
+      const __amdgpu_buffer_rsrc_t gc_rs = make_rsrc(a.grad_cost + img, plane_bytes);
+      const __amdgpu_buffer_rsrc_t gu_rs = make_rsrc(gm_p + UP * HWC, plane_bytes);
+      const __amdgpu_buffer_rsrc_t gd_rs = make_rsrc(gm_p + DOWN * HWC, plane_bytes);
+      const __amdgpu_buffer_rsrc_t gl_rs = make_rsrc(gm_p + LEFT * HWC, plane_bytes);
+      const __amdgpu_buffer_rsrc_t gr_rs = make_rsrc(gm_p + RIGHT * HWC, plane_bytes);
       auto step = [&](int i, int t, auto second_tag) {
         constexpr bool SECOND = decltype(second_tag)::value;
         const int s = nsteps - 1 - t;
@@ -634,8 +684,7 @@ __global__ __launch_bounds__(PAIR ? 2 * kWave : kWave) void lbp_bwd_stereo_w64(B
         __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(bin_here == 0 ? 0.f : w * tot), gj_rs,
                                               bin_lane ? (bin_here * (int)HW + qi) * 4 : kDrop, 0, 0);
         __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(tot), ge_rs, lane == 0 ? qi * 4 : kDrop, 0, 0);
-        const int64_t HWC = HW * C;
-        float *gm = a.grad_msg + (int64_t)n * 4 * HWC + q * C;
+        const uint32_t so = row_off(q);
         if constexpr (SECOND) {
           // second visitor: the pixel's grad_cost and the U and D slots of its input-message
           // gradient, X_R + X_L (X_D and X_U at the sender are the next step's receiver terms,
@@ -649,26 +698,26 @@ __global__ __launch_bounds__(PAIR ? 2 * kWave : kWave) void lbp_bwd_stereo_w64(B
             cst[k] = ((pd[ni][k] + pu[ni][k]) + xr) + xl;
             hr[k] = xr + xl;
           }
-          st_row(a.grad_cost + img + q * C, cst);
-          st_row(gm + UP * HWC, hr);
-          st_row(gm + DOWN * HWC, hr);
+          st_row_b(gc_rs, so, cst);
+          st_row_b(gu_rs, so, hr);
+          st_row_b(gd_rs, so, hr);
         } else {
-          st_row(Xo + q * C, gh);
+          st_row_b(xo_rs, so, gh);
           if constexpr (PAIR && !kOsp) {
             // first visitor: the L and R slots, 0 away from the row ends (which are combined
             // after the sweep); written here so that both halves store three rows per step
             float zero[LPL];
 #pragma unroll
             for (int k = 0; k < LPL; ++k) zero[k] = 0.f;
-            st_row(gm + LEFT * HWC, zero);
-            st_row(gm + RIGHT * HWC, zero);
+            st_row_b(gl_rs, so, zero);
+            st_row_b(gr_rs, so, zero);
           }
         }
         tsprev = ts;
 #pragma unroll
         for (int k = 0; k < LPL; ++k) { xprev[k] = X[k]; smprev[k] = sm[k]; }
       };
-      auto load_xo = [&](int s, float *d) { ld_row(Xt_p + (q0 + (int64_t)s * dq) * C, d); };
+      auto load_xo = [&](int s, float *d) { ld_row_b(xt_rs, row_off(q0 + (int64_t)s * dq), d); };
       // a group of PF steps; each ring slot is refilled after its use (with the other pass's
       // h-gradient too in the second half)
       auto group = [&](int t0, auto second_tag) {
@@ -1423,9 +1472,10 @@ void launch_kind(const BwdArgs &a, bool vert, int ngroups, hipStream_t st) {
 
 // the fast step needs the window half-width (argmins within 3 labels), a stereo packing's
 // jump length and 32-bit buffer offsets into the (direction) planes of the pairwise gradient
+// and 32-bit byte offsets into every [H,W,C] plane (its row streams are buffer-addressed)
 bool stereo_fast_ok(const BwdArgs &a) {
   return a.delta >= 0 && a.delta <= 3 && !a.accum && (a.K == 6 || a.K == 8) &&
-         (int64_t)a.K * a.H * a.W * 4 < (1LL << 31);
+         (int64_t)a.K * a.H * a.W * 4 < (1LL << 31) && (int64_t)a.H * a.W * a.C * 4 < (1LL << 31);
 }
 template <int LPL, bool FULL>
 void launch_stereo_w64(const BwdArgs &a, bool vert, bool pair, int ngroups, hipStream_t st) {
