@@ -18,7 +18,6 @@
 #include "bpl_common.h"
 
 namespace bpl {
-namespace {
 
 struct BwdArgs {
   const float *jump, *edge, *in_grad, *msgs;
@@ -33,6 +32,15 @@ struct BwdArgs {
   // (igz), grad_msg's LEFT/RIGHT slots need writing only at the row ends (osp)
   int igz, osp;
 };
+
+// Paired horizontal launches (stereo, exact mode, fast step).  Their instantiations dominate
+// the build, so they live in one translation unit per labels-per-lane count
+// (lbp_bwd_pair<L>.hip defines BPL_BWD_PAIR_LPL = L and includes this file) and compile in
+// parallel with this one.
+template <int LPL>
+void launch_bwd_pair(const BwdArgs &a, bool full, int fk, int ngroups, hipStream_t st);
+
+namespace {
 
 __device__ __forceinline__ int64_t px(int n, int y, int x, int H, int W) {
   return ((int64_t)n * H + y) * (int64_t)W + x;
@@ -1446,6 +1454,7 @@ __global__ void lbp_bwd_reduce_chunks(const float *__restrict__ chunk_sums, floa
   out[i] = s;
 }
 
+#ifndef BPL_BWD_PAIR_LPL
 template <int KIND, int SEG, int LPL>
 void launch_phase(const BwdArgs &a, bool vert, int ngroups, hipStream_t st) {
   constexpr int PF = 4;
@@ -1479,6 +1488,7 @@ bool stereo_fast_ok(const BwdArgs &a) {
 }
 template <int LPL, bool FULL>
 void launch_stereo_w64(const BwdArgs &a, bool vert, bool pair, int ngroups, hipStream_t st) {
+  // prefetch depth 4 (the paired horizontal kernel: launch_bwd_pair)
   constexpr int PF = 4;
   const bool fast = stereo_fast_ok(a);
   auto go = [&](auto fk) {
@@ -1487,19 +1497,8 @@ void launch_stereo_w64(const BwdArgs &a, bool vert, bool pair, int ngroups, hipS
       const dim3 grid((unsigned)(ngroups * 2 * a.W));
       hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, true, FULL, FK, false, PF>), grid, dim3(kWave), 0, st, a);
     } else if (pair) {
-      // pairing needs the fast step (stereo_pair_ok), so only FK > 0 is instantiated
-      if constexpr (FK > 0) {
-        const dim3 grid((unsigned)(ngroups * a.H));
-        const int hs = (a.igz ? 1 : 0) | (a.osp ? 2 : 0);
-        if (hs == 0)
-          hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, FK, true, PF, 0>), grid, dim3(2 * kWave), 0, st, a);
-        else if (hs == 1)
-          hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, FK, true, PF, 1>), grid, dim3(2 * kWave), 0, st, a);
-        else if (hs == 2)
-          hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, FK, true, PF, 2>), grid, dim3(2 * kWave), 0, st, a);
-        else
-          hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, FK, true, PF, 3>), grid, dim3(2 * kWave), 0, st, a);
-      }
+      // pairing needs the fast step (stereo_pair_ok), so FK > 0
+      launch_bwd_pair<LPL>(a, FULL, FK, ngroups, st);
     } else {
       const dim3 grid((unsigned)(ngroups * 2 * a.H));
       hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, FK, false, PF>), grid, dim3(kWave), 0, st, a);
@@ -1511,7 +1510,8 @@ void launch_stereo_w64(const BwdArgs &a, bool vert, bool pair, int ngroups, hipS
 }
 template <int LPL>
 void launch_stereo_w64_any(const BwdArgs &a, bool vert, bool pair, int ngroups, hipStream_t st) {
-  if (a.C == kWave * LPL) launch_stereo_w64<LPL, true>(a, vert, pair, ngroups, st);
+  // full rows of 4 labels per lane would be C = 256, above the 255-label limit
+  if (LPL < 4 && a.C == kWave * LPL) launch_stereo_w64<LPL, LPL < 4>(a, vert, pair, ngroups, st);
   else launch_stereo_w64<LPL, false>(a, vert, pair, ngroups, st);
 }
 // the 64-lane stereo kernel covers 16 < C <= 256 and up to 8 pairwise bins
@@ -1677,3 +1677,47 @@ extern "C" int bpl_lbp_bwd(int kind, int mode, const float *jump, int64_t jump_k
   }
   return BPL_OK;
 }
+
+#else  // BPL_BWD_PAIR_LPL: the paired horizontal launches for one labels-per-lane count
+
+}  // namespace
+
+// Prefetch depth 12: the paired kernel has one wave per SIMD, so registers are spare, and a
+// shallower ring leaves its steps waiting on memory (384x1248x192: 1.205 ms per launch with 4
+// steps, 1.147 with 8, 1.084 with 12).  HS carries the chained-iteration flags.
+template <int LPL>
+void launch_bwd_pair(const BwdArgs &a, bool full, int fk, int ngroups, hipStream_t st) {
+  constexpr int PFH = 12;
+  const dim3 grid((unsigned)(ngroups * a.H));
+  const int hs = (a.igz ? 1 : 0) | (a.osp ? 2 : 0);
+  auto go = [&](auto full_c, auto fk_c) {
+    constexpr bool FULL = decltype(full_c)::value;
+    constexpr int FK = decltype(fk_c)::value;
+    if (hs == 0)
+      hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, FK, true, PFH, 0>), grid, dim3(2 * kWave), 0, st, a);
+    else if (hs == 1)
+      hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, FK, true, PFH, 1>), grid, dim3(2 * kWave), 0, st, a);
+    else if (hs == 2)
+      hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, FK, true, PFH, 2>), grid, dim3(2 * kWave), 0, st, a);
+    else
+      hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, FK, true, PFH, 3>), grid, dim3(2 * kWave), 0, st, a);
+  };
+  using T = std::true_type;
+  using F = std::false_type;
+  using K6 = std::integral_constant<int, 6>;
+  using K8 = std::integral_constant<int, 8>;
+  // full rows of 4 labels per lane would be C = 256, above the 255-label limit
+  if constexpr (LPL < 4) {
+    if (full) {
+      if (fk == 6) go(T(), K6());
+      else go(T(), K8());
+      return;
+    }
+  }
+  if (fk == 6) go(F(), K6());
+  else go(F(), K8());
+}
+template void launch_bwd_pair<BPL_BWD_PAIR_LPL>(const BwdArgs &, bool, int, int, hipStream_t);
+
+}  // namespace bpl
+#endif  // BPL_BWD_PAIR_LPL
