@@ -368,10 +368,10 @@ __device__ __forceinline__ int shl1_i(int v, int fill) {
 // RIGHT slots are zero inside the sweep (not loaded), bit 1 = grad_msg's LEFT and RIGHT slots
 // are written at the row ends only.  The other kernels read the same flags from a.igz / a.osp.
 template <int LPL, bool VERT, bool FULL, int FASTK, bool PAIR, int PF, int HS = 0>
-__global__ __launch_bounds__(PAIR ? 2 * kWave : kWave) void lbp_bwd_stereo_w64(BwdArgs a) {
+__global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(BwdArgs a) {
   static_assert(!(PAIR && VERT), "pairing is for the horizontal phase");
   constexpr int ND = (3 + LPL - 1) / LPL;  // neighbour lanes reached by offsets up to 3
-  constexpr int NWG = PAIR ? 2 : 1;        // waves per workgroup
+  constexpr int NWG = PAIR ? 4 : 1;        // waves per workgroup (PAIR: two rows, two passes each)
   __shared__ float lds_g_all[NWG][2][kWave * LPL];
   __shared__ int lds_a_all[NWG][2][kWave * LPL];
 
@@ -380,17 +380,23 @@ __global__ __launch_bounds__(PAIR ? 2 * kWave : kWave) void lbp_bwd_stereo_w64(B
   const int len = VERT ? H : W;
   const int nsteps = len - 1;
   // wave index in the workgroup, wave-uniform: everything derived from it (direction, start
-  // pixel, stream pointers) then stays in scalar registers
-  const int wv = PAIR ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) : 0;
-  const int g_ = PAIR ? 2 * (int)(blockIdx.x / L) + wv : (int)(blockIdx.x / L);
-  const int ln = blockIdx.x - (PAIR ? g_ >> 1 : g_) * L;
+  // pixel, stream pointers) then stays in scalar registers.  A paired workgroup holds two rows
+  // (four waves: one wave per SIMD; two-wave workgroups put two waves on some SIMDs); with an
+  // odd number of rows the last workgroup's second row repeats row L-1, whose stores then
+  // write the same values twice (the paired kernel only stores, never accumulates).
+  const int wq = PAIR ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) : 0;
+  const int wv = wq & 1;  // pass of the row (PAIR)
+  const int L2 = (L + 1) / 2;
+  const int g_ = PAIR ? 2 * (int)(blockIdx.x / L2) + wv : (int)(blockIdx.x / L);
+  const int ln = PAIR ? min(2 * ((int)blockIdx.x - (g_ >> 1) * L2) + (wq >> 1), L - 1)
+                      : (int)blockIdx.x - g_ * L;
   const int n = a.n_base + (g_ >> 1);
   const int dir = VERT ? ((g_ & 1) ? UP : DOWN) : ((g_ & 1) ? LEFT : RIGHT);
   const int rslot = dir == RIGHT ? LEFT : dir == LEFT ? RIGHT : dir == DOWN ? UP : DOWN;
   const bool fwd = (dir == RIGHT || dir == DOWN);
   const int lane = threadIdx.x & (kWave - 1);
-  float(&lds_g)[2][kWave * LPL] = lds_g_all[wv];
-  int(&lds_a)[2][kWave * LPL] = lds_a_all[wv];
+  float(&lds_g)[2][kWave * LPL] = lds_g_all[wq];
+  int(&lds_a)[2][kWave * LPL] = lds_a_all[wq];
   const int c0 = lane * LPL;
   const int nj = a.Nj > 1 ? n : 0;
   const bool exact = PAIR || a.mode == BPL_GRAD_EXACT;  // the paired kernel runs in exact mode only
@@ -558,11 +564,11 @@ __global__ __launch_bounds__(PAIR ? 2 * kWave : kWave) void lbp_bwd_stereo_w64(B
     const float *o_p = a.jump + ((int64_t)nj * 4 + dir) * K * HW;
     for (int p = lane; p < len; p += kWave) off = off || o_p[VERT ? (int64_t)p * W + ln : (int64_t)ln * W + p] != 0.f;
     bool fast = __ballot(off) == 0;
-    if constexpr (PAIR) {  // both passes of the row must take the fast step
-      __shared__ int pair_fast[2];
-      if (lane == 0) pair_fast[wv] = fast ? 1 : 0;
+    if constexpr (PAIR) {  // all four waves (both passes of both rows) take the same path,
+      __shared__ int pair_fast[4];  // so that they reach the same barriers
+      if (lane == 0) pair_fast[wq] = fast ? 1 : 0;
       __syncthreads();
-      fast = pair_fast[0] != 0 && pair_fast[1] != 0;
+      fast = pair_fast[0] != 0 && pair_fast[1] != 0 && pair_fast[2] != 0 && pair_fast[3] != 0;
     }
     if (fast) {
       const int hw4 = __builtin_amdgcn_readfirstlane((int)(HW * 4));
@@ -1688,19 +1694,19 @@ extern "C" int bpl_lbp_bwd(int kind, int mode, const float *jump, int64_t jump_k
 template <int LPL>
 void launch_bwd_pair(const BwdArgs &a, bool full, int fk, int ngroups, hipStream_t st) {
   constexpr int PFH = 12;
-  const dim3 grid((unsigned)(ngroups * a.H));
+  const dim3 grid((unsigned)(ngroups * ((a.H + 1) / 2)));  // two rows per workgroup
   const int hs = (a.igz ? 1 : 0) | (a.osp ? 2 : 0);
   auto go = [&](auto full_c, auto fk_c) {
     constexpr bool FULL = decltype(full_c)::value;
     constexpr int FK = decltype(fk_c)::value;
     if (hs == 0)
-      hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, FK, true, PFH, 0>), grid, dim3(2 * kWave), 0, st, a);
+      hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, FK, true, PFH, 0>), grid, dim3(4 * kWave), 0, st, a);
     else if (hs == 1)
-      hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, FK, true, PFH, 1>), grid, dim3(2 * kWave), 0, st, a);
+      hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, FK, true, PFH, 1>), grid, dim3(4 * kWave), 0, st, a);
     else if (hs == 2)
-      hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, FK, true, PFH, 2>), grid, dim3(2 * kWave), 0, st, a);
+      hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, FK, true, PFH, 2>), grid, dim3(4 * kWave), 0, st, a);
     else
-      hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, FK, true, PFH, 3>), grid, dim3(2 * kWave), 0, st, a);
+      hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, FK, true, PFH, 3>), grid, dim3(4 * kWave), 0, st, a);
   };
   using T = std::true_type;
   using F = std::false_type;
