@@ -83,3 +83,17 @@ def test_loads_the_reference_matching_checkpoints(path):
     formats.load_checkpoint(m, path, "cpu")
     sd = torch.load(path, map_location="cpu", weights_only=True)
     assert torch.equal(m._softmax.T.detach(), sd["_softmax.T"])
+
+
+FLOW_MATCH = [p for p in MATCH if os.sep + "flow" + os.sep in p]
+
+
+@pytest.mark.skipif(not FLOW_MATCH, reason="reference checkpoints absent")
+@pytest.mark.parametrize("path", FLOW_MATCH)
+def test_loads_the_reference_flow_matching_checkpoints(path):
+    from bplayers_amd.flow_matching import FlowMatchingSad
+    m = FlowMatchingSad(24, device="cpu")
+    formats.load_checkpoint(m, path, "cpu")
+    sd = torch.load(path, map_location="cpu", weights_only=True)
+    assert torch.equal(m._softmax.T.detach(), sd["_softmax.T"])
+

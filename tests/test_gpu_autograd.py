@@ -41,7 +41,7 @@ def test_stereo_message_passing_exact_gradients_match_fd(iters):
     loss = (beliefs * torch.tensor(Wb, dtype=torch.float32, device=DEV)).sum()
     loss.backward()
     L0, Mo, _ = oracle_pipeline(prob, aff, edge, iters, 3, Wb)
-    np.testing.assert_allclose(float(loss), L0, rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(float(loss.detach()), L0, rtol=1e-5, atol=1e-5)
     eps = 1e-5
     for name, arr, g in (("prob", prob, tp.grad), ("aff", aff, ta.grad), ("edge", edge, te.grad)):
         a64 = arr.astype(np.float64)
@@ -212,3 +212,34 @@ def test_flow_bp_pipeline_runs_and_matches_oracle_forward():
     Mv, *_ = O.lbp_forward(-sm(ocv), O.StereoPairwise(jump), edge, Mu, 3)
     bv = O.beliefs(-sm(ocv), Mv)
     np.testing.assert_allclose(beliefs[:, 1].detach().cpu().numpy(), bv, rtol=1e-4, atol=1e-5)
+
+
+def test_flow_matching_layer_matches_oracle():
+    """FlowMatchingSad: softmax(-min-projected SAD / T) of both flow components, stacked as
+    [N,2,H,W,K], and the feature gradients of a weighted sum of it, against the FP64 oracle
+    (flow SAD forward/backward restatement plus the softmax Jacobian)."""
+    from bplayers_amd.flow_matching import FlowMatchingSad
+    from oracle import sad as S
+    rng = np.random.default_rng(9)
+    N, Cf, H, W, sws, T = 1, 6, 10, 14, 6, 1.7
+    f0 = rng.normal(size=(N, Cf, H, W)).astype(np.float32)
+    f1 = rng.normal(size=(N, Cf, H, W)).astype(np.float32)
+    K = sws + 1
+    Wt = rng.normal(size=(N, 2, H, W, K))
+    m = FlowMatchingSad(sws, init_temp=T, device=DEV)
+    t0, t1 = _t(f0, True), _t(f1, True)
+    prob = m(t0, t1)
+    (prob * torch.tensor(Wt, dtype=torch.float32, device=DEV)).sum().backward()
+    cu, cv, us, vs = S.flow_sad_forward(f0.astype(np.float64), f1.astype(np.float64), sws)
+
+    def softmax(x):
+        e = np.exp(x - x.max(-1, keepdims=True))
+        return e / e.sum(-1, keepdims=True)
+    pu, pv = softmax(-cu / T), softmax(-cv / T)
+    np.testing.assert_allclose(prob.detach().cpu().numpy(), np.stack([pu, pv], axis=1), rtol=1e-5, atol=1e-6)
+    # d/dcv of sum(W * softmax(-cv/T)) = -(1/T) p (W - sum(W p))
+    gu = -(pu * (Wt[:, 0] - (Wt[:, 0] * pu).sum(-1, keepdims=True))) / T
+    gv = -(pv * (Wt[:, 1] - (Wt[:, 1] * pv).sum(-1, keepdims=True))) / T
+    d0, d1 = S.flow_sad_backward(f0.astype(np.float64), f1.astype(np.float64), sws, gu, gv, us, vs)
+    np.testing.assert_allclose(t0.grad.cpu().numpy(), d0, rtol=1e-4, atol=1e-5)
+    np.testing.assert_allclose(t1.grad.cpu().numpy(), d1, rtol=1e-4, atol=1e-5)
