@@ -243,3 +243,45 @@ def test_flow_matching_layer_matches_oracle():
     d0, d1 = S.flow_sad_backward(f0.astype(np.float64), f1.astype(np.float64), sws, gu, gv, us, vs)
     np.testing.assert_allclose(t0.grad.cpu().numpy(), d0, rtol=1e-4, atol=1e-5)
     np.testing.assert_allclose(t1.grad.cpu().numpy(), d1, rtol=1e-4, atol=1e-5)
+
+
+def test_multiscale_bp_matches_oracle():
+    """Coarse-to-fine BP over a 3-level pyramid (stereo.multiscale_bp) against the same loop on
+    the FP64 oracle: trilinear x2 upsampling of the coarser beliefs, half of them added to the
+    finer probabilities (in place, without gradients), BP with the default P1/P2 per level;
+    the returned lists keep the reference's order [level 0, level 2, level 1]."""
+    import torch.nn.functional as F
+    from bplayers_amd.ops.lbp_stereo.bp_op_cuda import BP
+    from bplayers_amd.stereo import multiscale_bp
+    rng = np.random.default_rng(12)
+    shapes = [(1, 16, 24, 16), (1, 8, 12, 8), (1, 4, 6, 4)]
+    probs = [rng.dirichlet(np.ones(s[-1]), size=s[:3]) for s in shapes]
+    pv = [torch.tensor(p, dtype=torch.float32, device=DEV) for p in probs]
+    crfs = [[BP(DEV, max_iter=1, num_labels=s[-1], delta=3, mode_inference='wta')] for s in shapes]
+    with torch.no_grad():
+        disps, beliefs = multiscale_bp(pv, crfs, min_disps=[0, 0, 0])
+    torch.cuda.synchronize()
+    assert [tuple(b.shape) for b in beliefs] == [shapes[0], shapes[2], shapes[1]]
+    b_in, ob = None, {}
+    for lvl in (2, 1, 0):
+        N, H, W, D = shapes[lvl]
+        p = probs[lvl].astype(np.float64)
+        if b_in is not None:
+            up = F.interpolate(torch.from_numpy(b_in)[:, None], scale_factor=2.0, mode='trilinear')[:, 0].numpy()
+            p = p + up / 2.0
+            np.testing.assert_allclose(pv[lvl].cpu().numpy(), p, rtol=1e-5, atol=1e-6)  # the in-place addition
+        aff = np.zeros((N, 4, 3, H, W))
+        aff[:, :, :2] = 0.1
+        aff[:, :, 2] = 2.5
+        cost = -p
+        M, _ = O.lbp_forward_iters(cost, O.StereoPairwise(pack_stereo_jump(aff)), np.ones((N, 4, H, W)),
+                                   np.zeros((N, 4, H, W, D)), 3, 1)
+        ob[lvl] = b_in = O.beliefs(cost, M)
+    np.testing.assert_allclose(beliefs[0].cpu().numpy(), ob[0], rtol=0, atol=5e-5)
+    np.testing.assert_allclose(beliefs[1].cpu().numpy(), ob[2], rtol=0, atol=5e-5)
+    np.testing.assert_allclose(beliefs[2].cpu().numpy(), ob[1], rtol=0, atol=5e-5)
+    srt = np.sort(ob[0], axis=-1)
+    clear = (srt[..., -1] - srt[..., -2]) > 1e-4
+    assert clear.mean() > 0.9
+    d0 = disps[0].cpu().numpy()[:, 0]
+    assert np.array_equal(d0[clear], ob[0].argmax(-1)[clear])
