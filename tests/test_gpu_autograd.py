@@ -285,3 +285,47 @@ def test_multiscale_bp_matches_oracle():
     assert clear.mean() > 0.9
     d0 = disps[0].cpu().numpy()[:, 0]
     assert np.array_equal(d0[clear], ob[0].argmax(-1)[clear])
+
+
+def test_multiscale_flow_bp_matches_oracle():
+    """Coarse-to-fine flow BP over 2 levels (flow.multiscale_flow_bp) against the same loop on
+    the FP64 oracle: per level, u from zero messages and v from u's messages (the 5-D BP path),
+    beliefs of each component upsampled trilinearly to (2H, 2W, 2K-1) and half of them added to
+    the finer probabilities; flow = winner-take-all label - sws // 2."""
+    import torch.nn.functional as F
+    from bplayers_amd.ops.lbp_stereo.bp_op_cuda import BP
+    from bplayers_amd.flow import multiscale_flow_bp
+    rng = np.random.default_rng(13)
+    shapes = [(1, 2, 8, 12, 9), (1, 2, 4, 6, 5)]
+    sws = [8, 4]
+    probs = [rng.dirichlet(np.ones(s[-1]), size=s[:4]) for s in shapes]
+    pv = [torch.tensor(p, dtype=torch.float32, device=DEV) for p in probs]
+    crfs = [[BP(DEV, max_iter=1, num_labels=s[-1], delta=3, mode_inference='wta')] for s in shapes]
+    with torch.no_grad():
+        flows, beliefs = multiscale_flow_bp(pv, crfs, sws)
+    torch.cuda.synchronize()
+    b_in, ob = None, {}
+    for lvl in (1, 0):
+        N, _, H, W, K = shapes[lvl]
+        p = probs[lvl].astype(np.float64)
+        if b_in is not None:
+            size = (2 * b_in.shape[2], 2 * b_in.shape[3], 2 * b_in.shape[4] - 1)
+            up = np.stack([F.interpolate(torch.from_numpy(b_in[:, c])[:, None], size=size, mode='trilinear')[:, 0]
+                           .numpy() for c in range(2)], axis=1)
+            p = p + up / 2.0
+        aff = np.zeros((N, 4, 3, H, W))
+        aff[:, :, :2] = 0.1
+        aff[:, :, 2] = 2.5
+        pw = O.StereoPairwise(pack_stereo_jump(aff))
+        ones = np.ones((N, 4, H, W))
+        cu, cv = -p[:, 0], -p[:, 1]
+        Mu, _ = O.lbp_forward_iters(cu, pw, ones, np.zeros((N, 4, H, W, K)), 3, 1)
+        Mv, _ = O.lbp_forward_iters(cv, pw, ones, Mu, 3, 1)
+        ob[lvl] = b_in = np.stack([O.beliefs(cu, Mu), O.beliefs(cv, Mv)], axis=1)
+    for lvl in (0, 1):
+        np.testing.assert_allclose(beliefs[lvl].cpu().numpy(), ob[lvl], rtol=0, atol=5e-5)
+    srt = np.sort(ob[0], axis=-1)
+    clear = (srt[..., -1] - srt[..., -2]) > 1e-4
+    assert clear.mean() > 0.9
+    f0 = flows[0].cpu().numpy()
+    assert np.array_equal(f0[clear], (ob[0].argmax(-1) - sws[0] // 2)[clear])
