@@ -78,6 +78,9 @@ CASES = [
     ("stereo", 1, 2, 41, 192, 3, {}),
     ("stereo", 1, 4, 13, 64, 3, {"offset": True}),
     ("stereo", 1, 3, 2, 64, 3, {}),
+    # paired workgroups hold two rows: one row only, and a batch of two with an odd row count
+    ("stereo", 1, 1, 9, 64, 3, {}),
+    ("stereo", 2, 3, 11, 192, 3, {}),
     ("semantic_global", 2, 5, 6, 19, 19, {}),
     ("semantic_global", 1, 4, 5, 19, 3, {}),
     ("semantic_global", 1, 3, 30, 25, 25, {}),

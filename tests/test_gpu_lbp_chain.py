@@ -41,6 +41,8 @@ CASES = [
     ("stereo", 1, 5, 7, 40, 3, {"K_in": 5}, True),          # K = 8
     ("stereo", 1, 5, 6, 12, 3, {}, True),                   # C <= 16: generic kernels
     ("stereo", 1, 3, 2, 64, 3, {}, True),                   # two-pixel rows
+    ("stereo", 1, 1, 9, 64, 3, {}, True),                   # one row (paired workgroups hold two)
+    ("stereo", 2, 3, 11, 192, 3, {}, True),                 # batch of two, odd row count
     ("semantic_global", 2, 5, 6, 19, 19, {}, False),
     ("semantic_global", 1, 4, 5, 12, 12, {}, False),
     ("semantic_pixel", 2, 4, 5, 19, 19, {}, False),
