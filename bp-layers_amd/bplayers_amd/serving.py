@@ -14,6 +14,9 @@ Design:
     shape class into a batch (oldest first) whenever a slot of that class is free, and
     launches it asynchronously; batches of different slots overlap on the device;
   * completion is detected with per-slot events; latency = completion - arrival.
+  * ``max_inflight`` (optional) caps the batches on the device at once: batches of
+    different slots share the GPU's compute units, so every in-flight batch runs slower; the
+    cap trades that overlap for shorter individual solves;
   * ``sequential=True`` is the baseline: one request at a time, batch 1, same kernels.
 
 The kernels are batch-invariant (every batch element is solved by its own waves with a
@@ -113,13 +116,14 @@ def ctypes_stream(s):
 
 class StereoServer:
     def __init__(self, device="cuda:0", C=64, hbm_budget=32 << 30, max_batch=4, max_slots_per_class=2, iters=1,
-                 P1=0.1, P2=0.5, delta=3, T_match=1.0, T_bel=1.0, sequential=False):
+                 P1=0.1, P2=0.5, delta=3, T_match=1.0, T_bel=1.0, sequential=False, max_inflight=None):
         self.device = torch.device(device)
         self.C = C
         self.budget = int(hbm_budget)
         self.max_batch = 1 if sequential else max_batch
         self.max_slots = 1 if sequential else max_slots_per_class
         self.sequential = sequential
+        self.max_inflight = 1 if sequential else max_inflight
         self.params = dict(P1=P1, P2=P2, delta=delta, iters=iters, T_match=T_match, T_bel=T_bel)
         self.slots = collections.defaultdict(list)
         self.resident = 0
@@ -185,11 +189,12 @@ class StereoServer:
                         done.append(r)
                     inflight.remove(s)
             launched = False
-            if not (self.sequential and inflight):
+            cap = self.max_inflight
+            if cap is None or len(inflight) < cap:
                 # oldest waiting request first decides the next shape class
                 order = sorted((q[0].arrival, shp) for shp, q in queues.items() if q)
                 for _, shp in order:
-                    if self.sequential and inflight:
+                    if cap is not None and len(inflight) >= cap:
                         break
                     s = self._slot_for(shp)
                     if s is None:

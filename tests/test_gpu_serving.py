@@ -85,3 +85,32 @@ def test_budget_limits_resident_slots_and_still_serves_everything():
     stats = srv.serve(make_requests(pool, 16, rate_hz=1e5, seed=3))
     assert stats["frames"] == 16
     assert srv.resident <= budget
+
+
+def test_inflight_cap_keeps_results_and_bounds_concurrency():
+    """max_inflight=1 still batches requests of one class but keeps one batch on the device at
+    a time; every result equals the sequential server's."""
+    from bplayers_amd import serving
+    from bplayers_amd.serving import StereoServer, make_requests
+    pool = _pool(seed=3)
+    reqs_a = make_requests(pool, 20, rate_hz=1e9, seed=4)
+    reqs_b = make_requests(pool, 20, rate_hz=1e9, seed=4)
+    StereoServer(DEV, C=C_FEAT, hbm_budget=1 << 30, sequential=True).serve(reqs_a, keep_results=True)
+    srv = StereoServer(DEV, C=C_FEAT, hbm_budget=1 << 30, max_batch=4, max_slots_per_class=2, max_inflight=1)
+    peak = [0]
+    launch = serving.Slot.launch
+
+    def counting_launch(slot, reqs):
+        busy = sum(1 for lst in srv.slots.values() for s in lst if s.reqs is not None)
+        peak[0] = max(peak[0], busy + 1)
+        return launch(slot, reqs)
+    serving.Slot.launch = counting_launch
+    try:
+        st = srv.serve(reqs_b, keep_results=True)
+    finally:
+        serving.Slot.launch = launch
+    assert st["frames"] == 20 and peak[0] == 1
+    assert any(len(s.reqs or []) for lst in srv.slots.values() for s in lst) is False
+    for ra, rb in zip(sorted(reqs_a, key=lambda r: r.rid), sorted(reqs_b, key=lambda r: r.rid)):
+        assert torch.equal(ra.result, rb.result), f"request {ra.rid} differs"
+
