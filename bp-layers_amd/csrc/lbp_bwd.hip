@@ -595,8 +595,28 @@ __global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(B
       const __amdgpu_buffer_rsrc_t gd_rs = make_rsrc(gm_p + DOWN * HWC, plane_bytes);
       const __amdgpu_buffer_rsrc_t gl_rs = make_rsrc(gm_p + LEFT * HWC, plane_bytes);
       const __amdgpu_buffer_rsrc_t gr_rs = make_rsrc(gm_p + RIGHT * HWC, plane_bytes);
-      auto step = [&](int i, int t, auto second_tag) {
+      // horizontal phase: the pairwise-bin and edge gradients of four consecutive steps (four
+      // consecutive pixels of the row) are kept in registers and stored as one 16-byte access
+      // per lane (fewer memory instructions per step leave room in the 6-bit vmcnt counter
+      // for the prefetch ring); groups of PF steps start at multiples of 4
+      static_assert(VERT || PF % 4 == 0, "horizontal groups hold whole blocks of four steps");
+      float bq[4], eq[4];
+      auto flush4 = [&](int tb) {  // steps tb .. tb + 3
+        const int64_t sa = nsteps - 1 - tb;
+        const int qa = (int)(q0 + sa * dq), qd = (int)(q0 + (sa - 3) * dq);
+        const int qmin = min(qa, qd);
+        float vb[4], ve[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {  // element k = pixel qmin + k (RIGHT: steps run right to left)
+          vb[k] = fwd ? bq[3 - k] : bq[k];
+          ve[k] = fwd ? eq[3 - k] : eq[k];
+        }
+        buf_st_row<4>(gj_rs, bin_lane ? (uint32_t)(bin_here * (int)HW + qmin) * 4u : (uint32_t)kDrop, 0u, vb);
+        buf_st_row<4>(ge_rs, lane == 0 ? (uint32_t)qmin * 4u : (uint32_t)kDrop, 0u, ve);
+      };
+      auto step = [&](int i, int t, auto second_tag, auto batch_tag) {
         constexpr bool SECOND = decltype(second_tag)::value;
+        constexpr bool BATCH = !VERT && decltype(batch_tag)::value;
         const int s = nsteps - 1 - t;
         const int64_t q = q0 + (int64_t)s * dq;
         int A[LPL];
@@ -695,9 +715,15 @@ __global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(B
         reduce_scatter8(vb, even, odd);
         const float tot = (lane & 1) ? odd : even;
         const int qi = (int)q;
-        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(bin_here == 0 ? 0.f : w * tot), gj_rs,
-                                              bin_lane ? (bin_here * (int)HW + qi) * 4 : kDrop, 0, 0);
-        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(tot), ge_rs, lane == 0 ? qi * 4 : kDrop, 0, 0);
+        const float bval = bin_here == 0 ? 0.f : w * tot;
+        if constexpr (BATCH) {
+          bq[i & 3] = bval;
+          eq[i & 3] = tot;
+        } else {
+          __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(bval), gj_rs,
+                                                bin_lane ? (bin_here * (int)HW + qi) * 4 : kDrop, 0, 0);
+          __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(tot), ge_rs, lane == 0 ? qi * 4 : kDrop, 0, 0);
+        }
         const uint32_t so = row_off(q);
         if constexpr (SECOND) {
           // second visitor: the pixel's grad_cost and the U and D slots of its input-message
@@ -738,7 +764,10 @@ __global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(B
         constexpr bool SECOND = decltype(second_tag)::value;
 #pragma unroll
         for (int i = 0; i < PF; ++i) {
-          step(i, t0 + i, second_tag);
+          step(i, t0 + i, second_tag, std::true_type());
+          if constexpr (!VERT) {
+            if ((i & 3) == 3) flush4(t0 + i - 3);
+          }
           const int s = nsteps - 1 - (t0 + i);
           const int sn = s - PF >= 0 ? s - PF : 0;
           load_step(sn, pg[i], pm[i], pa[i], pd[i], pu[i], pp[i], pl[i]);
@@ -764,8 +793,8 @@ __global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(B
           for (; t0 + PF <= nl; t0 += PF) group(t0, std::true_type());
 #pragma unroll
           for (int i = 0; i < PF; ++i) {
-            if (t0 + i < nl) step(i, t0 + i, std::true_type());
-            else if (t0 + i == nl) step(i, t0 + i, std::false_type());
+            if (t0 + i < nl) step(i, t0 + i, std::true_type(), std::false_type());
+            else if (t0 + i == nl) step(i, t0 + i, std::false_type(), std::false_type());
           }
           fix_lo = W - 1 - Tb;
           fix_hi = Tb;
@@ -776,7 +805,7 @@ __global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(B
         for (; t0 + PF <= nsteps; t0 += PF) group(t0, std::false_type());
 #pragma unroll
         for (int i = 0; i < PF; ++i)
-          if (t0 + i < nsteps) step(i, t0 + i, std::false_type());
+          if (t0 + i < nsteps) step(i, t0 + i, std::false_type(), std::false_type());
       }
       if constexpr (!PAIR) return;
       fast_done = true;
