@@ -539,7 +539,15 @@ __global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(B
 #pragma unroll
       for (int k = 0; k < LPL; ++k) { dd[k] = 0.f; du[k] = 0.f; }
     }
-    dl = BUF ? (int)__builtin_amdgcn_raw_buffer_load_b8(gl_rs_in, 0u, (uint32_t)qr, 0) : (int)gl_p[qr];
+    if constexpr (BUF) {
+      // the receiver's global-argmin byte is wave-uniform: a scalar load of its aligned dword
+      // (read-only data; counted by lgkmcnt, so it takes no slot of the vector-memory counter)
+      const uintptr_t ga = reinterpret_cast<uintptr_t>(gl_p) + (uintptr_t)qr;
+      const uint32_t word = *reinterpret_cast<const __attribute__((address_space(4))) uint32_t *>(ga & ~(uintptr_t)3);
+      dl = (int)((word >> ((uint32_t)(ga & 3) * 8u)) & 0xFFu);
+    } else {
+      dl = (int)gl_p[qr];
+    }
     dp = par_p[q];
   };
   if (nsteps > 0) {
