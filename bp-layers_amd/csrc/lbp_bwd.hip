@@ -448,7 +448,6 @@ __global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(B
   const __amdgpu_buffer_rsrc_t xd_rs = make_rsrc(XDp, plane_bytes);
   const __amdgpu_buffer_rsrc_t xu_rs = make_rsrc(XUp, plane_bytes);
   const __amdgpu_buffer_rsrc_t am_rs = make_rsrc(am_p, plane_bytes / 4u);
-  const __amdgpu_buffer_rsrc_t gl_rs_in = make_rsrc(gl_p, (uint32_t)HW);
   auto row_off = [&](int64_t q) -> uint32_t { return (uint32_t)q * (uint32_t)C * 4u; };
   auto ld_row_b = [&](__amdgpu_buffer_rsrc_t r, uint32_t soff, float *d) {
     if (FULL) {
