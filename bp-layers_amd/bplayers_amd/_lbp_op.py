@@ -24,7 +24,9 @@ Deliberate differences (all internal to the autograd Functions that carry these 
     gradient are zero away from the row ends, so a call may skip reading them in in_grad
     (``in_h_row_ends``) or writing them in its grad_messages (``out_h_row_ends``; those
     entries are then left uninitialised and the tensor is only fit to be the in_grad of a
-    call with ``in_h_row_ends``).
+    call with ``in_h_row_ends``);
+  * ``grad_messages_out=`` (keyword) receives the input-message gradient; in the reference mode
+    it may be ``in_grad`` itself, which then passes through uncopied (for autograd callers).
 """
 import torch
 
@@ -97,7 +99,8 @@ def forward_minsum(kind, cost, jump, edge, messages, delta, out=None):
 
 
 def backward_minsum(kind, cost, jump, edge, in_grad, messages, messages_argmin, message_scale,
-                    mode="reference", delta=None, in_h_row_ends=False, out_h_row_ends=False):
+                    mode="reference", delta=None, in_h_row_ends=False, out_h_row_ends=False,
+                    grad_messages_out=None):
     N, H, W, C, K, Nj = _check_common(kind, cost, jump, edge, messages)
     nat.check_input(in_grad, "in_grad")
     if in_grad.shape != messages.shape:
@@ -125,7 +128,15 @@ def backward_minsum(kind, cost, jump, edge, in_grad, messages, messages_argmin, 
         grad_jump = torch.empty((4, H, W, C, C), dtype=torch.float32, device=dev)
         gj_ret = grad_jump
     grad_edge = torch.empty((N, 4, H, W), dtype=torch.float32, device=dev)
-    grad_msg = torch.empty_like(messages)
+    if grad_messages_out is not None:
+        nat.check_input(grad_messages_out, "grad_messages_out")
+        if grad_messages_out.shape != messages.shape:
+            raise RuntimeError("grad_messages_out must have the shape of messages")
+        if grad_messages_out.data_ptr() == in_grad.data_ptr() and mode != "reference":
+            raise RuntimeError("grad_messages_out may be in_grad in the reference mode only")
+        grad_msg = grad_messages_out
+    else:
+        grad_msg = torch.empty_like(messages)
     ws_bytes = int(nat.lib().bpl_lbp_bwd_workspace_size(kind, N, H, W, C))
     ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
     nat.call("bpl_lbp_bwd", kind, m, nat.ptr(jump), K, Nj, nat.ptr(edge), nat.ptr(in_grad), nat.ptr(messages),

@@ -43,9 +43,11 @@ class LBPMinSumFunction(torch.autograd.Function):
     @staticmethod
     def backward(ctx, in_grad):
         cost, jump, edge, messages, messages_argmin, message_scale = ctx.saved_tensors
+        g = in_grad.contiguous()
+        # reference mode: grad_messages = in_grad passes through without a copy
         grad_cost, grad_jump, grad_edge, grad_message = lbp.backward_minsum(
-            cost, jump.contiguous(), edge, in_grad.contiguous(), messages, messages_argmin, message_scale,
-            mode=ctx.grad_mode)
+            cost, jump.contiguous(), edge, g, messages, messages_argmin, message_scale,
+            mode=ctx.grad_mode, grad_messages_out=g if ctx.grad_mode == 'reference' else None)
         return (grad_cost, None, None, grad_edge, grad_message, None, unpack_pairwise_grad(grad_jump, ctx.single_pw),
                 None, None)
 

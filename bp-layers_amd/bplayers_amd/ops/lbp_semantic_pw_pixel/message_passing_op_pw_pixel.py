@@ -45,9 +45,12 @@ class LBPMinSumFunction(torch.autograd.Function):
     @staticmethod
     def backward(ctx, in_grad):
         cost, edge, messages, messages_argmin, message_scale, j = ctx.saved_tensors
+        g = in_grad.contiguous()
+        # reference mode: grad_messages = in_grad passes through without a copy
         grad_cost, grad_jump, grad_edge, grad_message = lbp.backward_minsum(
-            cost, edge, in_grad.contiguous(), messages, messages_argmin, message_scale, mode=ctx.grad_mode,
-            jump=j if ctx.grad_mode == 'exact' else None)
+            cost, edge, g, messages, messages_argmin, message_scale, mode=ctx.grad_mode,
+            jump=j if ctx.grad_mode == 'exact' else None,
+            grad_messages_out=g if ctx.grad_mode == 'reference' else None)
         return grad_cost, None, None, grad_edge, grad_message, None, fold_pairwise_grad(grad_jump), None
 
 

@@ -84,8 +84,10 @@ class LBPMinSumFunction(torch.autograd.Function):
     @staticmethod
     def backward(ctx, in_grad):
         cost, jump, edge, messages, messages_argmin, message_scale = ctx.saved_tensors
+        # grad_messages = in_grad (the reference's semantics) passes through without a copy
+        g = in_grad.contiguous()
         grad_cost, grad_jump, grad_edge, grad_message = lbp.backward_minsum(
-            cost, jump, edge, in_grad.contiguous(), messages, messages_argmin, message_scale, delta=ctx.delta)
+            cost, jump, edge, g, messages, messages_argmin, message_scale, delta=ctx.delta, grad_messages_out=g)
         grad_aff, grad_offset = unpack_affinity_grad(grad_jump)
         return grad_cost, grad_edge, grad_message, None, grad_aff, grad_offset
 

@@ -11,9 +11,10 @@ def forward_minsum(cost, jump, edge, messages, delta, *, out=None):
 
 
 def backward_minsum(cost, edge, in_grad, messages, messages_argmin, message_scale, *, mode="reference",
-                    jump=None, in_h_row_ends=False, out_h_row_ends=False):
+                    jump=None, in_h_row_ends=False, out_h_row_ends=False, grad_messages_out=None):
     """Gradients [grad_unary, grad_pairwise, grad_edge, grad_messages] of one iteration.
     The reference build computes no edge gradient (zeros); mode="exact" computes it and needs
     the pairwise tensor passed as ``jump=``."""
     return _lbp_op.backward_minsum(KIND, cost, jump, edge, in_grad, messages, messages_argmin, message_scale,
-                                   mode=mode, in_h_row_ends=in_h_row_ends, out_h_row_ends=out_h_row_ends)
+                                   mode=mode, in_h_row_ends=in_h_row_ends, out_h_row_ends=out_h_row_ends,
+                                   grad_messages_out=grad_messages_out)

@@ -39,6 +39,9 @@ struct BwdArgs {
 // parallel with this one.
 template <int LPL>
 void launch_bwd_pair(const BwdArgs &a, bool full, int fk, int ngroups, hipStream_t st);
+// the unpaired horizontal launches (reference mode, or no pairing), in the same units
+template <int LPL>
+void launch_bwd_h_unpaired(const BwdArgs &a, bool full, int fk, int ngroups, hipStream_t st);
 
 namespace {
 
@@ -1418,8 +1421,8 @@ __global__ void lbp_bwd_pair_rows(const float *__restrict__ in_grad, const float
 __global__ void lbp_bwd_combine(const float *__restrict__ XL, const float *__restrict__ XR,
                                 const float *__restrict__ XU, const float *__restrict__ XD,
                                 const float *__restrict__ in_grad, float *__restrict__ grad_cost,
-                                float *__restrict__ grad_msg, int N, int H, int W, int C, int exact,
-                                int osp) {
+                                float *grad_msg, int N, int H, int W, int C, int exact,
+                                int osp, int copy_msgs) {
   const int64_t total = (int64_t)N * H * W * C;
   const int64_t HWC = (int64_t)H * W * C;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
@@ -1428,11 +1431,13 @@ __global__ void lbp_bwd_combine(const float *__restrict__ XL, const float *__res
     const int64_t n = i / HWC;
     const int64_t r = i - n * HWC;            // (y, x, c) inside the image
     const int64_t base = n * 4 * HWC + r;     // slot 0 of this element
-    if (!exact) {
-      grad_msg[base] = in_grad[base];
-      grad_msg[base + HWC] = in_grad[base + HWC];
-      grad_msg[base + 2 * HWC] = in_grad[base + 2 * HWC];
-      grad_msg[base + 3 * HWC] = in_grad[base + 3 * HWC];
+    if (!exact) {  // the reference's grad_messages = in_grad; nothing to copy when they alias
+      if (copy_msgs) {
+        grad_msg[base] = in_grad[base];
+        grad_msg[base + HWC] = in_grad[base + HWC];
+        grad_msg[base + 2 * HWC] = in_grad[base + 2 * HWC];
+        grad_msg[base + 3 * HWC] = in_grad[base + 3 * HWC];
+      }
       continue;
     }
     const int64_t yx = r / C;
@@ -1530,7 +1535,7 @@ bool stereo_fast_ok(const BwdArgs &a) {
 }
 template <int LPL, bool FULL>
 void launch_stereo_w64(const BwdArgs &a, bool vert, bool pair, int ngroups, hipStream_t st) {
-  // prefetch depth 4 (the paired horizontal kernel: launch_bwd_pair)
+  // prefetch depth 4 (the horizontal kernels: launch_bwd_pair, launch_bwd_h_unpaired)
   constexpr int PF = 4;
   const bool fast = stereo_fast_ok(a);
   auto go = [&](auto fk) {
@@ -1542,8 +1547,7 @@ void launch_stereo_w64(const BwdArgs &a, bool vert, bool pair, int ngroups, hipS
       // pairing needs the fast step (stereo_pair_ok), so FK > 0
       launch_bwd_pair<LPL>(a, FULL, FK, ngroups, st);
     } else {
-      const dim3 grid((unsigned)(ngroups * 2 * a.H));
-      hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, FK, false, PF>), grid, dim3(kWave), 0, st, a);
+      launch_bwd_h_unpaired<LPL>(a, FULL, FK, ngroups, st);
     }
   };
   if (fast && a.K == 6) go(std::integral_constant<int, 6>());
@@ -1616,6 +1620,8 @@ extern "C" int bpl_lbp_bwd(int kind, int mode, const float *jump, int64_t jump_k
   BPL_REQUIRE(mode == BPL_GRAD_REFERENCE || mode == BPL_GRAD_EXACT, BPL_E_ARG, "bpl_lbp_bwd: unknown mode %d", mode);
   BPL_REQUIRE(hflags == 0 || mode == BPL_GRAD_EXACT, BPL_E_ARG,
               "bpl_lbp_bwd: the row-end flags apply to the exact mode only");
+  BPL_REQUIRE(grad_msg != in_grad || mode == BPL_GRAD_REFERENCE, BPL_E_ARG,
+              "bpl_lbp_bwd: grad_msg may alias in_grad in the reference mode only");
   BPL_REQUIRE(N >= 1 && H >= 1 && W >= 1 && C >= 1, BPL_E_ARG, "bpl_lbp_bwd: sizes must be positive");
   BPL_REQUIRE(C <= 255, BPL_E_LIMIT, "bpl_lbp_bwd: C=%lld labels exceeds the supported 255", (long long)C);
   BPL_REQUIRE(edge && in_grad && messages && argmin && gargmin && grad_cost && grad_jump && grad_edge && grad_msg,
@@ -1699,7 +1705,7 @@ extern "C" int bpl_lbp_bwd(int kind, int mode, const float *jump, int64_t jump_k
     KernelTimer kt(BPL_KERNEL_LBP_BWD_COMBINE, st);
     hipLaunchKernelGGL(lbp_bwd_combine, dim3((unsigned)blocks), dim3(threads), 0, st, a.X[LEFT], a.X[RIGHT],
                        a.X[UP], a.X[DOWN], in_grad, grad_cost, grad_msg, (int)N, (int)H, (int)W, (int)C,
-                       mode == BPL_GRAD_EXACT ? 1 : 0, a.osp);
+                       mode == BPL_GRAD_EXACT ? 1 : 0, a.osp, grad_msg != in_grad ? 1 : 0);
     int rc = check_hip(hipGetLastError(), "bpl_lbp_bwd combine launch");
     if (rc) return rc;
   }
@@ -1760,6 +1766,34 @@ void launch_bwd_pair(const BwdArgs &a, bool full, int fk, int ngroups, hipStream
   else go(F(), K8());
 }
 template void launch_bwd_pair<BPL_BWD_PAIR_LPL>(const BwdArgs &, bool, int, int, hipStream_t);
+
+// Unpaired horizontal adjoints (one wave per row and pass): the fast step with the paired
+// kernel's 12-step ring (one wave per SIMD here too), the general step with 4.
+template <int LPL>
+void launch_bwd_h_unpaired(const BwdArgs &a, bool full, int fk, int ngroups, hipStream_t st) {
+  const dim3 grid((unsigned)(ngroups * 2 * a.H));
+  auto go = [&](auto full_c, auto fk_c) {
+    constexpr bool FULL = decltype(full_c)::value;
+    constexpr int FK = decltype(fk_c)::value;
+    constexpr int PFU = FK > 0 ? 12 : 4;
+    hipLaunchKernelGGL((lbp_bwd_stereo_w64<LPL, false, FULL, FK, false, PFU>), grid, dim3(kWave), 0, st, a);
+  };
+  using T = std::true_type;
+  using F = std::false_type;
+  auto by_k = [&](auto full_c) {
+    if (fk == 6) go(full_c, std::integral_constant<int, 6>());
+    else if (fk == 8) go(full_c, std::integral_constant<int, 8>());
+    else go(full_c, std::integral_constant<int, 0>());
+  };
+  if constexpr (LPL < 4) {
+    if (full) {
+      by_k(T());
+      return;
+    }
+  }
+  by_k(F());
+}
+template void launch_bwd_h_unpaired<BPL_BWD_PAIR_LPL>(const BwdArgs &, bool, int, int, hipStream_t);
 
 }  // namespace bpl
 #endif  // BPL_BWD_PAIR_LPL

@@ -1,4 +1,5 @@
-// Paired horizontal LBP backward launches for 1 label(s) per lane (see launch_bwd_pair in
-// lbp_bwd.hip): a translation unit of its own so that the build compiles them in parallel.
+// Horizontal LBP backward launches (paired and unpaired) for 1 label(s) per lane (see
+// launch_bwd_pair in lbp_bwd.hip): a translation unit of its own so that the build compiles them
+// in parallel.
 #define BPL_BWD_PAIR_LPL 1
 #include "lbp_bwd.hip"

@@ -67,9 +67,14 @@ def test_argument_validation_returns_codes(lib):
     rc = lib.bpl_lbp_bwd(nat.LBP_STEREO, nat.GRAD_EXACT | 0x40, P, 6, 1, P, P, P, P, P, P, P, P, P, P,
                          1 << 40, 1, 4, 4, 8, 3, None)
     assert rc == -1 and b"unknown mode" in lib.bpl_last_error()
+    # grad_msg may alias in_grad only in the reference mode (arguments: ..., in_grad, ..., grad_msg)
+    G = ctypes.c_void_p(4096)
+    rc = lib.bpl_lbp_bwd(nat.LBP_STEREO, nat.GRAD_EXACT, P, 6, 1, P, G, P, P, P, P, P, P, G, P,
+                         1 << 40, 1, 4, 4, 8, 3, None)
+    assert rc == -1 and b"alias" in lib.bpl_last_error()
     need = lib.bpl_lbp_bwd_workspace_size(nat.LBP_STEREO, 1, 4, 4, 8)
     assert need == 4 * 1 * 4 * 4 * 8 * 4
-    rc = lib.bpl_lbp_bwd(nat.LBP_STEREO, 1, P, 6, 1, P, P, P, P, P, P, P, P, P, P, need - 1, 1, 4, 4, 8, 3, None)
+    rc = lib.bpl_lbp_bwd(nat.LBP_STEREO, 1, P, 6, 1, P, P, P, P, P, P, P, P, G, P, need - 1, 1, 4, 4, 8, 3, None)
     assert rc == -1 and b"workspace" in lib.bpl_last_error()
     rc = lib.bpl_sad_stereo_fwd(P, P, P, 1, 3, 4, 4, 5, 2, 1.0, None)
     assert rc == -1 and b"empty" in lib.bpl_last_error()

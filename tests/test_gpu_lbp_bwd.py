@@ -115,3 +115,33 @@ def test_lbp_backward_matches_oracle(case, mode, pass_delta):
     kind, N, H, W, C, delta, extra = case
     print(f"case {case} mode={mode} pass_delta={pass_delta}", flush=True)
     run_case(kind, N, H, W, C, delta, mode, seed=100 + C + H, pass_delta=pass_delta, **extra)
+
+
+@pytest.mark.parametrize("kind,C", [("stereo", 192), ("stereo", 64), ("semantic_global", 19), ("semantic_pixel", 19)])
+def test_reference_mode_passes_in_grad_through(kind, C):
+    """Reference mode with grad_messages_out = in_grad: no copy is made, in_grad is left as it
+    was, and every other gradient equals the copying call's bit for bit."""
+    cost, jump, edge, M0 = make_case(kind, 1, 5, 13, C, 3 if kind == "stereo" else C, seed=31)
+    mod = _mods()[kind]
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    cost, jump, edge = t(cost), t(jump), t(edge)
+    msgs, am, sc = mod.forward_minsum(cost, jump, edge, t(M0), 3 if kind == "stereo" else C, out=torch.empty_like(t(M0)))
+    G = t(np.random.default_rng(3).normal(size=M0.shape).astype(np.float32))
+    G0 = G.clone()
+    if kind == "semantic_pixel":
+        a = mod.backward_minsum(cost, edge, G, msgs, am, sc)
+        b = mod.backward_minsum(cost, edge, G, msgs, am, sc, grad_messages_out=G)
+    else:
+        a = mod.backward_minsum(cost, jump, edge, G, msgs, am, sc)
+        b = mod.backward_minsum(cost, jump, edge, G, msgs, am, sc, grad_messages_out=G)
+    torch.cuda.synchronize()
+    assert b[3].data_ptr() == G.data_ptr() and torch.equal(G, G0) and torch.equal(a[3], G0)
+    for x, y in zip(a[:3], b[:3]):
+        assert torch.equal(x, y)
+    with pytest.raises(RuntimeError):
+        if kind == "semantic_pixel":
+            mod.backward_minsum(cost, edge, G, msgs, am, sc, mode="exact", jump=jump, grad_messages_out=G)
+        else:
+            mod.backward_minsum(cost, jump, edge, G, msgs, am, sc, mode="exact", grad_messages_out=G)
+
