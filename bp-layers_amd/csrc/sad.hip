@@ -231,10 +231,19 @@ __global__ void sad_stereo_fwd_step_k(const float *__restrict__ f0, const float 
 constexpr int SB_TX = 32, SB_CB = 16, SB_GP = SB_TX + 1, SB_DC = 64, SB_CG = 4;
 constexpr int SB_SPAN = SB_TX + SB_DC - 1;
 
-// sign(t) * g with the reference's dead zone |t| <= 1e-15 -> 0 (a sign-bit flip: exact)
-__device__ __forceinline__ float sgn_mul(float t, float g) {
+// acc + sign(t) * g with the reference's dead zone |t| <= 1e-15 -> 0, in two VALU
+// instructions after the difference t.  The indicator [|t| > eps] is one clamped FMA:
+// 2^127 |t| - 2^127 eps has the sign of |t| - eps exactly (power-of-two scaling, one
+// rounding), and when positive it is at least 2^127 ulp(eps) = 2^54, so the clamp to [0, 1]
+// yields exactly 0 or 1 (NaN t clamps to 0).  The sign of t is a sign-bit flip of g (exact),
+// and fma(1, ±g, acc) = acc ± g, fma(0, ±g, acc) = acc for finite g: the sums are those of
+// adding the selected term.  NEG subtracts the term instead.
+template <bool NEG>
+__device__ __forceinline__ float acc_sgn(float acc, float t, float g) {
+  constexpr float kScale = 0x1p127f, kScaledEps = kSignEps * 0x1p127f;
+  const float on = __builtin_amdgcn_fmed3f(fmaf(fabsf(t), kScale, -kScaledEps), 0.f, 1.f);
   const float v = __int_as_float(__float_as_int(g) ^ (__float_as_int(t) & (int)0x80000000));
-  return fabsf(t) > kSignEps ? v : 0.f;
+  return fmaf(NEG ? -on : on, v, acc);
 }
 
 __global__ __launch_bounds__(256) void sad_stereo_bwd_k(const float *__restrict__ f0, const float *__restrict__ f1,
@@ -310,10 +319,10 @@ __global__ __launch_bounds__(256) void sad_stereo_bwd_k(const float *__restrict_
           const float v0n = F0c[k + 1];  // f0[xa + 1 + d]; f0[xa + d] is the previous step's
           const float g00 = G0c[k * SB_GP], g01 = G0c[k * SB_GP + 1];
           const float g10 = G1c[k * SB_GP], g11 = G1c[k * SB_GP + 1];
-          s00 += sgn_mul(fa0 - v1n, g00);
-          s01 += sgn_mul(fa1 - v1, g01);
-          s10 -= sgn_mul(v0 - fb0, g10);
-          s11 -= sgn_mul(v0n - fb1, g11);
+          s00 = acc_sgn<false>(s00, fa0 - v1n, g00);
+          s01 = acc_sgn<false>(s01, fa1 - v1, g01);
+          s10 = acc_sgn<true>(s10, v0 - fb0, g10);
+          s11 = acc_sgn<true>(s11, v0n - fb1, g11);
           v1 = v1n;
           v0 = v0n;
         }
