@@ -246,7 +246,7 @@ __device__ __forceinline__ float acc_sgn(float acc, float t, float g) {
   return fmaf(NEG ? -on : on, v, acc);
 }
 
-__global__ __launch_bounds__(256) void sad_stereo_bwd_k(const float *__restrict__ f0, const float *__restrict__ f1,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void sad_stereo_bwd_k(const float *__restrict__ f0, const float *__restrict__ f1,
                                                         const float *__restrict__ g, float *__restrict__ df0,
                                                         float *__restrict__ df1, int C, int H, int W, int D, int mind) {
   __shared__ float G0[SB_DC * SB_GP];      // [k - d0][x]
@@ -277,31 +277,42 @@ __global__ __launch_bounds__(256) void sad_stereo_bwd_k(const float *__restrict_
     for (int d0 = 0; d0 < D; d0 += SB_DC) {
       const int dn = min(SB_DC, D - d0);
       __syncthreads();  // the previous chunk's readers are done
-      for (int i = t; i < SB_TX * dn; i += 256) {
-        const int xl = i / dn, kk = i - xl * dn;
+      // compile-time trip counts and branch-free bodies: every load reads a clamped, in-bounds
+      // address and a select stores 0 for the invalid entries, so the unrolled loads are in
+      // flight together instead of one per branch
+#pragma unroll 4
+      for (int i = t; i < SB_TX * SB_DC; i += 256) {
+        const int xl = i / SB_DC, kk = i - xl * SB_DC;
         const int x = x0 + xl, xm = x - (mind + d0 + kk);
-        G0[kk * SB_GP + xl] = (x < W && xm >= 0 && xm < W) ? Gr[(int64_t)x * D + d0 + kk] : 0.f;
+        const bool ok = kk < dn && x < W && xm >= 0 && xm < W;
+        const float v = Gr[(int64_t)min(x, W - 1) * D + d0 + min(kk, dn - 1)];
+        G0[kk * SB_GP + xl] = ok ? v : 0.f;
       }
       // source pixel p = x0 + mind + m carries the entries k = m - xl of the tile's pixels x0 + xl
-      for (int i = t; i < (dn + SB_TX - 1) * SB_TX; i += 256) {
+#pragma unroll 4
+      for (int i = t; i < (SB_DC + SB_TX - 1) * SB_TX; i += 256) {
         const int mm = i / SB_TX, xl = i - mm * SB_TX;
         const int kk = mm - xl;  // k - d0
-        if (kk < 0 || kk >= dn) continue;
         const int x = x0 + xl, pp = x0 + mind + d0 + mm;
-        G1[kk * SB_GP + xl] = (x < W && pp >= 0 && pp < W) ? Gr[(int64_t)pp * D + d0 + kk] : 0.f;
+        const bool in_chunk = kk >= 0 && kk < dn;
+        const bool ok = in_chunk && x < W && pp >= 0 && pp < W;
+        const float v = Gr[(int64_t)min(max(pp, 0), W - 1) * D + d0 + min(max(kk, 0), dn - 1)];
+        if (in_chunk) G1[kk * SB_GP + xl] = ok ? v : 0.f;
       }
 #pragma unroll
       for (int p = 0; p < SB_CG; ++p) {
         const int cb = cg0 + p * SB_CB;
         if (cb >= C) break;
         if (p > 0) __syncthreads();  // the previous part's readers of F are done
+#pragma unroll 3
         for (int i = t; i < SB_CB * SB_SPAN; i += 256) {
           const int cc = i / SB_SPAN, j = i - cc * SB_SPAN;
           const int c = cb + cc;
-          const int64_t b = ((int64_t)n * C + c) * HW + (int64_t)y * W;
+          const int64_t b = ((int64_t)n * C + min(c, C - 1)) * HW + (int64_t)y * W;
           const int xs1 = x0 - (mind + d0 + SB_DC - 1) + j, xs0 = x0 + mind + d0 + j;
-          F1[i] = (c < C && xs1 >= 0 && xs1 < W) ? f1[b + xs1] : 0.f;
-          F0[i] = (c < C && xs0 >= 0 && xs0 < W) ? f0[b + xs0] : 0.f;
+          const float v1 = f1[b + min(max(xs1, 0), W - 1)], v0 = f0[b + min(max(xs0, 0), W - 1)];
+          F1[i] = (c < C && xs1 >= 0 && xs1 < W) ? v1 : 0.f;
+          F0[i] = (c < C && xs0 >= 0 && xs0 < W) ? v0 : 0.f;
         }
         __syncthreads();
         // f1[x - d] = F1c[-k], f0[x + d] = F0c[k] for pixel xa (k = d - mind, absolute)
