@@ -132,8 +132,11 @@ def backward_minsum(kind, cost, jump, edge, in_grad, messages, messages_argmin, 
         nat.check_input(grad_messages_out, "grad_messages_out")
         if grad_messages_out.shape != messages.shape:
             raise RuntimeError("grad_messages_out must have the shape of messages")
-        if grad_messages_out.data_ptr() == in_grad.data_ptr() and mode != "reference":
+        g0, i0, nb = grad_messages_out.data_ptr(), in_grad.data_ptr(), in_grad.numel() * 4
+        if g0 == i0 and mode != "reference":
             raise RuntimeError("grad_messages_out may be in_grad in the reference mode only")
+        if g0 != i0 and not (g0 + nb <= i0 or i0 + nb <= g0):
+            raise RuntimeError("grad_messages_out partly overlaps in_grad")
         grad_msg = grad_messages_out
     else:
         grad_msg = torch.empty_like(messages)

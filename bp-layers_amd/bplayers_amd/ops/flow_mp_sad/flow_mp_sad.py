@@ -14,7 +14,7 @@ class FlowMpSadFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, f0, f1, sws):
         if not (sws <= 108 or sws == 2 * 108):
-            raise ValueError("Unsupported sws: ", sws, "only allowd: <= 108 or 216")
+            raise ValueError(f"search window sws={sws} is not supported (sws <= 108 or sws = 216)")
         cv_u, cv_v, u_star, v_star = pytorch_cuda_flow_mp_sad_op.forward(f0, f1, sws, 0, 0, 0, 0)
         ctx.save_for_backward(f0, f1, u_star, v_star)
         ctx.sws = int(sws)

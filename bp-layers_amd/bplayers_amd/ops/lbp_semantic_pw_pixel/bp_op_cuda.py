@@ -3,14 +3,15 @@
 import torch
 import torch.nn as nn
 
+from .._pairwise import smoothness_prob, truncated_linear
 from ..lbp_stereo.bp_op_cuda import BP as _StereoBP
 from .inference_op import Inference
 from .message_passing_op_pw_pixel import MessagePassing
 
 
 class BP(nn.Module):
-    construct_pw_energy_weights = staticmethod(_StereoBP.construct_pw_energy_weights)
-    construct_pw_prob_weights = staticmethod(_StereoBP.construct_pw_prob_weights)
+    construct_pw_energy_weights = staticmethod(truncated_linear)
+    construct_pw_prob_weights = staticmethod(smoothness_prob)
     adjust_input_weights = _StereoBP.adjust_input_weights
 
     def __init__(self, device, args=None, max_iter=1, num_labels=None, delta=None, mode_inference='expectation',

@@ -7,7 +7,7 @@ from ... import _lbp_op
 from ... import _native as nat
 from ... import pytorch_cuda_lbp_op_semantic_pw_pixel as lbp
 from ...corenet import TemperatureSoftmin
-from ..lbp_stereo.message_passing_op_cuda import construct_pw_tensor  # noqa: F401
+from ..lbp_stereo.message_passing_op_cuda import check_modes, construct_pw_tensor  # noqa: F401
 
 
 def shift_pairwise(jump):
@@ -80,34 +80,30 @@ class LBPMinSumExactChainFunction(torch.autograd.Function):
 class MessagePassing(nn.Module):
     def __init__(self, device, max_iter, num_labels, delta, mode='min-sum', grad_mode='reference'):
         super(MessagePassing, self).__init__()
+        check_modes(mode, grad_mode)
         self.device = device
         self.max_iter = max_iter
-        if mode != 'min-sum':
-            raise ValueError("Unknown message parsing mode " + mode)
         self.mode = mode
         self.grad_mode = grad_mode
+        self.delta = delta
+        self.rescaleT = None
         self.L1 = nn.Parameter(torch.tensor(0.1, device=device), requires_grad=True)
         self.L2 = nn.Parameter(torch.tensor(2.5, device=device), requires_grad=True)
         self.softmin = TemperatureSoftmin(dim=3, init_temp=1.0, device=device)
-        self.delta = delta
-        self.rescaleT = None
 
     def projectL1L2(self):
         self.L2.data = torch.max(self.L1.data, self.L2.data)
 
     def forward(self, prob_vol, edge_weights, messages, jump):
         N, H, W, C = prob_vol.shape
-        if edge_weights is None:
-            edge_weights = torch.ones((N, 4, H, W), device=prob_vol.device)
-        if self.mode != 'min-sum':
-            raise NotImplementedError("message parsing mode " + self.mode + " is currently not implemented!")
+        edge_weights = torch.ones((N, 4, H, W), device=prob_vol.device) if edge_weights is None else edge_weights
         cost = -prob_vol
         if self.grad_mode == 'exact':
             if self.max_iter > 0:
                 messages = LBPMinSumExactChainFunction.apply(cost, edge_weights, messages, self.delta, jump,
                                                              self.max_iter)
         else:
-            for it in range(self.max_iter):
+            for _ in range(self.max_iter):
                 messages = LBPMinSumFunction.apply(cost, self.L1, self.L2, edge_weights, messages, self.delta, jump,
                                                    self.grad_mode)
         beliefs = messages.sum(dim=1) + cost

@@ -3,57 +3,22 @@
 The reference derives BP from its network base class (checkpoint loading through
 command-line args); here BP is a plain nn.Module with an optional ``checkpoint`` path that
 is loaded with ``torch.load(weights_only=True)`` and the reference's key adjustments."""
-import numpy as np
 import torch
 import torch.nn as nn
 
+from .._pairwise import jump_index, jump_value, smoothness_prob, truncated_linear
 from .inference_op import Inference
 from .message_passing_op_cuda import MessagePassing
 
 
 class BP(nn.Module):
 
-    @staticmethod
-    def get_linear_idx(pos, L_vec_size):
-        vec_idx = L_vec_size // 2 + pos
-        vec_idx = np.maximum(1, vec_idx)
-        vec_idx = np.minimum(L_vec_size - 1, vec_idx)
-        return vec_idx
-
-    @staticmethod
-    def f_func(t, s, L_vec):
-        """Pairwise value of labels (t, s) for a jump vector, linearly interpolated for a
-        fractional offset (bp_op_cuda.py:25-51)."""
-        o = L_vec[0]
-        input_pos = t - s + o
-        lower_pos = int(np.floor(input_pos))
-        upper_pos = int(np.ceil(input_pos))
-        if lower_pos == upper_pos:
-            return L_vec[BP.get_linear_idx(lower_pos, L_vec.shape[0])]
-        lower_val = L_vec[BP.get_linear_idx(lower_pos, L_vec.shape[0])]
-        upper_val = L_vec[BP.get_linear_idx(upper_pos, L_vec.shape[0])]
-        return (upper_pos - input_pos) * lower_val + (input_pos - lower_pos) * upper_val
-
-    @staticmethod
-    def construct_pw_energy_weights(K, L1, L2):
-        fij = np.ones((K, K)) * L2
-        for i in range(-1, 2):
-            if i == 0:
-                fij -= np.diag(L2 * np.ones(K - np.abs(i)), i)
-            else:
-                fij -= np.diag((L2 - L1) * np.ones(K - np.abs(i)), i)
-        return fij
-
-    @staticmethod
-    def construct_pw_prob_weights(K, L1, L2):
-        fij = np.zeros((K, K))
-        for i in range(-1, 2):
-            if i == 0:
-                fij += np.diag(L1 * np.ones(K - np.abs(i)), i)
-            else:
-                fij += np.diag(L2 * np.ones(K - np.abs(i)), i)
-        fij[fij == 0] = 1.0 - L1 - L2
-        return fij
+    # closed-form pairwise tables of the global P1/P2 model and the jump-vector lookup, for
+    # callers of the reference's static helpers (bp_op_cuda.py:18-78)
+    get_linear_idx = staticmethod(jump_index)
+    f_func = staticmethod(jump_value)
+    construct_pw_energy_weights = staticmethod(truncated_linear)
+    construct_pw_prob_weights = staticmethod(smoothness_prob)
 
     def __init__(self, device, args=None, max_iter=1, num_labels=None, delta=3, mode_inference='expectation',
                  mode_message_passing='min-sum', layer_idx=0, level=0, grad_mode='reference', checkpoint=None):

@@ -19,55 +19,26 @@ from ... import _lbp_op
 from ... import _native as nat
 from ... import pytorch_cuda_lbp_op_stereo as lbp
 from ...corenet import TemperatureSoftmin
+from .._pairwise import pack_stereo_jump, truncated_linear, unpack_stereo_jump_grad
 
 
 def construct_pw_tensor(L1, L2, K):
-    """Dense K x K P1/P2 pairwise matrix (message_passing_op_cuda.py:9-20)."""
-    pw = torch.ones((K, K), dtype=torch.float) * L2
-    for i in range(-1, 2):
-        if i == 0:
-            pw -= torch.diag(L2 * torch.ones(K - abs(i)), i)
-        else:
-            pw -= torch.diag((L2 - L1) * torch.ones(K - abs(i)), i)
-    return pw.unsqueeze(0).unsqueeze(0)
+    """[1,1,K,K] P1/P2 energy matrix (0 on the diagonal, L1 one label apart, L2 further;
+    message_passing_op_cuda.py:9-20)."""
+    return torch.tensor(truncated_linear(K, float(L1), float(L2)), dtype=torch.float)[None, None]
 
 
 def pack_affinities(affinities, offset):
-    """[N,4,K_in,H,W] affinities + [N,4,H,W] offset -> [N,4,K_in+3,H,W] jump vector
-    (message_passing_op_cuda.py:35-56): index 0 the window offset, 1 and -1 the truncation
-    (last affinity), the middle 0, then pairs outward; directions 1 and 3 get each pair
-    swapped so that their term is the transpose of directions 0 and 2."""
-    N, D4, Ki, H, W = affinities.shape
-    Kp = Ki + 3
-    J = torch.zeros((N, D4, Kp, H, W), dtype=affinities.dtype, device=affinities.device)
-    J[:, :, 0] = offset
-    J[:, :, 1] = affinities[:, :, -1]
-    J[:, :, -1] = affinities[:, :, -1]
-    ic = 0
-    for i in range(2, Kp // 2):
-        J[:, 0::2, i] = affinities[:, 0::2, ic]
-        J[:, 0::2, -i] = affinities[:, 0::2, ic + 1]
-        J[:, 1::2, i] = affinities[:, 1::2, ic + 1]
-        J[:, 1::2, -i] = affinities[:, 1::2, ic]
-        ic += 2
-    return J.contiguous()
+    """[N,4,K_in,H,W] affinities (K_in = 3: (P1-, P1+, P2); 5: (L2-, L2+, L1-, L1+, L3)) plus
+    the [N,4,H,W] window offset -> [N,4,K_in+3,H,W] jump vectors (message_passing_op_cuda.py:35-56;
+    slot layout in ops._pairwise.stereo_jump_layout)."""
+    return pack_stereo_jump(affinities, offset)
 
 
 def unpack_affinity_grad(gJ):
-    """Adjoint of pack_affinities for the affinities (message_passing_op_cuda.py:75-91);
-    returns (grad_affinities, grad_offset)."""
-    N, D4, Kp, H, W = gJ.shape
-    g = torch.zeros((N, D4, Kp - 3, H, W), dtype=gJ.dtype, device=gJ.device)
-    g[:, :, -1] += gJ[:, :, 1]
-    g[:, :, -1] += gJ[:, :, -1]
-    ic = 0
-    for i in range(2, Kp // 2):
-        g[:, 0::2, ic] = gJ[:, 0::2, i]
-        g[:, 0::2, ic + 1] = gJ[:, 0::2, -i]
-        g[:, 1::2, ic + 1] = gJ[:, 1::2, i]
-        g[:, 1::2, ic] = gJ[:, 1::2, -i]
-        ic += 2
-    return g, gJ[:, :, 0]
+    """Adjoint of pack_affinities (message_passing_op_cuda.py:75-91): (grad_affinities,
+    grad_offset); the gradient of the constant centre slot is dropped."""
+    return unpack_stereo_jump_grad(gJ)
 
 
 class LBPMinSumFunction(torch.autograd.Function):
@@ -140,8 +111,33 @@ class LBPMinSumExactChainFunction(torch.autograd.Function):
         return grad_cost, grad_edge, grad_message, None, grad_aff, grad_offset, None
 
 
+def check_modes(mode, grad_mode):
+    """The layers implement min-sum message passing only, with the reference's gradient or the
+    exact adjoint."""
+    if mode != 'min-sum':
+        raise ValueError(f"message passing mode {mode!r} is not supported (only 'min-sum')")
+    if grad_mode not in ('reference', 'exact'):
+        raise ValueError(f"gradient mode {grad_mode!r} is not supported ('reference' or 'exact')")
+
+
 class MessagePassing(nn.Module):
-    """Min-sum message passing layer (message_passing_op_cuda.py:95-178)."""
+    """Min-sum message passing layer (message_passing_op_cuda.py:95-178): ``max_iter`` LBP
+    iterations on cost = -prob_vol, then beliefs = softmin(cost + sum of the four messages).
+    Without per-pixel affinities the global parameters ``L1`` / ``L2`` (P1 for both
+    neighbours, P2) are used for every pixel and direction."""
+
+    def __init__(self, device, max_iter, num_labels, delta, mode='min-sum', grad_mode='reference'):
+        super(MessagePassing, self).__init__()
+        check_modes(mode, grad_mode)
+        self.device = device
+        self.max_iter = max_iter
+        self.mode = mode
+        self.grad_mode = grad_mode
+        self.delta = delta
+        self.rescaleT = None
+        self._L1 = nn.Parameter(torch.tensor(0.1, device=device), requires_grad=True)
+        self._L2 = nn.Parameter(torch.tensor(2.5, device=device), requires_grad=True)
+        self.softmin = TemperatureSoftmin(dim=3, init_temp=1.0, device=device)
 
     @property
     def L1(self):
@@ -152,59 +148,42 @@ class MessagePassing(nn.Module):
         return self._L2
 
     def setL1L2(self, value_L1, value_L2):
-        if value_L1 > 0 and value_L2 > 0 and value_L1 <= value_L2:
-            self._L1 = value_L1
-            self._L2 = value_L2
-        elif value_L1 < 0 or value_L2 < 0:
-            raise ValueError("L1 or L2 is < 0!")
-        elif value_L1 > value_L2:
-            raise ValueError("L1 must be smaller than or equal L2!")
-
-    def __init__(self, device, max_iter, num_labels, delta, mode='min-sum', grad_mode='reference'):
-        super(MessagePassing, self).__init__()
-        self.device = device
-        self.max_iter = max_iter
-        if mode != 'min-sum':
-            raise ValueError("Unknown message parsing mode " + mode)
-        if grad_mode not in ('reference', 'exact'):
-            raise ValueError("Unknown gradient mode " + grad_mode)
-        self.mode = mode
-        self.grad_mode = grad_mode
-        self._L1 = nn.Parameter(torch.tensor(0.1, device=device), requires_grad=True)
-        self._L2 = nn.Parameter(torch.tensor(2.5, device=device), requires_grad=True)
-        self.softmin = TemperatureSoftmin(dim=3, init_temp=1.0, device=device)
-        self.delta = delta
-        self.rescaleT = None
+        """Replace both parameters when 0 < L1 <= L2; negative values or L1 > L2 raise, and a
+        zero leaves the parameters unchanged (the reference's behaviour)."""
+        if min(value_L1, value_L2) < 0:
+            raise ValueError(f"L1 and L2 must be non-negative (got {value_L1}, {value_L2})")
+        if value_L1 > value_L2:
+            raise ValueError(f"L1 must not exceed L2 (got {value_L1} > {value_L2})")
+        if value_L1 > 0 and value_L2 > 0:
+            self._L1, self._L2 = value_L1, value_L2
 
     def projectL1L2(self):
         self.L2.data = torch.max(self.L1.data, self.L2.data)
 
+    def global_affinities(self, N, H, W, device):
+        """(P1, P1, P2) = (L1, L1, L2) at every pixel and direction: [N,4,3,H,W]."""
+        aff = torch.zeros((N, 4, 3, H, W), dtype=torch.float, device=device)
+        aff[:, :, :2] = self._L1
+        aff[:, :, 2] = self._L2
+        return aff
+
     def forward(self, prob_vol, edge_weights, affinities, offset, messages):
         N, H, W, C = prob_vol.shape
-        NUM_DIR = 4
         dev = prob_vol.device
-        if edge_weights is None:
-            edge_weights = torch.ones((N, NUM_DIR, H, W), device=dev)
-        if affinities is None:
-            if (self._L1 is not None) and (self._L2 is not None):
-                affinities = torch.zeros((N, NUM_DIR, 3, H, W), dtype=torch.float, device=dev)
-                affinities[:, :, :2, :, :] = self._L1
-                affinities[:, :, 2, :, :] = self._L2
-        else:
-            if (self._L1 is not None and self._L2 is None) or (self._L1 is None and self._L2 is not None):
-                raise ValueError("L1 or L2 is None and affinities are not set!")
-        if offset is None:
-            offset = torch.zeros((N, NUM_DIR, H, W), device=dev)
-        if self.mode != 'min-sum':
-            raise NotImplementedError("message parsing mode " + self.mode + " is currently not implemented!")
+        have = (self._L1 is not None, self._L2 is not None)
+        if affinities is None and all(have):
+            affinities = self.global_affinities(N, H, W, dev)
+        elif affinities is not None and any(have) and not all(have):
+            raise ValueError("per-pixel affinities were given while only one of L1 / L2 is set")
+        edge_weights = torch.ones((N, 4, H, W), device=dev) if edge_weights is None else edge_weights
+        offset = torch.zeros((N, 4, H, W), device=dev) if offset is None else offset
         cost = -prob_vol
         if self.grad_mode == 'exact':
             if self.max_iter > 0:
                 messages = LBPMinSumExactChainFunction.apply(cost, edge_weights, messages, self.delta, affinities,
                                                              offset, self.max_iter)
         else:
-            for it in range(self.max_iter):
+            for _ in range(self.max_iter):
                 messages = LBPMinSumFunction.apply(cost, edge_weights, messages, self.delta, affinities, offset)
-        beliefs = messages.sum(dim=1) + cost
-        beliefs = self.softmin.forward(beliefs)
+        beliefs = self.softmin.forward(messages.sum(dim=1) + cost)
         return beliefs, messages

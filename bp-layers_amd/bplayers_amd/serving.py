@@ -82,6 +82,10 @@ class Slot:
         H, W, D = self.shape
         C = self.C
         p = nat.ptr
+        # the slot's stream first waits for the caller's stream: that orders this batch after
+        # the copy of the previous batch's results (finish() clones them on the caller's
+        # stream) and after whatever produced the requests' features
+        self.stream.wait_stream(torch.cuda.current_stream(self.stream.device))
         with torch.cuda.stream(self.stream):
             st = ctypes_stream(self.stream)
             for i, r in enumerate(reqs):
@@ -101,6 +105,8 @@ class Slot:
         return self.reqs is not None and self.event.query()
 
     def finish(self, keep_results):
+        """Called once the slot's event has completed; the results are copied on the caller's
+        current stream (the next launch() of this slot waits for that stream)."""
         reqs = self.reqs
         if keep_results:
             for i, r in enumerate(reqs):

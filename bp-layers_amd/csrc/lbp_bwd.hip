@@ -1620,10 +1620,21 @@ extern "C" int bpl_lbp_bwd(int kind, int mode, const float *jump, int64_t jump_k
   BPL_REQUIRE(mode == BPL_GRAD_REFERENCE || mode == BPL_GRAD_EXACT, BPL_E_ARG, "bpl_lbp_bwd: unknown mode %d", mode);
   BPL_REQUIRE(hflags == 0 || mode == BPL_GRAD_EXACT, BPL_E_ARG,
               "bpl_lbp_bwd: the row-end flags apply to the exact mode only");
-  BPL_REQUIRE(grad_msg != in_grad || mode == BPL_GRAD_REFERENCE, BPL_E_ARG,
-              "bpl_lbp_bwd: grad_msg may alias in_grad in the reference mode only");
   BPL_REQUIRE(N >= 1 && H >= 1 && W >= 1 && C >= 1, BPL_E_ARG, "bpl_lbp_bwd: sizes must be positive");
   BPL_REQUIRE(C <= 255, BPL_E_LIMIT, "bpl_lbp_bwd: C=%lld labels exceeds the supported 255", (long long)C);
+  BPL_REQUIRE(N * 2 * (H > W ? H : W) < (1LL << 31) && H < (1 << 30) && W < (1 << 30), BPL_E_LIMIT,
+              "bpl_lbp_bwd: grid too large");
+  {
+    // grad_msg and in_grad are [N,4,H,W,C] each: they must be disjoint, or (reference mode
+    // only, where nothing is written to grad_msg) the same tensor; a partial overlap would
+    // race with kernels that read in_grad while others write grad_msg
+    const uintptr_t g0 = (uintptr_t)grad_msg, i0 = (uintptr_t)in_grad;
+    const uintptr_t bytes = (uintptr_t)(N * 4 * H * W * C) * sizeof(float);
+    const bool same = g0 == i0, disjoint = g0 + bytes <= i0 || i0 + bytes <= g0;
+    BPL_REQUIRE(disjoint || (same && mode == BPL_GRAD_REFERENCE), BPL_E_ARG,
+                same ? "bpl_lbp_bwd: grad_msg may alias in_grad in the reference mode only"
+                     : "bpl_lbp_bwd: grad_msg partly overlaps in_grad");
+  }
   BPL_REQUIRE(edge && in_grad && messages && argmin && gargmin && grad_cost && grad_jump && grad_edge && grad_msg,
               BPL_E_ARG, "bpl_lbp_bwd: null pointer");
   BPL_REQUIRE(jump || (kind == BPL_LBP_SEMANTIC_PIXEL && mode == BPL_GRAD_REFERENCE), BPL_E_ARG,

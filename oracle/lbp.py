@@ -345,6 +345,139 @@ def lbp_forward(cost, pw, edge, messages, delta, want_margin=False):
     return M, argmin, scale, from_global, gargmin
 
 
+# --------------------------------------------------------------------------------------
+# argmins of one fp32 iteration, replayed step by step from its own inputs
+# --------------------------------------------------------------------------------------
+U32 = 2.0 ** -24    # unit roundoff of fp32
+ROUND_KAPPA = 6.0   # roundings bounded per candidate value (see local_argmins)
+
+
+def _step_decisions(ops, fw_of, pw, n, d, ys, xs, w, delta):
+    """FP64 decisions of one step from its fp32 operands, with an fp32 ambiguity mask.
+
+    ops [P,C,4]: the four fp32 summands of h per sender label (cost, chain, two cross
+    messages); fw_of(c, l) -> f(c,l) * w [P,C] for receiver labels c and sender labels l.
+    Returns (label [P,C] of the winning sender (the global argmin where the truncation term
+    won), from_global [P,C], clear [P,C]): ``clear`` is False where another candidate, or the
+    global argmin's runner-up, lies within the fp32 rounding bound of the winner."""
+    P, C, _ = ops.shape
+    h = ops.sum(axis=2)
+    A = np.abs(ops).sum(axis=2)
+    bits = np.ascontiguousarray(ops.astype(np.float32)).view(np.uint32)   # operand identity
+    rows = np.arange(P)[:, None]
+    cs = np.arange(C)
+
+    def same_ops(l1, l2):
+        return (bits[rows, l1] == bits[rows, l2]).all(axis=-1)
+
+    res = send(h, pw, n, d, ys, xs, w, delta)
+    arg, fg, lstar = res[1], res[3], res[4]
+    # candidates in the order of send(): (value, sender label, is_global, f*w)
+    cands = []
+    if delta >= C:
+        for l in range(C):
+            ll = np.full((P, C), l)
+            fw = fw_of(cs[None, :], ll)
+            cands.append((h[:, l:l + 1] + fw, ll, False, fw, np.ones((P, C), bool)))
+    else:
+        ls = np.broadcast_to(lstar[:, None], (P, C))
+        ft = (pw.trunc(n, d, ys, xs) * w)[:, None]
+        cands.append((np.broadcast_to(h[rows, ls] + ft, (P, C)), ls, True, np.broadcast_to(ft, (P, C)),
+                      np.ones((P, C), bool)))
+        start, stop = pw.window(n, d, ys, xs, delta, C)
+        kmin = int((start - cs[None, :]).min())
+        kmax = int((stop - 1 - cs[None, :]).max())
+        for k in range(kmin, kmax + 1):
+            l = cs + k
+            valid = (l[None, :] >= start) & (l[None, :] < stop) & (l >= 0)[None, :] & (l < C)[None, :]
+            lc = np.broadcast_to(np.clip(l, 0, C - 1)[None, :], (P, C))
+            fw = fw_of(cs[None, :], lc)
+            cands.append((h[rows, lc] + fw, lc, False, fw, valid))
+    vals = np.stack([c[0] for c in cands])                  # [M,P,C]
+    labs = np.stack([c[1] for c in cands])
+    isg = np.array([c[2] for c in cands])[:, None, None]
+    fws = np.stack([c[3] for c in cands])
+    ok = np.stack([c[4] for c in cands])
+    # frame allowance: an fp32 kernel may carry the chain shifted by up to log C plus the
+    # largest pairwise value (a normalisation it has not applied yet)
+    fmax = np.where(ok, np.abs(fws), 0.0).max(axis=(0, 2))[:, None]
+    err_h = ROUND_KAPPA * U32 * (A + np.maximum(np.log(C), fmax))   # per sender label [P,C]
+    wi = np.argmax((labs == arg[None]) & (isg == fg[None]) & ok, axis=0)   # the winning candidate
+    take = lambda a: np.take_along_axis(a, wi[None], axis=0)[0]
+    v_w, l_w, fw_w = take(vals), take(labs), take(fws)
+    e_w = err_h[rows, l_w] + ROUND_KAPPA * U32 * np.abs(fw_w)
+    clear = np.ones((P, C), dtype=bool)
+    for m in range(len(cands)):
+        other = ok[m] & ((labs[m] != arg) | (isg[m, 0, 0] != fg))
+        e_m = err_h[rows, labs[m]] + ROUND_KAPPA * U32 * np.abs(fws[m])
+        gap = np.abs(vals[m] - v_w)
+        near = gap <= e_m + e_w
+        exact = (vals[m] == v_w) & same_ops(labs[m], l_w) & (fws[m] == fw_w)
+        clear &= ~(other & near & ~exact)
+    if delta < C:
+        # the label stored for truncation winners is the global argmin: its runner-up must be
+        # outside the rounding bound too (or an identical-operand tie, resolved by the tree)
+        ls = lstar[:, None]
+        gap = np.abs(h - h[rows, ls])
+        near = (gap <= err_h + err_h[rows, ls]) & (cs[None, :] != ls)
+        exact = (h == h[rows, ls]) & same_ops(np.broadcast_to(cs[None, :], (P, C)), np.broadcast_to(ls, (P, C)))
+        lab_ok = ~np.any(near & ~exact, axis=1)
+        clear &= ~fg | lab_ok[:, None]
+    label = np.where(fg, np.broadcast_to(lstar[:, None], (P, C)), arg)
+    return label, fg, clear
+
+
+def local_argmins(cost, pw, edge, msg_in, msg_out, delta):
+    """Argmins of one LBP iteration recomputed in FP64 step by step from the fp32 inputs that
+    a kernel actually used: for every pass and sender, h = cost + the sender's chain message
+    (msg_out's slot of the pass, the value the pass wrote one step earlier, or the input at
+    the first sender) + the two cross messages (msg_in's UP/DOWN slots for the horizontal
+    passes, msg_out's LEFT/RIGHT slots for the vertical ones), exactly as the passes of
+    lbp_forward read them (lbp_min_sum_kernel.cu:66-73, in-place updates in pass order).
+
+    Returns (label [N,4,H,W,C], from_global [N,4,H,W,C], clear [N,4,H,W,C], written
+    [N,4,H,W]).  label is the sender label the winner came from (the global argmin for
+    truncation winners).  ``clear`` marks the receiver labels whose decision is fixed under
+    fp32 rounding: every other candidate (another sender label, or the same label through
+    the other source) differs from the winner by more than the sum of their rounding bounds,
+    or is an exact tie of identical fp32 operands (decided by the same rule on every
+    implementation).  A candidate value h_l + f w computed in fp32 from these inputs errs by
+    at most ROUND_KAPPA * 2^-24 * (sum |operands of h_l| + max(log C, max|f w|) + |f w|):
+    three additions and one fused multiply-add, the rounding of the stored chain message, and
+    a chain carried in a shifted frame (msg - min instead of msg - lse: the two differ by the
+    log-sum-exp of non-positive values, which lies in [-max|f w|, log C]; a constant shift of
+    h changes no decision but scales the roundings).
+    """
+    cost = np.asarray(cost, dtype=np.float32)
+    edge = np.asarray(edge, dtype=np.float64)
+    Mi = np.asarray(msg_in, dtype=np.float32)
+    Mo = np.asarray(msg_out, dtype=np.float32)
+    N, H, W, C = cost.shape
+    label = np.zeros((N, 4, H, W, C), dtype=np.int64)
+    fgl = np.zeros((N, 4, H, W, C), dtype=bool)
+    clear = np.ones((N, 4, H, W, C), dtype=bool)
+    written = np.zeros((N, 4, H, W), dtype=bool)
+    for n in range(N):
+        for d in PASS_ORDER:
+            r = RECV[d]
+            horiz = d in (LEFT, RIGHT)
+            cross = (UP, DOWN) if horiz else (LEFT, RIGHT)
+            for s in _steps(d, H, W):
+                ys, xs, yq, xq = _line(d, s, H, W)
+                ops = np.stack([cost[n, ys, xs, :], Mo[n, r, ys, xs, :],
+                                (Mi if horiz else Mo)[n, cross[0], ys, xs, :],
+                                (Mi if horiz else Mo)[n, cross[1], ys, xs, :]], axis=2).astype(np.float64)
+                w = edge[n, d, ys, xs]
+                fw_of = lambda c, l: pw.f(n, d, ys, xs, np.broadcast_to(c, l.shape) if np.ndim(l) > 1 else c, l) \
+                    * w[:, None]
+                lab, fg, ok = _step_decisions(ops, fw_of, pw, n, d, ys, xs, w, delta)
+                label[n, r, yq, xq] = lab
+                fgl[n, r, yq, xq] = fg
+                clear[n, r, yq, xq] = ok
+                written[n, r, yq, xq] = True
+    return label, fgl, clear, written
+
+
 def lbp_forward_iters(cost, pw, edge, messages, delta, iters):
     """``iters`` iterations; returns the final messages and the per-iteration records
     (msg_in, msg_out, argmin, scale, from_global) needed by the backward passes."""
@@ -360,7 +493,7 @@ def lbp_forward_iters(cost, pw, edge, messages, delta, iters):
 # --------------------------------------------------------------------------------------
 # backward, mode E: exact adjoint of lbp_forward
 # --------------------------------------------------------------------------------------
-def lbp_backward_exact(cost, pw, edge, msg_out, argmin, from_global, grad_out, delta=None):
+def lbp_backward_exact(cost, pw, edge, msg_out, argmin, from_global, grad_out, delta=None, absolute=False):
     """Exact reverse-mode derivative of one ``lbp_forward`` iteration (argmins held fixed).
 
     Replays the forward steps in reverse.  For each written message m = msg - lse(msg):
@@ -371,15 +504,27 @@ def lbp_backward_exact(cost, pw, edge, msg_out, argmin, from_global, grad_out, d
 
     Returns (grad_cost [N,H,W,C], grad_pairwise (shape of pw.zeros_grad(N)),
     grad_edge [N,4,H,W], grad_messages_in [N,4,H,W,C]).
+
+    ``absolute=True`` returns the componentwise condition magnitudes instead: the same
+    computation with every term replaced by its absolute value (|grad_out|, gw + p * sum(gw)
+    for the normalisation Jacobian, |f| and |w| for the pairwise and edge terms), i.e. the sum
+    of |term| over all the products that are added into each output.  An fp32 evaluation of
+    the adjoint errs on an output by at most (number of roundings along its path) * 2^-24
+    times this magnitude, so it is the scale against which a relative error of an output
+    that results from cancellation is measured.
     """
     cost = np.asarray(cost, dtype=np.float64)
     edge = np.asarray(edge, dtype=np.float64)
     N, H, W, C = cost.shape
     gM = np.array(grad_out, dtype=np.float64, copy=True)
+    if absolute:
+        gM = np.abs(gM)
     g_cost = np.zeros_like(cost)
     g_edge = np.zeros((N, 4, H, W))
     g_pw = pw.zeros_grad(N)
     cs = np.arange(C)
+    sgn = -1.0 if not absolute else 1.0
+    mag = np.abs if absolute else (lambda v: v)
     for n in range(N):
         for d in reversed(PASS_ORDER):
             r = RECV[d]
@@ -388,14 +533,14 @@ def lbp_backward_exact(cost, pw, edge, msg_out, argmin, from_global, grad_out, d
                 gw = gM[n, r, yq, xq, :].copy()
                 gM[n, r, yq, xq, :] = 0.0
                 p = np.exp(msg_out[n, r, yq, xq, :])
-                gmsg = gw - p * gw.sum(axis=1, keepdims=True)
+                gmsg = gw + sgn * p * gw.sum(axis=1, keepdims=True)
                 a = argmin[n, r, yq, xq, :]
                 fg = from_global[n, r, yq, xq, :]
                 P = len(ys)
                 gh = np.zeros((P, C))
                 np.add.at(gh, (np.repeat(np.arange(P), C), a.ravel()), gmsg.ravel())
-                w = edge[n, d, ys, xs]
-                fv = np.where(fg, pw.trunc(n, d, ys, xs)[:, None], pw.f(n, d, ys, xs, cs[None, :], a))
+                w = mag(edge[n, d, ys, xs])
+                fv = mag(np.where(fg, pw.trunc(n, d, ys, xs)[:, None], pw.f(n, d, ys, xs, cs[None, :], a)))
                 g_edge[n, d, ys, xs] += (fv * gmsg).sum(axis=1)
                 pw.add_grad(g_pw, n, d, ys, xs, cs[None, :], a, w[:, None] * gmsg, trunc_mask=fg)
                 g_cost[n, ys, xs, :] += gh
@@ -405,12 +550,13 @@ def lbp_backward_exact(cost, pw, edge, msg_out, argmin, from_global, grad_out, d
     return g_cost, g_pw, g_edge, gM
 
 
-def lbp_backward_exact_iters(cost, pw, edge, recs, grad_final):
-    """Chain ``lbp_backward_exact`` over the iterations recorded by lbp_forward_iters."""
+def lbp_backward_exact_iters(cost, pw, edge, recs, grad_final, absolute=False):
+    """Chain ``lbp_backward_exact`` over the iterations recorded by lbp_forward_iters
+    (``absolute``: the condition magnitudes of the chained adjoint)."""
     G = np.asarray(grad_final, dtype=np.float64)
     gc = gp = ge = None
     for (Mi, Mo, a, S, fg) in reversed(recs):
-        c_, p_, e_, G = lbp_backward_exact(cost, pw, edge, Mo, a, fg, G)
+        c_, p_, e_, G = lbp_backward_exact(cost, pw, edge, Mo, a, fg, G, absolute=absolute)
         gc = c_ if gc is None else gc + c_
         gp = p_ if gp is None else gp + p_
         ge = e_ if ge is None else ge + e_
@@ -424,7 +570,7 @@ _BWD_SHIFT = {LEFT: (0, 1), RIGHT: (0, -1), DOWN: (-1, 0), UP: (1, 0)}   # :279-
 _BWD_GXY = {UP: DOWN, DOWN: UP, LEFT: RIGHT, RIGHT: LEFT}                  # :302-318
 
 
-def lbp_backward_reference(cost, pw, edge, in_grad, messages, argmin, scale):
+def lbp_backward_reference(cost, pw, edge, in_grad, messages, argmin, scale, absolute=False):
     """The reference's backward_minsum (lbp_min_sum_kernel.cu:215-417, host :513-621).
 
     Adjoint passes in the order of the host loop: slot DOWN (adjoint of the UP pass,
@@ -437,10 +583,16 @@ def lbp_backward_reference(cost, pw, edge, in_grad, messages, argmin, scale):
     The pairwise bin is always idx(c - argmin) and the edge term f(c, argmin) (util.cuh:111-132).
     grad_messages = in_grad (:534-536).  Batch-shared parameter gradients are summed over
     the batch and edge gradients are per batch element (the reference indexes them with 0).
+    ``absolute=True``: the componentwise condition magnitudes (every term by its absolute
+    value; see lbp_backward_exact).
     """
     cost = np.asarray(cost, dtype=np.float64)
     edge = np.asarray(edge, dtype=np.float64)
     in_grad = np.asarray(in_grad, dtype=np.float64)
+    if absolute:
+        in_grad = np.abs(in_grad)
+    sgn = 1.0 if absolute else -1.0
+    mag = np.abs if absolute else (lambda v: v)
     messages = np.asarray(messages, dtype=np.float64)
     scale = np.asarray(scale, dtype=np.float64)
     N, H, W, C = cost.shape
@@ -465,13 +617,13 @@ def lbp_backward_reference(cost, pw, edge, in_grad, messages, argmin, scale):
                     ys = np.full(W, s)
                     xs = np.arange(W)
                 yq, xq = ys + dy, xs + dx
-                w = edge[n, gxy, ys, xs]
+                w = mag(edge[n, gxy, ys, xs])
                 Sq = scale[n, bdir, yq, xq]
                 orig = messages[n, bdir, yq, xq, :] + np.log(Sq)[:, None]
                 p = np.exp(orig) / Sq[:, None]
                 G = in_grad[n, bdir, yq, xq, :]
-                hor = (G - p * G.sum(axis=1, keepdims=True)) + \
-                      (acc[:, 0] - p * acc[:, 0].sum(axis=1, keepdims=True))
+                hor = (G + sgn * p * G.sum(axis=1, keepdims=True)) + \
+                      (acc[:, 0] + sgn * p * acc[:, 0].sum(axis=1, keepdims=True))
                 if cross:
                     up = saved[UP, yq, xq, :] + acc[:, 1]
                     down = saved[DOWN, yq, xq, :] + acc[:, 2]
@@ -485,7 +637,7 @@ def lbp_backward_reference(cost, pw, edge, in_grad, messages, argmin, scale):
                 gu = np.zeros((P, C))
                 np.add.at(gu, (rows, a.ravel()), tot.ravel())
                 g_unary[n, ys, xs, :] += gu
-                fv = pw.f(n, gxy, ys, xs, cs[None, :], a)
+                fv = mag(pw.f(n, gxy, ys, xs, cs[None, :], a))
                 pw.add_grad(g_pw, n, gxy, ys, xs, cs[None, :], a, w[:, None] * tot)
                 g_edge[n, gxy, ys, xs] += (fv * tot).sum(axis=1)
                 tmp = np.zeros((P, 3, C))

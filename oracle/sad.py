@@ -52,11 +52,14 @@ def stereo_sad_forward(f0, f1, min_disp, max_disp, step=1.0):
     return out
 
 
-def stereo_sad_backward(f0, f1, min_disp, max_disp, grad):
-    """(df0, df1) of stereo_sad_forward at step 1 (stereo_sad_kernel.cu:64-108)."""
+def stereo_sad_backward(f0, f1, min_disp, max_disp, grad, absolute=False):
+    """(df0, df1) of stereo_sad_forward at step 1 (stereo_sad_kernel.cu:64-108).
+    ``absolute=True``: the condition magnitudes sum_d |sign| |g| of both outputs."""
     f0 = np.asarray(f0, dtype=np.float64)
     f1 = np.asarray(f1, dtype=np.float64)
     g = np.asarray(grad, dtype=np.float64)
+    if absolute:
+        g = np.abs(g)
     N, C, H, W = f0.shape
     df0 = np.zeros_like(f0)
     df1 = np.zeros_like(f1)
@@ -66,13 +69,15 @@ def stereo_sad_backward(f0, f1, min_disp, max_disp, grad):
         if lo >= hi:
             continue
         s = _sign(f0[:, :, :, lo:hi] - f1[:, :, :, lo - d:hi - d])  # [N,C,H,w]
+        if absolute:
+            s = np.abs(s)
         gi = g[:, None, :, lo:hi, i]
         df0[:, :, :, lo:hi] += s * gi
-        df1[:, :, :, lo - d:hi - d] -= s * gi
+        df1[:, :, :, lo - d:hi - d] += (1.0 if absolute else -1.0) * s * gi
     return df0, df1
 
 
-def flow_sad_forward(f0, f1, sws, offset_u=0, offset_v=0, block_u=0, block_v=0):
+def flow_sad_forward(f0, f1, sws, offset_u=0, offset_v=0, block_u=0, block_v=0, want_volume=False):
     """Fused K x K window SAD with min-projections (flow_mp_sad_kernel.cu:10-124).
 
     K = sws + 1, h = sws // 2.  S(u,v) = sum_c |f0[c,y,x] - f1[c,y+ov+v,x+ou+u]| or FILL.
@@ -81,6 +86,7 @@ def flow_sad_forward(f0, f1, sws, offset_u=0, offset_v=0, block_u=0, block_v=0):
     cv_u[.., u+h] = min_v S, u_star = argmin_v (first) + block_v * sws;
     cv_v[.., v+h] = min_u S, v_star = argmin_u (first) + block_u * sws.
     Argmins are returned as float64 label ids (the reference stores them as fp32).
+    ``want_volume``: also return the window volume S [N,H,W,v_idx,u_idx].
     """
     f0 = np.asarray(f0, dtype=np.float64)
     f1 = np.asarray(f1, dtype=np.float64)
@@ -105,17 +111,20 @@ def flow_sad_forward(f0, f1, sws, offset_u=0, offset_v=0, block_u=0, block_v=0):
     cv_v = S.min(axis=4)
     v_star = S.argmin(axis=4).astype(np.float64) + block_u * sws
     # the reference initialises the running min at 9999999 (only matters if every S >= it)
+    if want_volume:
+        return cv_u, cv_v, u_star, v_star, S
     return cv_u, cv_v, u_star, v_star
 
 
-def flow_sad_backward(f0, f1, sws, grad_u, grad_v, u_star, v_star):
+def flow_sad_backward(f0, f1, sws, grad_u, grad_v, u_star, v_star, absolute=False):
     """(df0, df1) routed through the stored argmins (flow_mp_sad_kernel.cu:126-200).
 
     Labels run over [-h, h] (:152, :175): for odd sws the last label (K - 1) has no gradient.
 
     For each u: v = u_star[u] - h; if (x+u, y+v) is inside, upd = sign(f0 - f1[y+v,x+u]) *
     grad_u[u]; df0 += upd; df1[y+v, x+u] -= upd.  The same for v with v_star.  Offsets
-    and block indices are not applied (as in the reference)."""
+    and block indices are not applied (as in the reference).  ``absolute=True``: the
+    condition magnitudes (|sign| |grad| summed into both outputs)."""
     f0 = np.asarray(f0, dtype=np.float64)
     f1 = np.asarray(f1, dtype=np.float64)
     N, C, H, W = f0.shape
@@ -143,7 +152,9 @@ def flow_sad_backward(f0, f1, sws, grad_u, grad_v, u_star, v_star):
                 xx_c = np.clip(xx, 0, W - 1)
                 diff = f0[n] - f1[n][:, yy_c, xx_c]
                 upd = _sign(diff) * np.asarray(gsrc[n, :, :, k], dtype=np.float64)[None] * ok[None]
+                if absolute:
+                    upd = np.abs(upd)
                 df0[n] += upd
                 for c in range(C):
-                    np.add.at(df1[n, c], (yy_c[ok], xx_c[ok]), -upd[c][ok])
+                    np.add.at(df1[n, c], (yy_c[ok], xx_c[ok]), (upd if absolute else -upd)[c][ok])
     return df0, df1

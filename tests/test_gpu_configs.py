@@ -1,0 +1,398 @@
+"""Parity at the benchmark configurations themselves (GPU against the FP64 oracle).
+
+* config 0 at its full size: stereo forward [1,32,128,256] features, 64 disparities, global
+  P1 = 0.1 / P2 = 0.5, one iteration -- SAD volume, cost = -softmax(-SAD), messages, argmins,
+  beliefs and winner-take-all disparities;
+* config 1 on a 16 x 32 crop: 192 disparities, 3 exact iterations with per-pixel softplus
+  P1 / P2, gradients to the features, P1 and P2 through the drop-in autograd layers (the
+  fast forward step and the paired horizontal backward), against the oracle's analytic
+  gradient and FP64 central differences;
+* config 2 on a crop: semantic LBP, 2 x 19 labels, global 19 x 19 matrix, softplus edge
+  weights, 2 exact iterations, and the pixel-wise variant (matrix scaled per pixel and
+  direction);
+* config 3 on a crop: flow SAD over a 25 x 25 window (sws = 24), 2 iterations on u, then 2 on v
+  starting from u's messages.
+
+Bars as in _parity.py; each check prints its error.
+"""
+import numpy as np
+import pytest
+import torch
+
+import _parity as P
+from oracle import lbp as O
+from oracle import sad as S
+from _lbp_cases import pack_stereo_jump, unpack_stereo_grad
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def _t(a, grad=False):
+    return torch.tensor(np.asarray(a, dtype=np.float32), device=DEV, requires_grad=grad)
+
+
+def _softmax(z):
+    e = np.exp(z - z.max(-1, keepdims=True))
+    return e / e.sum(-1, keepdims=True)
+
+
+def _written(N, H, W):
+    w = np.ones((N, 4, H, W), dtype=bool)
+    w[:, 0, :, 0] = w[:, 1, :, -1] = w[:, 2, 0, :] = w[:, 3, -1, :] = False
+    return w
+
+
+def _labels(am, sc):
+    from bplayers_amd._lbp_op import _gargmin_of
+    ag = am.cpu().numpy()
+    gg = _gargmin_of(sc).cpu().numpy()
+    fg = ag == 255
+    return np.where(fg, gg[..., None], ag).astype(np.int64), fg
+
+
+def test_config0_full_size_forward():
+    """stereo_fwd_128x256_d64 at full size: each stage against the oracle fed the same fp32
+    inputs, then the whole chain from the features."""
+    from bplayers_amd import workloads, _native as nat
+    from bplayers_amd import pytorch_cuda_stereo_sad_op as sadop
+    from bplayers_amd import pytorch_cuda_lbp_op_stereo as lbp
+    from bplayers_amd.ops.lbp_stereo.message_passing_op_cuda import pack_affinities
+    from oracle import cpu as OC
+    wl = workloads.stereo_small()
+    f0, f1 = wl["f0"], wl["f1"]
+    N, Cf, H, W = f0.shape
+    D = wl["max_disp"] + 1
+    print(f"config 0: features {f0.shape}, D={D}", flush=True)
+    tf0, tf1 = _t(f0), _t(f1)
+    sad_g = sadop.forward(tf0, tf1, 0, D - 1).cpu().numpy()
+    cost_t = sadop.forward_softmax(tf0, tf1, 0, D - 1, -1.0, -1.0)        # -softmax(-SAD)
+    jump_t = pack_affinities(_t(wl["affinities"]), torch.zeros((N, 4, H, W), device=DEV))
+    edge_t = torch.ones((N, 4, H, W), device=DEV)
+    M0 = np.zeros((N, 4, H, W, D), dtype=np.float32)
+    M_t, am, sc = lbp.forward_minsum(cost_t, jump_t, edge_t, _t(M0), wl["delta"], out=torch.empty_like(_t(M0)))
+    bel_t = torch.empty((N, H, W, D), device=DEV)
+    lab_t = torch.empty((N, H, W), device=DEV)
+    nat.call("bpl_lbp_beliefs", nat.ptr(cost_t), nat.ptr(M_t), nat.ptr(bel_t), nat.ptr(lab_t), N, H, W, D, 1.0,
+             nat.stream_of(cost_t))
+    torch.cuda.synchronize()
+    cost_g, Mg = cost_t.cpu().numpy(), M_t.cpu().numpy()
+    jump = jump_t.cpu().numpy()
+    edge = np.ones((N, 4, H, W), dtype=np.float32)
+    # SAD and the matching softmax (condition magnitude of a softmax output under relative
+    # errors of its SAD inputs: p (SAD + <p, SAD>))
+    sad64 = S.stereo_sad_forward(f0, f1, 0, D - 1)
+    P.elementwise("SAD volume", sad_g, sad64)
+    p64 = _softmax(-sad64)
+    P.conditioned("cost = -softmax(-SAD)", cost_g, -p64, p64 * (sad64 + (p64 * sad64).sum(-1, keepdims=True)))
+    # LBP iteration on the GPU's cost volume
+    Mo, ao, fo = OC.lbp_forward(cost_g, jump, edge, M0, wl["delta"], dtype=np.float64)
+    wr = _written(N, H, W)
+    P.rows("messages", Mg[wr], Mo[wr])
+    lab_g, fg_g = _labels(am, sc)
+    lab, fg, clear, written = O.local_argmins(cost_g, O.StereoPairwise(jump), edge, M0, Mg, wl["delta"])
+    P.argmins("argmins vs FP64 step replay", lab_g, fg_g, lab, fg, clear, written)
+    E64 = cost_g.astype(np.float64) + Mo.sum(axis=1)
+    b64 = O.softmin(E64)
+    bel_g = bel_t.cpu().numpy()
+    P.elementwise("beliefs [1,64,128,256] (as NHWC)", bel_g, b64)
+    lgb = -np.log(np.maximum(bel_g.astype(np.float64), 1e-300))
+    dev = np.abs((lgb - lgb.min(-1, keepdims=True)) - (E64 - E64.min(-1, keepdims=True))).max()
+    clear_w = P.winner_clear(E64, dev)
+    disp_g = lab_t.cpu().numpy()
+    bad = (disp_g != E64.argmin(-1)) & clear_w
+    print(f"    disparities: {int(bad.sum())} mismatches; {1 - clear_w.mean():.2e} of pixels within "
+          f"{2 * dev:.3g} of a tie", flush=True)
+    assert bad.sum() == 0 and clear_w.mean() > 0.99
+    # the whole chain from the features in FP64 (errors of the SAD stage included)
+    Me, _, _ = OC.lbp_forward(-p64, jump, edge, M0, wl["delta"], dtype=np.float64)
+    be = O.softmin(-p64 + Me.sum(axis=1))
+    e2e = float((np.abs(bel_g - be)[be >= P.FP32_TINY] / be[be >= P.FP32_TINY]).max())
+    print(f"    end to end (features -> beliefs, FP64 throughout): beliefs max rel err {e2e:.3g}", flush=True)
+    assert e2e <= 1e-4
+
+
+def _config1_inputs(rows=16, cols=32):
+    from bplayers_amd import workloads
+    wl = workloads.stereo_full(H=rows, W=cols)   # same generator as the full image, smaller
+    aff = wl["affinities"]                        # [1,4,3,H,W] = (P1, P1, P2), softplus(N(0,1))
+    return wl["f0"], wl["f1"], aff[:, :, 0], aff[:, :, 2], wl["max_disp"] + 1
+
+
+def _chain_adjoint(cost, pw, edge, states, G_final, absolute=False):
+    """Exact adjoint of a chain of LBP iterations replayed on a given forward state
+    (per iteration: output messages, sender labels, truncation flags)."""
+    recs = [(None, Mo, lab, None, fg) for (Mo, lab, fg) in states]
+    return O.lbp_backward_exact_iters(cost, pw, edge, recs, G_final, absolute=absolute)
+
+
+def _softmax_adjoint(prob, g_prob, absolute=False):
+    """d/dz of a loss through prob = softmax(z) over the last axis (and magnitudes)."""
+    if absolute:
+        return prob * (g_prob + (prob * g_prob).sum(-1, keepdims=True))
+    return prob * (g_prob - (prob * g_prob).sum(-1, keepdims=True))
+
+
+def _oracle_config1(f0, f1, p1, p2, D, Wb, iters=3, delta=3, state=None, absolute=False, want_grad=True):
+    """Config-1 pipeline: SAD -> prob = softmax(-SAD) -> iters exact LBP iterations from zero
+    messages (cost = -prob) -> beliefs = softmin(cost + sum M) -> loss = sum(Wb * beliefs);
+    returns (loss, grads (f0, f1, P1, P2)).
+
+    state=None: all in FP64 from the features, with the oracle's own argmins.
+    state=(prob, [(messages, labels, from_trunc) per iteration]): the adjoint replayed on that
+    forward state (a GPU forward's), in FP64."""
+    aff = np.stack([p1, p1, p2], axis=2)
+    jump = pack_stereo_jump(aff)
+    pw = O.StereoPairwise(jump)
+    N, H, W = f0.shape[0], f0.shape[2], f0.shape[3]
+    edge = np.ones((N, 4, H, W))
+    if state is None:
+        prob = _softmax(-S.stereo_sad_forward(f0, f1, 0, D - 1))
+        M, recs = O.lbp_forward_iters(-prob, pw, edge, np.zeros((N, 4, H, W, D)), delta, iters)
+        states = [(Mo, a, fg) for (_, Mo, a, _, fg) in recs]
+    else:
+        prob, states = state
+        M = states[-1][0]
+    cost = -prob
+    E = cost + M.sum(axis=1)
+    loss = float((Wb * O.softmin(E)).sum())
+    if not want_grad:
+        return loss, None
+    gE, kE = P.loss_grad_of_beliefs(E, Wb)
+    g = kE if absolute else gE
+    gc, gj, _, _ = _chain_adjoint(cost, pw, edge, states, np.broadcast_to(g[:, None], M.shape), absolute)
+    g_prob = (gc + g) if absolute else -(gc + g)      # cost = -prob
+    g_sad = _softmax_adjoint(prob, g_prob, absolute)  # z = -SAD: the sign drops out of |.|
+    if not absolute:
+        g_sad = -g_sad
+    d0, d1 = S.stereo_sad_backward(f0, f1, 0, D - 1, g_sad, absolute=absolute)
+    ga = unpack_stereo_grad(gj, 3)
+    if absolute:
+        ga = np.abs(ga)
+    return loss, (d0, d1, ga[:, :, 0] + ga[:, :, 1], ga[:, :, 2])
+
+
+def test_config1_crop_gradients_fd():
+    """stereo_fb_384x1248_d192 on a 16 x 32 crop: gradients to f0, f1, P1, P2 of
+    sum(Wb * beliefs) after 3 exact iterations, through StereoMatchingSadFunction, softmax and
+    the exact MessagePassing layer (192 labels: fast forward step, paired horizontal backward).
+    (i) against the FP64 adjoint replayed on the GPU's forward state (the backward's parity,
+    bar 1e-5 x kappa); (ii) against the FP64 pipeline from the features (forward errors of 3
+    iterations included); (iii) the oracle's analytic gradient against FP64 central differences."""
+    from bplayers_amd.ops.sad.stereo_sad import StereoMatchingSadFunction
+    from bplayers_amd.ops.lbp_stereo.message_passing_op_cuda import MessagePassing, pack_affinities
+    from bplayers_amd import _lbp_op, _native as nat
+    f0, f1, p1, p2, D = _config1_inputs()
+    N, Cf, H, W = f0.shape
+    rng = np.random.default_rng(17)
+    Wb = rng.normal(size=(N, H, W, D))
+    print(f"config 1 crop: features {f0.shape}, D={D}, 3 iterations", flush=True)
+    t0, t1, tp1, tp2 = _t(f0, True), _t(f1, True), _t(p1, True), _t(p2, True)
+    sad = StereoMatchingSadFunction.apply(t0, t1, 0, D - 1)
+    prob = torch.softmax(-sad, dim=3).contiguous()
+    aff = torch.stack([tp1, tp1, tp2], dim=2)
+    mp = MessagePassing(DEV, 3, D, 3, grad_mode="exact")
+    beliefs, _ = mp(prob, None, aff, None, torch.zeros((N, 4, H, W, D), device=DEV))
+    loss = (beliefs * _t(Wb)).sum()
+    loss.backward()
+    torch.cuda.synchronize()
+    grads = [x.grad.cpu().numpy().astype(np.float64) for x in (t0, t1, tp1, tp2)]
+    # the GPU's forward state (deterministic kernels: the layer computed the same)
+    with torch.no_grad():
+        jump_t = pack_affinities(aff.detach(), torch.zeros((N, 4, H, W), device=DEV))
+        _, recs_g = _lbp_op.exact_chain_forward(nat.LBP_STEREO, -prob.detach(), jump_t,
+                                                torch.ones((N, 4, H, W), device=DEV),
+                                                torch.zeros((N, 4, H, W, D), device=DEV), 3, 3)
+    states = [(m.cpu().numpy().astype(np.float64),) + _labels(am, sc) for (m, am, sc) in recs_g]
+    state = (prob.detach().cpu().numpy().astype(np.float64), states)
+    names = ("grad f0", "grad f1", "grad P1", "grad P2")
+    L_s, ora_s = _oracle_config1(f0, f1, p1, p2, D, Wb, state=state)
+    _, kap = _oracle_config1(f0, f1, p1, p2, D, Wb, state=state, absolute=True)
+    P.elementwise("loss (GPU forward state)", np.array(float(loss.detach())), np.array(L_s))
+    for name, g, o, k in zip(names, grads, ora_s, kap):
+        P.conditioned(name + " vs FP64 adjoint on the GPU state", g, o, k)
+    # (ii) the FP64 pipeline from the features
+    L0, ora = _oracle_config1(f0, f1, p1, p2, D, Wb)
+    own = O.lbp_forward_iters(-_softmax(-S.stereo_sad_forward(f0, f1, 0, D - 1)),
+                              O.StereoPairwise(pack_stereo_jump(np.stack([p1, p1, p2], axis=2))),
+                              np.ones((N, 4, H, W)), np.zeros((N, 4, H, W, D)), 3, 3)[1]
+    nd = sum(int((st[1] != r[2]).sum() + (st[2] != r[4]).sum()) for st, r in zip(states, own))
+    print(f"    {nd} of {3 * N * 4 * H * W * D} routed decisions differ from the FP64 pipeline's "
+          f"argmins", flush=True)
+    print(f"    loss: GPU {float(loss.detach()):.9g}, FP64 pipeline {L0:.9g}", flush=True)
+    if nd == 0:
+        for name, g, o, k in zip(names, grads, ora, kap):
+            P.conditioned(name + " vs FP64 pipeline", g, o, k, tol=1e-4)
+    # (iii) central differences of the FP64 pipeline at sampled entries
+    eps = 1e-6
+    worst = 0.0
+    arrs = [f0.astype(np.float64), f1.astype(np.float64), p1.astype(np.float64), p2.astype(np.float64)]
+    for a_i, (name, arr) in enumerate(zip(names, arrs)):
+        flat = arr.reshape(-1)
+        og = ora[a_i].reshape(-1)
+        for i in rng.choice(flat.size, size=6, replace=False):
+            o = flat[i]
+            vals = []
+            for sgn in (1.0, -1.0):
+                flat[i] = o + sgn * eps
+                vals.append(_oracle_config1(*arrs, D, Wb, want_grad=False)[0])
+            flat[i] = o
+            fd = (vals[0] - vals[1]) / (2 * eps)
+            err = abs(fd - og[i]) / max(1.0, abs(fd))
+            worst = max(worst, err)
+            assert err <= 1e-5, (name, i, fd, og[i])
+    print(f"    FD: oracle analytic gradient vs central differences {worst:.3g} (bar 1e-5)", flush=True)
+
+
+def test_config2_crop_semantic():
+    """semantic_fb_2x512x1024_c19 on a 2 x 24 x 40 crop: 19 labels, the global matrix with
+    softplus edge weights and the pixel-wise matrix (scaled per pixel and direction), 2 exact
+    iterations through the op: messages and argmins per iteration, then the chained backward
+    of a loss on the final beliefs against the oracle on the GPU's forward state."""
+    from bplayers_amd import workloads, _lbp_op, _native as nat
+    from bplayers_amd.ops.lbp_semantic_pw.message_passing_op_cuda import pack_pairwise
+    wl = workloads.semantic(N=2, H=24, W=40)
+    prob = wl["prob"]
+    N, H, W, C = prob.shape
+    cost = -prob
+    M = wl["pw_mat"][0, 0]
+    rng = np.random.default_rng(23)
+    for variant in ("global", "pixel"):
+        print(f"config 2 crop, {variant} pairwise: {prob.shape}, 2 iterations", flush=True)
+        if variant == "global":
+            kind, nk = "semantic_global", nat.LBP_SEMANTIC_GLOBAL
+            jump = pack_pairwise(torch.from_numpy(wl["pw_mat"]), False).numpy()
+            edge = wl["edge"]
+        else:
+            kind, nk = "semantic_pixel", nat.LBP_SEMANTIC_PIXEL
+            jump = np.ascontiguousarray(wl["edge"][0][:, :, :, None, None] * M[None, None, None], dtype=np.float32)
+            edge = np.ones((N, 4, H, W), dtype=np.float32)
+        pw = O.make_pairwise(kind, jump)
+        ct, jt, et = _t(cost), _t(jump), _t(edge)
+        Mg_fin, recs = _lbp_op.exact_chain_forward(nk, ct, jt, et, torch.zeros((N, 4, H, W, C), device=DEV), C, 2)
+        Mi = np.zeros((N, 4, H, W, C))
+        Mo64 = np.zeros((N, 4, H, W, C))
+        wr = _written(N, H, W)
+        for it, (m, am, sc) in enumerate(recs):
+            Mg = m.cpu().numpy()
+            Mo64, *_ = O.lbp_forward(cost, pw, edge, Mo64, C)
+            P.rows(f"messages after iteration {it + 1}", Mg[wr], Mo64[wr])
+            lab_g, fg_g = _labels(am, sc)
+            lab, fg, clear, written = O.local_argmins(cost, pw, edge, Mi.astype(np.float32), Mg, C)
+            P.argmins(f"argmins iteration {it + 1}", lab_g, fg_g, lab, fg, clear, written)
+            Mi = Mg
+        # backward from a loss on the beliefs of the final messages
+        Wb = rng.normal(size=(N, H, W, C))
+        Mf = recs[-1][0].cpu().numpy().astype(np.float64)
+        gE, kE = P.loss_grad_of_beliefs(cost + Mf.sum(axis=1), Wb)
+        gM = np.broadcast_to(gE[:, None], Mf.shape).astype(np.float32)
+        gc, gj, ge, gm = _lbp_op.exact_chain_backward(nk, ct, jt, et, _t(np.ascontiguousarray(gM)), recs)
+        torch.cuda.synchronize()
+        recs64 = []
+        Mprev = np.zeros((N, 4, H, W, C))
+        for m, am, sc in recs:
+            lab, fg = _labels(am, sc)
+            Mo_ = m.cpu().numpy().astype(np.float64)
+            recs64.append((Mprev, Mo_, lab, None, fg))
+            Mprev = Mo_
+        ora = O.lbp_backward_exact_iters(cost, pw, edge, recs64, gM)
+        kap = O.lbp_backward_exact_iters(cost, pw, edge, recs64, np.broadcast_to(kE[:, None], Mf.shape),
+                                         absolute=True)
+        got = [gc, gj[:1] if variant == "global" else gj, ge, gm]
+        for name, g, o, k in zip(("grad cost", "grad pairwise", "grad edge", "grad messages"), got, ora, kap):
+            P.conditioned(name, g.cpu().numpy(), o, k)
+
+
+def _oracle_config3(f0, f1, sws, jump, Wu, Wv, star, state=None, absolute=False):
+    """Config-3 pipeline: flow SAD -> u and v volumes -> cost = -softmax(-volume) -> 2 exact
+    iterations on u from zero messages, then 2 on v from u's final messages -> loss =
+    sum(Wu * beliefs_u) + sum(Wv * beliefs_v); returns (beliefs_u, beliefs_v, (df0, df1)).
+    The SAD backward routes through ``star`` = (u_star, v_star).  state=None: FP64 from the
+    features; state=(pu, pv, states_u, states_v): the adjoint on that forward state."""
+    N, H, W = f0.shape[0], f0.shape[2], f0.shape[3]
+    K = sws + 1
+    pw = O.StereoPairwise(jump)
+    ones = np.ones((N, 4, H, W))
+    if state is None:
+        cu, cv, _, _ = S.flow_sad_forward(f0, f1, sws)
+        pu, pv = _softmax(-cu), _softmax(-cv)
+        Mu, ru = O.lbp_forward_iters(-pu, pw, ones, np.zeros((N, 4, H, W, K)), 3, 2)
+        Mv, rv = O.lbp_forward_iters(-pv, pw, ones, Mu, 3, 2)
+        su = [(Mo, a, fg) for (_, Mo, a, _, fg) in ru]
+        sv = [(Mo, a, fg) for (_, Mo, a, _, fg) in rv]
+    else:
+        pu, pv, su, sv = state
+    Eu, Ev = -pu + su[-1][0].sum(axis=1), -pv + sv[-1][0].sum(axis=1)
+    gEu, kEu = P.loss_grad_of_beliefs(Eu, Wu)
+    gEv, kEv = P.loss_grad_of_beliefs(Ev, Wv)
+    gu_, gv_ = (kEu, kEv) if absolute else (gEu, gEv)
+    shape = su[-1][0].shape
+    gcv, _, _, gMv0 = _chain_adjoint(-pv, pw, ones, sv, np.broadcast_to(gv_[:, None], shape), absolute)
+    gcu, _, _, _ = _chain_adjoint(-pu, pw, ones, su, np.broadcast_to(gu_[:, None], shape) + gMv0, absolute)
+    g_vol = []
+    for p_, gc_, g_ in ((pu, gcu, gu_), (pv, gcv, gv_)):
+        g_prob = (gc_ + g_) if absolute else -(gc_ + g_)
+        gz = _softmax_adjoint(p_, g_prob, absolute)
+        g_vol.append(gz if absolute else -gz)
+    grads = S.flow_sad_backward(f0, f1, sws, g_vol[0], g_vol[1], star[0], star[1], absolute=absolute)
+    return O.softmin(Eu), O.softmin(Ev), grads
+
+
+def test_config3_crop_flow():
+    """flow_fb_256x512_r12 on a 24 x 40 crop: flow SAD over the 25 x 25 window, u and v volumes,
+    cost = -softmax(-SAD), 2 exact iterations on u from zero messages, then 2 on v from u's
+    final messages; beliefs of both and the loss gradient to both frames' features, (i) against
+    the FP64 adjoint on the GPU's forward state (bar 1e-5 x kappa), (ii) against the FP64
+    pipeline from the features."""
+    from bplayers_amd import workloads, _lbp_op, _native as nat
+    from bplayers_amd.ops.flow_mp_sad.flow_mp_sad import FlowMpSadFunction
+    from bplayers_amd.ops.lbp_stereo.message_passing_op_cuda import LBPMinSumExactChainFunction, pack_affinities
+    import test_gpu_sad as TS
+    wl = workloads.flow(H=24, W=40)
+    f0, f1, sws = wl["f0"], wl["f1"], wl["sws"]
+    N, Cf, H, W = f0.shape
+    K = sws + 1
+    print(f"config 3 crop: frames {f0.shape}, {K}x{K} window, 2 + 2 iterations", flush=True)
+    t0, t1 = _t(f0, True), _t(f1, True)
+    cu, cv, us, vs = FlowMpSadFunction.apply(t0, t1, sws)
+    exp = S.flow_sad_forward(f0, f1, sws, want_volume=True)
+    TS.check_flow_forward([x.detach().cpu().numpy() for x in (cu, cv, us, vs)], exp, Cf, sws)
+    aff = torch.zeros((N, 4, 3, H, W), device=DEV)
+    aff[:, :, :2] = wl["P1"]
+    aff[:, :, 2] = wl["P2"]
+    off = torch.zeros((N, 4, H, W), device=DEV)
+    edge = torch.ones((N, 4, H, W), device=DEV)
+    probs = [torch.softmax(-x, dim=3).contiguous() for x in (cu, cv)]
+    Mu = LBPMinSumExactChainFunction.apply(-probs[0], edge, torch.zeros((N, 4, H, W, K), device=DEV), 3, aff, off, 2)
+    Mv = LBPMinSumExactChainFunction.apply(-probs[1], edge, Mu, 3, aff, off, 2)
+    rng = np.random.default_rng(29)
+    Wu, Wv = rng.normal(size=(N, H, W, K)), rng.normal(size=(N, H, W, K))
+    bu = torch.softmax(probs[0] - Mu.sum(dim=1), dim=3)
+    bv = torch.softmax(probs[1] - Mv.sum(dim=1), dim=3)
+    ((bu * _t(Wu)).sum() + (bv * _t(Wv)).sum()).backward()
+    torch.cuda.synchronize()
+    grads = (t0.grad.cpu().numpy(), t1.grad.cpu().numpy())
+    jump = pack_stereo_jump(aff.cpu().numpy())
+    star = (us.cpu().numpy(), vs.cpu().numpy())
+    with torch.no_grad():
+        jt = pack_affinities(aff, off)
+        pu_t, pv_t = probs[0].detach(), probs[1].detach()
+        mu_f, ru = _lbp_op.exact_chain_forward(nat.LBP_STEREO, -pu_t, jt, edge,
+                                               torch.zeros((N, 4, H, W, K), device=DEV), 3, 2)
+        _, rv = _lbp_op.exact_chain_forward(nat.LBP_STEREO, -pv_t, jt, edge, mu_f, 3, 2)
+    st = lambda recs: [(m.cpu().numpy().astype(np.float64),) + _labels(am, sc) for (m, am, sc) in recs]
+    state = (pu_t.cpu().numpy().astype(np.float64), pv_t.cpu().numpy().astype(np.float64), st(ru), st(rv))
+    bu_s, bv_s, ora_s = _oracle_config3(f0, f1, sws, jump, Wu, Wv, star, state=state)
+    _, _, kap = _oracle_config3(f0, f1, sws, jump, Wu, Wv, star, state=state, absolute=True)
+    P.elementwise("beliefs u (GPU forward state)", bu.detach().cpu().numpy(), bu_s)
+    P.elementwise("beliefs v (GPU forward state)", bv.detach().cpu().numpy(), bv_s)
+    for name, g, o, k in zip(("grad f0", "grad f1"), grads, ora_s, kap):
+        P.conditioned(name + " vs FP64 adjoint on the GPU state", g, o, k)
+    bu_o, bv_o, ora = _oracle_config3(f0, f1, sws, jump, Wu, Wv, star)
+    for name, g, o in (("beliefs u", bu, bu_o), ("beliefs v", bv, bv_o)):
+        g = g.detach().cpu().numpy()
+        m = o >= P.FP32_TINY
+        print(f"    {name} vs FP64 pipeline: max rel err {float((np.abs(g - o)[m] / o[m]).max()):.3g}", flush=True)
+    for name, g, o, k in zip(("grad f0", "grad f1"), grads, ora, kap):
+        print(f"    {name} vs FP64 pipeline: max |err|/kappa {float((np.abs(g - o) / np.maximum(k, 1e-300)).max()):.3g}",
+              flush=True)

@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 import torch
 
+import _parity as P
 from oracle import lbp as O
 from _lbp_cases import pack_semantic_global
 
@@ -73,12 +74,17 @@ def test_semantic_net_matches_oracle(kind):
     b = O.beliefs(cost, M, float(net._crf[0].message_passing.softmin.T.detach()))
     bg = beliefs.double().cpu().numpy()
     assert bg.shape == (1, H, W, C)
-    np.testing.assert_allclose(bg, b, rtol=0, atol=2e-5)
-    # winner take all, where the oracle's winner leads the runner-up by a margin
-    srt = np.sort(b, axis=-1)
-    clear = (srt[..., -1] - srt[..., -2]) > 1e-4
+    P.elementwise("beliefs", bg, b)
+    # winner take all: identical where the oracle's energy minimum leads the runner-up by more
+    # than twice the GPU's largest energy deviation (log-beliefs up to a constant)
+    Tb = float(net._crf[0].message_passing.softmin.T.detach())
+    E = (cost + M.sum(axis=1)) / Tb
+    lgb = -np.log(np.maximum(bg, 1e-300))
+    dev = np.abs((lgb - lgb.min(-1, keepdims=True)) - (E - E.min(-1, keepdims=True))).max()
+    clear = P.winner_clear(E, dev)
+    print(f"    WTA: energy deviation {dev:.3g}, {1 - clear.mean():.2e} of pixels within it of a tie", flush=True)
     lab = labels.cpu().numpy().reshape(1, H, W)
-    assert clear.mean() > 0.9
+    assert clear.mean() > 0.99
     assert np.array_equal(lab[clear], b.argmax(-1)[clear])
     sp = np.sort(prob[0], axis=-1)
     uclear = (sp[..., -1] - sp[..., -2]) > 1e-5

@@ -3,14 +3,11 @@ import numpy as np
 import pytest
 import torch
 
+import _parity as P
 from oracle import lbp as O
 from _lbp_cases import random_stereo_problem, semantic_matrix, pack_semantic_global, LEFT, RIGHT, UP, DOWN
 
 pytestmark = pytest.mark.gpu
-
-# Tolerances (fp32 kernel vs FP64 oracle on identical fp32 inputs):
-MSG_TOL = 2e-5     # |gpu - oracle| / max(1, |oracle|) on messages
-TIE_TOL = 1e-4     # argmins are compared where the oracle's winner leads every other candidate by this
 
 
 def _mods():
@@ -20,12 +17,14 @@ def _mods():
     return {"stereo": st, "semantic_global": sg, "semantic_pixel": sp}
 
 
-def make_case(kind, N, H, W, C, delta, seed, K_in=3, offset=False, ordered=False):
+def make_case(kind, N, H, W, C, delta, seed, K_in=3, offset=False, ordered=False, shared_jump=False):
     rng = np.random.default_rng(seed)
     if kind == "stereo":
         cost, aff, jump, edge = random_stereo_problem(rng, N, H, W, C, K_in=K_in, ordered=ordered)
         if offset:
             jump[:, :, 0] = rng.choice([0.0, 0.5, -1.0, 1.3, 2.5], size=jump[:, :, 0].shape)
+        if shared_jump:  # one pairwise tensor [1,4,K,H,W] for the whole batch
+            jump = jump[:1]
     elif kind == "semantic_global":
         cost = rng.normal(size=(N, H, W, C))
         jump = pack_semantic_global(np.stack([semantic_matrix(rng, C), semantic_matrix(rng, C)])[None])
@@ -56,30 +55,45 @@ def run_gpu(kind, cost, jump, edge, M0, delta, inplace=True):
             _gargmin_of(sc).cpu().numpy())
 
 
+def written_mask(N, H, W):
+    """Receiver slots a forward pass writes (the first pixel of each pass receives nothing)."""
+    w = np.ones((N, 4, H, W), dtype=bool)
+    w[:, LEFT, :, 0] = w[:, RIGHT, :, -1] = False
+    w[:, UP, 0, :] = w[:, DOWN, -1, :] = False
+    return w
+
+
+def check_argmins(kind, cost, jump, edge, M0, delta, Mg, ag, gg, oracle=None):
+    """GPU argmins against (i) the FP64 replay of every step from the GPU's own stored inputs,
+    under the fp32 rounding-bound tie mask, and (ii) the FP64 oracle's own chain, at the
+    decisions whose oracle margin exceeds what the GPU's input deviation can move."""
+    pw = O.make_pairwise(kind, jump)
+    gpu_fg = ag == 255
+    gpu_lab = np.where(gpu_fg, np.broadcast_to(gg[..., None], ag.shape), ag).astype(np.int64)
+    lab, fg, clear, written = O.local_argmins(cost, pw, edge, M0, Mg.astype(np.float32), delta)
+    P.argmins("argmins vs FP64 step replay", gpu_lab, gpu_fg, lab, fg, clear, written)
+    if oracle is not None:
+        Mo, ao, fgo, lso, margin = oracle
+        # a decision of the oracle's chain can only flip if its margin is within the GPU's
+        # deviation of the step's inputs (chain + two cross messages) plus rounding
+        dev = np.abs(Mg - Mo).max()
+        thr = 3.0 * dev + 2.0 * O.ROUND_KAPPA * O.U32 * (np.abs(cost).max() + 3 * np.abs(Mo).max() + 10.0)
+        ora_lab = np.where(fgo, np.broadcast_to(lso[..., None], ao.shape), ao)
+        P.argmins("argmins vs FP64 oracle chain", gpu_lab, gpu_fg, ora_lab, fgo, margin > thr, written)
+
+
 def check_forward(kind, cost, jump, edge, M0, delta, inplace=True):
     Mg, ag, sg, gg = run_gpu(kind, cost, jump, edge, M0, delta, inplace)
     pw = O.make_pairwise(kind, jump)
     Mo, ao, so, fgo, lso, margin = O.lbp_forward(cost, pw, edge, M0, delta, want_margin=True)
-    err = np.abs(Mg - Mo) / np.maximum(1.0, np.abs(Mo))
-    assert err.max() <= MSG_TOL, f"message error {err.max():.3g}"
     N, _, H, W, C = Mo.shape
-    written = np.ones((N, 4, H, W), dtype=bool)
-    written[:, LEFT, :, 0] = written[:, RIGHT, :, -1] = False
-    written[:, UP, 0, :] = written[:, DOWN, -1, :] = False
-    sc_err = np.abs(sg - so)[written] / so[written]
-    assert sc_err.max() <= 1e-4, f"scale error {sc_err.max():.3g}"
-    gpu_fg = ag == 255
-    gpu_lab = np.where(gpu_fg, np.broadcast_to(gg[..., None], ag.shape), ag).astype(np.int64)
-    ora_lab = np.where(fgo, np.broadcast_to(lso[..., None], ao.shape), ao)
-    clear = (margin > TIE_TOL) & written[..., None]
-    assert np.array_equal(gpu_lab[clear], ora_lab[clear]), \
-        f"argmin mismatches: {(gpu_lab[clear] != ora_lab[clear]).sum()} of {clear.sum()}"
-    assert np.array_equal(gpu_fg[clear], fgo[clear]), "truncation-term flags differ"
-    clear_frac = clear[np.broadcast_to(written[..., None], clear.shape)].mean()
-    assert clear_frac > 0.9, f"too many near-ties ({1 - clear_frac:.3f})"
+    written = written_mask(N, H, W)
+    err = P.rows("messages", Mg[written], Mo[written])
+    P.elementwise("message scale", sg[written], so[written])
+    check_argmins(kind, cost, jump, edge, M0, delta, Mg, ag, gg, (Mo, ao, fgo, lso, margin))
     # unwritten boundary slots are copies of the input
     np.testing.assert_array_equal(Mg[~written], M0.astype(np.float64)[~written])
-    return err.max(), 1 - clear[written].mean()
+    return err
 
 
 CASES = [
@@ -128,8 +142,7 @@ def test_lbp_forward_matches_oracle(case, inplace, impl, monkeypatch):
         monkeypatch.setenv("BPL_LBP_GENERIC", "1")
     print(f"case {case} inplace={inplace} impl={impl}", flush=True)
     cost, jump, edge, M0 = make_case(kind, N, H, W, C, delta, seed=hash((N, H, W, C)) % 1000, **extra)
-    e, tie = check_forward(kind, cost, jump, edge, M0, delta, inplace)
-    print(f"  max rel err {e:.3g}, near-tie fraction {tie:.4f}")
+    check_forward(kind, cost, jump, edge, M0, delta, inplace)
 
 
 @pytest.mark.parametrize("C", [64, 192])
@@ -145,9 +158,7 @@ def test_lbp_forward_two_iterations_long_rows(C):
     M = M0.astype(np.float64)
     for _ in range(2):
         M, *_ = O.lbp_forward(cost, O.StereoPairwise(jump), edge, M, 3)
-    err = np.abs(m.cpu().numpy() - M) / np.maximum(1.0, np.abs(M))
-    print(f"max rel err after 2 iterations, W=300: {err.max():.3g}")
-    assert err.max() <= MSG_TOL
+    P.rows("messages after 2 iterations, W=300", m.cpu().numpy(), M)
 
 
 @pytest.mark.parametrize("C", [40, 64, 100, 192])

@@ -3,6 +3,7 @@ import numpy as np
 import pytest
 import torch
 
+import _parity as P
 from oracle import lbp as O
 from oracle import sad as S
 from _lbp_cases import pack_stereo_jump
@@ -58,12 +59,15 @@ def test_served_disparity_matches_oracle_pipeline():
     aff[:, :, 2] = 0.5
     M, *_ = O.lbp_forward(cost, O.StereoPairwise(pack_stereo_jump(aff)), np.ones((1, 4, H, W)),
                           np.zeros((1, 4, H, W, D)), 3)
-    b = O.beliefs(cost, M)
-    lab = b.argmax(-1)[0]
-    srt = np.sort(b, -1)[0]
-    clear = (srt[..., -1] - srt[..., -2]) > 1e-5
+    E = (cost + M.sum(axis=1))[0]
+    lab = E.argmin(-1)
+    # a one-iteration solve's energies carry the rounding of the SAD sums through the softmax
+    # and of four messages: bounded by 1e-5 of their magnitudes (the per-vector message bar)
+    dev = 1e-5 * (np.abs(cost).max() + 4 * np.abs(M).max())
+    clear = P.winner_clear(E, dev)
     got = r.result.cpu().numpy()
-    assert clear.mean() > 0.9
+    print(f"    disparities: {1 - clear.mean():.2e} of pixels within {2 * dev:.3g} of a tie", flush=True)
+    assert clear.mean() > 0.99
     np.testing.assert_array_equal(got[clear], lab[clear])
 
 

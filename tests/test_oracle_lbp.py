@@ -334,3 +334,86 @@ def test_reference_backward_equals_exact_at_one_iteration(variant):
         ep[:, :, -1] += ep[:, :, 1]; ep[:, :, 1] = 0
     np.testing.assert_allclose(rp, ep, atol=1e-10)
     np.testing.assert_array_equal(rm, G)
+
+
+# --------------------------------------------------------------------------------------
+# the parity bars' own tools: the FP64 step replay with its fp32 tie mask, and the
+# condition magnitudes of the adjoint
+# --------------------------------------------------------------------------------------
+def test_local_argmins_reproduce_the_oracle_and_an_independent_fp32_implementation():
+    """On the oracle's own messages the step replay gives the oracle's argmins; on the C++
+    fp32 restatement's messages it gives that implementation's argmins wherever it calls the
+    decision clear (an fp32 evaluation with its own summation order)."""
+    from oracle import cpu
+    from _lbp_cases import random_stereo_problem
+    rng = np.random.default_rng(41)
+    for C, delta in ((24, 3), (9, 9)):
+        cost, aff, jump, edge = random_stereo_problem(rng, 1, 4, 30, C)
+        f32 = lambda a: np.asarray(a, dtype=np.float32)
+        cost, jump, edge = f32(cost), f32(jump), f32(edge)
+        M0 = f32(rng.normal(size=(1, 4, 4, 30, C)) * 0.3)
+        pw = O.StereoPairwise(jump)
+        Mo, ao, so, fo, lso = O.lbp_forward(cost, pw, edge, M0, delta)
+        lab, fg, clear, wr = O.local_argmins(cost, pw, edge, M0, f32(Mo), delta)
+        m = clear & wr[..., None]
+        assert m.sum() > 0.99 * wr.sum() * C
+        assert np.array_equal(lab[m], np.where(fo, lso[..., None], ao)[m]) and np.array_equal(fg[m], fo[m])
+        M32, a32, f32_ = cpu.lbp_forward(cost, jump, edge, M0, delta, dtype=np.float32)
+        lab, fg, clear, wr = O.local_argmins(cost, pw, edge, M0, M32, delta)
+        m = clear & wr[..., None]
+        assert np.array_equal(lab[m], a32[m]) and np.array_equal(fg[m], f32_[m].astype(bool))
+
+
+def test_local_argmins_flag_rounding_ties_but_not_identical_operands():
+    """Two window candidates one ulp apart are a rounding tie (not clear); candidates from
+    bitwise-identical operands are an exact tie every implementation resolves by the
+    reference's '<=' order (clear, last offset wins)."""
+    C, W = 8, 2
+    jump = np.zeros((1, 4, 6, 1, W), dtype=np.float32)
+    jump[:, :, 1] = jump[:, :, 5] = 5.0          # truncation far away
+    jump[:, :, 2] = jump[:, :, 4] = 0.25         # P1
+    edge = np.ones((1, 4, 1, W), dtype=np.float32)
+    M0 = np.zeros((1, 4, 1, W, C), dtype=np.float32)
+    pw = O.StereoPairwise(jump)
+    for kind in ("ulp", "identical"):
+        cost = np.full((1, 1, W, C), 1.0, dtype=np.float32)
+        cost[0, 0, 0, 3] = 0.0                    # sender x=0: minimum at label 3
+        if kind == "ulp":
+            cost[0, 0, 0, 5] = np.nextafter(np.float32(0.0), np.float32(1.0)) * 0 + np.float32(1e-7)
+            cost[0, 0, 0, 3] = 0.0
+        else:
+            cost[0, 0, 0, 5] = 0.0                # labels 3 and 5 identical
+        Mo, ao, *_ = O.lbp_forward(cost, pw, edge, M0, 3)
+        lab, fg, clear, wr = O.local_argmins(cost, pw, edge, M0, Mo.astype(np.float32), 3)
+        # receiver label 4 sees sender labels 3 and 5 through the same P1 entry
+        r = clear[0, 0, 0, 1, 4]                   # slot LEFT at x=1 (RIGHT pass)
+        if kind == "ulp":
+            assert not r
+        else:
+            assert r and lab[0, 0, 0, 1, 4] == 5 and ao[0, 0, 0, 1, 4] == 5
+
+
+def test_condition_magnitudes_bound_the_gradients_and_an_fp32_evaluation():
+    """kappa (absolute=True) dominates |gradient| elementwise, is linear in |G|, and the C++
+    fp32 restatement's exact backward errs by well under 1e-5 kappa on a realistic problem."""
+    from oracle import cpu
+    from _lbp_cases import random_stereo_problem
+    rng = np.random.default_rng(43)
+    cost, aff, jump, edge = random_stereo_problem(rng, 1, 5, 40, 32)
+    f32 = lambda a: np.asarray(a, dtype=np.float32)
+    cost, jump, edge = f32(cost), f32(jump), f32(edge)
+    M0 = np.zeros((1, 4, 5, 40, 32), dtype=np.float32)
+    pw = O.StereoPairwise(jump)
+    Mo, ao, so, fo, lso = O.lbp_forward(cost, pw, edge, M0, 3)
+    G = rng.normal(size=Mo.shape)
+    g = O.lbp_backward_exact(cost, pw, edge, Mo, ao, fo, G)
+    k = O.lbp_backward_exact(cost, pw, edge, Mo, ao, fo, G, absolute=True)
+    k2 = O.lbp_backward_exact(cost, pw, edge, Mo, ao, fo, -2 * G, absolute=True)
+    for a, b, c in zip(g, k, k2):
+        assert np.all(np.abs(a) <= b * (1 + 1e-12) + 1e-300)
+        np.testing.assert_allclose(c, 2 * b, rtol=1e-12)
+    lab = np.where(fo, lso[..., None], ao).astype(np.int32)
+    r32 = cpu.lbp_backward_exact(jump, edge, f32(G), f32(Mo), lab, fo.astype(np.uint8), 3, dtype=np.float32)
+    for a32, a64, kk in zip(r32, g, k):
+        nz = kk > 0
+        assert (np.abs(a32.astype(np.float64) - a64)[nz] / kk[nz]).max() < 1e-5

@@ -96,3 +96,24 @@ def test_sub_expectation_closed_form():
     vals = b[0, 0, 0, win]
     np.testing.assert_allclose(float(r), float((vals * torch.tensor(win, dtype=torch.float32)).sum() / vals.sum()),
                                rtol=1e-6)
+
+
+def test_pairwise_tables_known_answers():
+    """Closed-form P1/P2 tables and the jump-vector lookup on hand-computed values."""
+    from bplayers_amd.ops._pairwise import jump_index, jump_value, smoothness_prob, truncated_linear
+    from bplayers_amd.ops.lbp_stereo.bp_op_cuda import BP
+    E = truncated_linear(4, 0.1, 0.5)
+    np.testing.assert_allclose(E, [[0, .1, .5, .5], [.1, 0, .1, .5], [.5, .1, 0, .1], [.5, .5, .1, 0]])
+    P = smoothness_prob(3, 0.6, 0.15)
+    np.testing.assert_allclose(P, [[.6, .15, .25], [.15, .6, .15], [.25, .15, .6]])
+    # a zero weight is replaced by 1 - L1 - L2 like every other zero entry
+    np.testing.assert_allclose(smoothness_prob(2, 0.0, 0.3), [[.7, .3], [.3, .7]])
+    assert list(jump_index(np.arange(-4, 5), 6)) == [1, 1, 1, 2, 3, 4, 5, 5, 5]
+    J = np.array([0.5, 2.0, 1.0, 0.0, 1.5, 3.0])  # offset 0.5
+    assert jump_value(1, 1, J) == 0.75            # between J[3] = 0 and J[4] = 1.5
+    assert jump_value(3, 1, J) == 3.0             # 2.5: J[5] and the clamped J[5]
+    assert jump_value(0, 2, J) == 0.5 * 2.0 + 0.5 * 1.0  # -1.5: J[1] and J[2]
+    J0 = J.copy()
+    J0[0] = 0.0
+    assert jump_value(2, 4, J0) == J0[1] and jump_value(4, 3, J0) == J0[4]  # integer differences
+    assert BP.f_func is not None and BP.construct_pw_energy_weights(3, 0.1, 0.2)[0, 2] == 0.2
