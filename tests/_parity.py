@@ -8,11 +8,17 @@ arithmetic itself is accurate to, which differs by quantity:
   the rounding of the largest entries into every entry, so a message entry near 0 has no
   meaningful elementwise relative error;
 * beliefs (probabilities): elementwise |b_gpu - b_64| / b_64 (``elementwise``);
-* gradients: |g_gpu - g_64| / kappa elementwise, kappa being the componentwise condition
-  magnitude of the adjoint (the same sums with every term replaced by its absolute value,
-  oracle ``absolute=True``).  An output that results from cancellation between large terms
-  can only be as accurate as those terms, so a plain elementwise relative error of it is not
-  bounded in fp32; where no term reaches an output (kappa = 0) it must be exactly 0.
+* gradients of the LBP op (and of anything downstream of it): normwise over each output
+  tensor, max |g_gpu - g_64| / max |g_64| (``normwise``).  The adjoint of a message step
+  subtracts the probability-weighted mean (g = t - p sum(t)), so gradient entries are
+  differences of much larger sums and carry the rounding of those sums; the absolute-value
+  version of that step doubles sum|t| at every pixel of a chain, so componentwise condition
+  magnitudes grow like 2^(chain length) and bound nothing.  The elementwise relative error of
+  the entries above 10 % of the maximum is printed beside it;
+* gradients of single-step sums (the SAD backward, softmax Jacobians): |g_gpu - g_64| / kappa
+  elementwise, kappa being the componentwise condition magnitude (the same sums with every
+  term by its absolute value, oracle ``absolute=True``); where no term reaches an output
+  (kappa = 0) it must be exactly 0 (``conditioned``).
 
 Argmins are compared with the FP64 replay of each step from the kernel's own inputs
 (oracle.lbp.local_argmins): identical wherever no competing candidate lies within the fp32
@@ -23,6 +29,15 @@ Every check prints its maximum error, so a run's log holds the numbers.
 import numpy as np
 
 TOL = 1e-5
+
+
+def beliefs_bar(iters):
+    """Elementwise bar for beliefs after ``iters`` chained LBP iterations: 1e-5 for one, x3 per
+    further iteration.  Every message of an iteration sums three messages of the previous one,
+    so their rounding compounds: measured on the stereo workload (4 x 1248, 192 labels), the
+    GPU's beliefs err by 3.7e-6 / 1.2e-5 / 3.5e-5 after 1 / 2 / 3 iterations and the plain fp32
+    restatement of the reference by 5.2e-6 / 2.3e-5 / 5.6e-5."""
+    return TOL * 3.0 ** (max(1, iters) - 1)
 # Decisions within fp32 rounding of a tie are genuine ties at fp32 resolution (for instance
 # candidates h + P1 w whose h differ below the ulp of P1 w round to the same value); their
 # share is measured and printed per test (up to 0.6 % on the first iteration of a softmax cost
@@ -63,6 +78,26 @@ def elementwise(name, g, o, tol=TOL):
     return err
 
 
+def normwise(name, g, o, tol=TOL):
+    """max |g - o| / max |o| over the whole tensor; an all-zero oracle tensor must be matched
+    exactly."""
+    g = np.asarray(g, dtype=np.float64)
+    o = np.asarray(o, dtype=np.float64)
+    s = float(np.abs(o).max()) if o.size else 0.0
+    d = np.abs(g - o)
+    if s == 0:
+        assert np.all(g == 0), f"{name}: non-zero values where the oracle is all 0"
+        print(f"    {name}: all zero, matched", flush=True)
+        return 0.0
+    err = float(d.max()) / s
+    big = np.abs(o) >= 0.1 * s
+    rel = float((d[big] / np.abs(o[big])).max())
+    print(f"    {name}: normwise err {err:.3g} (bar {tol:.0e}); elementwise relative where "
+          f"|g| >= 0.1 max: {rel:.3g}", flush=True)
+    assert err <= tol, f"{name}: normwise relative error {err:.3g} > {tol:g}"
+    return err
+
+
 def conditioned(name, g, o, kappa, tol=TOL):
     """|g - o| <= tol * kappa elementwise (kappa: condition magnitudes of the oracle)."""
     g = np.asarray(g, dtype=np.float64)
@@ -96,10 +131,20 @@ def argmins(name, gpu_label, gpu_from_trunc, label, from_trunc, clear, written, 
 
 def winner_clear(energy, bound):
     """Pixels whose energy minimum (beliefs' maximum) over the label axis leads the runner-up
-    by more than 2 * bound, ``bound`` being the largest deviation of a GPU energy from the
-    oracle's (measured on the same run and printed)."""
+    by more than 2 * bound, ``bound`` being the deviation of the GPU's energies from the
+    oracle's (a scalar, or per pixel)."""
     e = np.sort(np.asarray(energy, dtype=np.float64), axis=-1)
-    return (e[..., 1] - e[..., 0]) > 2.0 * bound
+    return (e[..., 1] - e[..., 0]) > 2.0 * np.asarray(bound)
+
+
+def energy_deviation(beliefs_gpu, energy):
+    """Per pixel, the largest deviation of the GPU's energies, read from its beliefs
+    (-log b = E / T + const), from the oracle's energies (both shifted to their minimum)."""
+    lg = -np.log(np.maximum(np.asarray(beliefs_gpu, dtype=np.float64), 1e-300))
+    E = np.asarray(energy, dtype=np.float64)
+    ok = np.asarray(beliefs_gpu) >= FP32_TINY
+    d = np.abs((lg - lg.min(-1, keepdims=True)) - (E - E.min(-1, keepdims=True)))
+    return np.where(ok, d, 0.0).max(axis=-1)
 
 
 def loss_grad_of_beliefs(energy, Wb, T=1.0):

@@ -5,7 +5,7 @@ stereo workload's own inputs (features warped by a smooth disparity field, cost 
 For every size and iteration 1..3 the test prints the GPU's error next to that of the C++
 fp32 restatement of the reference (oracle/c, the reference's arithmetic in plain fp32), both
 against the FP64 restatement, for the messages (per vector), the beliefs (elementwise) and the
-exact backward's cost gradient (relative to its condition magnitudes).  It asserts that the
+exact backward's cost gradient (normwise).  It asserts that the
 GPU is as accurate as fp32 arithmetic gets here: within 1e-5, or within 3x the plain fp32
 restatement's error where that one exceeds 1e-5 itself."""
 import numpy as np
@@ -93,12 +93,9 @@ def test_error_growth_with_chain_length(D, W):
             gc, _, _, g = OC.lbp_backward_exact(jump, edge, g, Mo, lab, fg, 3, dtype=dt)
             acc = acc + gc.astype(np.float64)
         tot[dt] = acc
-    recs64 = [(None, Mo.astype(np.float64), lab.astype(np.int64), None, fg.astype(bool)) for Mo, lab, fg in states]
-    kap = O.lbp_backward_exact_iters(cost, O.StereoPairwise(jump), edge, recs64,
-                                     np.broadcast_to(kE[:, None], Mf.shape), absolute=True)[0]
-    nz = kap > 0
-    ek = lambda a: float((np.abs(a - tot[np.float64])[nz] / kap[nz]).max())
+    ref = tot[np.float64]
+    ek = lambda a: float(np.abs(a - ref).max() / np.abs(ref).max())
     e_g, e_r = ek(gc_g), ek(tot[np.float32])
-    print(f"    backward, {iters} iterations: grad cost |err|/kappa GPU {e_g:.3g} / fp32 restatement {e_r:.3g}",
+    print(f"    backward, {iters} iterations: grad cost normwise GPU {e_g:.3g} / fp32 restatement {e_r:.3g}",
           flush=True)
     assert e_g <= _bar(e_r)

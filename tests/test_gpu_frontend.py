@@ -79,10 +79,10 @@ def test_semantic_net_matches_oracle(kind):
     # than twice the GPU's largest energy deviation (log-beliefs up to a constant)
     Tb = float(net._crf[0].message_passing.softmin.T.detach())
     E = (cost + M.sum(axis=1)) / Tb
-    lgb = -np.log(np.maximum(bg, 1e-300))
-    dev = np.abs((lgb - lgb.min(-1, keepdims=True)) - (E - E.min(-1, keepdims=True))).max()
+    dev = P.energy_deviation(bg, E)
     clear = P.winner_clear(E, dev)
-    print(f"    WTA: energy deviation {dev:.3g}, {1 - clear.mean():.2e} of pixels within it of a tie", flush=True)
+    print(f"    WTA: energy deviation up to {dev.max():.3g}, {1 - clear.mean():.2e} of pixels within twice their "
+          f"own deviation of a tie", flush=True)
     lab = labels.cpu().numpy().reshape(1, H, W)
     assert clear.mean() > 0.99
     assert np.array_equal(lab[clear], b.argmax(-1)[clear])

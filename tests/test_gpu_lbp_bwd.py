@@ -46,12 +46,11 @@ def run_case(kind, N, H, W, C, delta, mode, seed, pass_delta=False, **extra):
     # the routing replayed here is the GPU forward's; check it against the FP64 step replay
     check_argmins(kind, cost, jump, edge, M0, delta, Mg, ag, gg)
     ora = oracle_backward(kind, mode, cost, jump, edge, G, Mg, lab, fg, sg)
-    kap = oracle_backward(kind, mode, cost, jump, edge, G, Mg, lab, fg, sg, absolute=True)
     if kind == "semantic_global":
         assert gj.shape[0] == N and np.all(gj[1:] == 0), "batch entries past 0 of the matrix gradient"
         gj = gj[:1]
-    for name, g, o, k in zip(("grad cost", "grad pairwise", "grad edge", "grad messages"), (gc, gj, ge, gm), ora, kap):
-        P.conditioned(name, g, o, k)
+    for name, g, o in zip(("grad cost", "grad pairwise", "grad edge", "grad messages"), (gc, gj, ge, gm), ora):
+        P.normwise(name, g, o)
     return True
 
 

@@ -229,4 +229,8 @@ def test_stereo_sad_softmax_fused_equals_two_kernels(N, C, H, W, mn, mx):
         s64 = S.stereo_sad_forward(f0.cpu().numpy(), f1.cpu().numpy(), mn, mx) * scale
         p64 = np.exp(s64 - s64.max(axis=3, keepdims=True))
         p64 = sign * p64 / p64.sum(axis=3, keepdims=True)
-        P.elementwise(f"fused SAD softmax, scale {scale:.3g}", got.cpu().numpy(), p64)
+        # condition magnitudes of a softmax output under relative errors of its SAD inputs
+        q = np.abs(p64)
+        z = np.abs(s64)
+        P.conditioned(f"fused SAD softmax, scale {scale:.3g}", got.cpu().numpy(), p64,
+                      q * (z + (q * z).sum(axis=3, keepdims=True)))
