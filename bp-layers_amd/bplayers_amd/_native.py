@@ -21,6 +21,7 @@ GRAD_EXACT = 1
 # chained exact iterations: in_grad's / grad_msg's LEFT and RIGHT slots at the row ends only
 GRAD_IN_H_ROW_ENDS = 0x10
 GRAD_OUT_H_ROW_ENDS = 0x20
+LBP_BWD_NOALLOC_MAX_BYTES = 128 << 20  # BPL_LBP_BWD_NOALLOC_MAX_BYTES
 
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
@@ -41,6 +42,8 @@ _SIGS = {
     "bpl_lbp_beliefs": ([_P, _P, _P, _P, _I64, _I64, _I64, _I64, _F, _P], _I),
     "bpl_lbp_bwd": ([_I, _I, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I64,
                      _I64, _I64, _I64, _I64, _I, _P], _I),
+    "bpl_lbp_bwd_noalloc": ([_I, _I, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                             _I64, _I64, _I64, _I64, _I, _P], _I),
     "bpl_timing_enable": ([_I], _I),
     "bpl_timing_reset": ([], _I),
     "bpl_timing_read": ([_I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I64)], _I),

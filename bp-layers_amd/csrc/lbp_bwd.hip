@@ -13,6 +13,7 @@
 // Routing a receiver-label gradient g(c) to the sender label argmin(c) is a gather over
 // the few label offsets that occur (|argmin(c) - c| <= D) plus one segment sum for the
 // labels whose value came from the truncation term (all routed to the global argmin).
+#include <mutex>
 #include <type_traits>
 
 #include "bpl_common.h"
@@ -1608,12 +1609,14 @@ extern "C" int64_t bpl_lbp_bwd_workspace_size(int kind, int64_t N, int64_t H, in
   return total;
 }
 
-extern "C" int bpl_lbp_bwd(int kind, int mode, const float *jump, int64_t jump_k, int64_t jump_n,
-                           const float *edge, const float *in_grad, const float *messages,
-                           const uint8_t *argmin, const uint8_t *gargmin,
-                           float *grad_cost, float *grad_jump, float *grad_edge, float *grad_msg,
-                           void *workspace, int64_t workspace_bytes,
-                           int64_t N, int64_t H, int64_t W, int64_t C, int delta, void *stream) {
+namespace {
+
+int lbp_bwd_impl(int kind, int mode, const float *jump, int64_t jump_k, int64_t jump_n,
+                 const float *edge, const float *in_grad, const float *messages,
+                 const uint8_t *argmin, const uint8_t *gargmin,
+                 float *grad_cost, float *grad_jump, float *grad_edge, float *grad_msg,
+                 void *workspace, int64_t workspace_bytes,
+                 int64_t N, int64_t H, int64_t W, int64_t C, int delta, void *stream) {
   BPL_REQUIRE(kind >= 0 && kind <= 2, BPL_E_ARG, "bpl_lbp_bwd: unknown kind %d", kind);
   const int hflags = mode & (BPL_GRAD_IN_H_ROW_ENDS | BPL_GRAD_OUT_H_ROW_ENDS);
   mode &= ~(BPL_GRAD_IN_H_ROW_ENDS | BPL_GRAD_OUT_H_ROW_ENDS);
@@ -1735,6 +1738,67 @@ extern "C" int bpl_lbp_bwd(int kind, int mode, const float *jump, int64_t jump_k
     if (rc) return rc;
   }
   return BPL_OK;
+}
+
+// Scratch memory of bpl_lbp_bwd_noalloc: one array per GPU, in the code object, so the entry
+// point needs no allocation.  In host code `scratch` names a host-side stand-in: its device
+// address comes from hipGetSymbolAddress, made once per device with that device current.
+constexpr int64_t kScratchBytes = BPL_LBP_BWD_NOALLOC_MAX_BYTES;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_of_current_device(void **out) {
+  int dev = -1;
+  int rc = check_hip(hipGetDevice(&dev), "bpl_lbp_bwd_noalloc: hipGetDevice");
+  if (rc) return rc;
+  BPL_REQUIRE(dev >= 0 && dev < kMaxDevices, BPL_E_LIMIT, "bpl_lbp_bwd_noalloc: device %d beyond %d", dev,
+              kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    rc = check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "bpl_lbp_bwd_noalloc: hipGetSymbolAddress");
+    if (rc) return rc;
+    BPL_REQUIRE(p != nullptr, BPL_E_HIP, "bpl_lbp_bwd_noalloc: no device address for the scratch array");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return BPL_OK;
+}
+
+}  // namespace
+
+extern "C" int bpl_lbp_bwd(int kind, int mode, const float *jump, int64_t jump_k, int64_t jump_n,
+                           const float *edge, const float *in_grad, const float *messages,
+                           const uint8_t *argmin, const uint8_t *gargmin,
+                           float *grad_cost, float *grad_jump, float *grad_edge, float *grad_msg,
+                           void *workspace, int64_t workspace_bytes,
+                           int64_t N, int64_t H, int64_t W, int64_t C, int delta, void *stream) {
+  return lbp_bwd_impl(kind, mode, jump, jump_k, jump_n, edge, in_grad, messages, argmin, gargmin, grad_cost,
+                      grad_jump, grad_edge, grad_msg, workspace, workspace_bytes, N, H, W, C, delta, stream);
+}
+
+extern "C" int bpl_lbp_bwd_noalloc(int kind, int mode, const float *jump, int64_t jump_k, int64_t jump_n,
+                                   const float *edge, const float *in_grad, const float *messages,
+                                   const uint8_t *argmin, const uint8_t *gargmin,
+                                   float *grad_cost, float *grad_jump, float *grad_edge, float *grad_msg,
+                                   int64_t N, int64_t H, int64_t W, int64_t C, int delta, void *stream) {
+  BPL_REQUIRE(kind >= 0 && kind <= 2, BPL_E_ARG, "bpl_lbp_bwd_noalloc: unknown kind %d", kind);
+  BPL_REQUIRE(N >= 1 && H >= 1 && W >= 1 && C >= 1 && C <= 255, BPL_E_ARG,
+              "bpl_lbp_bwd_noalloc: sizes must be positive and C <= 255");
+  int64_t planes, pv, ph, total;
+  ws_layout(kind, N, H, W, C, &planes, &pv, &ph, &total);
+  BPL_REQUIRE(total <= kScratchBytes, BPL_E_LIMIT,
+              "bpl_lbp_bwd_noalloc: these sizes need %lld bytes of workspace, above the %lld bytes of the "
+              "built-in scratch array (use bpl_lbp_bwd with a caller-owned workspace)",
+              (long long)total, (long long)kScratchBytes);
+  void *ws = nullptr;
+  int rc = scratch_of_current_device(&ws);
+  if (rc) return rc;
+  return lbp_bwd_impl(kind, mode, jump, jump_k, jump_n, edge, in_grad, messages, argmin, gargmin, grad_cost,
+                      grad_jump, grad_edge, grad_msg, ws, kScratchBytes, N, H, W, C, delta, stream);
 }
 
 #else  // BPL_BWD_PAIR_LPL: the paired horizontal launches for one labels-per-lane count

@@ -190,6 +190,24 @@ int bpl_lbp_bwd(int kind, int mode, const float *jump, int64_t jump_k, int64_t j
                 int64_t N, int64_t H, int64_t W, int64_t C, int delta, void *stream);
 
 /*
+ * bpl_lbp_bwd without a workspace argument: no counterpart in the reference.  For callers that
+ * cannot allocate device memory (code under HIP-graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The workspace is a scratch array of BPL_LBP_BWD_NOALLOC_MAX_BYTES
+ * built into the library, one per GPU (held from the library's first use on that GPU until the
+ * process ends); sizes whose bpl_lbp_bwd_workspace_size exceeds it are rejected with
+ * BPL_E_LIMIT.  The entry point calls no allocating routine.  It uses the array of the current
+ * device, so the stream must belong to that device.  Every call on one GPU shares that array:
+ * calls that overlap on one GPU (on different streams, or from different threads) race and
+ * must be ordered by the caller.  Results are those of bpl_lbp_bwd, bit for bit.
+ */
+#define BPL_LBP_BWD_NOALLOC_MAX_BYTES (128LL << 20)
+int bpl_lbp_bwd_noalloc(int kind, int mode, const float *jump, int64_t jump_k, int64_t jump_n,
+                        const float *edge, const float *in_grad, const float *messages,
+                        const uint8_t *argmin, const uint8_t *gargmin,
+                        float *grad_cost, float *grad_jump, float *grad_edge, float *grad_msg,
+                        int64_t N, int64_t H, int64_t W, int64_t C, int delta, void *stream);
+
+/*
  * y = sign * softmax(scale * x) over the contiguous last dimension of a [rows, C] tensor
  * (C <= 256).  With scale = -1/T and sign = -1 this is the reference's matching softmax
  * softmax(-SAD/T) (matching.py:22-25, corenet.py:97-104) followed by the LBP input
