@@ -53,6 +53,12 @@ inline bool generic_only() {
   const char *e = getenv("BPL_LBP_GENERIC");
   return e && e[0] == '1';
 }
+// BPL_LBP_NO_P1P2=1 turns the P1/P2 sweep of the stereo forward off (its scanlines take the
+// 7-candidate fast step), for comparisons of the two.
+inline bool p1p2_off() {
+  const char *e = getenv("BPL_LBP_NO_P1P2");
+  return e && e[0] == '1';
+}
 
 // ------------------------------------------------------------------ lane exchange
 // DPP controls (GFX9 encoding): quad_perm [1,0,3,2] / [2,3,0,1], row_half_mirror, row_mirror.
@@ -137,6 +143,20 @@ __device__ __forceinline__ int wave_max_i(int v) {
   v = max(lo, hi);
   swap_halves(v, lo, hi);
   return max(lo, hi);
+}
+// Float minimum over the wave.  Built with -fno-honor-nans (the LBP translation units) the
+// DPP moves fold into v_min_f32_dpp; otherwise the compiler adds canonicalisations (same
+// values).
+__device__ __forceinline__ float wave_min_f(float v) {
+  v = fminf(v, dpp_src_f<kDppXor1>(v));
+  v = fminf(v, dpp_src_f<kDppXor2>(v));
+  v = fminf(v, dpp_src_f<kDppHalfMirror>(v));
+  v = fminf(v, dpp_src_f<kDppMirror>(v));
+  float lo, hi;
+  swap_rows(v, lo, hi);
+  v = fminf(lo, hi);
+  swap_halves(v, lo, hi);
+  return fminf(lo, hi);
 }
 __device__ __forceinline__ float wave_sum_f(float v) {
   v = v + dpp_src_f<kDppXor1>(v);

@@ -589,9 +589,9 @@ __global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(B
       const int bin_here = 2 * row16 + (lane & 1);  // bin held by this lane after the reduce-scatter
       const bool bin_lane = (lane & 15) < 2 && bin_here < KK;
       constexpr int kDrop = 0x7FFFFFFF;              // out-of-range buffer offset: store dropped
-      float tsprev = 0.f, xprev[LPL], smprev[LPL];
+      float gvprev[LPL];  // the previous step's h-gradient, before routing
 #pragma unroll
-      for (int k = 0; k < LPL; ++k) { xprev[k] = 0.f; smprev[k] = 0.f; }
+      for (int k = 0; k < LPL; ++k) gvprev[k] = 0.f;
       // PAIR: the other pass's h-gradient at the sender, read by the second visitor
       const float *Xt_p = a.X[dir == RIGHT ? LEFT : RIGHT] + img;
       const bool isR = dir == RIGHT;
@@ -638,11 +638,15 @@ __global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(B
         asm("" : "+v"(pv) : "v"(acc[0]));
         int gl = pl[i];
         asm("" : "+v"(gl) : "v"(acc[0]));
-        // normalisation sum by the scalar recursion
+        // normalisation sum sum_c (g + acc)(c): the routing moves every entry of the previous
+        // step's h-gradient to one sender label, so sum acc = sum gvprev, and the reduction
+        // runs beside the routing instead of after it.  Summing the actual vectors each step
+        // projects out all the mass present, rounding included, as a direct sum does; a
+        // recursion on the sums alone would let the rounding accumulate along the scanline.
         float r = 0.f;
 #pragma unroll
-        for (int k = 0; k < LPL; ++k) r += valid[k] ? (pg[i][k] + xprev[k]) - tsprev * smprev[k] : 0.f;
-        const float ts = tsprev + wave_sum_f(r);
+        for (int k = 0; k < LPL; ++k) r += valid[k] ? pg[i][k] + gvprev[k] : 0.f;
+        const float ts = wave_sum_f(r);
         float sm[LPL], X[LPL], gv[LPL], fs = 0.f;
         int ar[LPL];
 #pragma unroll
@@ -764,9 +768,8 @@ __global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(B
             st_row_b(gr_rs, so, zero);
           }
         }
-        tsprev = ts;
 #pragma unroll
-        for (int k = 0; k < LPL; ++k) { xprev[k] = X[k]; smprev[k] = sm[k]; }
+        for (int k = 0; k < LPL; ++k) gvprev[k] = gv[k];
       };
       auto load_xo = [&](int s, float *d) { ld_row_b(xt_rs, row_off(q0 + (int64_t)s * dq), d); };
       // a group of PF steps; each ring slot is refilled after its use (with the other pass's

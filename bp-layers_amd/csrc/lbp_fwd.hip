@@ -24,6 +24,7 @@ struct FwdArgs {
   uint8_t *argmin, *gargmin;
   float *scale;
   int N, H, W, C, K, Nj, delta, bits, mat_lds;
+  int buf_ok;  // every [H,W,C] plane and the K jump planes of a direction below 2^31 bytes
 };
 
 __device__ __forceinline__ int64_t px(int n, int y, int x, int H, int W) {
@@ -604,6 +605,253 @@ __device__ __forceinline__ float shl1_inf(float v) {  // lane i <- lane i+1, lan
 //     largest tie key (an integer max-reduction), the reference's tree order;
 //   * the log-sum-exp needs no shift: every u lies in [(Fmin - JK) w, 0], so unless
 //     (JK - Fmin) w > 80 no exp(u) underflows; otherwise (rare) it is shifted by max u.
+// Sweep of the P1/P2 form: jump length 6 with J[1] == J[5] == JK (the truncation value) and
+// J[3] == 0 at every pixel of the scanline, zero window offsets, delta 3 -- the packing of
+// per-pixel (P1-, P1+, P2) affinities (message_passing_op_cuda.py:35-56) -- and no
+// log-sum-exp term that can underflow ((JK - J[j]) w <= 80), all checked by the caller.
+//   * The window candidates at offsets -3, -2, 2, 3 carry the truncation value JK, so they are
+//     never below the global term gv = JK w + min h, and equal it exactly when their sender
+//     label attains min h: they only decide the argmin at such ties.  The window minimum is
+//     therefore min(J[4] w + h[c-1], h[c], J[2] w + h[c+1]) and the message min(that, gv).
+//   * The argmin is the last candidate attaining the message in the reference's order (gv,
+//     then offsets -3 .. 3).  An offset +-2 / +-3 attains it exactly when the message is gv
+//     and its sender attains min h; its sender label is then the global argmin itself.  (This
+//     is the FP64 semantics of those ties; an fp32 fused multiply-add of a slightly larger h
+//     could also round onto gv.)  With several labels attaining min h (rare) every one is
+//     tested.
+//   * The chain carried to the next pixel is u = msg - gv (messages are shift-invariant in h);
+//     every u lies in [-80, 0], so the log-sum-exp needs no shift.
+//   * Row streams are buffer-addressed: one wave-uniform row offset shared by the cost and
+//     both cross-message streams (their planes are per-lane offsets of one resource per
+//     tensor); ring loads past the last sender run off the scanline and are never used.
+template <int LPL, bool VERT, bool FULL, bool OUTS, int PF>
+__device__ __forceinline__ void fwd_sweep_p1p2(const FwdArgs &a, int n, int dir, int rslot, int nj, int64_t q0,
+                                               int64_t dq, int nsteps, int lane, int c0, const bool *valid) {
+  constexpr int ND = (3 + LPL - 1) / LPL;
+  const int H = a.H, W = a.W, C = a.C, K = a.K;
+  const int64_t HW = (int64_t)H * W;
+  const uint32_t pb = (uint32_t)(HW * C * 4);        // bytes of one [H,W,C] plane
+  const int64_t img4 = (int64_t)n * 4 * HW * C;      // the image's 4-slot message block
+  const __amdgpu_buffer_rsrc_t c_rs = make_rsrc(a.cost + (int64_t)n * HW * C, pb);
+  const __amdgpu_buffer_rsrc_t mi_rs = make_rsrc(a.msg_in + img4, 4u * pb);
+  const __amdgpu_buffer_rsrc_t mo_rs = make_rsrc(a.msg_out + img4, 4u * pb);
+  const __amdgpu_buffer_rsrc_t x_rs = VERT ? mo_rs : mi_rs;     // the cross messages
+  const int64_t rs_base = ((int64_t)n * 4 + rslot) * HW;
+  const __amdgpu_buffer_rsrc_t j_rs = make_rsrc(a.jump + ((int64_t)nj * 4 + dir) * K * HW, (uint32_t)(K * HW * 4));
+  const __amdgpu_buffer_rsrc_t e_rs = make_rsrc(a.edge + ((int64_t)n * 4 + dir) * HW, (uint32_t)(HW * 4));
+  const void *a_base = OUTS ? (const void *)(a.argmin + rs_base * C) : (const void *)a.cost;
+  const void *s_base = OUTS ? (const void *)(a.scale + rs_base) : (const void *)a.cost;
+  const void *g_base = OUTS ? (const void *)(a.gargmin + rs_base) : (const void *)a.cost;
+  const __amdgpu_buffer_rsrc_t a_rs = make_rsrc(a_base, OUTS ? pb / 4u : 0u);
+  const __amdgpu_buffer_rsrc_t s_rs = make_rsrc(s_base, OUTS ? (uint32_t)(HW * 4) : 0u);
+  const __amdgpu_buffer_rsrc_t g_rs = make_rsrc(g_base, OUTS ? (uint32_t)HW : 0u);
+  // per-lane byte offsets: the lane's labels within a row, plus the plane of the stream
+  uint32_t vl[LPL];
+#pragma unroll
+  for (int k = 0; k < LPL; ++k) vl[k] = (uint32_t)(FULL ? c0 + k : min(c0 + k, C - 1)) * 4u;
+  const uint32_t p_x1 = (uint32_t)(VERT ? RIGHT : UP) * pb, p_x2 = (uint32_t)(VERT ? LEFT : DOWN) * pb;
+  const uint32_t p_o = (uint32_t)rslot * pb;
+  const uint32_t jv = lane < K ? (uint32_t)lane * (uint32_t)(HW * 4) : kBufDrop;  // lane j: jump entry j
+  const uint32_t ev = lane == K ? 0u : kBufDrop;                                   // lane K: edge weight
+  const uint32_t one = lane == 0 ? 0u : kBufDrop;   // per-receiver values: one lane stores
+  const uint32_t rowb = (uint32_t)C * 4u;
+  const uint32_t drow = (uint32_t)((int)dq * (int)rowb);    // row offset step (two's complement)
+  const uint32_t dpix = (uint32_t)((int)dq * 4);
+  auto ld_row = [&](__amdgpu_buffer_rsrc_t r, uint32_t plane, uint32_t soff, float *d) {
+    if (FULL) {
+      buf_ld_row<LPL>(r, vl[0] + plane, soff, d);
+    } else {
+#pragma unroll
+      for (int k = 0; k < LPL; ++k) d[k] = buf_ld(r, vl[k] + plane, soff);
+    }
+  };
+  auto st_row = [&](uint32_t soff, const float *v) {
+    if (FULL) {
+      buf_st_row<LPL>(mo_rs, vl[0] + p_o, soff, v);
+    } else {
+#pragma unroll
+      for (int k = 0; k < LPL; ++k) buf_st(mo_rs, valid[k] ? vl[k] + p_o : kBufDrop, soff, v[k]);
+    }
+  };
+  auto st_args = [&](uint32_t soff_b, const uint32_t *b) {   // soff_b: byte offset of the row
+    if constexpr (FULL && LPL == 3) {
+      __builtin_amdgcn_raw_buffer_store_b16((uint16_t)(b[0] | (b[1] << 8)), a_rs, (uint32_t)c0, soff_b, 0);
+      __builtin_amdgcn_raw_buffer_store_b8((uint8_t)b[2], a_rs, (uint32_t)c0 + 2u, soff_b, 0);
+    } else if constexpr (FULL && LPL == 2) {
+      __builtin_amdgcn_raw_buffer_store_b16((uint16_t)(b[0] | (b[1] << 8)), a_rs, (uint32_t)c0, soff_b, 0);
+    } else {
+#pragma unroll
+      for (int k = 0; k < LPL; ++k)
+        __builtin_amdgcn_raw_buffer_store_b8((uint8_t)b[k], a_rs, valid[k] ? (uint32_t)(c0 + k) : kBufDrop, soff_b, 0);
+    }
+  };
+
+  // the first sender's chain is its input message, copied through
+  const uint32_t so0 = (uint32_t)q0 * rowb;
+  float chain[LPL];
+  ld_row(mi_rs, p_o, so0, chain);
+  st_row(so0, chain);
+  if constexpr (OUTS) {
+    uint32_t z[LPL];
+#pragma unroll
+    for (int k = 0; k < LPL; ++k) z[k] = 0u;
+    st_args(so0 / 4u, z);
+    buf_st(s_rs, one, (uint32_t)q0 * 4u, 0.f);
+    __builtin_amdgcn_raw_buffer_store_b8((uint8_t)0, g_rs, one, (uint32_t)q0, 0);
+  }
+  // labels of this lane and their neighbours (the argmin candidates c-1, c, c+1)
+  uint32_t lab_c[LPL + 2];
+#pragma unroll
+  for (int k = 0; k < LPL + 2; ++k) lab_c[k] = (uint32_t)(c0 + k - 1);
+
+  float pc[PF][LPL], q1[PF][LPL], q2[PF][LPL], pj[PF], pe[PF];
+  uint32_t ring_so = so0, ring_sp = (uint32_t)q0 * 4u;   // offsets of the next step to load
+  auto load_ring = [&](float *dc, float *d1, float *d2, float &dj, float &de) {
+    ld_row(c_rs, 0u, ring_so, dc);
+    ld_row(x_rs, p_x1, ring_so, d1);
+    ld_row(x_rs, p_x2, ring_so, d2);
+    dj = buf_ld(j_rs, jv, ring_sp);
+    de = buf_ld(e_rs, ev, ring_sp);
+    ring_so += drow;
+    ring_sp += dpix;
+  };
+#pragma unroll
+  for (int i = 0; i < PF; ++i) {
+    load_ring(pc[i], q1[i], q2[i], pj[i], pe[i]);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  float Lprev = 0.f;
+  uint32_t rso = so0 + drow, rsp = (uint32_t)q0 * 4u + dpix;   // receiver of step 0
+  auto step = [&](int i) {
+    float h[LPL];
+#pragma unroll
+    for (int k = 0; k < LPL; ++k) {
+      float c = pc[i][k];
+      if (VERT) asm("" : "+v"(c) : "v"(chain[k]));  // the vertical order adds the chain last
+      const float v = VERT ? ((c + q1[i][k]) + q2[i][k]) + chain[k] : ((c + chain[k]) + q1[i][k]) + q2[i][k];
+      h[k] = valid[k] ? v : __builtin_inff();
+    }
+    float pv = pj[i] + pe[i];
+    asm("" : "+v"(pv) : "v"(chain[0]));  // no instruction: orders pv after the chain
+    const float w = readlane_f(pv, 6);
+    const float JK = readlane_f(pv, 5);
+    const float Jm = readlane_f(pv, 4);   // offset -1: f = J[K/2 + 1]
+    const float Jp = readlane_f(pv, 2);   // offset +1: f = J[K/2 - 1]
+    float hmin = h[0];
+#pragma unroll
+    for (int k = 1; k < LPL; ++k) hmin = fminf(hmin, h[k]);
+    const float Hs = wave_min_f(hmin);
+    const float hl = shr1_inf(h[LPL - 1]);   // h[c0 - 1]
+    const float hr = shl1_inf(h[0]);         // h[c0 + LPL]
+    const float gv = fmaf(JK, w, Hs);
+    float cm[LPL], cp[LPL], msg[LPL];
+#pragma unroll
+    for (int k = 0; k < LPL; ++k) {
+      cm[k] = fmaf(Jm, w, k == 0 ? hl : h[k > 0 ? k - 1 : 0]);
+      cp[k] = fmaf(Jp, w, k == LPL - 1 ? hr : h[k < LPL - 1 ? k + 1 : 0]);
+      msg[k] = fminf(fminf(fminf(cm[k], h[k]), cp[k]), gv);
+      chain[k] = msg[k] - gv;
+    }
+    // ---- tail: argmins, normalisation, stores
+    if constexpr (OUTS) {
+      uint64_t mk[LPL];
+      int cnt = 0;
+#pragma unroll
+      for (int k = 0; k < LPL; ++k) {
+        mk[k] = __ballot(valid[k] && h[k] == Hs);
+        cnt += __popcll(mk[k]);
+      }
+      uint32_t lab[LPL];
+      int lstar = 0;
+      if (cnt == 1) {
+#pragma unroll
+        for (int k = 0; k < LPL; ++k)
+          if (mk[k]) lstar = (__ffsll((long long)mk[k]) - 1) * LPL + k;
+        // receivers 2-3 labels above the global argmin tie through offsets -3 / -2 (before
+        // the c-1, c, c+1 candidates), those 2-3 below through +2 / +3 (after them)
+        const uint32_t above = (uint32_t)(lstar + 2), below = (uint32_t)(lstar - 3);
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) {
+          const bool atg = !(msg[k] < gv);   // the message is gv
+          const uint32_t c = lab_c[k + 1];
+          uint32_t l = atg && c - above <= 1u ? (uint32_t)lstar : (uint32_t)kFromTrunc;
+          uint32_t hit = atg && c - below <= 1u ? 1u : 0u;
+          // materialised before the selects (no instruction): otherwise the compiler guards
+          // them with exec masks around the later overriding selects
+          asm("" : "+v"(l), "+v"(hit));
+          l = cm[k] == msg[k] ? lab_c[k] : l;
+          l = h[k] == msg[k] ? c : l;
+          l = cp[k] == msg[k] ? lab_c[k + 2] : l;
+          lab[k] = hit ? (uint32_t)lstar : l;
+        }
+      } else {
+        // several labels attain min h (rare): the tree order picks the stored one, and every
+        // receiver whose offset-+-2/+-3 sender attains min h ties with gv
+        int tk = -1;
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) tk = max(tk, (valid[k] && h[k] == Hs) ? tie_key(c0 + k, a.bits) : -1);
+        lstar = tie_key(wave_max_i(tk), a.bits);
+        float Lh[ND][LPL], Rh[ND][LPL];
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) { Lh[0][k] = shr1_inf(h[k]); Rh[0][k] = shl1_inf(h[k]); }
+#pragma unroll
+        for (int dd = 1; dd < ND; ++dd)
+#pragma unroll
+          for (int k = 0; k < LPL; ++k) { Lh[dd][k] = shr1_inf(Lh[dd - 1][k]); Rh[dd][k] = shl1_inf(Rh[dd - 1][k]); }
+        auto hat = [&](int k, int kk) -> float {   // h at label c0 + k + kk, |kk| <= 3
+          const int e = k + kk;
+          if (e >= 0 && e < LPL) return h[e >= 0 && e < LPL ? e : 0];
+          if (e < 0) {
+            const int j = -e, dd = (j + LPL - 1) / LPL, slot = dd * LPL - j;
+            return Lh[dd - 1][slot];
+          }
+          const int j = e - LPL + 1, dd = (j + LPL - 1) / LPL, slot = j - 1 - (dd - 1) * LPL;
+          return Rh[dd - 1][slot];
+        };
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) {
+          const bool atg = !(msg[k] < gv);
+          const uint32_t c = lab_c[k + 1];
+          uint32_t l = (uint32_t)kFromTrunc;
+          l = atg && hat(k, -3) == Hs ? c - 3u : l;
+          l = atg && hat(k, -2) == Hs ? c - 2u : l;
+          l = cm[k] == msg[k] ? lab_c[k] : l;
+          l = h[k] == msg[k] ? c : l;
+          l = cp[k] == msg[k] ? lab_c[k + 2] : l;
+          l = atg && hat(k, 2) == Hs ? c + 2u : l;
+          lab[k] = atg && hat(k, 3) == Hs ? c + 3u : l;
+        }
+      }
+      st_args(rso / 4u, lab);
+      __builtin_amdgcn_raw_buffer_store_b8((uint8_t)lstar, g_rs, one, rsp / 4u, 0);
+    }
+    float se = 0.f;
+#pragma unroll
+    for (int k = 0; k < LPL; ++k) se += valid[k] ? fast_exp(chain[k]) : 0.f;
+    se = wave_sum_f(se);
+    const float Lc = fast_log(se);
+    float outv[LPL];
+#pragma unroll
+    for (int k = 0; k < LPL; ++k) outv[k] = chain[k] - Lc;
+    st_row(rso, outv);
+    if constexpr (OUTS) buf_st(s_rs, one, rsp, fast_exp(gv + Lc - Lprev));
+    Lprev = Lc;
+    rso += drow;
+    rsp += dpix;
+  };
+  int s0 = 0;
+  for (; s0 + PF <= nsteps; s0 += PF) {
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      step(i);
+      load_ring(pc[i], q1[i], q2[i], pj[i], pe[i]);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < PF; ++i)
+    if (s0 + i < nsteps) step(i);
+}
+
 template <int LPL, bool VERT, bool FULL, int FASTK, bool OUTS, int PF>
 __global__ __launch_bounds__(64) void lbp_fwd_stereo_w64(FwdArgs a) {
   constexpr int ND = (3 + LPL - 1) / LPL;  // neighbour lanes reached by a +-3 window
@@ -657,6 +905,23 @@ __global__ __launch_bounds__(64) void lbp_fwd_stereo_w64(FwdArgs a) {
     constexpr int KK = FASTK;  // jump length of the fast step (K = a.K at launch)
     bool off = false;  // any non-zero window offset on this scanline (jump entry 0)
     const float *o_p = a.jump + ((int64_t)nj * 4 + dir) * K * HW;
+    if constexpr (FASTK == 6) {
+      // P1/P2 form: besides zero offsets, J[1] == J[5] (the truncation value) and J[3] == 0
+      // at every pixel, and no log-sum-exp term can underflow, (JK - J[j]) w <= 80
+      const float *e_p = a.edge + ((int64_t)n * 4 + dir) * HW;
+      bool form = true, big = false;
+      for (int p = lane; p < len; p += kWave) {
+        const int64_t q = VERT ? (int64_t)p * W + ln : (int64_t)ln * W + p;
+        const float j0 = o_p[q], j1 = o_p[HW + q], j2 = o_p[2 * HW + q], j3 = o_p[3 * HW + q];
+        const float j4 = o_p[4 * HW + q], j5 = o_p[5 * HW + q], w = e_p[q];
+        form = form && j0 == 0.f && j1 == j5 && j3 == 0.f;
+        big = big || (j5 - j2) * w > 80.f || (j5 - j4) * w > 80.f || j5 * w > 80.f;
+      }
+      if (a.buf_ok && __ballot(!form) == 0 && __ballot(big) == 0) {
+        fwd_sweep_p1p2<LPL, VERT, FULL, OUTS, PF>(a, n, dir, rslot, nj, q0, dq, nsteps, lane, c0, valid);
+        return;
+      }
+    }
     for (int p = lane; p < len; p += kWave) off = off || o_p[VERT ? (int64_t)p * W + ln : (int64_t)ln * W + p] != 0.f;
     if (__ballot(off) == 0) {
       float chain[LPL];
@@ -1010,13 +1275,20 @@ __global__ __launch_bounds__(64) void lbp_fwd_stereo_w64(FwdArgs a) {
   }
 }
 
+// prefetch depth (steps) of the stereo kernels' register ring, per phase
+#ifndef BPL_FWD_PF_H
+#define BPL_FWD_PF_H 6
+#endif
+#ifndef BPL_FWD_PF_V
+#define BPL_FWD_PF_V 4
+#endif
 template <int LPL, bool FULL, int FASTK, bool OUTS>
 void launch_stereo_w64_k(const FwdArgs &a, bool vert, hipStream_t st) {
   const dim3 grid((unsigned)(a.N * 2 * (vert ? a.W : a.H)));
   if (vert)
-    hipLaunchKernelGGL((lbp_fwd_stereo_w64<LPL, true, FULL, FASTK, OUTS, 4>), grid, dim3(kWave), 0, st, a);
+    hipLaunchKernelGGL((lbp_fwd_stereo_w64<LPL, true, FULL, FASTK, OUTS, BPL_FWD_PF_V>), grid, dim3(kWave), 0, st, a);
   else
-    hipLaunchKernelGGL((lbp_fwd_stereo_w64<LPL, false, FULL, FASTK, OUTS, 4>), grid, dim3(kWave), 0, st, a);
+    hipLaunchKernelGGL((lbp_fwd_stereo_w64<LPL, false, FULL, FASTK, OUTS, BPL_FWD_PF_H>), grid, dim3(kWave), 0, st, a);
 }
 template <int LPL, bool FULL>
 void launch_stereo_w64(const FwdArgs &a, bool vert, hipStream_t st) {
@@ -1110,6 +1382,8 @@ extern "C" int bpl_lbp_fwd(int kind, const float *cost, const float *jump, int64
   a.K = kind == BPL_LBP_STEREO ? (int)jump_k : 0;
   a.Nj = kind == BPL_LBP_STEREO ? (int)jump_n : 1;
   a.delta = delta; a.bits = tie_bits((int)C);
+  // the P1/P2 sweep addresses an image's four message planes through one 32-bit offset range
+  a.buf_ok = 4 * H * W * C * 4 < (1LL << 31) && jump_k * H * W * 4 < (1LL << 31) && !p1p2_off();
   a.mat_lds = (kind == BPL_LBP_SEMANTIC_GLOBAL && C <= 64) ? 1 : 0;
   const size_t shm = a.mat_lds ? (size_t)C * C * sizeof(float) : 0;
   hipStream_t st = as_stream(stream);

@@ -95,7 +95,7 @@ def test_config0_full_size_forward():
     E64 = cost_g.astype(np.float64) + Mo.sum(axis=1)
     b64 = O.softmin(E64)
     bel_g = bel_t.cpu().numpy()
-    P.elementwise("beliefs [1,64,128,256] (as NHWC)", bel_g, b64)
+    P.elementwise("beliefs [1,64,128,256] (as NHWC)", bel_g, b64, tol=P.beliefs_bar(1))
     dev = P.energy_deviation(bel_g, E64)
     clear_w = P.winner_clear(E64, dev)
     disp_g = lab_t.cpu().numpy()
