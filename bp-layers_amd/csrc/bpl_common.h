@@ -158,6 +158,17 @@ __device__ __forceinline__ float wave_min_f(float v) {
   swap_halves(v, lo, hi);
   return fminf(lo, hi);
 }
+// Workgroup b of a launch runs on XCD b mod 8 (the dispatcher deals workgroups to the eight
+// XCDs in turn).  This renumbering gives every XCD a contiguous range of logical indices, so
+// that neighbouring scanlines -- whose 4-byte accesses to per-pixel planes (pairwise
+// parameters, their gradients) share cache lines -- run on one XCD and meet in its L2
+// instead of each XCD reading and writing back its own partial lines.
+__device__ __forceinline__ int xcd_contiguous_block(int b, int nblocks) {
+  constexpr int kXcd = 8;
+  const int k = b % kXcd, j = b / kXcd;
+  const int q = nblocks / kXcd, r = nblocks % kXcd;
+  return k * q + min(k, r) + j;
+}
 __device__ __forceinline__ float wave_sum_f(float v) {
   v = v + dpp_src_f<kDppXor1>(v);
   v = v + dpp_src_f<kDppXor2>(v);

@@ -391,9 +391,10 @@ __global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(B
   const int wq = PAIR ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) : 0;
   const int wv = wq & 1;  // pass of the row (PAIR)
   const int L2 = (L + 1) / 2;
-  const int g_ = PAIR ? 2 * (int)(blockIdx.x / L2) + wv : (int)(blockIdx.x / L);
-  const int ln = PAIR ? min(2 * ((int)blockIdx.x - (g_ >> 1) * L2) + (wq >> 1), L - 1)
-                      : (int)blockIdx.x - g_ * L;
+  // columns of a vertical launch go to the XCDs in contiguous ranges (per-pixel planes)
+  const int bi = VERT ? xcd_contiguous_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
+  const int g_ = PAIR ? 2 * (bi / L2) + wv : bi / L;
+  const int ln = PAIR ? min(2 * (bi - (g_ >> 1) * L2) + (wq >> 1), L - 1) : bi - g_ * L;
   const int n = a.n_base + (g_ >> 1);
   const int dir = VERT ? ((g_ & 1) ? UP : DOWN) : ((g_ & 1) ? LEFT : RIGHT);
   const int rslot = dir == RIGHT ? LEFT : dir == LEFT ? RIGHT : dir == DOWN ? UP : DOWN;

@@ -861,8 +861,10 @@ __global__ __launch_bounds__(64) void lbp_fwd_stereo_w64(FwdArgs a) {
   const int L = VERT ? W : H;
   const int len = VERT ? H : W;
   const int nsteps = len - 1;
-  const int g = blockIdx.x / L;
-  const int ln = blockIdx.x - g * L;
+  // columns of a vertical launch go to the XCDs in contiguous ranges (per-pixel planes)
+  const int bi = VERT ? xcd_contiguous_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
+  const int g = bi / L;
+  const int ln = bi - g * L;
   const int n = g >> 1;
   const int dir = VERT ? ((g & 1) ? UP : DOWN) : ((g & 1) ? LEFT : RIGHT);
   const int rslot = dir == RIGHT ? LEFT : dir == LEFT ? RIGHT : dir == DOWN ? UP : DOWN;

@@ -95,7 +95,15 @@ def test_config0_full_size_forward():
     E64 = cost_g.astype(np.float64) + Mo.sum(axis=1)
     b64 = O.softmin(E64)
     bel_g = bel_t.cpu().numpy()
-    P.elementwise("beliefs [1,64,128,256] (as NHWC)", bel_g, b64, tol=P.beliefs_bar(1))
+    # beliefs carry the absolute error of their energies (|E| up to ~20): the bar is 1e-5 or
+    # 4x what plain fp32 arithmetic reaches on this image (the C++ fp32 restatement's
+    # messages, FP64 softmin), whichever is larger
+    M32, _, _ = OC.lbp_forward(cost_g, jump, edge, M0, wl["delta"], dtype=np.float32)
+    b32 = O.softmin(cost_g.astype(np.float64) + M32.astype(np.float64).sum(axis=1))
+    m = b64 >= P.FP32_TINY
+    r32 = float((np.abs(b32 - b64)[m] / b64[m]).max())
+    print(f"    beliefs of the fp32 restatement: max rel err {r32:.3g}", flush=True)
+    P.elementwise("beliefs [1,64,128,256] (as NHWC)", bel_g, b64, tol=max(P.TOL, 4.0 * r32))
     dev = P.energy_deviation(bel_g, E64)
     clear_w = P.winner_clear(E64, dev)
     disp_g = lab_t.cpu().numpy()
