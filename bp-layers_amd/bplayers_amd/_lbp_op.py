@@ -100,7 +100,10 @@ def forward_minsum(kind, cost, jump, edge, messages, delta, out=None):
 
 def backward_minsum(kind, cost, jump, edge, in_grad, messages, messages_argmin, message_scale,
                     mode="reference", delta=None, in_h_row_ends=False, out_h_row_ends=False,
-                    grad_messages_out=None):
+                    grad_messages_out=None, accumulate_into=None):
+    """Backward of one LBP iteration (bpl_lbp_bwd).  ``accumulate_into``: the (grad_cost,
+    grad_pairwise, grad_edge) of an earlier call with these sizes, to which this call's
+    gradients are added in place (BPL_GRAD_ACCUMULATE) and which are returned."""
     N, H, W, C, K, Nj = _check_common(kind, cost, jump, edge, messages)
     nat.check_input(in_grad, "in_grad")
     if in_grad.shape != messages.shape:
@@ -128,6 +131,17 @@ def backward_minsum(kind, cost, jump, edge, in_grad, messages, messages_argmin, 
         grad_jump = torch.empty((4, H, W, C, C), dtype=torch.float32, device=dev)
         gj_ret = grad_jump
     grad_edge = torch.empty((N, 4, H, W), dtype=torch.float32, device=dev)
+    if accumulate_into is not None:
+        if kind == nat.LBP_SEMANTIC_PIXEL:
+            raise RuntimeError("accumulate_into applies to the stereo and semantic-global kinds")
+        acc = tuple(accumulate_into)
+        for t_, ref, name in zip(acc, (grad_cost, gj_ret, grad_edge), ("grad_cost", "grad_pairwise", "grad_edge")):
+            nat.check_input(t_, name)
+            if t_.shape != ref.shape:
+                raise RuntimeError(f"accumulate_into: {name} has shape {tuple(t_.shape)}, expected {tuple(ref.shape)}")
+        grad_cost, gj_ret, grad_edge = acc
+        grad_jump = gj_ret[:1] if kind == nat.LBP_SEMANTIC_GLOBAL else gj_ret
+        m |= nat.GRAD_ACCUMULATE
     if grad_messages_out is not None:
         nat.check_input(grad_messages_out, "grad_messages_out")
         if grad_messages_out.shape != messages.shape:
@@ -167,17 +181,20 @@ def exact_chain_backward(kind, cost, jump, edge, in_grad, recs, delta=None):
     only (zero elsewhere in the exact adjoint); the top call reads the full ``in_grad`` and the
     bottom call writes the full gradient w.r.t. the initial messages.  Returns
     (grad_cost, grad_pairwise, grad_edge, grad_messages) with the first three summed over the
-    iterations (top iteration first)."""
+    iterations (top iteration first; every later call adds to them in the kernels, except for
+    the per-pixel variant, whose pairwise gradient is summed here)."""
     T = len(recs)
     if T == 0:
         raise RuntimeError("exact_chain_backward needs at least one iteration")
     g = in_grad.contiguous()
     tot = None
+    in_kernel = kind != nat.LBP_SEMANTIC_PIXEL
     for t in reversed(range(T)):
         m, am, sc = recs[t]
         gc, gj, ge, g = backward_minsum(kind, cost, jump, edge, g, m, am, sc, mode="exact", delta=delta,
-                                        in_h_row_ends=t < T - 1, out_h_row_ends=t > 0)
-        if tot is None:
+                                        in_h_row_ends=t < T - 1, out_h_row_ends=t > 0,
+                                        accumulate_into=tot if in_kernel else None)
+        if tot is None or in_kernel:
             tot = [gc, gj, ge]
         else:
             tot[0] += gc

@@ -21,6 +21,7 @@ GRAD_EXACT = 1
 # chained exact iterations: in_grad's / grad_msg's LEFT and RIGHT slots at the row ends only
 GRAD_IN_H_ROW_ENDS = 0x10
 GRAD_OUT_H_ROW_ENDS = 0x20
+GRAD_ACCUMULATE = 0x40
 LBP_BWD_NOALLOC_MAX_BYTES = 128 << 20  # BPL_LBP_BWD_NOALLOC_MAX_BYTES
 
 _P = ctypes.c_void_p

@@ -32,6 +32,7 @@ struct BwdArgs {
   // chained exact iterations: in_grad's LEFT/RIGHT slots are zero away from the row ends
   // (igz), grad_msg's LEFT/RIGHT slots need writing only at the row ends (osp)
   int igz, osp;
+  int accum_cost;  // grad_cost is added to (BPL_GRAD_ACCUMULATE) by its final writers
 };
 
 // Paired horizontal launches (stereo, exact mode, fast step).  Their instantiations dominate
@@ -597,6 +598,8 @@ __global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(B
       const float *Xt_p = a.X[dir == RIGHT ? LEFT : RIGHT] + img;
       const bool isR = dir == RIGHT;
       float xo[PAIR ? PF : 1][LPL];
+      float gco[PAIR ? PF : 1][LPL];  // PAIR, accumulating: grad_cost rows of the second visits
+      const bool accg = PAIR && a.accum_cost != 0;
       // row streams of the fast step, buffer-addressed (see BUF)
       const int64_t HWC = HW * C;
       float *gm_p = a.grad_msg + (int64_t)n * 4 * HWC;
@@ -751,7 +754,8 @@ __global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(B
           for (int k = 0; k < LPL; ++k) {
             const float xr = isR ? gh[k] : xo[i][k];
             const float xl = isR ? xo[i][k] : gh[k];
-            cst[k] = ((pd[ni][k] + pu[ni][k]) + xr) + xl;
+            const float c = ((pd[ni][k] + pu[ni][k]) + xr) + xl;
+            cst[k] = accg ? gco[i][k] + c : c;
             hr[k] = xr + xl;
           }
           st_row_b(gc_rs, so, cst);
@@ -772,7 +776,11 @@ __global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(B
 #pragma unroll
         for (int k = 0; k < LPL; ++k) gvprev[k] = gv[k];
       };
-      auto load_xo = [&](int s, float *d) { ld_row_b(xt_rs, row_off(q0 + (int64_t)s * dq), d); };
+      auto load_xo = [&](int s, float *d, float *g) {
+        const uint32_t so = row_off(q0 + (int64_t)s * dq);
+        ld_row_b(xt_rs, so, d);
+        if (accg) ld_row_b(gc_rs, so, g);
+      };
       // a group of PF steps; each ring slot is refilled after its use (with the other pass's
       // h-gradient too in the second half)
       auto group = [&](int t0, auto second_tag) {
@@ -786,7 +794,7 @@ __global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(B
           const int s = nsteps - 1 - (t0 + i);
           const int sn = s - PF >= 0 ? s - PF : 0;
           load_step(sn, pg[i], pm[i], pa[i], pd[i], pu[i], pp[i], pl[i]);
-          if constexpr (SECOND) load_xo(sn, xo[i]);
+          if constexpr (SECOND) load_xo(sn, xo[i], gco[i]);
         }
       };
       int t0 = 0;
@@ -803,7 +811,7 @@ __global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(B
 #pragma unroll
           for (int i = 0; i < PF; ++i) {
             const int s = nsteps - 1 - (Tb + i);
-            load_xo(s >= 0 ? s : 0, xo[i]);
+            load_xo(s >= 0 ? s : 0, xo[i], gco[i]);
           }
           for (; t0 + PF <= nl; t0 += PF) group(t0, std::true_type());
 #pragma unroll
@@ -1028,6 +1036,12 @@ __global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(B
         for (int k = 0; k < LPL; ++k)
           if (valid[k]) row[c0 + k] = v[k];
       };
+      if (a.accum_cost) {
+        float old[LPL];
+        ld_row(a.grad_cost + img + q * C, old);
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) cst[k] = old[k] + cst[k];
+      }
       st(a.grad_cost + img + q * C, cst);
       st(gm + UP * HWC, hr);
       st(gm + DOWN * HWC, hr);
@@ -1427,12 +1441,13 @@ __global__ void lbp_bwd_combine(const float *__restrict__ XL, const float *__res
                                 const float *__restrict__ XU, const float *__restrict__ XD,
                                 const float *__restrict__ in_grad, float *__restrict__ grad_cost,
                                 float *grad_msg, int N, int H, int W, int C, int exact,
-                                int osp, int copy_msgs) {
+                                int osp, int copy_msgs, int accum_cost) {
   const int64_t total = (int64_t)N * H * W * C;
   const int64_t HWC = (int64_t)H * W * C;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
     const float xl = XL[i], xr = XR[i], xu = XU[i], xd = XD[i];
-    grad_cost[i] = ((xd + xu) + xr) + xl;
+    const float gc = ((xd + xu) + xr) + xl;
+    grad_cost[i] = accum_cost ? grad_cost[i] + gc : gc;
     const int64_t n = i / HWC;
     const int64_t r = i - n * HWC;            // (y, x, c) inside the image
     const int64_t base = n * 4 * HWC + r;     // slot 0 of this element
@@ -1455,6 +1470,12 @@ __global__ void lbp_bwd_combine(const float *__restrict__ XL, const float *__res
     grad_msg[base + UP * HWC] = y == 0 ? (in_grad[base + UP * HWC] + xd) + hr : hr;
     grad_msg[base + DOWN * HWC] = y == H - 1 ? (in_grad[base + DOWN * HWC] + xu) + hr : hr;
   }
+}
+
+// dst += src over n elements (BPL_GRAD_ACCUMULATE: this call's grad_jump and grad_edge)
+__global__ void lbp_bwd_accumulate(float *__restrict__ dst, const float *__restrict__ src, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = dst[i] + src[i];
 }
 
 // Sum of the per-(wave, segment) C x C partials of each direction, in two deterministic stages.
@@ -1586,6 +1607,18 @@ int seg_width(int C) { return C <= 16 ? 16 : C <= 32 ? 32 : 64; }
 // generic layout
 bool semg_ok(const BwdArgs &a) { return a.C > 16 && a.C <= 32 && !generic_only(); }
 
+// BPL_GRAD_ACCUMULATE: the kernels write this call's grad_jump and grad_edge into workspace
+// copies, which lbp_bwd_accumulate then adds to the caller's tensors (stereo jumps of up to
+// 15 entries for each of N images; semantic global: one 4 x C x C matrix set)
+constexpr int kMaxJumpK = 15;
+int64_t acc_jump_floats(int kind, int64_t N, int64_t H, int64_t W, int64_t C) {
+  if (kind == BPL_LBP_STEREO) return N * 4 * kMaxJumpK * H * W;
+  if (kind == BPL_LBP_SEMANTIC_GLOBAL) return 4 * C * C;
+  return 0;
+}
+int64_t acc_floats(int kind, int64_t N, int64_t H, int64_t W, int64_t C) {
+  return kind == BPL_LBP_SEMANTIC_PIXEL ? 0 : acc_jump_floats(kind, N, H, W, C) + N * 4 * H * W;
+}
 void ws_layout(int kind, int64_t N, int64_t H, int64_t W, int64_t C, int64_t *planes, int64_t *part_v,
                int64_t *part_h, int64_t *total) {
   const int64_t plane = N * H * W * C;
@@ -1599,7 +1632,7 @@ void ws_layout(int kind, int64_t N, int64_t H, int64_t W, int64_t C, int64_t *pl
   }
   *part_v = pv;
   *part_h = ph;
-  *total = (*planes + pv + ph) * (int64_t)sizeof(float);
+  *total = (*planes + pv + ph + acc_floats(kind, N, H, W, C)) * (int64_t)sizeof(float);
 }
 
 }  // namespace
@@ -1623,7 +1656,10 @@ int lbp_bwd_impl(int kind, int mode, const float *jump, int64_t jump_k, int64_t 
                  int64_t N, int64_t H, int64_t W, int64_t C, int delta, void *stream) {
   BPL_REQUIRE(kind >= 0 && kind <= 2, BPL_E_ARG, "bpl_lbp_bwd: unknown kind %d", kind);
   const int hflags = mode & (BPL_GRAD_IN_H_ROW_ENDS | BPL_GRAD_OUT_H_ROW_ENDS);
-  mode &= ~(BPL_GRAD_IN_H_ROW_ENDS | BPL_GRAD_OUT_H_ROW_ENDS);
+  const bool accumulate = (mode & BPL_GRAD_ACCUMULATE) != 0;
+  mode &= ~(BPL_GRAD_IN_H_ROW_ENDS | BPL_GRAD_OUT_H_ROW_ENDS | BPL_GRAD_ACCUMULATE);
+  BPL_REQUIRE(!accumulate || kind != BPL_LBP_SEMANTIC_PIXEL, BPL_E_ARG,
+              "bpl_lbp_bwd: BPL_GRAD_ACCUMULATE applies to the stereo and semantic-global kinds");
   BPL_REQUIRE(mode == BPL_GRAD_REFERENCE || mode == BPL_GRAD_EXACT, BPL_E_ARG, "bpl_lbp_bwd: unknown mode %d", mode);
   BPL_REQUIRE(hflags == 0 || mode == BPL_GRAD_EXACT, BPL_E_ARG,
               "bpl_lbp_bwd: the row-end flags apply to the exact mode only");
@@ -1672,6 +1708,17 @@ int lbp_bwd_impl(int kind, int mode, const float *jump, int64_t jump_k, int64_t 
   a.igz = (hflags & BPL_GRAD_IN_H_ROW_ENDS) != 0;
   a.osp = (hflags & BPL_GRAD_OUT_H_ROW_ENDS) != 0;
   a.grad_cost = grad_cost; a.grad_msg = grad_msg;
+  a.accum_cost = accumulate ? 1 : 0;
+  // accumulating: this call's pairwise and edge gradients go to workspace copies first
+  float *acc_jump = ws + planes + pv + ph;
+  float *acc_edge = acc_jump + acc_jump_floats(kind, N, H, W, C);
+  const int64_t jump_elems = kind == BPL_LBP_STEREO ? jump_n * 4 * jump_k * H * W : 4 * C * C;
+  const int64_t edge_elems = N * 4 * H * W;
+  if (accumulate) {
+    a.grad_jump = acc_jump;
+    a.grad_edge = acc_edge;
+  }
+  float *out_jump = a.grad_jump;
   hipStream_t st = as_stream(stream);
   // parameter gradients shared across the batch accumulate over n in launch order
   const bool semp = kind == BPL_LBP_SEMANTIC_PIXEL && semp_ok(a);
@@ -1723,7 +1770,7 @@ int lbp_bwd_impl(int kind, int mode, const float *jump, int64_t jump_k, int64_t 
     KernelTimer kt(BPL_KERNEL_LBP_BWD_COMBINE, st);
     hipLaunchKernelGGL(lbp_bwd_combine, dim3((unsigned)blocks), dim3(threads), 0, st, a.X[LEFT], a.X[RIGHT],
                        a.X[UP], a.X[DOWN], in_grad, grad_cost, grad_msg, (int)N, (int)H, (int)W, (int)C,
-                       mode == BPL_GRAD_EXACT ? 1 : 0, a.osp, grad_msg != in_grad ? 1 : 0);
+                       mode == BPL_GRAD_EXACT ? 1 : 0, a.osp, grad_msg != in_grad ? 1 : 0, a.accum_cost);
     int rc = check_hip(hipGetLastError(), "bpl_lbp_bwd combine launch");
     if (rc) return rc;
   }
@@ -1737,8 +1784,21 @@ int lbp_bwd_impl(int kind, int mode, const float *jump, int64_t jump_k, int64_t 
     int rc = check_hip(hipGetLastError(), "bpl_lbp_bwd reduce launch");
     if (rc) return rc;
     hipLaunchKernelGGL(lbp_bwd_reduce_chunks, dim3((unsigned)((4 * C * C + 255) / 256)), dim3(256), 0, st,
-                       chunk_sums, grad_jump, (int)C);
+                       chunk_sums, out_jump, (int)C);
     rc = check_hip(hipGetLastError(), "bpl_lbp_bwd reduce launch");
+    if (rc) return rc;
+  }
+  if (accumulate) {
+    KernelTimer kt(BPL_KERNEL_LBP_BWD_COMBINE, st);
+    auto add = [&](float *dst, const float *src, int64_t n) {
+      int64_t blocks = (n + 255) / 256;
+      if (blocks > 65536) blocks = 65536;
+      hipLaunchKernelGGL(lbp_bwd_accumulate, dim3((unsigned)blocks), dim3(256), 0, st, dst, src, n);
+      return check_hip(hipGetLastError(), "bpl_lbp_bwd accumulate launch");
+    };
+    int rc = add(grad_jump, acc_jump, jump_elems);
+    if (rc) return rc;
+    rc = add(grad_edge, acc_edge, edge_elems);
     if (rc) return rc;
   }
   return BPL_OK;

@@ -59,6 +59,14 @@ extern "C" {
  */
 #define BPL_GRAD_IN_H_ROW_ENDS 0x10
 #define BPL_GRAD_OUT_H_ROW_ENDS 0x20
+/*
+ * BPL_GRAD_ACCUMULATE, OR-ed into either mode: grad_cost, grad_jump and grad_edge are added
+ * to (their contents plus this iteration's gradient) instead of overwritten, so the calls of
+ * a chain of iterations sum the parameter gradients in place -- what autograd does across the
+ * reference's iterations (ops/lbp_stereo/message_passing_op_cuda.py:165-167).  grad_msg is
+ * written as without the flag.  Kinds BPL_LBP_STEREO and BPL_LBP_SEMANTIC_GLOBAL.
+ */
+#define BPL_GRAD_ACCUMULATE 0x40
 
 /* Library identification and diagnostics. */
 int bpl_version(void);
@@ -171,7 +179,8 @@ int64_t bpl_lbp_bwd_workspace_size(int kind, int64_t N, int64_t H, int64_t W, in
  * lbp_min_sum_kernel.cu:215-417, host loop :513-621).
  * messages: the message tensor the probabilities exp(m) are read from (the
  * reference passes the in-place-updated tensor); argmin/gargmin as written by
- * bpl_lbp_fwd.  Outputs are fully overwritten: grad_cost [N,H,W,C]; grad_jump
+ * bpl_lbp_fwd.  Outputs are fully overwritten (or, BPL_GRAD_ACCUMULATE, the first three
+ * added to): grad_cost [N,H,W,C]; grad_jump
  * (shape of jump); grad_edge [N,4,H,W] (SEMANTIC_PIXEL in the reference mode: zeros, as
  * the reference, and jump may be NULL); grad_msg [N,4,H,W,C] (BPL_GRAD_REFERENCE: a copy
  * of in_grad; BPL_GRAD_EXACT: the gradient w.r.t. the iteration's input messages).

@@ -64,16 +64,22 @@ def test_argument_validation_returns_codes(lib):
         rc = lib.bpl_lbp_bwd(nat.LBP_STEREO, nat.GRAD_REFERENCE | fl, P, 6, 1, P, P, P, P, P, P, P, P, P, P,
                              1 << 40, 1, 4, 4, 8, 3, None)
         assert rc == -1 and b"exact mode only" in lib.bpl_last_error()
-    rc = lib.bpl_lbp_bwd(nat.LBP_STEREO, nat.GRAD_EXACT | 0x40, P, 6, 1, P, P, P, P, P, P, P, P, P, P,
+    rc = lib.bpl_lbp_bwd(nat.LBP_STEREO, nat.GRAD_EXACT | 0x80, P, 6, 1, P, P, P, P, P, P, P, P, P, P,
                          1 << 40, 1, 4, 4, 8, 3, None)
     assert rc == -1 and b"unknown mode" in lib.bpl_last_error()
+    # accumulation across iterations: stereo and semantic-global kinds only
+    rc = lib.bpl_lbp_bwd(nat.LBP_SEMANTIC_PIXEL, nat.GRAD_EXACT | nat.GRAD_ACCUMULATE, P, 6, 1, P, P, P, P, P,
+                         P, P, P, P, P, 1 << 40, 1, 4, 4, 8, 3, None)
+    assert rc == -1 and b"ACCUMULATE" in lib.bpl_last_error()
     # grad_msg may alias in_grad only in the reference mode (arguments: ..., in_grad, ..., grad_msg)
     G = ctypes.c_void_p(4096)
     rc = lib.bpl_lbp_bwd(nat.LBP_STEREO, nat.GRAD_EXACT, P, 6, 1, P, G, P, P, P, P, P, P, G, P,
                          1 << 40, 1, 4, 4, 8, 3, None)
     assert rc == -1 and b"alias" in lib.bpl_last_error()
     need = lib.bpl_lbp_bwd_workspace_size(nat.LBP_STEREO, 1, 4, 4, 8)
-    assert need == 4 * 1 * 4 * 4 * 8 * 4
+    # four h-gradient planes, then the accumulation copies of grad_jump (up to 15 entries per
+    # direction) and grad_edge
+    assert need == (4 * 1 * 4 * 4 * 8 + 1 * 4 * 15 * 4 * 4 + 1 * 4 * 4 * 4) * 4
     rc = lib.bpl_lbp_bwd(nat.LBP_STEREO, 1, P, 6, 1, P, P, P, P, P, P, P, P, G, P, need - 1, 1, 4, 4, 8, 3, None)
     assert rc == -1 and b"workspace" in lib.bpl_last_error()
     rc = lib.bpl_sad_stereo_fwd(P, P, P, 1, 3, 4, 4, 5, 2, 1.0, None)

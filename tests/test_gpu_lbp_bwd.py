@@ -144,3 +144,64 @@ def test_reference_mode_passes_in_grad_through(kind, C):
         else:
             mod.backward_minsum(cost, jump, edge, G, msgs, am, sc, mode="exact", grad_messages_out=G)
 
+
+
+@pytest.mark.parametrize("kind,N,H,W,C,jn,mode", [
+    ("stereo", 1, 6, 40, 192, None, "exact"),      # paired horizontal kernel (fast step)
+    ("stereo", 2, 5, 9, 100, 1, "exact"),          # one jump for the batch: per-image launches
+    ("stereo", 1, 4, 7, 64, None, "reference"),    # unpaired fast kernels and the combine pass
+    ("semantic_global", 2, 5, 6, 19, None, "exact"),
+])
+def test_accumulate_adds_to_the_outputs(kind, N, H, W, C, jn, mode):
+    """BPL_GRAD_ACCUMULATE: grad_cost, grad_pairwise and grad_edge come out as the tensors
+    passed in plus the plain call's results (bit for bit: one fp32 addition each); the
+    message gradient is the plain call's."""
+    from bplayers_amd import _lbp_op, _native as nat
+    knd = {"stereo": nat.LBP_STEREO, "semantic_global": nat.LBP_SEMANTIC_GLOBAL}[kind]
+    cost, jump, edge, M0 = make_case(kind, N, H, W, C, 3, seed=7 + C)
+    if jn == 1:
+        jump = jump[:1]
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    rng = np.random.default_rng(C)
+    G = t(rng.normal(size=M0.shape).astype(np.float32))
+    msgs, am, sc = _lbp_op.forward_minsum(knd, t(cost), t(jump), t(edge), t(M0), 3, out=torch.empty_like(t(M0)))
+    plain = _lbp_op.backward_minsum(knd, t(cost), t(jump), t(edge), G, msgs, am, sc, mode=mode, delta=3)
+    prior = [torch.randn(r.shape, generator=torch.Generator().manual_seed(i)).to(dev) for i, r in enumerate(plain[:3])]
+    if kind == "semantic_global":
+        prior[1][1:] = 0   # the matrix gradient lives in batch entry 0
+    expect = [p + r for p, r in zip(prior, plain[:3])]
+    acc = _lbp_op.backward_minsum(knd, t(cost), t(jump), t(edge), G, msgs, am, sc, mode=mode, delta=3,
+                                  accumulate_into=[p.clone() for p in prior])
+    torch.cuda.synchronize()
+    for name, a, e in zip(("grad cost", "grad pairwise", "grad edge"), acc[:3], expect):
+        d = (a - e).abs().max().item()
+        print(f"    {kind} {mode}: accumulated {name} max |diff| {d:.3g}", flush=True)
+        assert torch.equal(a, e), f"accumulated {name} differs from prior + plain"
+    assert torch.equal(acc[3], plain[3]), "message gradient changed by accumulation"
+
+
+@pytest.mark.parametrize("C", [192, 255])
+def test_chain_accumulates_like_explicit_sums(C):
+    """exact_chain_backward (accumulating in the kernels, chained row-end flags) against the
+    same three calls without accumulation summed in torch, in the same order."""
+    from bplayers_amd import _lbp_op, _native as nat
+    N, H, W, iters = 1, 6, 37, 3
+    cost, jump, edge, M0 = make_case("stereo", N, H, W, C, 3, seed=11)
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    G = t(np.random.default_rng(3).normal(size=M0.shape).astype(np.float32))
+    _, recs = _lbp_op.exact_chain_forward(nat.LBP_STEREO, t(cost), t(jump), t(edge), t(M0), 3, iters)
+    got = _lbp_op.exact_chain_backward(nat.LBP_STEREO, t(cost), t(jump), t(edge), G, recs, delta=3)
+    g, tot = G, None
+    for it in reversed(range(iters)):
+        m, am, sc = recs[it]
+        gc, gj, ge, g = _lbp_op.backward_minsum(nat.LBP_STEREO, t(cost), t(jump), t(edge), g, m, am, sc,
+                                                mode="exact", delta=3, in_h_row_ends=it < iters - 1,
+                                                out_h_row_ends=it > 0)
+        tot = [gc, gj, ge] if tot is None else [tot[0] + gc, tot[1] + gj, tot[2] + ge]
+    torch.cuda.synchronize()
+    for name, a, e in zip(("grad cost", "grad pairwise", "grad edge"), got[:3], tot):
+        print(f"    C={C} chain {name}: max |diff| {(a - e).abs().max().item():.3g}", flush=True)
+        assert torch.equal(a, e), f"chain {name}: in-kernel accumulation differs from the explicit sum"
+    assert torch.equal(got[3], g)
