@@ -31,6 +31,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 #                       write g_cost, g_msg (with the combine kernel)         30 B/pl
 FWD_B_PL = 44.0
 BWD_B_PL = 52.0
+ACC_B_PL = 4.0  # summing grad_cost over a chain's iterations: read the running sum
 PHASE_B_PL = {"lbp_fwd_horizontal": (22.0, 24.0), "lbp_fwd_vertical": (22.0, 24.0),
               "lbp_bwd_vertical": (22.0, 48.0), "lbp_bwd_horizontal": (30.0, 48.0)}  # (per pl, per px / C)
 LBP_CLASSES = ("lbp_fwd_horizontal", "lbp_fwd_vertical", "lbp_bwd_vertical", "lbp_bwd_horizontal",
@@ -49,6 +50,8 @@ def parse():
     ap.add_argument("--requests", type=int, default=96, help="serving_c4: requests per measurement")
     ap.add_argument("--rows", type=int, default=0, help="crop the image to this many rows (debug)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-others", action="store_true",
+                    help="default workload: skip the compact semantic / flow / serving results")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--pmc-summary", nargs="+", metavar="DIR",
                     help="summarise rocprofv3 --pmc runs of `bench.py --pmc` (directories holding "
@@ -146,17 +149,29 @@ class StereoPlan:
                 g_in = g_out
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(wl, plan_shape, seconds, iters_per_sample=1):
     """The C++/OpenMP restatement (oracle/c) on a crop of the same workload, fp32, all host
-    cores: forward + exact backward of one LBP iteration; returns G pl/s."""
+    cores: forward + exact backward of one LBP iteration; returns G pl/s, with a single-thread
+    figure on a smaller crop (about a third of the time budget) and the CPU model."""
     import numpy as np
     sys.path.insert(0, ROOT)
     from oracle import cpu
     N, H, W, C = plan_shape
-    threads = len(os.sched_getaffinity(0))
+    all_threads = len(os.sched_getaffinity(0))
     rng = np.random.default_rng(0)
 
-    def run(rows):
+    def run(rows, threads):
         cost = -rng.uniform(0, 1, size=(N, rows, W, C)).astype(np.float32)
         aff = wl["affinities"][..., :rows, :]
         from bplayers_amd.ops.lbp_stereo.message_passing_op_cuda import pack_affinities
@@ -170,21 +185,29 @@ def cpu_baseline(wl, plan_shape, seconds, iters_per_sample=1):
         cpu.lbp_backward_exact(jump, edge, G, M, am, fg, wl["delta"], threads=threads)
         return time.perf_counter() - t0
 
-    rows = 4
-    t = run(rows)
-    # scale the crop so the timed sample takes about `seconds`
-    rows = int(max(4, min(H, rows * seconds / max(t, 1e-3))))
-    t = run(rows)
-    # the whole image takes less than the budget: repeat it (fresh inputs each time) and average
-    reps = max(1, int(seconds / max(t, 1e-3))) if rows == H else 1
-    if reps > 1:
-        t = sum(run(rows) for _ in range(reps)) / reps
+    def sized(threads, budget):
+        rows = 4
+        t = run(rows, threads)
+        # scale the crop so the timed sample takes about `budget`
+        rows = int(max(4, min(H, rows * budget / max(t, 1e-3))))
+        t = run(rows, threads)
+        # the whole image takes less than the budget: repeat it (fresh inputs each time) and average
+        reps = max(1, int(budget / max(t, 1e-3))) if rows == H else 1
+        if reps > 1:
+            t = sum(run(rows, threads) for _ in range(reps)) / reps
+        return rows, t, reps
+
+    rows, t, reps = sized(all_threads, seconds)
     pl = N * rows * W * C
-    return {"value": pl / t / 1e9, "unit": "G(pixel*label)/s per LBP iter fwd+bwd", "cores": threads,
-            "kind": "port",
+    rows1, t1, _ = sized(1, seconds / 3)
+    pl1 = N * rows1 * W * C
+    return {"value": pl / t / 1e9, "unit": "G(pixel*label)/s per LBP iter fwd+bwd", "cores": all_threads,
+            "kind": "port", "cpu_model": cpu_model(),
             "sample": f"C++/OpenMP fp32 restatement (oracle/c), 1 iteration fwd+exact bwd on a "
                       f"{rows}x{W} image with {C} labels ({pl / 1e6:.1f} M pl), mean of {reps} runs "
-                      f"of {t:.2f} s ({reps * t:.0f} s of CPU time)"}
+                      f"of {t:.2f} s ({reps * t:.0f} s of CPU time)",
+            "single_thread": {"value": pl1 / t1 / 1e9, "cores": 1,
+                              "sample": f"the same on a {rows1}x{W} crop, one thread, {t1:.2f} s"}}
 
 
 class LBPPlan:
@@ -302,7 +325,7 @@ def bench_semantic(args, torch):
                                   "alg_bytes_per_pl": round(it_bytes / plan.pl(), 3), "iteration_ms": round(it_ms, 4)}
     line["timings_ms"] = {f"{k}_ms": round(v, 4) for k, v in avg.items()}
     line["wall_s"] = round(wall, 3)
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def bench_semantic_pixel(args, torch):
@@ -336,7 +359,7 @@ def bench_semantic_pixel(args, torch):
                                   "alg_bytes_per_pl": round(it_bytes / plan.pl(), 3), "iteration_ms": round(it_ms, 4)}
     line["timings_ms"] = {f"{k}_ms": round(v, 4) for k, v in avg.items()}
     line["wall_s"] = round(wall, 3)
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def bench_flow(args, torch):
@@ -404,7 +427,7 @@ def bench_flow(args, torch):
     line["sad"] = {"flow_fwd_G_pixel_window_per_s": round(N * H * W * K * K / (avg["sad_flow_fwd"] * 1e-3) / 1e9, 3),
                    "window": f"{K}x{K}", "feature_channels": Cf}
     line["wall_s"] = round(wall, 3)
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def bench_serving(args, torch):
@@ -444,7 +467,42 @@ def bench_serving(args, torch):
                                           "continuous": {"p50": round(mix_l["p50_ms"], 2), "p99": round(mix_l["p99_ms"], 2)},
                                           "sequential": {"p50": round(seq_l["p50_ms"], 2), "p99": round(seq_l["p99_ms"], 2)}}
     line["resident_bytes"] = mix["resident_bytes"]
-    print(json.dumps(line), flush=True)
+    return line
+
+
+def other_workloads(args, torch, benches):
+    """Compact results of the semantic, flow and serving workloads for the default line (a few
+    steps each; every workload also runs on its own with --workload)."""
+    import copy
+    sub = copy.copy(args)
+    sub.steps, sub.warmup = 5, 2
+    out = {}
+    for name in ("semantic_fb_2x512x1024_c19", "semantic_pixel_fb_2x512x1024_c19", "flow_fb_256x512_r12",
+                 "serving_c4"):
+        sub.workload = name
+        t0 = time.perf_counter()
+        try:
+            r = benches[name](sub, torch)
+        except Exception as e:  # reported, never allowed to cost the headline number
+            out[name] = {"error": repr(e)}
+            continue
+        if name == "serving_c4":
+            c = {"frames_per_s": r["value"], "sequential_frames_per_s": r["sequential_frames_per_s"],
+                 "speedup_vs_sequential": r["speedup_vs_sequential"],
+                 "latency_ms_at_offered_load": r["latency_ms_at_offered_load"]}
+        else:
+            c = {"metric": r["metric"], "value": r["value"], "unit": r["unit"], "ms_per_step": r["ms_per_step"],
+                 "steps": sub.steps, "config": r["config"], "timings_ms": r.get("timings_ms")}
+            ri = r.get("roofline_iteration")
+            if ri:
+                c["roofline_iteration"] = {k: ri[k] for k in ("achieved", "peak", "unit", "frac", "alg_bytes_per_pl",
+                                                              "iteration_ms")}
+            if "sad" in r:
+                c["sad"] = r["sad"]
+        c["wall_s"] = round(time.perf_counter() - t0, 2)
+        out[name] = c
+        torch.cuda.empty_cache()
+    return out
 
 
 def measured_traffic(workload, kernel):
@@ -518,6 +576,7 @@ def pmc_summary(dirs):
                            "fetch_scale": round(fs, 4) if fs else None, "write_scale": round(ws, 4) if ws else None},
            "kernels": {}}
     canon = {k: v[0] + v[1] / 192 for k, v in PHASE_B_PL.items()}
+    canon["lbp_bwd_horizontal"] += ACC_B_PL * 2 / 3  # 2 of the 3 chained calls accumulate
     rows = []
     for cls in sorted(mean):
         if cls == "calibration_copy":
@@ -589,7 +648,7 @@ def main():
               "flow_fb_256x512_r12": bench_flow,
               "serving_c4": bench_serving}
     if args.workload in others:
-        others[args.workload](args, torch)
+        print(json.dumps(others[args.workload](args, torch)), flush=True)
         return
     if args.workload == "stereo_fb_384x1248_d192":
         wl = workloads.stereo_full()
@@ -635,13 +694,18 @@ def main():
     lbp_k = {k: kt[k][0] for k in kt if k in PHASE_B_PL}
     dom = max(lbp_k, key=lbp_k.get)
     per_pl, per_px = PHASE_B_PL[dom]
+    # the backward calls after the top one add to grad_cost (read + write): +4 B/pl on average
+    # over the chain's calls (iters - 1 of iters)
+    acc_pl = ACC_B_PL * (iters - 1) / iters if backward else 0.0
+    if dom == "lbp_bwd_horizontal":
+        per_pl += acc_pl
     dom_bytes = pl * (per_pl + per_px / C)
     dom_achieved = dom_bytes / (avg[dom] * 1e-3) / 1e9
     fwd_b = FWD_B_PL + (32.0 + 16.0) / C
     bwd_b = BWD_B_PL + (64.0 + 32.0) / C
     it_classes = LBP_CLASSES if backward else LBP_CLASSES[:2]
     it_ms = sum(avg[k] for k in it_classes if k in avg)
-    it_bytes = pl * (fwd_b + (bwd_b if backward else -4.0))  # forward-only: no argmin writes
+    it_bytes = pl * (fwd_b + (bwd_b + acc_pl if backward else -4.0))  # forward-only: no argmin writes
     it_achieved = it_bytes / (it_ms * 1e-3) / 1e9
     traffic = measured_traffic(args.workload, dom)
     res = {f"{k}_ms": v for k, v in avg.items()}
@@ -691,13 +755,15 @@ def main():
                      "traffic_source": traffic["source"] if traffic else None},
         "roofline_iteration": {"bound": "hbm", "kernels": list(it_classes), "achieved": round(it_achieved, 2),
                                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(it_achieved / HBM_PEAK_GBS, 4),
-                               "alg_bytes_per_pl": round(fwd_b + (bwd_b if backward else -4.0), 3),
+                               "alg_bytes_per_pl": round(fwd_b + (bwd_b + acc_pl if backward else -4.0), 3),
                                "iteration_ms": round(it_ms, 4)},
         "timings_ms": {k: round(v, 4) for k, v in res.items()},
         "sad": {"fwd_G_pd_per_s": round(pl / (t_sad * 1e-3) / 1e9, 3),
                 "bwd_G_pd_per_s": round(pl / (res["sad_stereo_bwd_ms"] * 1e-3) / 1e9, 3) if backward else None},
         "wall_s": round(wall, 3),
     }
+    if args.workload == "stereo_fb_384x1248_d192" and not args.no_others and not args.rows:
+        line["other_workloads"] = other_workloads(args, torch, others)
     if not args.no_cpu:
         try:
             line["cpu_baseline"] = cpu_baseline(wl, (plan.N, plan.H, plan.W, plan.C), args.cpu_seconds)
