@@ -309,6 +309,8 @@ def bench_semantic(args, torch):
     jump = pack_pairwise(torch.from_numpy(wl["pw_mat"]).to(dev), False)
     edge = torch.from_numpy(wl["edge"]).to(dev)
     plan = LBPPlan(nat.LBP_SEMANTIC_GLOBAL, cost, jump, 0, 1, edge, wl["delta"], wl["iters"])
+    if args.pmc:
+        return pmc_steps(plan.step, torch, dev, {"workload": args.workload, "pl": plan.pl()})
     ms, wall, kt = timed_steps(args.steps, args.warmup, plan.step, nat, torch)
     value = plan.pl() * plan.iters / (ms * 1e-3) / 1e9
     avg = {k: v[0] / v[1] for k, v in kt.items()}
@@ -343,6 +345,8 @@ def bench_semantic_pixel(args, torch):
     jump = (w[:, :, :, None, None] * M).contiguous()
     edge = torch.ones((N, 4, H, W), device=dev)
     plan = LBPPlan(nat.LBP_SEMANTIC_PIXEL, cost, jump, 0, 1, edge, wl["delta"], wl["iters"])
+    if args.pmc:
+        return pmc_steps(plan.step, torch, dev, {"workload": args.workload, "pl": plan.pl()})
     ms, wall, kt = timed_steps(args.steps, args.warmup, plan.step, nat, torch)
     value = plan.pl() * plan.iters / (ms * 1e-3) / 1e9
     avg = {k: v[0] / v[1] for k, v in kt.items()}
@@ -403,10 +407,6 @@ def bench_flow(args, torch):
                 plan.bwd(t, g_in, g_out)
                 g_in = g_out
 
-    ms, wall, kt = timed_steps(args.steps, args.warmup, step, nat, torch)
-    pl_step = (pu.pl() + pv.pl()) * wl["iters"]
-    avg = {k: v[0] / v[1] for k, v in kt.items()}
-    # flow SAD forward and backward, timed the same way
     df0, df1 = torch.empty_like(f0), torch.empty_like(f1)
     gu, gv = torch.randn_like(cu), torch.randn_like(cv)
 
@@ -415,6 +415,15 @@ def bench_flow(args, torch):
         nat.call("bpl_sad_flow_bwd", nat.ptr(f0), nat.ptr(f1), nat.ptr(gu), nat.ptr(gv), nat.ptr(us), nat.ptr(vs),
                  nat.ptr(df0), nat.ptr(df1), N, Cf, H, W, wl["sws"], st)
 
+    if args.pmc:
+        def both():
+            sad_step()
+            step()
+        return pmc_steps(both, torch, dev, {"workload": args.workload, "pl": pu.pl() + pv.pl()})
+    ms, wall, kt = timed_steps(args.steps, args.warmup, step, nat, torch)
+    pl_step = (pu.pl() + pv.pl()) * wl["iters"]
+    avg = {k: v[0] / v[1] for k, v in kt.items()}
+    # flow SAD forward and backward, timed the same way
     _, _, kt2 = timed_steps(5, 1, sad_step, nat, torch)
     for k, v in kt2.items():
         avg[k] = v[0] / v[1]
@@ -543,7 +552,18 @@ def kernel_class(name):
     return f"lbp_{m.group(1)}_{'vertical' if vert == 'true' else 'horizontal'}"
 
 
-def pmc_summary(dirs):
+# per-phase canonical bytes of the semantic workloads (per pixel: edge weights only, read in
+# the forward, read and their gradient written in the backward)
+PHASE_B_PL_SEMANTIC = {"lbp_fwd_horizontal": (22.0, 8.0), "lbp_fwd_vertical": (22.0, 8.0),
+                       "lbp_bwd_vertical": (22.0, 16.0), "lbp_bwd_horizontal": (30.0, 16.0)}
+PMC_WORKLOADS = {  # pixel*labels of one LBP iteration, labels, canonical phase bytes, output file
+    "stereo_fb_384x1248_d192": (384 * 1248 * 192, 192, PHASE_B_PL, TRAFFIC_FILE),
+    "semantic_fb_2x512x1024_c19": (2 * 512 * 1024 * 19, 19, PHASE_B_PL_SEMANTIC, None),
+    "flow_fb_256x512_r12": (256 * 512 * 25, 25, PHASE_B_PL, None),  # one volume (u or v) per launch
+}
+
+
+def pmc_summary(dirs, workload="stereo_fb_384x1248_d192"):
     """profiles/traffic.json from rocprofv3 --pmc runs of `bench.py --pmc`: per kernel class the
     HBM bytes per launch (FETCH_SIZE and WRITE_SIZE, in KiB per dispatch, each scaled by what it
     reads for the 256 MiB calibration copy) and the SQ counters collected."""
@@ -564,19 +584,23 @@ def pmc_summary(dirs):
     copy_bytes = 256 << 20
     fs = copy_bytes / (cal["FETCH_SIZE"] * 1024) if "FETCH_SIZE" in cal else None
     ws = copy_bytes / (cal["WRITE_SIZE"] * 1024) if "WRITE_SIZE" in cal else None
-    pl = 384 * 1248 * 192
-    out = {"workload": "stereo_fb_384x1248_d192",
+    pl, n_lab, phase_b, out_file = PMC_WORKLOADS[workload]
+    out_file = out_file or os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    out = {"workload": workload,
            "command": "cd /tmp && export TMPDIR=/tmp && rocprofv3 --pmc <counters> --output-format csv "
-                      "-d $REPO/gpurun_out/pmc_<name> -o run -- python3 $REPO/bench.py --pmc  (one run per "
-                      "counter set: FETCH_SIZE; WRITE_SIZE; SQ counters), then "
-                      "python3 bench.py --pmc-summary gpurun_out/pmc_*",
+                      f"-d $REPO/gpurun_out/pmc_<name> -o run -- python3 $REPO/bench.py --pmc --workload {workload} "
+                      "(one run per counter set: FETCH_SIZE; WRITE_SIZE; SQ counters; TCC counters), then "
+                      f"python3 bench.py --pmc-summary gpurun_out/pmc_* --workload {workload}",
            "calibration": {"kernel": "__amd_rocclr_copyBuffer (torch copy_ of 256 MiB fp32)",
                            "bytes_read": copy_bytes, "bytes_written": copy_bytes,
                            "FETCH_SIZE_KiB": cal.get("FETCH_SIZE"), "WRITE_SIZE_KiB": cal.get("WRITE_SIZE"),
                            "fetch_scale": round(fs, 4) if fs else None, "write_scale": round(ws, 4) if ws else None},
            "kernels": {}}
-    canon = {k: v[0] + v[1] / 192 for k, v in PHASE_B_PL.items()}
-    canon["lbp_bwd_horizontal"] += ACC_B_PL * 2 / 3  # 2 of the 3 chained calls accumulate
+    canon = {k: v[0] + v[1] / n_lab for k, v in phase_b.items()}
+    if workload == "stereo_fb_384x1248_d192":
+        canon["lbp_bwd_horizontal"] += ACC_B_PL * 2 / 3  # 2 of the 3 chained calls accumulate
+    else:
+        canon["lbp_bwd_horizontal"] += ACC_B_PL / 2      # 1 of the 2 chained calls accumulates
     rows = []
     for cls in sorted(mean):
         if cls == "calibration_copy":
@@ -592,9 +616,12 @@ def pmc_summary(dirs):
                 if cls in canon:
                     k["canonical_bytes_per_pl"] = round(canon[cls], 3)
         k["sq"] = {c: int(v) for c, v in m.items() if c.startswith("SQ_")}
+        tcc = {c: int(v) for c, v in m.items() if c.startswith("TCC_")}
+        if tcc:
+            k["tcc"] = tcc
         out["kernels"][cls] = k
         rows.append((cls, k))
-    with open(TRAFFIC_FILE, "w") as f:
+    with open(out_file, "w") as f:
         json.dump(out, f, indent=1)
     print(f"calibration: fetch x{out['calibration']['fetch_scale']}, write x{out['calibration']['write_scale']}")
     print("| kernel class | HBM bytes / launch | fetch | write | B/pl measured | B/pl canonical |")
@@ -612,6 +639,21 @@ def pmc_summary(dirs):
             w = q["SQ_WAVES"]
             print(f"| {cls} | {w} | {q.get('SQ_INSTS_VALU', 0) / w:.0f} | {q.get('SQ_INSTS_SALU', 0) / w:.0f} | "
                   f"{q.get('SQ_WAVE_CYCLES', 0) / w:.0f} |")
+
+
+def pmc_steps(step, torch, dev, info):
+    """pmc_run for a step function: one warm-up step, one step, the 256 MiB calibration copy."""
+    step()
+    torch.cuda.synchronize()
+    step()
+    torch.cuda.synchronize()
+    src = torch.ones(64 * 1024 * 1024, device=dev)  # 256 MiB of fp32
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"pmc_run": True, "steps": 2, **info, "calibration_copy_bytes": 2 * src.numel() * 4}), flush=True)
+    return {"pmc_run": True}
 
 
 def pmc_run(plan, torch):
@@ -634,7 +676,7 @@ def pmc_run(plan, torch):
 def main():
     args = parse()
     if args.pmc_summary:
-        pmc_summary(args.pmc_summary)
+        pmc_summary(args.pmc_summary, args.workload)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:

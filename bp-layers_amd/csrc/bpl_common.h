@@ -160,9 +160,10 @@ __device__ __forceinline__ float wave_min_f(float v) {
 }
 // Workgroup b of a launch runs on XCD b mod 8 (the dispatcher deals workgroups to the eight
 // XCDs in turn).  This renumbering gives every XCD a contiguous range of logical indices, so
-// that neighbouring scanlines -- whose 4-byte accesses to per-pixel planes (pairwise
-// parameters, their gradients) share cache lines -- run on one XCD and meet in its L2
-// instead of each XCD reading and writing back its own partial lines.
+// that neighbouring scanlines of a vertical phase, which share cache lines -- 4-byte accesses
+// to per-pixel planes (pairwise parameters, their gradients), and label rows that are not a
+// whole number of lines (19 or 25 labels: 76 or 100 bytes) -- run on one XCD and meet in
+// its L2 instead of each XCD reading and writing back its own partial lines.
 __device__ __forceinline__ int xcd_contiguous_block(int b, int nblocks) {
   constexpr int kXcd = 8;
   const int k = b % kXcd, j = b / kXcd;

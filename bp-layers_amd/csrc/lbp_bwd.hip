@@ -67,8 +67,10 @@ __global__ __launch_bounds__(64) void lbp_bwd_phase(BwdArgs a) {
   const int len = VERT ? H : W;
   const int nsteps = len - 1;
   const int wpg = (L + SPW - 1) / SPW;
-  const int g_ = blockIdx.x / wpg;
-  const int wl = blockIdx.x - g_ * wpg;
+  // columns of a vertical launch go to the XCDs in contiguous ranges (shared cache lines)
+  const int bi = VERT ? xcd_contiguous_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
+  const int g_ = bi / wpg;
+  const int wl = bi - g_ * wpg;
   const int n = a.n_base + (g_ >> 1);
   const int dsel = g_ & 1;
   const int dir = VERT ? (dsel ? UP : DOWN) : (dsel ? LEFT : RIGHT);
@@ -84,7 +86,7 @@ __global__ __launch_bounds__(64) void lbp_bwd_phase(BwdArgs a) {
   const bool exact = a.mode == BPL_GRAD_EXACT;
   float *X = a.X[dir];
   float *part = nullptr;
-  if (KIND == BPL_LBP_SEMANTIC_GLOBAL) part = a.partial + ((int64_t)blockIdx.x * SPW + seg) * C * C;
+  if (KIND == BPL_LBP_SEMANTIC_GLOBAL) part = a.partial + ((int64_t)bi * SPW + seg) * C * C;
 
   auto pos_of = [&](int s) { return fwd ? s : len - 1 - s; };
   auto yx_of = [&](int p, int &y, int &x) {
@@ -392,7 +394,7 @@ __global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(B
   const int wq = PAIR ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) : 0;
   const int wv = wq & 1;  // pass of the row (PAIR)
   const int L2 = (L + 1) / 2;
-  // columns of a vertical launch go to the XCDs in contiguous ranges (per-pixel planes)
+  // columns of a vertical launch go to the XCDs in contiguous ranges (shared cache lines)
   const int bi = VERT ? xcd_contiguous_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
   const int g_ = PAIR ? 2 * (bi / L2) + wv : bi / L;
   const int ln = PAIR ? min(2 * (bi - (g_ >> 1) * L2) + (wq >> 1), L - 1) : bi - g_ * L;
@@ -1082,8 +1084,10 @@ __global__ __launch_bounds__(64) void lbp_bwd_semg(BwdArgs a) {
   const int len = VERT ? H : W;
   const int nsteps = len - 1;
   const int wpg = (L + SPW - 1) / SPW;
-  const int g_ = blockIdx.x / wpg;
-  const int wl = blockIdx.x - g_ * wpg;
+  // columns of a vertical launch go to the XCDs in contiguous ranges (shared cache lines)
+  const int bi = VERT ? xcd_contiguous_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
+  const int g_ = bi / wpg;
+  const int wl = bi - g_ * wpg;
   const int n = a.n_base + (g_ >> 1);
   const int dir = VERT ? ((g_ & 1) ? UP : DOWN) : ((g_ & 1) ? LEFT : RIGHT);
   const int rslot = dir == RIGHT ? LEFT : dir == LEFT ? RIGHT : dir == DOWN ? UP : DOWN;
@@ -1204,7 +1208,7 @@ __global__ __launch_bounds__(64) void lbp_bwd_semg(BwdArgs a) {
   }
   // the segment's partial matrix, in lbp_bwd_reduce_partials' layout
   wave_lds_sync();
-  float *part = a.partial + ((int64_t)blockIdx.x * SPW + seg) * C * C;
+  float *part = a.partial + ((int64_t)bi * SPW + seg) * C * C;
   for (int i = c; i < C * C; i += SEG) {
     const int r = i / C, cc = i - r * C;
     part[i] = lds_part[seg][r * CM + cc];
@@ -1241,8 +1245,10 @@ __global__ __launch_bounds__(64) void lbp_bwd_semp(BwdArgs a) {
   const int seg = lane / SEG;
   const int c = lane - seg * SEG;
   const int wpg = IMGS ? L : (L + SPW - 1) / SPW;
-  const int g_ = blockIdx.x / wpg;
-  const int wl = blockIdx.x - g_ * wpg;
+  // columns of a vertical launch go to the XCDs in contiguous ranges (shared cache lines)
+  const int bi = VERT ? xcd_contiguous_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
+  const int g_ = bi / wpg;
+  const int wl = bi - g_ * wpg;
   const int n = IMGS ? seg : (g_ >> 1);
   const int dir = VERT ? ((g_ & 1) ? UP : DOWN) : ((g_ & 1) ? LEFT : RIGHT);
   const int rslot = dir == RIGHT ? LEFT : dir == LEFT ? RIGHT : dir == DOWN ? UP : DOWN;

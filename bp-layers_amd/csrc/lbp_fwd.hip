@@ -48,8 +48,10 @@ __global__ __launch_bounds__(64) void lbp_fwd_phase(FwdArgs a) {
   const int len = VERT ? H : W;             // senders + 1 per scanline
   const int nsteps = len - 1;
   const int wpg = (L + SPW - 1) / SPW;
-  const int g = blockIdx.x / wpg;
-  const int wl = blockIdx.x - g * wpg;
+  // columns of a vertical launch go to the XCDs in contiguous ranges (shared cache lines)
+  const int bi = VERT ? xcd_contiguous_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
+  const int g = bi / wpg;
+  const int wl = bi - g * wpg;
   const int n = g >> 1;
   const int dsel = g & 1;
   const int dir = VERT ? (dsel ? UP : DOWN) : (dsel ? LEFT : RIGHT);
@@ -289,8 +291,10 @@ __global__ __launch_bounds__(64) void lbp_fwd_semg(FwdArgs a) {
   const int len = VERT ? H : W;
   const int nsteps = len - 1;
   const int wpg = (L + SPW - 1) / SPW;
-  const int g = blockIdx.x / wpg;
-  const int wl = blockIdx.x - g * wpg;
+  // columns of a vertical launch go to the XCDs in contiguous ranges (shared cache lines)
+  const int bi = VERT ? xcd_contiguous_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
+  const int g = bi / wpg;
+  const int wl = bi - g * wpg;
   const int n = g >> 1;
   const int dir = VERT ? ((g & 1) ? UP : DOWN) : ((g & 1) ? LEFT : RIGHT);
   const int rslot = dir == RIGHT ? LEFT : dir == LEFT ? RIGHT : dir == DOWN ? UP : DOWN;
@@ -431,8 +435,10 @@ __global__ __launch_bounds__(64) void lbp_fwd_semp(FwdArgs a) {
   const int seg = lane / SEG;
   const int c = lane - seg * SEG;
   const int wpg = IMGS ? L : (L + SPW - 1) / SPW;
-  const int g = blockIdx.x / wpg;
-  const int wl = blockIdx.x - g * wpg;
+  // columns of a vertical launch go to the XCDs in contiguous ranges (shared cache lines)
+  const int bi = VERT ? xcd_contiguous_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
+  const int g = bi / wpg;
+  const int wl = bi - g * wpg;
   const int n = IMGS ? seg : g >> 1;
   const int dir = VERT ? ((g & 1) ? UP : DOWN) : ((g & 1) ? LEFT : RIGHT);
   const int rslot = dir == RIGHT ? LEFT : dir == LEFT ? RIGHT : dir == DOWN ? UP : DOWN;
@@ -861,7 +867,7 @@ __global__ __launch_bounds__(64) void lbp_fwd_stereo_w64(FwdArgs a) {
   const int L = VERT ? W : H;
   const int len = VERT ? H : W;
   const int nsteps = len - 1;
-  // columns of a vertical launch go to the XCDs in contiguous ranges (per-pixel planes)
+  // columns of a vertical launch go to the XCDs in contiguous ranges (shared cache lines)
   const int bi = VERT ? xcd_contiguous_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
   const int g = bi / L;
   const int ln = bi - g * L;
