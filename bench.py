@@ -111,6 +111,8 @@ class StereoPlan:
             ws = int(nat.lib().bpl_lbp_bwd_workspace_size(nat.LBP_STEREO, N, H, W, C))
             self.ws = torch.empty(ws, dtype=torch.uint8, device=dev)
             self.ws_bytes = ws
+        self.fws_bytes = int(nat.lib().bpl_lbp_fwd_workspace_size(nat.LBP_STEREO, N, H, W, C))
+        self.fws = torch.empty(self.fws_bytes, dtype=torch.uint8, device=dev)
         self.stream = torch.cuda.current_stream(dev)
         self.sp = self.nat.ptr  # shorthand
 
@@ -119,9 +121,11 @@ class StereoPlan:
 
     def fwd(self, t):
         nat, p = self.nat, self.sp
-        nat.call("bpl_lbp_fwd", nat.LBP_STEREO, p(self.cost), p(self.jump), self.K, self.jump.shape[0], p(self.edge),
-                 p(self.msgs[t]), p(self.msgs[t + 1]), p(self.argmin[t]), p(self.garg[t]), p(self.scale[t]),
-                 self.N, self.H, self.W, self.C, self.delta, nat.stream_of(self.cost))
+        # the workspace form: the vertical phase reads the horizontal phase's base plane
+        nat.call("bpl_lbp_fwd_ws", nat.LBP_STEREO, p(self.cost), p(self.jump), self.K, self.jump.shape[0],
+                 p(self.edge), p(self.msgs[t]), p(self.msgs[t + 1]), p(self.argmin[t]), p(self.garg[t]),
+                 p(self.scale[t]), self.N, self.H, self.W, self.C, self.delta, p(self.fws), self.fws_bytes,
+                 nat.stream_of(self.cost))
 
     def bwd(self, t, g_in, g_out):
         nat, p = self.nat, self.sp
@@ -228,6 +232,8 @@ class LBPPlan:
         self.argmin = [torch.empty((N, 4, H, W, C), dtype=torch.uint8, device=dev) for _ in range(iters)]
         self.garg = [torch.empty((N, 4, H, W), dtype=torch.uint8, device=dev) for _ in range(iters)]
         self.scale = [torch.empty((N, 4, H, W), device=dev) for _ in range(iters)]
+        self.fws_bytes = int(nat.lib().bpl_lbp_fwd_workspace_size(kind, N, H, W, C))
+        self.fws = torch.empty(self.fws_bytes, dtype=torch.uint8, device=dev) if self.fws_bytes else None
         if backward:
             g = torch.Generator(device="cpu").manual_seed(5)
             self.G = torch.randn((N, 4, H, W, C), generator=g).to(dev)
@@ -243,9 +249,10 @@ class LBPPlan:
 
     def fwd(self, t):
         nat, p = self.nat, self.sp
-        nat.call("bpl_lbp_fwd", self.kind, p(self.cost), p(self.jump), self.K, self.Nj, p(self.edge),
+        nat.call("bpl_lbp_fwd_ws", self.kind, p(self.cost), p(self.jump), self.K, self.Nj, p(self.edge),
                  p(self.msgs[t]), p(self.msgs[t + 1]), p(self.argmin[t]), p(self.garg[t]), p(self.scale[t]),
-                 self.N, self.H, self.W, self.C, self.delta, nat.stream_of(self.cost))
+                 self.N, self.H, self.W, self.C, self.delta, p(self.fws), self.fws_bytes,
+                 nat.stream_of(self.cost))
 
     def bwd(self, t, g_in, g_out):
         nat, p = self.nat, self.sp

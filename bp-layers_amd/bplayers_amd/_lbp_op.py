@@ -92,9 +92,15 @@ def forward_minsum(kind, cost, jump, edge, messages, delta, out=None):
     argmin = torch.empty((N, 4, H, W, C), dtype=torch.uint8, device=cost.device)
     scale, garg = _alloc_scale_and_gargmin(N, H, W, cost.device)
     delta = max(0, min(int(delta), 65535))
-    nat.call("bpl_lbp_fwd", kind, nat.ptr(cost), nat.ptr(jump), K, Nj, nat.ptr(edge), nat.ptr(messages),
+    # stereo iterations of the fast step's shape run with a workspace plane (bpl_lbp_fwd_ws:
+    # the vertical phase reads (cost + M_R) + M_L as one plane; the same outputs bit for bit)
+    ws = None
+    if kind == nat.LBP_STEREO and K == 6 and delta == 3 and C > 16:
+        ws = torch.empty(int(nat.lib().bpl_lbp_fwd_workspace_size(kind, N, H, W, C)), dtype=torch.uint8,
+                         device=cost.device)
+    nat.call("bpl_lbp_fwd_ws", kind, nat.ptr(cost), nat.ptr(jump), K, Nj, nat.ptr(edge), nat.ptr(messages),
              nat.ptr(msg_out), nat.ptr(argmin), nat.ptr(garg), nat.ptr(scale), N, H, W, C, delta,
-             nat.stream_of(cost))
+             nat.ptr(ws), 0 if ws is None else ws.numel(), nat.stream_of(cost))
     return [msg_out, argmin, scale]
 
 

@@ -38,6 +38,8 @@ _SIGS = {
     "bpl_sad_flow_fwd": ([_P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I, _I, _I, _I, _I, _P], _I),
     "bpl_sad_flow_bwd": ([_P, _P, _P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I, _P], _I),
     "bpl_lbp_fwd": ([_I, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I, _P], _I),
+    "bpl_lbp_fwd_workspace_size": ([_I, _I64, _I64, _I64, _I64], _I64),
+    "bpl_lbp_fwd_ws": ([_I, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I, _P, _I64, _P], _I),
     "bpl_lbp_bwd_workspace_size": ([_I, _I64, _I64, _I64, _I64], _I64),
     "bpl_softmax_labels": ([_P, _P, _I64, _I64, _F, _F, _P], _I),
     "bpl_lbp_beliefs": ([_P, _P, _P, _P, _I64, _I64, _I64, _I64, _F, _P], _I),

@@ -51,7 +51,8 @@ class Request:
 def slot_bytes(shape, C, batch, iters=1):
     H, W, D = shape
     px = H * W
-    return batch * (2 * C * px * 4 + px * D * 4 + 4 * px * D * 4 + 4 * px * 4 + px * 4) + 4 * 6 * px * 4
+    # features, cost, messages, the forward's base-plane workspace, edges, disparities; jump
+    return batch * (2 * C * px * 4 + px * D * 4 + 4 * px * D * 4 + px * D * 4 + 4 * px * 4 + px * 4) + 4 * 6 * px * 4
 
 
 class Slot:
@@ -66,6 +67,7 @@ class Slot:
         self.f1 = torch.empty((B, C, H, W), device=dev)
         self.cost = torch.empty((B, H, W, D), device=dev)
         self.msgs = torch.empty((B, 4, H, W, D), device=dev)
+        self.base = torch.empty((B, H, W, D), device=dev)  # bpl_lbp_fwd_ws workspace
         self.edge = torch.ones((B, 4, H, W), device=dev)
         self.disp = torch.empty((B, H, W), device=dev)
         aff = torch.zeros((1, 4, 3, H, W), device=dev)
@@ -95,8 +97,9 @@ class Slot:
                      -1.0 / self.T_match, -1.0, st)
             self.msgs[:n].zero_()
             for _ in range(self.iters):
-                nat.call("bpl_lbp_fwd", nat.LBP_STEREO, p(self.cost), p(self.jump), 6, 1, p(self.edge), p(self.msgs),
-                         p(self.msgs), None, None, None, n, H, W, D, self.delta, st)
+                nat.call("bpl_lbp_fwd_ws", nat.LBP_STEREO, p(self.cost), p(self.jump), 6, 1, p(self.edge),
+                         p(self.msgs), p(self.msgs), None, None, None, n, H, W, D, self.delta, p(self.base),
+                         self.base.numel() * 4, st)
             nat.call("bpl_lbp_beliefs", p(self.cost), p(self.msgs), None, p(self.disp), n, H, W, D, self.T_bel, st)
             self.event.record(self.stream)
         self.reqs = reqs

@@ -13,6 +13,8 @@
 //     per-segment LDS copy of h, the stereo kernels from neighbouring lanes (DPP shifts);
 //   * the next PF steps' streamed inputs (cost and the two cross messages) are prefetched
 //     into a register ring.
+#include <type_traits>
+
 #include "bpl_common.h"
 
 namespace bpl {
@@ -25,6 +27,9 @@ struct FwdArgs {
   float *scale;
   int N, H, W, C, K, Nj, delta, bits, mat_lds;
   int buf_ok;  // every [H,W,C] plane and the K jump planes of a direction below 2^31 bytes
+  // [N,H,W,C] plane (cost + M_R) + M_L: written by the paired horizontal kernel, read by the
+  // vertical P1/P2 sweep in place of the cost and both horizontal messages (null: not used)
+  float *base;
 };
 
 __device__ __forceinline__ int64_t px(int n, int y, int x, int H, int W) {
@@ -630,9 +635,19 @@ __device__ __forceinline__ float shl1_inf(float v) {  // lane i <- lane i+1, lan
 //   * Row streams are buffer-addressed: one wave-uniform row offset shared by the cost and
 //     both cross-message streams (their planes are per-lane offsets of one resource per
 //     tensor); ring loads past the last sender run off the scanline and are never used.
-template <int LPL, bool VERT, bool FULL, bool OUTS, int PF>
+//   * BV (vertical): the cost and both horizontal messages come as one base plane,
+//     (cost + M_R) + M_L, so h = base + chain is the vertical order's sum bit for bit.
+//   * PAIR (horizontal, both passes of the row in one workgroup): steps from Tb on (after a
+//     workgroup barrier; Tb >= (W - 2) / 2, a multiple of PF) are second visits, whose receiver
+//     the other pass has already reached: they write the receiver's base plane entry from
+//     their own message, the other pass's stored message and the receiver's cost (the next
+//     ring slot).  The caller writes the entries of the row ends and of the pixels both passes
+//     reach before the barrier.
+template <int LPL, bool VERT, bool FULL, bool OUTS, int PF, bool BV = false, bool PAIR = false>
 __device__ __forceinline__ void fwd_sweep_p1p2(const FwdArgs &a, int n, int dir, int rslot, int nj, int64_t q0,
-                                               int64_t dq, int nsteps, int lane, int c0, const bool *valid) {
+                                               int64_t dq, int nsteps, int lane, int c0, const bool *valid,
+                                               int Tb = 0) {
+  static_assert(!(BV && !VERT) && !(PAIR && VERT), "base plane: read by vertical, written by horizontal sweeps");
   constexpr int ND = (3 + LPL - 1) / LPL;
   const int H = a.H, W = a.W, C = a.C, K = a.K;
   const int64_t HW = (int64_t)H * W;
@@ -649,6 +664,8 @@ __device__ __forceinline__ void fwd_sweep_p1p2(const FwdArgs &a, int n, int dir,
   const void *s_base = OUTS ? (const void *)(a.scale + rs_base) : (const void *)a.cost;
   const void *g_base = OUTS ? (const void *)(a.gargmin + rs_base) : (const void *)a.cost;
   const __amdgpu_buffer_rsrc_t a_rs = make_rsrc(a_base, OUTS ? pb / 4u : 0u);
+  const __amdgpu_buffer_rsrc_t b_rs = make_rsrc((BV || PAIR) ? (const void *)(a.base + (int64_t)n * HW * C)
+                                                             : (const void *)a.cost, (BV || PAIR) ? pb : 0u);
   const __amdgpu_buffer_rsrc_t s_rs = make_rsrc(s_base, OUTS ? (uint32_t)(HW * 4) : 0u);
   const __amdgpu_buffer_rsrc_t g_rs = make_rsrc(g_base, OUTS ? (uint32_t)HW : 0u);
   // per-lane byte offsets: the lane's labels within a row, plus the plane of the stream
@@ -671,14 +688,15 @@ __device__ __forceinline__ void fwd_sweep_p1p2(const FwdArgs &a, int n, int dir,
       for (int k = 0; k < LPL; ++k) d[k] = buf_ld(r, vl[k] + plane, soff);
     }
   };
-  auto st_row = [&](uint32_t soff, const float *v) {
+  auto st_plane = [&](__amdgpu_buffer_rsrc_t r, uint32_t plane, uint32_t soff, const float *v) {
     if (FULL) {
-      buf_st_row<LPL>(mo_rs, vl[0] + p_o, soff, v);
+      buf_st_row<LPL>(r, vl[0] + plane, soff, v);
     } else {
 #pragma unroll
-      for (int k = 0; k < LPL; ++k) buf_st(mo_rs, valid[k] ? vl[k] + p_o : kBufDrop, soff, v[k]);
+      for (int k = 0; k < LPL; ++k) buf_st(r, valid[k] ? vl[k] + plane : kBufDrop, soff, v[k]);
     }
   };
+  auto st_row = [&](uint32_t soff, const float *v) { st_plane(mo_rs, p_o, soff, v); };
   auto st_args = [&](uint32_t soff_b, const uint32_t *b) {   // soff_b: byte offset of the row
     if constexpr (FULL && LPL == 3) {
       __builtin_amdgcn_raw_buffer_store_b16((uint16_t)(b[0] | (b[1] << 8)), a_rs, (uint32_t)c0, soff_b, 0);
@@ -713,28 +731,57 @@ __device__ __forceinline__ void fwd_sweep_p1p2(const FwdArgs &a, int n, int dir,
   float pc[PF][LPL], q1[PF][LPL], q2[PF][LPL], pj[PF], pe[PF];
   uint32_t ring_so = so0, ring_sp = (uint32_t)q0 * 4u;   // offsets of the next step to load
   auto load_ring = [&](float *dc, float *d1, float *d2, float &dj, float &de) {
-    ld_row(c_rs, 0u, ring_so, dc);
-    ld_row(x_rs, p_x1, ring_so, d1);
-    ld_row(x_rs, p_x2, ring_so, d2);
+    if constexpr (BV) {
+      ld_row(b_rs, 0u, ring_so, dc);
+    } else {
+      ld_row(c_rs, 0u, ring_so, dc);
+      ld_row(x_rs, p_x1, ring_so, d1);
+      ld_row(x_rs, p_x2, ring_so, d2);
+    }
     dj = buf_ld(j_rs, jv, ring_sp);
     de = buf_ld(e_rs, ev, ring_sp);
     ring_so += drow;
     ring_sp += dpix;
   };
+  // Stores of one step: the vector-memory counter counts them with the loads, and the wait
+  // counts of the loop are the smaller of those of its two entries (the prologue and the back
+  // edge).  A prologue of loads alone would make every wait in the loop stricter than the ring
+  // needs (the loads of a slot then must have returned three steps after their issue instead of
+  // about PF - 1), so the prologue issues as many dropped stores between its slots as a step
+  // issues between two refills.
+  // (A resource of 0 bytes drops every store; distinct offsets keep the stores from being
+  // merged, as overwrites of one location or as one wider access.)
+  constexpr int kArgSt = FULL ? (LPL == 3 ? 2 : LPL == 2 ? 1 : LPL) : LPL;
+  constexpr int kStepSt = (OUTS ? kArgSt + 2 : 0) + (FULL ? 1 : LPL);
+  const __amdgpu_buffer_rsrc_t none_rs = make_rsrc(a.msg_out, 0u);
 #pragma unroll
   for (int i = 0; i < PF; ++i) {
     load_ring(pc[i], q1[i], q2[i], pj[i], pe[i]);
+    if (i + 1 < PF) {
+#pragma unroll
+      for (int d = 0; d < kStepSt; ++d) buf_st(none_rs, (uint32_t)(i * kStepSt + d) * 256u, 0u, 0.f);
+    }
     __builtin_amdgcn_sched_barrier(0);
   }
   float Lprev = 0.f;
   uint32_t rso = so0 + drow, rsp = (uint32_t)q0 * 4u + dpix;   // receiver of step 0
-  auto step = [&](int i) {
+  // PAIR: the other pass's message at the receiver of each ring step (second visits)
+  float po[PAIR ? PF : 1][LPL];
+  const uint32_t p_oth = (uint32_t)(dir == RIGHT ? RIGHT : LEFT) * pb;
+  uint32_t oth_so = 0;
+  auto load_other = [&](float *d) {
+    ld_row(mo_rs, p_oth, oth_so, d);
+    oth_so += drow;
+  };
+  auto step = [&](int i, auto second_tag) {
+    constexpr bool SECOND = PAIR && decltype(second_tag)::value;
     float h[LPL];
 #pragma unroll
     for (int k = 0; k < LPL; ++k) {
       float c = pc[i][k];
       if (VERT) asm("" : "+v"(c) : "v"(chain[k]));  // the vertical order adds the chain last
-      const float v = VERT ? ((c + q1[i][k]) + q2[i][k]) + chain[k] : ((c + chain[k]) + q1[i][k]) + q2[i][k];
+      const float v = BV ? c + chain[k]
+                    : VERT ? ((c + q1[i][k]) + q2[i][k]) + chain[k] : ((c + chain[k]) + q1[i][k]) + q2[i][k];
       h[k] = valid[k] ? v : __builtin_inff();
     }
     float pv = pj[i] + pe[i];
@@ -841,41 +888,89 @@ __device__ __forceinline__ void fwd_sweep_p1p2(const FwdArgs &a, int n, int dir,
     for (int k = 0; k < LPL; ++k) outv[k] = chain[k] - Lc;
     st_row(rso, outv);
     if constexpr (OUTS) buf_st(s_rs, one, rsp, fast_exp(gv + Lc - Lprev));
+    if constexpr (SECOND) {
+      // base entry of the receiver: its cost is the next ring slot's; the sums in the order
+      // (cost + M_R) + M_L of the vertical h (this pass writes the L slot when it runs RIGHT)
+      float bv[LPL];
+#pragma unroll
+      for (int k = 0; k < LPL; ++k) {
+        float cn = pc[(i + 1) % PF][k], ot = po[i][k];
+        asm("" : "+v"(cn), "+v"(ot) : "v"(outv[0]));
+        bv[k] = dir == RIGHT ? (cn + ot) + outv[k] : (cn + outv[k]) + ot;
+      }
+      st_plane(b_rs, 0u, rso, bv);
+    }
     Lprev = Lc;
     rso += drow;
     rsp += dpix;
   };
-  int s0 = 0;
-  for (; s0 + PF <= nsteps; s0 += PF) {
+  auto group = [&](int i0, auto second_tag) {
 #pragma unroll
     for (int i = 0; i < PF; ++i) {
-      step(i);
+      step(i, second_tag);
       load_ring(pc[i], q1[i], q2[i], pj[i], pe[i]);
+      if constexpr (decltype(second_tag)::value) load_other(po[i]);
+    }
+  };
+  int s0 = 0;
+  if constexpr (PAIR) {
+    if (Tb + PF <= nsteps - 1) {
+      for (; s0 < Tb; s0 += PF) group(0, std::false_type());
+      __syncthreads();  // the other pass's first visits are written
+      oth_so = (uint32_t)(q0 + (int64_t)(Tb + 1) * dq) * rowb;
+#pragma unroll
+      for (int i = 0; i < PF; ++i) load_other(po[i]);
+      for (; s0 + PF <= nsteps - 1; s0 += PF) group(0, std::true_type());
+#pragma unroll
+      for (int i = 0; i < PF; ++i) {
+        if (s0 + i < nsteps - 1) step(i, std::true_type());
+        else if (s0 + i == nsteps - 1) step(i, std::false_type());
+      }
+      return;
     }
   }
+  for (; s0 + PF <= nsteps; s0 += PF) group(0, std::false_type());
 #pragma unroll
   for (int i = 0; i < PF; ++i)
-    if (s0 + i < nsteps) step(i);
+    if (s0 + i < nsteps) step(i, std::false_type());
 }
 
-template <int LPL, bool VERT, bool FULL, int FASTK, bool OUTS, int PF>
-__global__ __launch_bounds__(64) void lbp_fwd_stereo_w64(FwdArgs a) {
+// P1/P2 form of scanline `ln` of pass `dir` (wave-uniform): zero window offsets, J[1] == J[5]
+// (the truncation value) and J[3] == 0 at every pixel, and no log-sum-exp term that can
+// underflow, (JK - J[j]) w <= 80; the jump length is 6.
+template <bool VERT>
+__device__ __forceinline__ bool p1p2_scanline(const FwdArgs &a, int n, int dir, int ln, int lane) {
+  const int H = a.H, W = a.W, K = a.K;
+  const int len = VERT ? H : W;
+  const int64_t HW = (int64_t)H * W;
+  const int nj = a.Nj > 1 ? n : 0;
+  const float *o_p = a.jump + ((int64_t)nj * 4 + dir) * K * HW;
+  const float *e_p = a.edge + ((int64_t)n * 4 + dir) * HW;
+  bool form = true, big = false;
+  for (int p = lane; p < len; p += kWave) {
+    const int64_t q = VERT ? (int64_t)p * W + ln : (int64_t)ln * W + p;
+    const float j0 = o_p[q], j1 = o_p[HW + q], j2 = o_p[2 * HW + q], j3 = o_p[3 * HW + q];
+    const float j4 = o_p[4 * HW + q], j5 = o_p[5 * HW + q], w = e_p[q];
+    form = form && j0 == 0.f && j1 == j5 && j3 == 0.f;
+    big = big || (j5 - j2) * w > 80.f || (j5 - j4) * w > 80.f || j5 * w > 80.f;
+  }
+  return a.buf_ok && __ballot(!form) == 0 && __ballot(big) == 0;
+}
+
+// One scanline (pass `dir`, row or column `ln` of image n) of the stereo kernels: the P1/P2
+// sweep, the fast step or the general step.  `lds_h` is this wave's own LDS window copy.
+// BV: a vertical P1/P2 sweep reads the base plane a.base (the other paths read the cost and
+// both horizontal messages; the two give the same h bit for bit).
+template <int LPL, bool VERT, bool FULL, int FASTK, bool OUTS, int PF, bool BV = false>
+__device__ __forceinline__ void fwd_scanline(const FwdArgs &a, int n, int dir, int ln, float (&lds_h)[2][kWave * LPL]) {
   constexpr int ND = (3 + LPL - 1) / LPL;  // neighbour lanes reached by a +-3 window
-  __shared__ float lds_h[2][kWave * LPL];
 
   const int H = a.H, W = a.W, C = a.C, K = a.K;
-  const int L = VERT ? W : H;
   const int len = VERT ? H : W;
   const int nsteps = len - 1;
-  // columns of a vertical launch go to the XCDs in contiguous ranges (shared cache lines)
-  const int bi = VERT ? xcd_contiguous_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
-  const int g = bi / L;
-  const int ln = bi - g * L;
-  const int n = g >> 1;
-  const int dir = VERT ? ((g & 1) ? UP : DOWN) : ((g & 1) ? LEFT : RIGHT);
   const int rslot = dir == RIGHT ? LEFT : dir == LEFT ? RIGHT : dir == DOWN ? UP : DOWN;
   const bool fwd = (dir == RIGHT || dir == DOWN);
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
   const int c0 = lane * LPL;
   const int delta = a.delta;
   const int nj = a.Nj > 1 ? n : 0;
@@ -914,19 +1009,8 @@ __global__ __launch_bounds__(64) void lbp_fwd_stereo_w64(FwdArgs a) {
     bool off = false;  // any non-zero window offset on this scanline (jump entry 0)
     const float *o_p = a.jump + ((int64_t)nj * 4 + dir) * K * HW;
     if constexpr (FASTK == 6) {
-      // P1/P2 form: besides zero offsets, J[1] == J[5] (the truncation value) and J[3] == 0
-      // at every pixel, and no log-sum-exp term can underflow, (JK - J[j]) w <= 80
-      const float *e_p = a.edge + ((int64_t)n * 4 + dir) * HW;
-      bool form = true, big = false;
-      for (int p = lane; p < len; p += kWave) {
-        const int64_t q = VERT ? (int64_t)p * W + ln : (int64_t)ln * W + p;
-        const float j0 = o_p[q], j1 = o_p[HW + q], j2 = o_p[2 * HW + q], j3 = o_p[3 * HW + q];
-        const float j4 = o_p[4 * HW + q], j5 = o_p[5 * HW + q], w = e_p[q];
-        form = form && j0 == 0.f && j1 == j5 && j3 == 0.f;
-        big = big || (j5 - j2) * w > 80.f || (j5 - j4) * w > 80.f || j5 * w > 80.f;
-      }
-      if (a.buf_ok && __ballot(!form) == 0 && __ballot(big) == 0) {
-        fwd_sweep_p1p2<LPL, VERT, FULL, OUTS, PF>(a, n, dir, rslot, nj, q0, dq, nsteps, lane, c0, valid);
+      if (p1p2_scanline<VERT>(a, n, dir, ln, lane)) {
+        fwd_sweep_p1p2<LPL, VERT, FULL, OUTS, PF, BV>(a, n, dir, rslot, nj, q0, dq, nsteps, lane, c0, valid);
         return;
       }
     }
@@ -1219,7 +1303,7 @@ __global__ __launch_bounds__(64) void lbp_fwd_stereo_w64(FwdArgs a) {
           const int buf = s & 1;
 #pragma unroll
           for (int k = 0; k < LPL; ++k) lds_h[buf][c0 + k] = h[k];
-          __syncthreads();
+          wave_lds_sync();  // this wave's own window copy
           const int k_lo = (int)floorf(o - 0.5f) - delta;
           const int k_hi = (int)ceilf(o + 0.5f) + delta;
           int st[LPL], sp[LPL];
@@ -1283,6 +1367,87 @@ __global__ __launch_bounds__(64) void lbp_fwd_stereo_w64(FwdArgs a) {
   }
 }
 
+template <int LPL, bool VERT, bool FULL, int FASTK, bool OUTS, int PF, bool BV = false>
+__global__ __launch_bounds__(64) void lbp_fwd_stereo_w64(FwdArgs a) {
+  __shared__ float lds_h[2][kWave * LPL];
+  const int L = VERT ? a.W : a.H;
+  // columns of a vertical launch go to the XCDs in contiguous ranges (shared cache lines)
+  const int bi = VERT ? xcd_contiguous_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
+  const int g = bi / L;
+  const int dir = VERT ? ((g & 1) ? UP : DOWN) : ((g & 1) ? LEFT : RIGHT);
+  fwd_scanline<LPL, VERT, FULL, FASTK, OUTS, PF, BV>(a, g >> 1, dir, bi - g * L, lds_h);
+}
+
+// Horizontal phase that also writes the base plane (cost + M_R) + M_L for the vertical phase
+// (FwdArgs::base), with the fast step's launch conditions (delta 3, jump length 6).  A
+// workgroup holds two rows of one image, four waves (the RIGHT and LEFT passes of each row; one
+// wave per SIMD); with an odd row count the last workgroup's second row repeats row H - 1,
+// whose stores then write the same values twice.  When all four scanlines have the P1/P2 form
+// the passes run paired (the second visitor of a pixel writes its base entry, see
+// fwd_sweep_p1p2); otherwise each wave runs its scanline alone.  After a final barrier the
+// base entries not yet written (every entry when unpaired) are formed from the stored
+// messages; the two waves of a row take alternate pixels.
+template <int LPL, bool FULL, bool OUTS, int PF>
+__global__ __launch_bounds__(4 * kWave) void lbp_fwd_stereo_pair(FwdArgs a) {
+  __shared__ float lds_h_all[4][2][kWave * LPL];
+  __shared__ int ok_all[4];
+  const int wq = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & (kWave - 1);
+  const int H = a.H, W = a.W, C = a.C;
+  const int H2 = (H + 1) / 2;
+  const int n = (int)blockIdx.x / H2;
+  const int y = min(2 * ((int)blockIdx.x - n * H2) + (wq >> 1), H - 1);
+  const int dir = (wq & 1) ? LEFT : RIGHT;
+  const int nsteps = W - 1;
+  const bool ok = p1p2_scanline<false>(a, n, dir, y, lane);
+  if (lane == 0) ok_all[wq] = ok ? 1 : 0;
+  __syncthreads();
+  const bool paired = ok_all[0] && ok_all[1] && ok_all[2] && ok_all[3];
+  // second visits from step Tb on: the other pass reaches pixel x at step W - 2 - x, so every
+  // receiver of a step >= Tb >= (W - 2) / 2 was reached by it before the barrier
+  const int Tb = PF * (((W - 1) / 2 + PF - 1) / PF);
+  int fix_lo = 0, fix_hi = W - 1;
+  if (paired) {
+    const int c0 = lane * LPL;
+    bool valid[LPL];
+#pragma unroll
+    for (int k = 0; k < LPL; ++k) valid[k] = FULL || c0 + k < C;
+    const int rslot = dir == RIGHT ? LEFT : RIGHT;
+    const int64_t dq = dir == RIGHT ? 1 : -1;
+    const int64_t q0 = (int64_t)y * W + (dir == RIGHT ? 0 : W - 1);
+    fwd_sweep_p1p2<LPL, false, FULL, OUTS, PF, false, true>(a, n, dir, rslot, a.Nj > 1 ? n : 0, q0, dq, nsteps,
+                                                            lane, c0, valid, Tb);
+    if (Tb + PF <= nsteps - 1) {
+      fix_lo = W - 1 - Tb;
+      fix_hi = Tb;
+    }
+  } else {
+    fwd_scanline<LPL, false, FULL, 6, OUTS, PF>(a, n, dir, y, lds_h_all[wq]);
+  }
+  __syncthreads();  // every message of both passes is written
+  const int64_t HW = (int64_t)H * W, HWC = HW * C;
+  const float *cp = a.cost + (int64_t)n * HWC;
+  const float *mr = a.msg_out + ((int64_t)n * 4 + RIGHT) * HWC;
+  const float *ml = a.msg_out + ((int64_t)n * 4 + LEFT) * HWC;
+  float *bp = a.base + (int64_t)n * HWC;
+  auto fix = [&](int x) {
+    const int64_t o = ((int64_t)y * W + x) * C;
+#pragma unroll
+    for (int k = 0; k < LPL; ++k) {
+      const int c = lane * LPL + k;
+      if (FULL || c < C) bp[o + c] = (cp[o + c] + mr[o + c]) + ml[o + c];
+    }
+  };
+  const int wv = wq & 1;
+  int idx = 0;
+  auto visit = [&](int x) {
+    if ((idx++ & 1) == wv) fix(x);
+  };
+  if (fix_lo > 0) visit(0);
+  for (int x = fix_lo; x <= fix_hi; ++x) visit(x);
+  if (fix_hi < W - 1) visit(W - 1);
+}
+
 // prefetch depth (steps) of the stereo kernels' register ring, per phase
 #ifndef BPL_FWD_PF_H
 #define BPL_FWD_PF_H 6
@@ -1298,13 +1463,28 @@ void launch_stereo_w64_k(const FwdArgs &a, bool vert, hipStream_t st) {
   else
     hipLaunchKernelGGL((lbp_fwd_stereo_w64<LPL, false, FULL, FASTK, OUTS, BPL_FWD_PF_H>), grid, dim3(kWave), 0, st, a);
 }
+// base-plane launches (a.base set by the host only with the fast step's conditions, K = 6):
+// the horizontal phase writes the plane, the vertical P1/P2 sweeps read it
+template <int LPL, bool FULL, bool OUTS>
+void launch_stereo_base(const FwdArgs &a, bool vert, hipStream_t st) {
+  if (vert) {
+    const dim3 grid((unsigned)(a.N * 2 * a.W));
+    hipLaunchKernelGGL((lbp_fwd_stereo_w64<LPL, true, FULL, 6, OUTS, BPL_FWD_PF_V, true>), grid, dim3(kWave), 0, st, a);
+  } else {
+    const dim3 grid((unsigned)(a.N * ((a.H + 1) / 2)));
+    hipLaunchKernelGGL((lbp_fwd_stereo_pair<LPL, FULL, OUTS, BPL_FWD_PF_H>), grid, dim3(4 * kWave), 0, st, a);
+  }
+}
 template <int LPL, bool FULL>
 void launch_stereo_w64(const FwdArgs &a, bool vert, hipStream_t st) {
   // fast step: delta 3, the jump lengths of the stereo packings (K_in = 3 -> 6, K_in = 5 ->
   // 8), and argmins, labels and scales all requested (training) or none (inference)
   const bool all = a.argmin && a.gargmin && a.scale, none = !a.argmin && !a.gargmin && !a.scale;
   const bool fast = a.delta == 3 && (all || none);
-  if (fast && a.K == 6) {
+  if (fast && a.K == 6 && a.base) {
+    if (all) launch_stereo_base<LPL, FULL, true>(a, vert, st);
+    else launch_stereo_base<LPL, FULL, false>(a, vert, st);
+  } else if (fast && a.K == 6) {
     if (all) launch_stereo_w64_k<LPL, FULL, 6, true>(a, vert, st);
     else launch_stereo_w64_k<LPL, FULL, 6, false>(a, vert, st);
   } else if (fast && a.K == 8) {
@@ -1362,10 +1542,17 @@ void launch_kind(const FwdArgs &a, bool vert, hipStream_t st, size_t shm) {
 
 using namespace bpl;
 
-extern "C" int bpl_lbp_fwd(int kind, const float *cost, const float *jump, int64_t jump_k, int64_t jump_n,
-                           const float *edge, const float *msg_in, float *msg_out,
-                           uint8_t *argmin, uint8_t *gargmin, float *scale,
-                           int64_t N, int64_t H, int64_t W, int64_t C, int delta, void *stream) {
+// BPL_LBP_NO_BASE=1 turns the base plane of bpl_lbp_fwd_ws off (the plain path), for comparisons.
+static bool base_off() {
+  const char *e = getenv("BPL_LBP_NO_BASE");
+  return e && e[0] == '1';
+}
+
+static int lbp_fwd_impl(int kind, const float *cost, const float *jump, int64_t jump_k, int64_t jump_n,
+                        const float *edge, const float *msg_in, float *msg_out,
+                        uint8_t *argmin, uint8_t *gargmin, float *scale,
+                        int64_t N, int64_t H, int64_t W, int64_t C, int delta, void *ws, int64_t ws_bytes,
+                        void *stream) {
   BPL_REQUIRE(kind >= 0 && kind <= 2, BPL_E_ARG, "bpl_lbp_fwd: unknown kind %d", kind);
   BPL_REQUIRE(N >= 1 && H >= 1 && W >= 1 && C >= 1, BPL_E_ARG,
               "bpl_lbp_fwd: sizes must be positive (N=%lld H=%lld W=%lld C=%lld)",
@@ -1393,6 +1580,12 @@ extern "C" int bpl_lbp_fwd(int kind, const float *cost, const float *jump, int64
   // the P1/P2 sweep addresses an image's four message planes through one 32-bit offset range
   a.buf_ok = 4 * H * W * C * 4 < (1LL << 31) && jump_k * H * W * 4 < (1LL << 31) && !p1p2_off();
   a.mat_lds = (kind == BPL_LBP_SEMANTIC_GLOBAL && C <= 64) ? 1 : 0;
+  // the base plane: stereo fast-step launches whose planes fit the buffer addressing, with a
+  // workspace of one [N,H,W,C] plane that overlaps no tensor the call reads or writes
+  a.base = nullptr;
+  if (ws && kind == BPL_LBP_STEREO && jump_k == 6 && delta == 3 && stereo_w64_ok(a) && a.buf_ok &&
+      !generic_only() && !base_off() && ws_bytes >= N * H * W * C * (int64_t)sizeof(float))
+    a.base = static_cast<float *>(ws);
   const size_t shm = a.mat_lds ? (size_t)C * C * sizeof(float) : 0;
   hipStream_t st = as_stream(stream);
   for (int phase = 0; phase < 2; ++phase) {
@@ -1418,4 +1611,41 @@ extern "C" int bpl_lbp_fwd(int kind, const float *cost, const float *jump, int64
     if (rc) return rc;
   }
   return BPL_OK;
+}
+
+extern "C" int bpl_lbp_fwd(int kind, const float *cost, const float *jump, int64_t jump_k, int64_t jump_n,
+                           const float *edge, const float *msg_in, float *msg_out,
+                           uint8_t *argmin, uint8_t *gargmin, float *scale,
+                           int64_t N, int64_t H, int64_t W, int64_t C, int delta, void *stream) {
+  return lbp_fwd_impl(kind, cost, jump, jump_k, jump_n, edge, msg_in, msg_out, argmin, gargmin, scale, N, H, W, C,
+                      delta, nullptr, 0, stream);
+}
+
+extern "C" int64_t bpl_lbp_fwd_workspace_size(int kind, int64_t N, int64_t H, int64_t W, int64_t C) {
+  if (kind != BPL_LBP_STEREO || N < 1 || H < 1 || W < 1 || C < 1) return 0;
+  return N * H * W * C * (int64_t)sizeof(float);
+}
+
+extern "C" int bpl_lbp_fwd_ws(int kind, const float *cost, const float *jump, int64_t jump_k, int64_t jump_n,
+                              const float *edge, const float *msg_in, float *msg_out,
+                              uint8_t *argmin, uint8_t *gargmin, float *scale,
+                              int64_t N, int64_t H, int64_t W, int64_t C, int delta, void *workspace,
+                              int64_t workspace_bytes, void *stream) {
+  BPL_REQUIRE(workspace_bytes >= 0, BPL_E_ARG, "bpl_lbp_fwd_ws: negative workspace size");
+  if (workspace && N >= 1 && H >= 1 && W >= 1 && C >= 1) {
+    // the workspace is written while the cost and both message tensors are read
+    const auto apart = [](const void *p, int64_t pb, const void *q, int64_t qb) {
+      const uintptr_t a0 = (uintptr_t)p, b0 = (uintptr_t)q;
+      return !q || a0 + (uintptr_t)pb <= b0 || b0 + (uintptr_t)qb <= a0;
+    };
+    const int64_t plane = N * H * W * C * (int64_t)sizeof(float);
+    BPL_REQUIRE(apart(workspace, workspace_bytes, cost, plane) && apart(workspace, workspace_bytes, msg_in, 4 * plane) &&
+                    apart(workspace, workspace_bytes, msg_out, 4 * plane),
+                BPL_E_ARG, "bpl_lbp_fwd_ws: the workspace overlaps the cost or a message tensor");
+  }
+  BPL_REQUIRE(!workspace || workspace_bytes >= bpl_lbp_fwd_workspace_size(kind, N, H, W, C), BPL_E_ARG,
+              "bpl_lbp_fwd_ws: workspace of %lld bytes, %lld needed", (long long)workspace_bytes,
+              (long long)bpl_lbp_fwd_workspace_size(kind, N, H, W, C));
+  return lbp_fwd_impl(kind, cost, jump, jump_k, jump_n, edge, msg_in, msg_out, argmin, gargmin, scale, N, H, W, C,
+                      delta, workspace, workspace_bytes, stream);
 }

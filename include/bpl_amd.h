@@ -170,6 +170,22 @@ int bpl_lbp_fwd(int kind, const float *cost, const float *jump, int64_t jump_k, 
                 uint8_t *argmin, uint8_t *gargmin, float *scale,
                 int64_t N, int64_t H, int64_t W, int64_t C, int delta, void *stream);
 
+/*
+ * bpl_lbp_fwd with a workspace; no counterpart in the reference (a faster form of the same
+ * iteration, for its callers that can provide device memory).  For the stereo kind with
+ * jump_k == 6 and delta == 3 the horizontal phase also writes the plane
+ * (cost + M_R) + M_L into the workspace and the vertical phase reads it in place of the cost
+ * and both horizontal messages (16 bytes per pixel-label fewer reads): outputs are bit for bit
+ * those of bpl_lbp_fwd.  Other calls, or a NULL workspace, run bpl_lbp_fwd.  The workspace
+ * must hold bpl_lbp_fwd_workspace_size bytes and overlap no tensor of the call.
+ */
+int64_t bpl_lbp_fwd_workspace_size(int kind, int64_t N, int64_t H, int64_t W, int64_t C);
+int bpl_lbp_fwd_ws(int kind, const float *cost, const float *jump, int64_t jump_k, int64_t jump_n,
+                   const float *edge, const float *msg_in, float *msg_out,
+                   uint8_t *argmin, uint8_t *gargmin, float *scale,
+                   int64_t N, int64_t H, int64_t W, int64_t C, int delta, void *workspace,
+                   int64_t workspace_bytes, void *stream);
+
 /* Workspace bytes needed by bpl_lbp_bwd for these sizes. */
 int64_t bpl_lbp_bwd_workspace_size(int kind, int64_t N, int64_t H, int64_t W, int64_t C);
 

@@ -180,3 +180,77 @@ def test_lbp_forward_messages_only(C, offset):
              nat.ptr(out), None, None, None, N, H, W, C, 3, nat.stream_of(c_))
     torch.cuda.synchronize()
     assert torch.equal(out, full)
+
+
+BASE_CASES = [  # (N, H, W, C, shared jump, argmins, in place, break): bpl_lbp_fwd_ws vs bpl_lbp_fwd
+    (1, 5, 40, 64, False, True, False, None),        # one label per lane, second visits present
+    (2, 3, 41, 100, True, True, True, None),         # partial lanes, odd row count, shared jump, in place
+    (1, 4, 9, 192, False, True, False, None),        # rows too short for second visits
+    (1, 6, 57, 255, False, False, False, None),      # four labels per lane, messages only
+    (2, 5, 40, 192, False, True, False, "offset"),   # a pixel with a window offset: unpaired rows
+    (1, 4, 40, 64, False, True, False, "big"),       # a scanline whose truncation term can underflow
+    (1, 8, 128, 192, False, True, False, None),      # several ring groups of second visits
+]
+
+
+@pytest.mark.parametrize("case", BASE_CASES, ids=[f"{c[0]}x{c[1]}x{c[2]}x{c[3]}-{c[7]}" for c in BASE_CASES])
+def test_lbp_forward_base_plane_bitwise(case):
+    """The workspace form (horizontal phase writes (cost + M_R) + M_L, vertical P1/P2 sweeps read
+    it) gives the outputs of bpl_lbp_fwd bit for bit, paired and unpaired rows alike."""
+    from bplayers_amd import _native as nat
+    from bplayers_amd._lbp_op import _alloc_scale_and_gargmin
+    N, H, W, C, shared, outs, inplace, brk = case
+    print(f"base plane case {case}", flush=True)
+    cost, jump, edge, M0 = make_case("stereo", N, H, W, C, 3, seed=W + C, shared_jump=shared)
+    if brk == "offset":
+        jump[0, RIGHT, 0, 2, 17] = 1.0
+    elif brk == "big":
+        edge[0, LEFT, 1, :] = 1.0
+        edge[0, LEFT, 1, 20] = 200.0
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.from_numpy(a).to(dev)
+    c_, j_, e_ = t(cost), t(jump), t(edge)
+    K, Nj = j_.shape[2], j_.shape[0]
+    res = []
+    for use_ws in (False, True):
+        m_in = t(M0)
+        out = m_in if inplace else torch.empty_like(m_in)
+        am = torch.empty((N, 4, H, W, C), dtype=torch.uint8, device=dev) if outs else None
+        sc, ga = _alloc_scale_and_gargmin(N, H, W, dev) if outs else (None, None)
+        wsb = int(nat.lib().bpl_lbp_fwd_workspace_size(nat.LBP_STEREO, N, H, W, C))
+        assert wsb == N * H * W * C * 4
+        ws = torch.full((wsb // 4,), float("nan"), device=dev) if use_ws else None
+        args = [nat.LBP_STEREO, nat.ptr(c_), nat.ptr(j_), K, Nj, nat.ptr(e_), nat.ptr(m_in), nat.ptr(out),
+                nat.ptr(am), nat.ptr(ga), nat.ptr(sc), N, H, W, C, 3]
+        if use_ws:
+            nat.call("bpl_lbp_fwd_ws", *args, nat.ptr(ws), wsb, nat.stream_of(c_))
+        else:
+            nat.call("bpl_lbp_fwd", *args, nat.stream_of(c_))
+        torch.cuda.synchronize()
+        res.append((out.clone(), am, sc, ga, ws))
+    (m0, a0, s0, g0, _), (m1, a1, s1, g1, ws) = res
+    assert torch.equal(m0, m1), "messages differ"
+    if outs:
+        assert torch.equal(a0, a1) and torch.equal(s0, s1) and torch.equal(g0, g1), "argmins or scales differ"
+    # every base entry was written: (cost + M_R) + M_L of the output messages
+    base = ws.view(N, H, W, C)
+    ref = (c_ + m1[:, RIGHT]) + m1[:, LEFT]
+    assert torch.equal(base, ref), "base plane differs from (cost + M_R) + M_L"
+
+
+def test_lbp_forward_ws_rejects_overlap():
+    from bplayers_amd import _native as nat
+    N, H, W, C = 1, 3, 8, 64
+    dev = torch.device("cuda:0")
+    cost = torch.zeros((N, H, W, C), device=dev)
+    m = torch.zeros((N, 4, H, W, C), device=dev)
+    jump = torch.zeros((N, 4, 6, H, W), device=dev)
+    edge = torch.ones((N, 4, H, W), device=dev)
+    wsb = N * H * W * C * 4
+    with pytest.raises(nat.BplError, match="overlaps"):
+        nat.call("bpl_lbp_fwd_ws", nat.LBP_STEREO, nat.ptr(cost), nat.ptr(jump), 6, 1, nat.ptr(edge), nat.ptr(m),
+                 nat.ptr(m), None, None, None, N, H, W, C, 3, nat.ptr(m), wsb, nat.stream_of(cost))
+    with pytest.raises(nat.BplError, match="needed"):
+        nat.call("bpl_lbp_fwd_ws", nat.LBP_STEREO, nat.ptr(cost), nat.ptr(jump), 6, 1, nat.ptr(edge), nat.ptr(m),
+                 nat.ptr(m), None, None, None, N, H, W, C, 3, nat.ptr(torch.empty(4, device=dev)), 16,
+                 nat.stream_of(cost))
