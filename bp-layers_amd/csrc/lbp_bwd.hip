@@ -636,10 +636,22 @@ __global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(B
         constexpr bool BATCH = !VERT && decltype(batch_tag)::value;
         const int s = nsteps - 1 - t;
         const int64_t q = q0 + (int64_t)s * dq;
+        // Uses of ring slot i wait for the previous step (no instruction; see the forward's
+        // fast step): the work on a slot that does not depend on the chain (argmin bytes,
+        // exp of the messages, X_D + X_U) would otherwise be hoisted to the top of the loop
+        // body for every slot of the group, with waits for all of the group's loads there.
+        // The argmin bytes, messages and vertical terms are ordered after the previous step's
+        // h-gradient before routing, so that they overlap that step's routing.
+#pragma unroll
+        for (int r = 0; r < NRAW; ++r) asm("" : "+v"(pa[i][r]) : "v"(gvprev[0]));
+#pragma unroll
+        for (int k = 0; k < LPL; ++k) {
+          asm("" : "+v"(pm[i][k]) : "v"(gvprev[0]));
+          if constexpr (!VERT) asm("" : "+v"(pd[i][k]), "+v"(pu[i][k]) : "v"(gvprev[0]));
+        }
         int A[LPL];
 #pragma unroll
         for (int k = 0; k < LPL; ++k) A[k] = arg_of(pa[i], k);
-        // uses of ring slot i wait for the previous step (see the forward's fast step)
         float pv = pp[i];
         asm("" : "+v"(pv) : "v"(acc[0]));
         int gl = pl[i];
@@ -756,7 +768,9 @@ __global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(B
           for (int k = 0; k < LPL; ++k) {
             const float xr = isR ? gh[k] : xo[i][k];
             const float xl = isR ? xo[i][k] : gh[k];
-            const float c = ((pd[ni][k] + pu[ni][k]) + xr) + xl;
+            float xdn = pd[ni][k], xun = pu[ni][k];   // ordered after this step's h-gradient
+            asm("" : "+v"(xdn), "+v"(xun) : "v"(gv[0]));
+            const float c = ((xdn + xun) + xr) + xl;
             cst[k] = accg ? gco[i][k] + c : c;
             hr[k] = xr + xl;
           }
