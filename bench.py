@@ -551,6 +551,8 @@ def kernel_class(name):
                       ("beliefs_k", "lbp_beliefs")):
         if stem in base:
             return cls
+    if "lbp_fwd_stereo_pair" in base:  # the paired horizontal forward (base-plane launches)
+        return "lbp_fwd_horizontal"
     m = re.search(r"lbp_(fwd|bwd)_(stereo_w64|semg|semp|phase)<([^>]*)>", base)
     if not m:
         return None
