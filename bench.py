@@ -46,7 +46,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="stereo_fb_384x1248_d192",
                     choices=["stereo_fb_384x1248_d192", "stereo_fwd_128x256_d64", "semantic_fb_2x512x1024_c19",
-                             "semantic_pixel_fb_2x512x1024_c19", "flow_fb_256x512_r12", "serving_c4"])
+                             "semantic_pixel_fb_2x512x1024_c19", "flow_fb_256x512_r12", "serving_c4",
+                             "serving_batch"])
     ap.add_argument("--requests", type=int, default=96, help="serving_c4: requests per measurement")
     ap.add_argument("--rows", type=int, default=0, help="crop the image to this many rows (debug)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -486,6 +487,60 @@ def bench_serving(args, torch):
     return line
 
 
+def bench_serving_batch(args, torch):
+    """serving_batch: GPU time per frame of one serving slot (SAD + softmax, 1 LBP iteration,
+    beliefs; C = 64 features, forward only, messages without argmins) at batch 1, 2 and 4 for
+    each shape class: ``args.steps`` back-to-back launches timed with events, and the library's
+    per-kernel times.  With --pmc: one launch of the 384x1248 D192 class at batch 1 and one at
+    batch 4 (for rocprofv3 counter passes; dispatches of the two differ in grid size)."""
+    from bplayers_amd import workloads
+    from bplayers_amd import _native as nat
+    from bplayers_amd.serving import Slot, Request
+    dev = torch.device("cuda:0")
+    shapes = list(workloads.SERVING_SHAPES)
+    if args.pmc:
+        shapes = [(384, 1248, 192)]
+    out = {}
+    for i, shp in enumerate(shapes):
+        r = workloads.serving_request(40 + i, workloads.SERVING_SHAPES.index(shp))
+        f0, f1 = torch.from_numpy(r["f0"]).to(dev), torch.from_numpy(r["f1"]).to(dev)
+        slot = Slot(shp, 64, 4, 0.1, 0.5, 3, 1, 1.0, 1.0, dev)
+        res = {}
+        for b in ((1, 4) if args.pmc else (1, 2, 4)):
+            reqs = [Request(j, shp, f0, f1) for j in range(b)]
+            slot.launch(reqs)
+            slot.stream.synchronize()
+            slot.reqs = None
+            if args.pmc:
+                continue
+            nat.timing_reset()
+            nat.timing_enable(True)
+            st, en = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            st.record(slot.stream)
+            for _ in range(args.steps):
+                slot.launch(reqs)
+                slot.reqs = None
+            en.record(slot.stream)
+            slot.stream.synchronize()
+            nat.timing_enable(False)
+            kt = nat.timing_read()
+            res[f"batch_{b}"] = {"ms_per_frame": round(st.elapsed_time(en) / args.steps / b, 4),
+                                 "kernel_ms_per_frame": {k: round(v[0] / v[1] / b, 4) for k, v in kt.items()}}
+        out["x".join(str(v) for v in shp)] = res
+        del slot
+        torch.cuda.empty_cache()
+    if args.pmc:
+        print(json.dumps({"pmc_run": True, "workload": "serving_batch", "shape": [384, 1248, 192],
+                          "batches": [1, 4]}), flush=True)
+        return {"pmc_run": True}
+    line = common_line(args, "ms per frame by batch size (one serving slot)", out["384x1248x192"]["batch_4"]["ms_per_frame"],
+                       "ms/frame", out["384x1248x192"]["batch_4"]["ms_per_frame"],
+                       {"workload": args.workload, "feature_channels": 64, "iters": 1, "launches": args.steps},
+                       higher=False)
+    line["by_shape"] = out
+    return line
+
+
 def other_workloads(args, torch, benches):
     """Compact results of the semantic, flow and serving workloads for the default line (a few
     steps each; every workload also runs on its own with --workload)."""
@@ -697,7 +752,7 @@ def main():
 
     others = {"semantic_fb_2x512x1024_c19": bench_semantic, "semantic_pixel_fb_2x512x1024_c19": bench_semantic_pixel,
               "flow_fb_256x512_r12": bench_flow,
-              "serving_c4": bench_serving}
+              "serving_c4": bench_serving, "serving_batch": bench_serving_batch}
     if args.workload in others:
         print(json.dumps(others[args.workload](args, torch)), flush=True)
         return
