@@ -126,7 +126,7 @@ class StereoPlan:
         nat.call("bpl_lbp_fwd_ws", nat.LBP_STEREO, p(self.cost), p(self.jump), self.K, self.jump.shape[0],
                  p(self.edge), p(self.msgs[t]), p(self.msgs[t + 1]), p(self.argmin[t]), p(self.garg[t]),
                  p(self.scale[t]), self.N, self.H, self.W, self.C, self.delta, p(self.fws), self.fws_bytes,
-                 nat.stream_of(self.cost))
+                 nat.FWD_MSG_IN_ZERO if t == 0 else 0, nat.stream_of(self.cost))
 
     def bwd(self, t, g_in, g_out):
         nat, p = self.nat, self.sp
@@ -230,6 +230,7 @@ class LBPPlan:
         dev = cost.device
         m0 = msgs0 if msgs0 is not None else torch.zeros((N, 4, H, W, C), device=dev)
         self.msgs = [m0] + [torch.empty((N, 4, H, W, C), device=dev) for _ in range(iters)]
+        self.m0_zero = msgs0 is None
         self.argmin = [torch.empty((N, 4, H, W, C), dtype=torch.uint8, device=dev) for _ in range(iters)]
         self.garg = [torch.empty((N, 4, H, W), dtype=torch.uint8, device=dev) for _ in range(iters)]
         self.scale = [torch.empty((N, 4, H, W), device=dev) for _ in range(iters)]
@@ -253,7 +254,7 @@ class LBPPlan:
         nat.call("bpl_lbp_fwd_ws", self.kind, p(self.cost), p(self.jump), self.K, self.Nj, p(self.edge),
                  p(self.msgs[t]), p(self.msgs[t + 1]), p(self.argmin[t]), p(self.garg[t]), p(self.scale[t]),
                  self.N, self.H, self.W, self.C, self.delta, p(self.fws), self.fws_bytes,
-                 nat.stream_of(self.cost))
+                 nat.FWD_MSG_IN_ZERO if t == 0 and self.m0_zero else 0, nat.stream_of(self.cost))
 
     def bwd(self, t, g_in, g_out):
         nat, p = self.nat, self.sp

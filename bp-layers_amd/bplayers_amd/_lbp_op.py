@@ -19,6 +19,8 @@ Deliberate differences (all internal to the autograd Functions that carry these 
   * every batch element is processed; batch-shared pairwise gradients are summed;
   * ``mode="exact"`` (keyword) returns the exact adjoint w.r.t. the input messages;
   * ``out=`` (keyword) writes the messages out of place, leaving ``messages`` untouched;
+  * ``msg_in_zero=True`` (keyword) declares ``messages`` all zeros (the first iteration from
+    zero messages), which the stereo kernels then need not read;
   * ``in_h_row_ends`` / ``out_h_row_ends`` (keywords, exact mode) chain the backward calls of
     consecutive iterations: in the exact adjoint the LEFT/RIGHT slots of the input-message
     gradient are zero away from the row ends, so a call may skip reading them in in_grad
@@ -81,7 +83,7 @@ def _check_common(kind, cost, jump, edge, messages):
     return N, H, W, C, K, Nj
 
 
-def forward_minsum(kind, cost, jump, edge, messages, delta, out=None):
+def forward_minsum(kind, cost, jump, edge, messages, delta, out=None, msg_in_zero=False):
     N, H, W, C, K, Nj = _check_common(kind, cost, jump, edge, messages)
     msg_out = messages
     if out is not None:
@@ -100,7 +102,8 @@ def forward_minsum(kind, cost, jump, edge, messages, delta, out=None):
                          device=cost.device)
     nat.call("bpl_lbp_fwd_ws", kind, nat.ptr(cost), nat.ptr(jump), K, Nj, nat.ptr(edge), nat.ptr(messages),
              nat.ptr(msg_out), nat.ptr(argmin), nat.ptr(garg), nat.ptr(scale), N, H, W, C, delta,
-             nat.ptr(ws), 0 if ws is None else ws.numel(), nat.stream_of(cost))
+             nat.ptr(ws), 0 if ws is None else ws.numel(), nat.FWD_MSG_IN_ZERO if msg_in_zero else 0,
+             nat.stream_of(cost))
     return [msg_out, argmin, scale]
 
 
@@ -169,14 +172,15 @@ def backward_minsum(kind, cost, jump, edge, in_grad, messages, messages_argmin, 
     return [grad_cost, gj_ret, grad_edge, grad_msg]
 
 
-def exact_chain_forward(kind, cost, jump, edge, messages, delta, iters):
+def exact_chain_forward(kind, cost, jump, edge, messages, delta, iters, msg_in_zero=False):
     """``iters`` iterations from ``messages`` (left untouched), each into fresh messages.
     Returns the final messages and the per-iteration records (messages, argmin, scale) that
     exact_chain_backward replays."""
     recs = []
     m = messages
-    for _ in range(iters):
-        m, am, sc = forward_minsum(kind, cost, jump, edge, m, delta, out=torch.empty_like(m))
+    for t in range(iters):
+        m, am, sc = forward_minsum(kind, cost, jump, edge, m, delta, out=torch.empty_like(m),
+                                   msg_in_zero=msg_in_zero and t == 0)
         recs.append((m, am, sc))
     return m, recs
 

@@ -22,6 +22,7 @@ GRAD_EXACT = 1
 GRAD_IN_H_ROW_ENDS = 0x10
 GRAD_OUT_H_ROW_ENDS = 0x20
 GRAD_ACCUMULATE = 0x40
+FWD_MSG_IN_ZERO = 0x1  # BPL_FWD_MSG_IN_ZERO
 LBP_BWD_NOALLOC_MAX_BYTES = 128 << 20  # BPL_LBP_BWD_NOALLOC_MAX_BYTES
 
 _P = ctypes.c_void_p
@@ -39,7 +40,8 @@ _SIGS = {
     "bpl_sad_flow_bwd": ([_P, _P, _P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I, _P], _I),
     "bpl_lbp_fwd": ([_I, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I, _P], _I),
     "bpl_lbp_fwd_workspace_size": ([_I, _I64, _I64, _I64, _I64], _I64),
-    "bpl_lbp_fwd_ws": ([_I, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I, _P, _I64, _P], _I),
+    "bpl_lbp_fwd_ws": ([_I, _P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I, _P, _I64, _I, _P],
+                       _I),
     "bpl_lbp_bwd_workspace_size": ([_I, _I64, _I64, _I64, _I64], _I64),
     "bpl_softmax_labels": ([_P, _P, _I64, _I64, _F, _F, _P], _I),
     "bpl_lbp_beliefs": ([_P, _P, _P, _P, _I64, _I64, _I64, _I64, _F, _P], _I),

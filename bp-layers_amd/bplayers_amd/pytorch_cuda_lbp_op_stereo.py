@@ -4,10 +4,12 @@ from . import _lbp_op
 from ._native import LBP_STEREO as KIND
 
 
-def forward_minsum(cost, jump, edge, messages, delta, *, out=None):
+def forward_minsum(cost, jump, edge, messages, delta, *, out=None, msg_in_zero=False):
     """One LBP iteration (RIGHT, LEFT, DOWN, UP); ``messages`` is updated in place unless
-    ``out`` is given.  Returns [messages, messages_argmin, message_scale]."""
-    return _lbp_op.forward_minsum(KIND, cost, jump, edge, messages, delta, out=out)
+    ``out`` is given.  Returns [messages, messages_argmin, message_scale].  ``msg_in_zero``
+    (keyword, not in the reference's signature): ``messages`` are all zeros, so the kernels
+    need not read them."""
+    return _lbp_op.forward_minsum(KIND, cost, jump, edge, messages, delta, out=out, msg_in_zero=msg_in_zero)
 
 
 def backward_minsum(cost, jump, edge, in_grad, messages, messages_argmin, message_scale, *, mode="reference",

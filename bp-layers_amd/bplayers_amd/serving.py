@@ -96,10 +96,10 @@ class Slot:
             nat.call("bpl_sad_stereo_softmax_fwd", p(self.f0), p(self.f1), p(self.cost), n, C, H, W, 0, D - 1,
                      -1.0 / self.T_match, -1.0, st)
             self.msgs[:n].zero_()
-            for _ in range(self.iters):
+            for it in range(self.iters):
                 nat.call("bpl_lbp_fwd_ws", nat.LBP_STEREO, p(self.cost), p(self.jump), 6, 1, p(self.edge),
                          p(self.msgs), p(self.msgs), None, None, None, n, H, W, D, self.delta, p(self.base),
-                         self.base.numel() * 4, st)
+                         self.base.numel() * 4, nat.FWD_MSG_IN_ZERO if it == 0 else 0, st)
             nat.call("bpl_lbp_beliefs", p(self.cost), p(self.msgs), None, p(self.disp), n, H, W, D, self.T_bel, st)
             self.event.record(self.stream)
         self.reqs = reqs

@@ -30,6 +30,9 @@ struct FwdArgs {
   // [N,H,W,C] plane (cost + M_R) + M_L: written by the paired horizontal kernel, read by the
   // vertical P1/P2 sweep in place of the cost and both horizontal messages (null: not used)
   float *base;
+  // msg_in is all zeros (the initial messages of an iteration chain): the P1/P2 sweeps read
+  // them through a resource of 0 bytes, which returns zeros without memory traffic
+  int in_zero;
 };
 
 __device__ __forceinline__ int64_t px(int n, int y, int x, int H, int W) {
@@ -654,7 +657,7 @@ __device__ __forceinline__ void fwd_sweep_p1p2(const FwdArgs &a, int n, int dir,
   const uint32_t pb = (uint32_t)(HW * C * 4);        // bytes of one [H,W,C] plane
   const int64_t img4 = (int64_t)n * 4 * HW * C;      // the image's 4-slot message block
   const __amdgpu_buffer_rsrc_t c_rs = make_rsrc(a.cost + (int64_t)n * HW * C, pb);
-  const __amdgpu_buffer_rsrc_t mi_rs = make_rsrc(a.msg_in + img4, 4u * pb);
+  const __amdgpu_buffer_rsrc_t mi_rs = make_rsrc(a.msg_in + img4, a.in_zero ? 0u : 4u * pb);
   const __amdgpu_buffer_rsrc_t mo_rs = make_rsrc(a.msg_out + img4, 4u * pb);
   const __amdgpu_buffer_rsrc_t x_rs = VERT ? mo_rs : mi_rs;     // the cross messages
   const int64_t rs_base = ((int64_t)n * 4 + rslot) * HW;
@@ -1552,7 +1555,7 @@ static int lbp_fwd_impl(int kind, const float *cost, const float *jump, int64_t 
                         const float *edge, const float *msg_in, float *msg_out,
                         uint8_t *argmin, uint8_t *gargmin, float *scale,
                         int64_t N, int64_t H, int64_t W, int64_t C, int delta, void *ws, int64_t ws_bytes,
-                        void *stream) {
+                        int flags, void *stream) {
   BPL_REQUIRE(kind >= 0 && kind <= 2, BPL_E_ARG, "bpl_lbp_fwd: unknown kind %d", kind);
   BPL_REQUIRE(N >= 1 && H >= 1 && W >= 1 && C >= 1, BPL_E_ARG,
               "bpl_lbp_fwd: sizes must be positive (N=%lld H=%lld W=%lld C=%lld)",
@@ -1583,6 +1586,7 @@ static int lbp_fwd_impl(int kind, const float *cost, const float *jump, int64_t 
   // the base plane: stereo fast-step launches whose planes fit the buffer addressing, with a
   // workspace of one [N,H,W,C] plane that overlaps no tensor the call reads or writes
   a.base = nullptr;
+  a.in_zero = (flags & BPL_FWD_MSG_IN_ZERO) ? 1 : 0;
   if (ws && kind == BPL_LBP_STEREO && jump_k == 6 && delta == 3 && stereo_w64_ok(a) && a.buf_ok &&
       !generic_only() && !base_off() && ws_bytes >= N * H * W * C * (int64_t)sizeof(float))
     a.base = static_cast<float *>(ws);
@@ -1618,7 +1622,7 @@ extern "C" int bpl_lbp_fwd(int kind, const float *cost, const float *jump, int64
                            uint8_t *argmin, uint8_t *gargmin, float *scale,
                            int64_t N, int64_t H, int64_t W, int64_t C, int delta, void *stream) {
   return lbp_fwd_impl(kind, cost, jump, jump_k, jump_n, edge, msg_in, msg_out, argmin, gargmin, scale, N, H, W, C,
-                      delta, nullptr, 0, stream);
+                      delta, nullptr, 0, 0, stream);
 }
 
 extern "C" int64_t bpl_lbp_fwd_workspace_size(int kind, int64_t N, int64_t H, int64_t W, int64_t C) {
@@ -1630,7 +1634,8 @@ extern "C" int bpl_lbp_fwd_ws(int kind, const float *cost, const float *jump, in
                               const float *edge, const float *msg_in, float *msg_out,
                               uint8_t *argmin, uint8_t *gargmin, float *scale,
                               int64_t N, int64_t H, int64_t W, int64_t C, int delta, void *workspace,
-                              int64_t workspace_bytes, void *stream) {
+                              int64_t workspace_bytes, int flags, void *stream) {
+  BPL_REQUIRE((flags & ~BPL_FWD_MSG_IN_ZERO) == 0, BPL_E_ARG, "bpl_lbp_fwd_ws: unknown flags 0x%x", flags);
   BPL_REQUIRE(workspace_bytes >= 0, BPL_E_ARG, "bpl_lbp_fwd_ws: negative workspace size");
   if (workspace && N >= 1 && H >= 1 && W >= 1 && C >= 1) {
     // the workspace is written while the cost and both message tensors are read
@@ -1647,5 +1652,5 @@ extern "C" int bpl_lbp_fwd_ws(int kind, const float *cost, const float *jump, in
               "bpl_lbp_fwd_ws: workspace of %lld bytes, %lld needed", (long long)workspace_bytes,
               (long long)bpl_lbp_fwd_workspace_size(kind, N, H, W, C));
   return lbp_fwd_impl(kind, cost, jump, jump_k, jump_n, edge, msg_in, msg_out, argmin, gargmin, scale, N, H, W, C,
-                      delta, workspace, workspace_bytes, stream);
+                      delta, workspace, workspace_bytes, flags, stream);
 }

@@ -223,7 +223,7 @@ def test_lbp_forward_base_plane_bitwise(case):
         args = [nat.LBP_STEREO, nat.ptr(c_), nat.ptr(j_), K, Nj, nat.ptr(e_), nat.ptr(m_in), nat.ptr(out),
                 nat.ptr(am), nat.ptr(ga), nat.ptr(sc), N, H, W, C, 3]
         if use_ws:
-            nat.call("bpl_lbp_fwd_ws", *args, nat.ptr(ws), wsb, nat.stream_of(c_))
+            nat.call("bpl_lbp_fwd_ws", *args, nat.ptr(ws), wsb, 0, nat.stream_of(c_))
         else:
             nat.call("bpl_lbp_fwd", *args, nat.stream_of(c_))
         torch.cuda.synchronize()
@@ -249,8 +249,41 @@ def test_lbp_forward_ws_rejects_overlap():
     wsb = N * H * W * C * 4
     with pytest.raises(nat.BplError, match="overlaps"):
         nat.call("bpl_lbp_fwd_ws", nat.LBP_STEREO, nat.ptr(cost), nat.ptr(jump), 6, 1, nat.ptr(edge), nat.ptr(m),
-                 nat.ptr(m), None, None, None, N, H, W, C, 3, nat.ptr(m), wsb, nat.stream_of(cost))
+                 nat.ptr(m), None, None, None, N, H, W, C, 3, nat.ptr(m), wsb, 0, nat.stream_of(cost))
     with pytest.raises(nat.BplError, match="needed"):
         nat.call("bpl_lbp_fwd_ws", nat.LBP_STEREO, nat.ptr(cost), nat.ptr(jump), 6, 1, nat.ptr(edge), nat.ptr(m),
-                 nat.ptr(m), None, None, None, N, H, W, C, 3, nat.ptr(torch.empty(4, device=dev)), 16,
+                 nat.ptr(m), None, None, None, N, H, W, C, 3, nat.ptr(torch.empty(4, device=dev)), 16, 0,
                  nat.stream_of(cost))
+
+
+@pytest.mark.parametrize("case", [(1, 5, 40, 192, True), (2, 3, 41, 100, False), (1, 4, 9, 64, True)])
+def test_lbp_forward_zero_input_messages(case):
+    """BPL_FWD_MSG_IN_ZERO (the first iteration from zero messages, whose planes the P1/P2 sweeps
+    then do not read): the outputs of bpl_lbp_fwd on the same zero messages, bit for bit, with and
+    without the base-plane workspace."""
+    from bplayers_amd import _native as nat
+    from bplayers_amd._lbp_op import _alloc_scale_and_gargmin
+    N, H, W, C, inplace = case
+    print(f"zero-input case {case}", flush=True)
+    cost, jump, edge, _ = make_case("stereo", N, H, W, C, 3, seed=7 + W)
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.from_numpy(a).to(dev)
+    c_, j_, e_ = t(cost), t(jump), t(edge)
+    res = []
+    for mode in ("plain", "flag", "flag+ws"):
+        m_in = torch.zeros((N, 4, H, W, C), device=dev)
+        out = m_in if inplace else torch.empty_like(m_in)
+        am = torch.empty((N, 4, H, W, C), dtype=torch.uint8, device=dev)
+        sc, ga = _alloc_scale_and_gargmin(N, H, W, dev)
+        args = [nat.LBP_STEREO, nat.ptr(c_), nat.ptr(j_), 6, N, nat.ptr(e_), nat.ptr(m_in), nat.ptr(out),
+                nat.ptr(am), nat.ptr(ga), nat.ptr(sc), N, H, W, C, 3]
+        if mode == "plain":
+            nat.call("bpl_lbp_fwd", *args, nat.stream_of(c_))
+        else:
+            ws = torch.empty(N * H * W * C, device=dev) if mode == "flag+ws" else None
+            nat.call("bpl_lbp_fwd_ws", *args, nat.ptr(ws), 0 if ws is None else ws.numel() * 4,
+                     nat.FWD_MSG_IN_ZERO, nat.stream_of(c_))
+        torch.cuda.synchronize()
+        res.append((out.clone(), am, sc, ga))
+    for r in res[1:]:
+        assert all(torch.equal(a, b) for a, b in zip(res[0], r)), "outputs differ"
