@@ -47,7 +47,7 @@ class BP(nn.Module):
             # u-flow
             messages_u = torch.zeros((N, 4, H, W, K), requires_grad=True, device=prob_vol.device, dtype=torch.float)
             beliefs_u, messages_u = self.message_passing.forward(prob_vol[:, 0].contiguous(), edge_weights,
-                                                                 affinities, offsets, messages_u)
+                                                                 affinities, offsets, messages_u, messages_zero=True)
             result_u = self.inference.forward(beliefs_u)
             # v-flow: starts from the u messages (bp_op_cuda.py:110)
             beliefs_v, messages_v = self.message_passing.forward(prob_vol[:, 1].contiguous(), edge_weights,
@@ -60,7 +60,7 @@ class BP(nn.Module):
         N, H, W, K = prob_vol.shape
         messages = torch.zeros((N, 4, H, W, K), requires_grad=True, device=prob_vol.device, dtype=torch.float)
         beliefs, messages = self.message_passing.forward(prob_vol.contiguous(), edge_weights, affinities, offsets,
-                                                         messages)
+                                                         messages, messages_zero=True)
         result = self.inference.forward(beliefs)
         disps = result.permute(0, 3, 1, 2)
         return disps, beliefs, messages

@@ -45,9 +45,10 @@ class LBPMinSumFunction(torch.autograd.Function):
     """One LBP iteration with the reference's semantics (in-place messages)."""
 
     @staticmethod
-    def forward(ctx, cost, edge, messages, delta, affinities, offset):
+    def forward(ctx, cost, edge, messages, delta, affinities, offset, msg_zero=False):
         jump = pack_affinities(affinities, offset.to(affinities.device))
-        messages, messages_argmin, message_scale = lbp.forward_minsum(cost, jump, edge, messages, delta)
+        messages, messages_argmin, message_scale = lbp.forward_minsum(cost, jump, edge, messages, delta,
+                                                                      msg_in_zero=bool(msg_zero))
         ctx.save_for_backward(cost, jump, edge, messages, messages_argmin, message_scale)
         ctx.delta = int(delta)
         return messages
@@ -60,7 +61,7 @@ class LBPMinSumFunction(torch.autograd.Function):
         grad_cost, grad_jump, grad_edge, grad_message = lbp.backward_minsum(
             cost, jump, edge, g, messages, messages_argmin, message_scale, delta=ctx.delta, grad_messages_out=g)
         grad_aff, grad_offset = unpack_affinity_grad(grad_jump)
-        return grad_cost, grad_edge, grad_message, None, grad_aff, grad_offset
+        return grad_cost, grad_edge, grad_message, None, grad_aff, grad_offset, None
 
 
 class LBPMinSumExactFunction(torch.autograd.Function):
@@ -92,10 +93,10 @@ class LBPMinSumExactChainFunction(torch.autograd.Function):
     backward calls chained (bpl_lbp_bwd's row-end flags)."""
 
     @staticmethod
-    def forward(ctx, cost, edge, messages, delta, affinities, offset, max_iter):
+    def forward(ctx, cost, edge, messages, delta, affinities, offset, max_iter, msg_zero=False):
         jump = pack_affinities(affinities, offset.to(affinities.device))
         out, recs = _lbp_op.exact_chain_forward(nat.LBP_STEREO, cost, jump, edge, messages.contiguous(), delta,
-                                                int(max_iter))
+                                                int(max_iter), msg_in_zero=bool(msg_zero))
         ctx.save_for_backward(cost, jump, edge, *[x for r in recs for x in r])
         ctx.delta = int(delta)
         ctx.iters = int(max_iter)
@@ -108,7 +109,7 @@ class LBPMinSumExactChainFunction(torch.autograd.Function):
         grad_cost, grad_jump, grad_edge, grad_message = _lbp_op.exact_chain_backward(
             nat.LBP_STEREO, cost, jump, edge, in_grad, recs, delta=ctx.delta)
         grad_aff, grad_offset = unpack_affinity_grad(grad_jump)
-        return grad_cost, grad_edge, grad_message, None, grad_aff, grad_offset, None
+        return grad_cost, grad_edge, grad_message, None, grad_aff, grad_offset, None, None
 
 
 def check_modes(mode, grad_mode):
@@ -167,7 +168,9 @@ class MessagePassing(nn.Module):
         aff[:, :, 2] = self._L2
         return aff
 
-    def forward(self, prob_vol, edge_weights, affinities, offset, messages):
+    def forward(self, prob_vol, edge_weights, affinities, offset, messages, messages_zero=False):
+        """``messages_zero`` (keyword, not in the reference's signature): ``messages`` are all
+        zeros (as BP creates them), which the first iteration then need not read."""
         N, H, W, C = prob_vol.shape
         dev = prob_vol.device
         have = (self._L1 is not None, self._L2 is not None)
@@ -181,9 +184,10 @@ class MessagePassing(nn.Module):
         if self.grad_mode == 'exact':
             if self.max_iter > 0:
                 messages = LBPMinSumExactChainFunction.apply(cost, edge_weights, messages, self.delta, affinities,
-                                                             offset, self.max_iter)
+                                                             offset, self.max_iter, messages_zero)
         else:
-            for _ in range(self.max_iter):
-                messages = LBPMinSumFunction.apply(cost, edge_weights, messages, self.delta, affinities, offset)
+            for it in range(self.max_iter):
+                messages = LBPMinSumFunction.apply(cost, edge_weights, messages, self.delta, affinities, offset,
+                                                   messages_zero and it == 0)
         beliefs = self.softmin.forward(messages.sum(dim=1) + cost)
         return beliefs, messages
