@@ -132,16 +132,17 @@ class StereoPlan:
         nat, p = self.nat, self.sp
         # chained iterations: the message gradients between two backward calls carry their
         # LEFT/RIGHT slots at the row ends only (zero elsewhere in the exact adjoint); the top
-        # call reads the full loss gradient, the bottom one writes the full initial-message
-        # gradient; the calls after the top one add their cost, pairwise and edge gradients to
-        # the top one's (the sum over iterations of a training step)
+        # call reads the full loss gradient; the bottom one computes no gradient w.r.t. the
+        # initial messages, constant zeros that nothing trains (as BP's); the calls after the
+        # top one add their cost, pairwise and edge gradients to the top one's (the sum over
+        # iterations of a training step)
         top = t == self.iters - 1
         mode = nat.GRAD_EXACT | (0 if top else nat.GRAD_IN_H_ROW_ENDS | nat.GRAD_ACCUMULATE) | \
             (nat.GRAD_OUT_H_ROW_ENDS if t > 0 else 0)
         nat.call("bpl_lbp_bwd", nat.LBP_STEREO, mode, p(self.jump), self.K, self.jump.shape[0],
                  p(self.edge), p(g_in), p(self.msgs[t + 1]), p(self.argmin[t]), p(self.garg[t]), p(self.gcost),
-                 p(self.gjump), p(self.gedge), p(g_out), p(self.ws), self.ws_bytes, self.N, self.H, self.W, self.C,
-                 self.delta, nat.stream_of(self.cost))
+                 p(self.gjump), p(self.gedge), p(g_out) if t > 0 else None, p(self.ws), self.ws_bytes, self.N,
+                 self.H, self.W, self.C, self.delta, nat.stream_of(self.cost))
 
     def step(self):
         for t in range(self.iters):
@@ -258,16 +259,17 @@ class LBPPlan:
 
     def bwd(self, t, g_in, g_out):
         nat, p = self.nat, self.sp
-        # chained iterations (see StereoPlan.bwd): the bottom call writes a full gradient; the
-        # per-pixel variant's pairwise gradient is not accumulated by the library
+        # chained iterations (see StereoPlan.bwd): from zero initial messages the bottom call
+        # computes no gradient w.r.t. them; the per-pixel variant's pairwise gradient is not
+        # accumulated by the library
         top = t == self.iters - 1
         acc = 0 if top or self.kind == nat.LBP_SEMANTIC_PIXEL else nat.GRAD_ACCUMULATE
         mode = nat.GRAD_EXACT | (0 if top else nat.GRAD_IN_H_ROW_ENDS) | acc | \
             (nat.GRAD_OUT_H_ROW_ENDS if t > 0 else 0)
         nat.call("bpl_lbp_bwd", self.kind, mode, p(self.jump), self.K, self.Nj, p(self.edge), p(g_in),
                  p(self.msgs[t + 1]), p(self.argmin[t]), p(self.garg[t]), p(self.gcost), p(self.gjump),
-                 p(self.gedge), p(g_out), p(self.ws), self.ws_bytes, self.N, self.H, self.W, self.C, self.delta,
-                 nat.stream_of(self.cost))
+                 p(self.gedge), p(g_out) if t > 0 or not self.m0_zero else None, p(self.ws), self.ws_bytes,
+                 self.N, self.H, self.W, self.C, self.delta, nat.stream_of(self.cost))
 
     def step(self):
         for t in range(self.iters):

@@ -109,10 +109,11 @@ def forward_minsum(kind, cost, jump, edge, messages, delta, out=None, msg_in_zer
 
 def backward_minsum(kind, cost, jump, edge, in_grad, messages, messages_argmin, message_scale,
                     mode="reference", delta=None, in_h_row_ends=False, out_h_row_ends=False,
-                    grad_messages_out=None, accumulate_into=None):
+                    grad_messages_out=None, accumulate_into=None, want_grad_messages=True):
     """Backward of one LBP iteration (bpl_lbp_bwd).  ``accumulate_into``: the (grad_cost,
     grad_pairwise, grad_edge) of an earlier call with these sizes, to which this call's
-    gradients are added in place (BPL_GRAD_ACCUMULATE) and which are returned."""
+    gradients are added in place (BPL_GRAD_ACCUMULATE) and which are returned.
+    ``want_grad_messages=False``: the input-message gradient is not computed (returned as None)."""
     N, H, W, C, K, Nj = _check_common(kind, cost, jump, edge, messages)
     nat.check_input(in_grad, "in_grad")
     if in_grad.shape != messages.shape:
@@ -151,7 +152,9 @@ def backward_minsum(kind, cost, jump, edge, in_grad, messages, messages_argmin, 
         grad_cost, gj_ret, grad_edge = acc
         grad_jump = gj_ret[:1] if kind == nat.LBP_SEMANTIC_GLOBAL else gj_ret
         m |= nat.GRAD_ACCUMULATE
-    if grad_messages_out is not None:
+    if not want_grad_messages:
+        grad_msg = None
+    elif grad_messages_out is not None:
         nat.check_input(grad_messages_out, "grad_messages_out")
         if grad_messages_out.shape != messages.shape:
             raise RuntimeError("grad_messages_out must have the shape of messages")
@@ -185,14 +188,15 @@ def exact_chain_forward(kind, cost, jump, edge, messages, delta, iters, msg_in_z
     return m, recs
 
 
-def exact_chain_backward(kind, cost, jump, edge, in_grad, recs, delta=None):
+def exact_chain_backward(kind, cost, jump, edge, in_grad, recs, delta=None, want_grad_messages=True):
     """Exact adjoint of exact_chain_forward: the backward calls of consecutive iterations pass
     each other message gradients whose LEFT/RIGHT slots are written and read at the row ends
     only (zero elsewhere in the exact adjoint); the top call reads the full ``in_grad`` and the
     bottom call writes the full gradient w.r.t. the initial messages.  Returns
     (grad_cost, grad_pairwise, grad_edge, grad_messages) with the first three summed over the
     iterations (top iteration first; every later call adds to them in the kernels, except for
-    the per-pixel variant, whose pairwise gradient is summed here)."""
+    the per-pixel variant, whose pairwise gradient is summed here).  ``want_grad_messages=False``:
+    the bottom call computes no initial-message gradient (returned as None)."""
     T = len(recs)
     if T == 0:
         raise RuntimeError("exact_chain_backward needs at least one iteration")
@@ -203,7 +207,8 @@ def exact_chain_backward(kind, cost, jump, edge, in_grad, recs, delta=None):
         m, am, sc = recs[t]
         gc, gj, ge, g = backward_minsum(kind, cost, jump, edge, g, m, am, sc, mode="exact", delta=delta,
                                         in_h_row_ends=t < T - 1, out_h_row_ends=t > 0,
-                                        accumulate_into=tot if in_kernel else None)
+                                        accumulate_into=tot if in_kernel else None,
+                                        want_grad_messages=want_grad_messages or t > 0)
         if tot is None or in_kernel:
             tot = [gc, gj, ge]
         else:

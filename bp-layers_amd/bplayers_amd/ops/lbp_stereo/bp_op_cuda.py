@@ -58,7 +58,9 @@ class BP(nn.Module):
             messages = torch.cat((messages_u.unsqueeze(1), messages_v.unsqueeze(1)), dim=1)
             return flow, beliefs, messages
         N, H, W, K = prob_vol.shape
-        messages = torch.zeros((N, 4, H, W, K), requires_grad=True, device=prob_vol.device, dtype=torch.float)
+        # the initial messages are constant zeros: no gradient is wanted for them (the
+        # reference marks them requires_grad; nothing can read that gradient)
+        messages = torch.zeros((N, 4, H, W, K), requires_grad=False, device=prob_vol.device, dtype=torch.float)
         beliefs, messages = self.message_passing.forward(prob_vol.contiguous(), edge_weights, affinities, offsets,
                                                          messages, messages_zero=True)
         result = self.inference.forward(beliefs)

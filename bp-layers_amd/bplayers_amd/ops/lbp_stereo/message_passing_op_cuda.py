@@ -107,7 +107,8 @@ class LBPMinSumExactChainFunction(torch.autograd.Function):
         cost, jump, edge, *flat = ctx.saved_tensors
         recs = [tuple(flat[3 * i:3 * i + 3]) for i in range(ctx.iters)]
         grad_cost, grad_jump, grad_edge, grad_message = _lbp_op.exact_chain_backward(
-            nat.LBP_STEREO, cost, jump, edge, in_grad, recs, delta=ctx.delta)
+            nat.LBP_STEREO, cost, jump, edge, in_grad, recs, delta=ctx.delta,
+            want_grad_messages=ctx.needs_input_grad[2])
         grad_aff, grad_offset = unpack_affinity_grad(grad_jump)
         return grad_cost, grad_edge, grad_message, None, grad_aff, grad_offset, None, None
 

@@ -604,14 +604,16 @@ __global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(B
       const bool accg = PAIR && a.accum_cost != 0;
       // row streams of the fast step, buffer-addressed (see BUF)
       const int64_t HWC = HW * C;
-      float *gm_p = a.grad_msg + (int64_t)n * 4 * HWC;
+      // a null grad_msg (not wanted): resources of 0 bytes, whose stores are dropped
+      float *gm_p = a.grad_msg ? a.grad_msg + (int64_t)n * 4 * HWC : a.grad_cost;
+      const uint32_t gm_bytes = a.grad_msg ? plane_bytes : 0u;
       const __amdgpu_buffer_rsrc_t xt_rs = make_rsrc(Xt_p, plane_bytes);
       const __amdgpu_buffer_rsrc_t xo_rs = make_rsrc(Xo, plane_bytes);
       const __amdgpu_buffer_rsrc_t gc_rs = make_rsrc(a.grad_cost + img, plane_bytes);
-      const __amdgpu_buffer_rsrc_t gu_rs = make_rsrc(gm_p + UP * HWC, plane_bytes);
-      const __amdgpu_buffer_rsrc_t gd_rs = make_rsrc(gm_p + DOWN * HWC, plane_bytes);
-      const __amdgpu_buffer_rsrc_t gl_rs = make_rsrc(gm_p + LEFT * HWC, plane_bytes);
-      const __amdgpu_buffer_rsrc_t gr_rs = make_rsrc(gm_p + RIGHT * HWC, plane_bytes);
+      const __amdgpu_buffer_rsrc_t gu_rs = make_rsrc(a.grad_msg ? gm_p + UP * HWC : gm_p, gm_bytes);
+      const __amdgpu_buffer_rsrc_t gd_rs = make_rsrc(a.grad_msg ? gm_p + DOWN * HWC : gm_p, gm_bytes);
+      const __amdgpu_buffer_rsrc_t gl_rs = make_rsrc(a.grad_msg ? gm_p + LEFT * HWC : gm_p, gm_bytes);
+      const __amdgpu_buffer_rsrc_t gr_rs = make_rsrc(a.grad_msg ? gm_p + RIGHT * HWC : gm_p, gm_bytes);
       // horizontal phase: the pairwise-bin and edge gradients of four consecutive steps (four
       // consecutive pixels of the row) are kept in registers and stored as one 16-byte access
       // per lane (fewer memory instructions per step leave room in the 6-bit vmcnt counter
@@ -1046,7 +1048,7 @@ __global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(B
 #pragma unroll
         for (int k = 0; k < LPL; ++k) sr[k] = (ge[k] + xl[k]) + (xd[k] + xu[k]);
       }
-      float *gm = a.grad_msg + (int64_t)n * 4 * HWC + q * C;
+      float *gm = a.grad_msg ? a.grad_msg + (int64_t)n * 4 * HWC + q * C : nullptr;
       auto st = [&](float *row, const float *v) {
 #pragma unroll
         for (int k = 0; k < LPL; ++k)
@@ -1059,10 +1061,12 @@ __global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(B
         for (int k = 0; k < LPL; ++k) cst[k] = old[k] + cst[k];
       }
       st(a.grad_cost + img + q * C, cst);
-      st(gm + UP * HWC, hr);
-      st(gm + DOWN * HWC, hr);
-      if (!kOsp || x == 0) st(gm + LEFT * HWC, sl);
-      if (!kOsp || x == W - 1) st(gm + RIGHT * HWC, sr);
+      if (gm) {
+        st(gm + UP * HWC, hr);
+        st(gm + DOWN * HWC, hr);
+        if (!kOsp || x == 0) st(gm + LEFT * HWC, sl);
+        if (!kOsp || x == W - 1) st(gm + RIGHT * HWC, sr);
+      }
     };
     // the two waves take alternate pixels of the list (row ends first)
     int idx = 0;
@@ -1468,6 +1472,7 @@ __global__ void lbp_bwd_combine(const float *__restrict__ XL, const float *__res
     const float xl = XL[i], xr = XR[i], xu = XU[i], xd = XD[i];
     const float gc = ((xd + xu) + xr) + xl;
     grad_cost[i] = accum_cost ? grad_cost[i] + gc : gc;
+    if (!grad_msg) continue;  // the input-message gradient is not wanted
     const int64_t n = i / HWC;
     const int64_t r = i - n * HWC;            // (y, x, c) inside the image
     const int64_t base = n * 4 * HWC + r;     // slot 0 of this element
@@ -1687,7 +1692,7 @@ int lbp_bwd_impl(int kind, int mode, const float *jump, int64_t jump_k, int64_t 
   BPL_REQUIRE(C <= 255, BPL_E_LIMIT, "bpl_lbp_bwd: C=%lld labels exceeds the supported 255", (long long)C);
   BPL_REQUIRE(N * 2 * (H > W ? H : W) < (1LL << 31) && H < (1 << 30) && W < (1 << 30), BPL_E_LIMIT,
               "bpl_lbp_bwd: grid too large");
-  {
+  if (grad_msg) {
     // grad_msg and in_grad are [N,4,H,W,C] each: they must be disjoint, or (reference mode
     // only, where nothing is written to grad_msg) the same tensor; a partial overlap would
     // race with kernels that read in_grad while others write grad_msg
@@ -1698,7 +1703,7 @@ int lbp_bwd_impl(int kind, int mode, const float *jump, int64_t jump_k, int64_t 
                 same ? "bpl_lbp_bwd: grad_msg may alias in_grad in the reference mode only"
                      : "bpl_lbp_bwd: grad_msg partly overlaps in_grad");
   }
-  BPL_REQUIRE(edge && in_grad && messages && argmin && gargmin && grad_cost && grad_jump && grad_edge && grad_msg,
+  BPL_REQUIRE(edge && in_grad && messages && argmin && gargmin && grad_cost && grad_jump && grad_edge,
               BPL_E_ARG, "bpl_lbp_bwd: null pointer");
   BPL_REQUIRE(jump || (kind == BPL_LBP_SEMANTIC_PIXEL && mode == BPL_GRAD_REFERENCE), BPL_E_ARG,
               "bpl_lbp_bwd: jump required");
@@ -1771,7 +1776,9 @@ int lbp_bwd_impl(int kind, int mode, const float *jump, int64_t jump_k, int64_t 
       if (rc) return rc;
     }
   }
-  if (pair) {
+  if (pair && !grad_msg) {
+    // the paired horizontal phase wrote grad_cost; the input-message gradient is not wanted
+  } else if (pair) {
     // the paired horizontal phase wrote grad_cost and grad_msg; add the y-boundary terms
     const int64_t total_el = N * 2 * W * C;
     const int threads = 256;

@@ -206,6 +206,9 @@ int64_t bpl_lbp_bwd_workspace_size(int kind, int64_t N, int64_t H, int64_t W, in
  * of in_grad; BPL_GRAD_EXACT: the gradient w.r.t. the iteration's input messages).
  * In the reference mode grad_msg may be in_grad itself: nothing is then written to it (an
  * autograd caller can pass its incoming gradient through instead of copying it).
+ * grad_msg may be NULL when the input-message gradient is not wanted (the first iteration
+ * from constant initial messages): the other outputs are unchanged and no message-gradient
+ * bytes are written.
  * delta: the delta the forward ran with, or -1 if unknown (an extra argument of this ABI,
  * not of the reference).  It only selects kernels: a value in [0,3] lets the stereo kernel
  * assume every non-truncation argmin lies within 3 labels of its receiver, which holds

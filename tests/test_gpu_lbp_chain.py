@@ -89,3 +89,34 @@ def test_chained_backward_equals_unchained(case):
         else:
             assert torch.equal(gr, gc), "bottom iteration: message gradient differs"
         g_ref = gr
+
+
+KIND_CODES = {"stereo": 0, "semantic_global": 1, "semantic_pixel": 2}
+
+
+@pytest.mark.parametrize("case", [c for c in CASES if c[0] != "semantic_pixel"] + [CASES[-2]],
+                         ids=lambda c: f"{c[0]}-N{c[1]}H{c[2]}W{c[3]}C{c[4]}{'-delta' if c[7] else ''}")
+def test_backward_without_message_gradient(case):
+    """A null grad_msg (want_grad_messages=False: the bottom call of a chain from constant initial
+    messages) leaves grad_cost, the pairwise and the edge gradients bitwise unchanged, in the
+    exact mode (paired kernel, or combine pass) and the reference mode."""
+    from bplayers_amd import _lbp_op
+    kind, N, H, W, C, delta, extra, pass_delta = case
+    print(f"no-message-gradient case {case}", flush=True)
+    cost, jump, edge, M0 = make_case(kind, N, H, W, C, delta, seed=3 + C + W, **extra)
+    mod = _mods()[kind]
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    cost, jump, edge = t(cost), t(jump), t(edge)
+    m, am, sc = mod.forward_minsum(cost, jump, edge, t(M0), delta, out=torch.empty_like(t(M0)))
+    G = torch.from_numpy(np.random.default_rng(5).normal(size=M0.shape).astype(np.float32)).to(dev)
+    d = delta if pass_delta else None
+    for mode in ("exact", "reference"):
+        if kind == "semantic_pixel" and mode == "reference":
+            continue
+        res = [_lbp_op.backward_minsum(KIND_CODES[kind], cost, jump, edge, G, m, am, sc, mode=mode, delta=d,
+                                       want_grad_messages=w) for w in (True, False)]
+        torch.cuda.synchronize()
+        assert res[1][3] is None
+        for name, a, b in zip(("cost", "pairwise", "edge"), res[0][:3], res[1][:3]):
+            assert torch.equal(a, b), f"{mode}: grad {name} differs without the message gradient"
