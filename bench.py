@@ -137,8 +137,8 @@ class StereoPlan:
         # top one add their cost, pairwise and edge gradients to the top one's (the sum over
         # iterations of a training step)
         top = t == self.iters - 1
-        mode = nat.GRAD_EXACT | (0 if top else nat.GRAD_IN_H_ROW_ENDS | nat.GRAD_ACCUMULATE) | \
-            (nat.GRAD_OUT_H_ROW_ENDS if t > 0 else 0)
+        mode = nat.GRAD_EXACT | (0 if top else nat.GRAD_IN_H_ROW_ENDS | nat.GRAD_IN_UD_SHARED | nat.GRAD_ACCUMULATE) | \
+            (nat.GRAD_OUT_H_ROW_ENDS | nat.GRAD_OUT_UD_SHARED if t > 0 else 0)
         nat.call("bpl_lbp_bwd", nat.LBP_STEREO, mode, p(self.jump), self.K, self.jump.shape[0],
                  p(self.edge), p(g_in), p(self.msgs[t + 1]), p(self.argmin[t]), p(self.garg[t]), p(self.gcost),
                  p(self.gjump), p(self.gedge), p(g_out) if t > 0 else None, p(self.ws), self.ws_bytes, self.N,
@@ -264,8 +264,8 @@ class LBPPlan:
         # accumulated by the library
         top = t == self.iters - 1
         acc = 0 if top or self.kind == nat.LBP_SEMANTIC_PIXEL else nat.GRAD_ACCUMULATE
-        mode = nat.GRAD_EXACT | (0 if top else nat.GRAD_IN_H_ROW_ENDS) | acc | \
-            (nat.GRAD_OUT_H_ROW_ENDS if t > 0 else 0)
+        mode = nat.GRAD_EXACT | (0 if top else nat.GRAD_IN_H_ROW_ENDS | nat.GRAD_IN_UD_SHARED) | acc | \
+            (nat.GRAD_OUT_H_ROW_ENDS | nat.GRAD_OUT_UD_SHARED if t > 0 else 0)
         nat.call("bpl_lbp_bwd", self.kind, mode, p(self.jump), self.K, self.Nj, p(self.edge), p(g_in),
                  p(self.msgs[t + 1]), p(self.argmin[t]), p(self.garg[t]), p(self.gcost), p(self.gjump),
                  p(self.gedge), p(g_out) if t > 0 or not self.m0_zero else None, p(self.ws), self.ws_bytes,

@@ -23,10 +23,10 @@ Deliberate differences (all internal to the autograd Functions that carry these 
     zero messages), which the stereo kernels then need not read;
   * ``in_h_row_ends`` / ``out_h_row_ends`` (keywords, exact mode) chain the backward calls of
     consecutive iterations: in the exact adjoint the LEFT/RIGHT slots of the input-message
-    gradient are zero away from the row ends, so a call may skip reading them in in_grad
-    (``in_h_row_ends``) or writing them in its grad_messages (``out_h_row_ends``; those
-    entries are then left uninitialised and the tensor is only fit to be the in_grad of a
-    call with ``in_h_row_ends``);
+    gradient are zero away from the row ends, and its DOWN slot equals its UP slot away from
+    rows 0 and H-1, so a call may skip reading those entries in in_grad (``in_h_row_ends``) or
+    writing them in its grad_messages (``out_h_row_ends``; they are then left uninitialised and
+    the tensor is only fit to be the in_grad of a call with ``in_h_row_ends``);
   * ``grad_messages_out=`` (keyword) receives the input-message gradient; in the reference mode
     it may be ``in_grad`` itself, which then passes through uncopied (for autograd callers).
 """
@@ -124,7 +124,10 @@ def backward_minsum(kind, cost, jump, edge, in_grad, messages, messages_argmin, 
     m = {"reference": nat.GRAD_REFERENCE, "exact": nat.GRAD_EXACT}[mode]
     if (in_h_row_ends or out_h_row_ends) and mode != "exact":
         raise RuntimeError("in_h_row_ends / out_h_row_ends apply to mode='exact' only")
-    m |= (nat.GRAD_IN_H_ROW_ENDS if in_h_row_ends else 0) | (nat.GRAD_OUT_H_ROW_ENDS if out_h_row_ends else 0)
+    # the U/D slots travel with the row-end protocol: equal away from rows 0 and H-1, so a chained
+    # gradient carries the DOWN slot at those rows only
+    m |= (nat.GRAD_IN_H_ROW_ENDS | nat.GRAD_IN_UD_SHARED if in_h_row_ends else 0) | \
+        (nat.GRAD_OUT_H_ROW_ENDS | nat.GRAD_OUT_UD_SHARED if out_h_row_ends else 0)
     dev = cost.device
     grad_cost = torch.empty_like(cost)
     if kind == nat.LBP_STEREO:

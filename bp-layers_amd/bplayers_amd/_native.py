@@ -22,6 +22,8 @@ GRAD_EXACT = 1
 GRAD_IN_H_ROW_ENDS = 0x10
 GRAD_OUT_H_ROW_ENDS = 0x20
 GRAD_ACCUMULATE = 0x40
+GRAD_OUT_UD_SHARED = 0x80
+GRAD_IN_UD_SHARED = 0x100
 FWD_MSG_IN_ZERO = 0x1  # BPL_FWD_MSG_IN_ZERO
 LBP_BWD_NOALLOC_MAX_BYTES = 128 << 20  # BPL_LBP_BWD_NOALLOC_MAX_BYTES
 

@@ -32,6 +32,10 @@ struct BwdArgs {
   // chained exact iterations: in_grad's LEFT/RIGHT slots are zero away from the row ends
   // (igz), grad_msg's LEFT/RIGHT slots need writing only at the row ends (osp)
   int igz, osp;
+  // chained U/D slots (BPL_GRAD_IN_UD_SHARED / BPL_GRAD_OUT_UD_SHARED): in_grad's DOWN slot is
+  // valid at rows 0 and H-1 only, its other rows being the UP slot's (uds_in); grad_msg's
+  // DOWN slot is written at rows 0 and H-1 only (uds_out)
+  int uds_in, uds_out;
   int accum_cost;  // grad_cost is added to (BPL_GRAD_ACCUMULATE) by its final writers
 };
 
@@ -52,6 +56,19 @@ __device__ __forceinline__ int64_t px(int n, int y, int x, int H, int W) {
 }
 __device__ __forceinline__ int64_t sl_px(int n, int s, int y, int x, int H, int W) {
   return (((int64_t)n * 4 + s) * H + y) * (int64_t)W + x;
+}
+
+// Slot of in_grad read at receiver row y of a pass whose messages land in slot rslot: with
+// BPL_GRAD_IN_UD_SHARED the DOWN slot holds its own values at rows 0 and H-1 only, its other
+// rows are those of the UP slot (the U and D slots of an input-message gradient are equal away
+// from those rows).
+__device__ __forceinline__ int ig_slot(const BwdArgs &a, int rslot, int y) {
+  return (a.uds_in && rslot == DOWN && y != 0 && y != a.H - 1) ? UP : rslot;
+}
+// the same as an element offset from the rslot plane, for in-plane pixel index qr
+__device__ __forceinline__ int64_t ig_shift(const BwdArgs &a, int rslot, int64_t qr) {
+  const int64_t W = a.W, H = a.H;
+  return (a.uds_in && rslot == DOWN && qr >= W && qr < (H - 1) * W) ? H * W * a.C : 0;
 }
 
 template <int KIND, int SEG, int LPL, bool VERT, int PF>
@@ -129,10 +146,11 @@ __global__ __launch_bounds__(64) void lbp_bwd_phase(BwdArgs a) {
     if (VERT) yq += fwd ? 1 : -1; else xq += fwd ? 1 : -1;
     const int64_t oq = sl_px(n, rslot, yq, xq, H, W);
     const int64_t pq = px(n, yq, xq, H, W) * C;
+    const int64_t og = sl_px(n, ig_slot(a, rslot, yq), yq, xq, H, W);
 #pragma unroll
     for (int k = 0; k < LPL; ++k) {
       const int c = min(sl + SEG * k, C - 1);  // clamped: loads stay unconditional
-      dg[k] = (!VERT && a.igz) ? 0.f : a.in_grad[oq * C + c];
+      dg[k] = (!VERT && a.igz) ? 0.f : a.in_grad[og * C + c];
       dm[k] = a.msgs[oq * C + c];
       da[k] = (int)a.argmin[oq * C + c];
       if (!VERT) {
@@ -451,7 +469,12 @@ __global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(B
   // 64-bit addresses.
   constexpr bool BUF = FASTK > 0;
   const uint32_t plane_bytes = (uint32_t)(HW * C * 4);
-  const __amdgpu_buffer_rsrc_t ig_rs = make_rsrc(ig_p, plane_bytes);
+  // BPL_GRAD_IN_UD_SHARED, UP pass (messages in the DOWN slot): rows 1 .. H-2 of in_grad come
+  // from the UP slot, the plane before (ig_slot); the resource then covers both planes
+  const bool uds = VERT && a.uds_in && rslot == DOWN;
+  const int64_t HWCi = HW * C;
+  auto ig_edge = [&](int64_t qr) { return qr < (int64_t)W || qr >= (int64_t)(H - 1) * W; };
+  const __amdgpu_buffer_rsrc_t ig_rs = uds ? make_rsrc(ig_p - HWCi, 2u * plane_bytes) : make_rsrc(ig_p, plane_bytes);
   const __amdgpu_buffer_rsrc_t m_rs = make_rsrc(m_p, plane_bytes);
   const __amdgpu_buffer_rsrc_t xd_rs = make_rsrc(XDp, plane_bytes);
   const __amdgpu_buffer_rsrc_t xu_rs = make_rsrc(XUp, plane_bytes);
@@ -526,9 +549,9 @@ __global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(B
 #pragma unroll
       for (int k = 0; k < LPL; ++k) dg[k] = 0.f;
     } else if (BUF) {
-      ld_row_b(ig_rs, so, dg);
+      ld_row_b(ig_rs, uds && ig_edge(qr) ? so + plane_bytes : so, dg);
     } else {
-      ld_row(ig_p + qr * C, dg);
+      ld_row(ig_p + qr * C - (uds && !ig_edge(qr) ? HWCi : 0), dg);
     }
     if (BUF) ld_row_b(m_rs, so, dm);
     else ld_row(m_p + qr * C, dm);
@@ -611,7 +634,9 @@ __global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(B
       const __amdgpu_buffer_rsrc_t xo_rs = make_rsrc(Xo, plane_bytes);
       const __amdgpu_buffer_rsrc_t gc_rs = make_rsrc(a.grad_cost + img, plane_bytes);
       const __amdgpu_buffer_rsrc_t gu_rs = make_rsrc(a.grad_msg ? gm_p + UP * HWC : gm_p, gm_bytes);
-      const __amdgpu_buffer_rsrc_t gd_rs = make_rsrc(a.grad_msg ? gm_p + DOWN * HWC : gm_p, gm_bytes);
+      // BPL_GRAD_OUT_UD_SHARED: the DOWN slot is written at rows 0 and H - 1 only
+      const bool gd_row = !a.uds_out || ln == 0 || ln == H - 1;
+      const __amdgpu_buffer_rsrc_t gd_rs = make_rsrc(a.grad_msg ? gm_p + DOWN * HWC : gm_p, gd_row ? gm_bytes : 0u);
       const __amdgpu_buffer_rsrc_t gl_rs = make_rsrc(a.grad_msg ? gm_p + LEFT * HWC : gm_p, gm_bytes);
       const __amdgpu_buffer_rsrc_t gr_rs = make_rsrc(a.grad_msg ? gm_p + RIGHT * HWC : gm_p, gm_bytes);
       // horizontal phase: the pairwise-bin and edge gradients of four consecutive steps (four
@@ -1063,7 +1088,7 @@ __global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(B
       st(a.grad_cost + img + q * C, cst);
       if (gm) {
         st(gm + UP * HWC, hr);
-        st(gm + DOWN * HWC, hr);
+        if (!a.uds_out || ln == 0 || ln == H - 1) st(gm + DOWN * HWC, hr);
         if (!kOsp || x == 0) st(gm + LEFT * HWC, sl);
         if (!kOsp || x == W - 1) st(gm + RIGHT * HWC, sr);
       }
@@ -1155,7 +1180,7 @@ __global__ __launch_bounds__(64) void lbp_bwd_semg(BwdArgs a) {
   auto load_step = [&](int s, int i) {
     const int64_t q = q0 + (int64_t)s * dq;
     const int64_t qr = q + dq;
-    pg[i] = (!VERT && a.igz) ? 0.f : ig_p[qr * C + cl];
+    pg[i] = (!VERT && a.igz) ? 0.f : ig_p[qr * C + cl - ig_shift(a, rslot, qr)];
     pm[i] = m_p[qr * C + cl];
     pa[i] = (int)am_p[qr * C + cl];
     pl[i] = (int)gl_p[qr];
@@ -1310,7 +1335,7 @@ __global__ __launch_bounds__(64) void lbp_bwd_semp(BwdArgs a) {
   auto load_step = [&](int s, int i) {
     const int64_t q = q0 + (int64_t)s * dq;
     const int64_t qr = q + dq;
-    pg[i] = (!VERT && a.igz) ? 0.f : ig_p[qr * C + cl];
+    pg[i] = (!VERT && a.igz) ? 0.f : ig_p[qr * C + cl - ig_shift(a, rslot, qr)];
     pm[i] = m_p[qr * C + cl];
     pa[i] = (int)am_p[qr * C + cl];
     pl[i] = (int)gl_p[qr];
@@ -1465,7 +1490,7 @@ __global__ void lbp_bwd_combine(const float *__restrict__ XL, const float *__res
                                 const float *__restrict__ XU, const float *__restrict__ XD,
                                 const float *__restrict__ in_grad, float *__restrict__ grad_cost,
                                 float *grad_msg, int N, int H, int W, int C, int exact,
-                                int osp, int copy_msgs, int accum_cost) {
+                                int osp, int copy_msgs, int accum_cost, int uds_out) {
   const int64_t total = (int64_t)N * H * W * C;
   const int64_t HWC = (int64_t)H * W * C;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
@@ -1493,7 +1518,8 @@ __global__ void lbp_bwd_combine(const float *__restrict__ XL, const float *__res
     if (!osp || x == 0) grad_msg[base + LEFT * HWC] = x == 0 ? (in_grad[base + LEFT * HWC] + xr) + sv : 0.f;
     if (!osp || x == W - 1) grad_msg[base + RIGHT * HWC] = x == W - 1 ? (in_grad[base + RIGHT * HWC] + xl) + sv : 0.f;
     grad_msg[base + UP * HWC] = y == 0 ? (in_grad[base + UP * HWC] + xd) + hr : hr;
-    grad_msg[base + DOWN * HWC] = y == H - 1 ? (in_grad[base + DOWN * HWC] + xu) + hr : hr;
+    if (!uds_out || y == 0 || y == H - 1)  // BPL_GRAD_OUT_UD_SHARED: the D slot at rows 0, H - 1 only
+      grad_msg[base + DOWN * HWC] = y == H - 1 ? (in_grad[base + DOWN * HWC] + xu) + hr : hr;
   }
 }
 
@@ -1680,14 +1706,16 @@ int lbp_bwd_impl(int kind, int mode, const float *jump, int64_t jump_k, int64_t 
                  void *workspace, int64_t workspace_bytes,
                  int64_t N, int64_t H, int64_t W, int64_t C, int delta, void *stream) {
   BPL_REQUIRE(kind >= 0 && kind <= 2, BPL_E_ARG, "bpl_lbp_bwd: unknown kind %d", kind);
-  const int hflags = mode & (BPL_GRAD_IN_H_ROW_ENDS | BPL_GRAD_OUT_H_ROW_ENDS);
+  const int hflags = mode & (BPL_GRAD_IN_H_ROW_ENDS | BPL_GRAD_OUT_H_ROW_ENDS | BPL_GRAD_IN_UD_SHARED |
+                            BPL_GRAD_OUT_UD_SHARED);
   const bool accumulate = (mode & BPL_GRAD_ACCUMULATE) != 0;
-  mode &= ~(BPL_GRAD_IN_H_ROW_ENDS | BPL_GRAD_OUT_H_ROW_ENDS | BPL_GRAD_ACCUMULATE);
+  mode &= ~(BPL_GRAD_IN_H_ROW_ENDS | BPL_GRAD_OUT_H_ROW_ENDS | BPL_GRAD_IN_UD_SHARED | BPL_GRAD_OUT_UD_SHARED |
+            BPL_GRAD_ACCUMULATE);
   BPL_REQUIRE(!accumulate || kind != BPL_LBP_SEMANTIC_PIXEL, BPL_E_ARG,
               "bpl_lbp_bwd: BPL_GRAD_ACCUMULATE applies to the stereo and semantic-global kinds");
   BPL_REQUIRE(mode == BPL_GRAD_REFERENCE || mode == BPL_GRAD_EXACT, BPL_E_ARG, "bpl_lbp_bwd: unknown mode %d", mode);
   BPL_REQUIRE(hflags == 0 || mode == BPL_GRAD_EXACT, BPL_E_ARG,
-              "bpl_lbp_bwd: the row-end flags apply to the exact mode only");
+              "bpl_lbp_bwd: the row-end and shared-slot flags apply to the exact mode only");
   BPL_REQUIRE(N >= 1 && H >= 1 && W >= 1 && C >= 1, BPL_E_ARG, "bpl_lbp_bwd: sizes must be positive");
   BPL_REQUIRE(C <= 255, BPL_E_LIMIT, "bpl_lbp_bwd: C=%lld labels exceeds the supported 255", (long long)C);
   BPL_REQUIRE(N * 2 * (H > W ? H : W) < (1LL << 31) && H < (1 << 30) && W < (1 << 30), BPL_E_LIMIT,
@@ -1732,6 +1760,8 @@ int lbp_bwd_impl(int kind, int mode, const float *jump, int64_t jump_k, int64_t 
   a.delta = delta;
   a.igz = (hflags & BPL_GRAD_IN_H_ROW_ENDS) != 0;
   a.osp = (hflags & BPL_GRAD_OUT_H_ROW_ENDS) != 0;
+  a.uds_in = (hflags & BPL_GRAD_IN_UD_SHARED) != 0;
+  a.uds_out = (hflags & BPL_GRAD_OUT_UD_SHARED) != 0;
   a.grad_cost = grad_cost; a.grad_msg = grad_msg;
   a.accum_cost = accumulate ? 1 : 0;
   // accumulating: this call's pairwise and edge gradients go to workspace copies first
@@ -1797,7 +1827,7 @@ int lbp_bwd_impl(int kind, int mode, const float *jump, int64_t jump_k, int64_t 
     KernelTimer kt(BPL_KERNEL_LBP_BWD_COMBINE, st);
     hipLaunchKernelGGL(lbp_bwd_combine, dim3((unsigned)blocks), dim3(threads), 0, st, a.X[LEFT], a.X[RIGHT],
                        a.X[UP], a.X[DOWN], in_grad, grad_cost, grad_msg, (int)N, (int)H, (int)W, (int)C,
-                       mode == BPL_GRAD_EXACT ? 1 : 0, a.osp, grad_msg != in_grad ? 1 : 0, a.accum_cost);
+                       mode == BPL_GRAD_EXACT ? 1 : 0, a.osp, grad_msg != in_grad ? 1 : 0, a.accum_cost, a.uds_out);
     int rc = check_hip(hipGetLastError(), "bpl_lbp_bwd combine launch");
     if (rc) return rc;
   }

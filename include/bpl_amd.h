@@ -60,6 +60,18 @@ extern "C" {
 #define BPL_GRAD_IN_H_ROW_ENDS 0x10
 #define BPL_GRAD_OUT_H_ROW_ENDS 0x20
 /*
+ * Likewise for the UP and DOWN slots, which are equal in the exact adjoint away from rows 0
+ * and H-1 (both are the horizontal passes' h-gradient; only the pass-through of a vertical
+ * pass's first pixel adds to one of them there):
+ *   BPL_GRAD_OUT_UD_SHARED   grad_msg's DOWN slot is written only at rows 0 and H-1; its
+ *                            other rows are left as they were (they equal the UP slot's);
+ *   BPL_GRAD_IN_UD_SHARED    in_grad's DOWN slot is read only at rows 0 and H-1; its other
+ *                            rows are read from the UP slot.
+ * A grad_msg written with OUT_UD_SHARED is a valid in_grad for IN_UD_SHARED.
+ */
+#define BPL_GRAD_OUT_UD_SHARED 0x80
+#define BPL_GRAD_IN_UD_SHARED 0x100
+/*
  * BPL_GRAD_ACCUMULATE, OR-ed into either mode: grad_cost, grad_jump and grad_edge are added
  * to (their contents plus this iteration's gradient) instead of overwritten, so the calls of
  * a chain of iterations sum the parameter gradients in place -- what autograd does across the

@@ -60,11 +60,11 @@ def test_argument_validation_returns_codes(lib):
     rc = lib.bpl_lbp_bwd(nat.LBP_STEREO, 5, P, 6, 1, P, P, P, P, P, P, P, P, P, P, 1 << 40, 1, 4, 4, 8, 3, None)
     assert rc == -1 and b"mode" in lib.bpl_last_error()
     # the chained-iteration row-end flags belong to the exact mode only
-    for fl in (nat.GRAD_IN_H_ROW_ENDS, nat.GRAD_OUT_H_ROW_ENDS):
+    for fl in (nat.GRAD_IN_H_ROW_ENDS, nat.GRAD_OUT_H_ROW_ENDS, nat.GRAD_IN_UD_SHARED, nat.GRAD_OUT_UD_SHARED):
         rc = lib.bpl_lbp_bwd(nat.LBP_STEREO, nat.GRAD_REFERENCE | fl, P, 6, 1, P, P, P, P, P, P, P, P, P, P,
                              1 << 40, 1, 4, 4, 8, 3, None)
         assert rc == -1 and b"exact mode only" in lib.bpl_last_error()
-    rc = lib.bpl_lbp_bwd(nat.LBP_STEREO, nat.GRAD_EXACT | 0x80, P, 6, 1, P, P, P, P, P, P, P, P, P, P,
+    rc = lib.bpl_lbp_bwd(nat.LBP_STEREO, nat.GRAD_EXACT | 0x200, P, 6, 1, P, P, P, P, P, P, P, P, P, P,
                          1 << 40, 1, 4, 4, 8, 3, None)
     assert rc == -1 and b"unknown mode" in lib.bpl_last_error()
     # accumulation across iterations: stereo and semantic-global kinds only

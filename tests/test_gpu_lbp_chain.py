@@ -1,6 +1,7 @@
 """GPU checks of the chained exact backward (in_h_row_ends / out_h_row_ends): a backward pass
 through several iterations whose calls skip the LEFT/RIGHT slots of the message gradients away
-from the row ends gives bit-identical gradients to the same chain without the flags.
+from the row ends, and the DOWN slot away from rows 0 and H-1 (equal to the UP slot there),
+gives bit-identical gradients to the same chain without the flags.
 
 The skipped entries of every intermediate gradient are overwritten with NaN before the next
 call reads it, so a kernel that still read one would put NaN into its outputs."""
@@ -12,7 +13,7 @@ from test_gpu_lbp_fwd import make_case, _mods
 
 pytestmark = pytest.mark.gpu
 
-LEFT, RIGHT = 0, 1
+LEFT, RIGHT, UP, DOWN = 0, 1, 2, 3
 ITERS = 3
 
 
@@ -25,10 +26,12 @@ def _backward(mod, kind, cost, jump, edge, g, msgs, am, sc, delta, **flags):
 
 
 def _poison_interior(gm):
-    """NaN into the LEFT/RIGHT entries a row-end-only gradient leaves unwritten."""
+    """NaN into the entries a chained gradient leaves unwritten: LEFT/RIGHT away from the row
+    ends, DOWN away from rows 0 and H-1."""
     gm = gm.clone()
     gm[:, LEFT, :, 1:, :] = float("nan")
     gm[:, RIGHT, :, :-1, :] = float("nan")
+    gm[:, DOWN, 1:-1] = float("nan")
     return gm
 
 
@@ -80,7 +83,10 @@ def test_chained_backward_equals_unchained(case):
             assert torch.equal(a, b), f"iteration {it}: grad {name} differs"
         gr, gc = ref[3], chain[3]
         if it > 0:
-            assert torch.equal(gr[:, 2:], gc[:, 2:]), f"iteration {it}: U/D message gradient differs"
+            assert torch.equal(gr[:, UP], gc[:, UP]), f"iteration {it}: U message gradient differs"
+            assert torch.equal(gr[:, DOWN, 0], gc[:, DOWN, 0]) and torch.equal(gr[:, DOWN, -1], gc[:, DOWN, -1])
+            # the exact adjoint's D slot equals the U slot away from rows 0 and H-1
+            assert torch.equal(gr[:, DOWN, 1:-1], gr[:, UP, 1:-1])
             assert torch.equal(gr[:, LEFT, :, 0], gc[:, LEFT, :, 0])
             assert torch.equal(gr[:, RIGHT, :, -1], gc[:, RIGHT, :, -1])
             # the exact adjoint's L/R slots really are zero away from the row ends
