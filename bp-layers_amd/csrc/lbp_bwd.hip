@@ -1223,13 +1223,7 @@ __global__ __launch_bounds__(64) void lbp_bwd_semg(BwdArgs a) {
         }
         wave_lds_sync();
         // route: sender label c gathers the receivers routed to it, in label order
-        float gh = 0.f;
-#pragma unroll
-        for (int rr = 0; rr < CM; rr += 2) {  // two entries per 16-byte read, in label order
-          const int4 v = *reinterpret_cast<const int4 *>(&lds_ag[seg][rr]);
-          gh += v.x == c ? __int_as_float(v.y) : 0.f;
-          gh += v.z == c ? __int_as_float(v.w) : 0.f;
-        }
+        const float gh = routed_gather<CM>(lds_ag[seg], c);  // two entries per 16-byte read
         // parameter gradients of (dir, sender): edge sum_c M[r][c] g(c) and matrix entry
         // [r][c] (a truncation winner uses the truncation entry [0][C-1] in the exact mode)
         const float w = pw[i];
@@ -1380,13 +1374,7 @@ __global__ __launch_bounds__(64) void lbp_bwd_semp(BwdArgs a) {
           lds_ag[seg][c] = make_int2(lab ? r : -1, __float_as_int(gk));
         }
         wave_lds_sync();
-        float gh = 0.f;
-#pragma unroll
-        for (int rr = 0; rr < CM; rr += 2) {  // two entries per 16-byte read, in label order
-          const int4 v = *reinterpret_cast<const int4 *>(&lds_ag[seg][rr]);
-          gh += v.x == c ? __int_as_float(v.y) : 0.f;
-          gh += v.z == c ? __int_as_float(v.w) : 0.f;
-        }
+        const float gh = routed_gather<CM>(lds_ag[seg], c);  // two entries per 16-byte read
         wave_lds_sync();  // the gathers are done before the next step's writes
         const float w = pw[i];
         const bool to_tr = tr && EXACT;
