@@ -238,10 +238,10 @@ struct OpMin {
 template <int SEG, class Op>
 __device__ __forceinline__ float seg_reduce(float v, Op op) {
   static_assert(SEG == 16 || SEG == 32 || SEG == 64, "segment width");
-  v = op(v, dpp_f<kDppXor1>(v));
-  v = op(v, dpp_f<kDppXor2>(v));
-  v = op(v, dpp_f<kDppHalfMirror>(v));
-  v = op(v, dpp_f<kDppMirror>(v));
+  v = op(v, dpp_src_f<kDppXor1>(v));  // the moves fold into the operation (v_add_f32_dpp, ...)
+  v = op(v, dpp_src_f<kDppXor2>(v));
+  v = op(v, dpp_src_f<kDppHalfMirror>(v));
+  v = op(v, dpp_src_f<kDppMirror>(v));
   if constexpr (SEG >= 32) {
     float lo, hi;
     swap_rows(v, lo, hi);

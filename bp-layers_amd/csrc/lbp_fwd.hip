@@ -286,14 +286,25 @@ __global__ __launch_bounds__(64) void lbp_fwd_phase(FwdArgs a) {
 // M[dir][.][c] of its label in registers (CM values, 0 past C); per step the segment's h row
 // goes through LDS (+inf past C) and comes back as wave-uniform vector reads, the CM
 // candidates h[l] + w M[l][c] form a min3 tree, and the winner is the last l equal to the
-// minimum (the reference's `<=` scan from msg = 1e15 in label order).  Normalisation and
-// outputs as in lbp_fwd_phase (max-shifted log-sum-exp, scale = exp(lse), label of the
-// minimum 0 in this branch).
+// minimum (the reference's `<=` scan from msg = 1e15 in label order).  Outputs as in
+// lbp_fwd_phase: msg - lse (max-shifted log-sum-exp), scale = exp(lse), label of the minimum 0.
+//
+// The chain runs in a shifted frame: the next sender's h takes u = msg - min_l h[l] in place of
+// the normalised msg - lse.  A min-plus step commutes with a constant shift of h, so the
+// decisions are those of the normalised chain up to rounding, and u lies in [w min M, w max M]
+// whatever the magnitudes of h.  The normalisation, the argmin and the stores then hang off the
+// step-to-step dependency chain (h -> LDS -> candidates -> minimum -> u) instead of lying on
+// it.  The shifted frame differs from the normalised one by a per-scanline constant k (k = 0 at
+// the first sender, k' = lse - shift per step), so the stored message msg - lse needs no
+// correction and scale = exp(lse - k).
+// Planes are addressed through buffer resources (every [H,W,C] plane below 2^31 bytes) with one
+// scalar offset per position; lanes past C or past the last scanline store to an offset past
+// the resource (dropped), so the steps have no branches.
 // ---------------------------------------------------------------------------------------
 template <int CM, bool VERT, int PF>
 __global__ __launch_bounds__(64) void lbp_fwd_semg(FwdArgs a) {
   constexpr int SEG = 32, SPW = 2;
-  __shared__ float lds_h[SPW][CM];
+  __shared__ float lds_h[SPW][SEG];  // a row per segment: every lane writes its slot
   const int H = a.H, W = a.W, C = a.C;
   const int L = VERT ? W : H;
   const int len = VERT ? H : W;
@@ -317,100 +328,160 @@ __global__ __launch_bounds__(64) void lbp_fwd_semg(FwdArgs a) {
   const bool st_ok = line_ok && lab;
   const int cl = lab ? c : C - 1;  // label loaded by this lane
   const int64_t HW = (int64_t)H * W;
-  const int64_t dq = VERT ? (fwd ? (int64_t)W : -(int64_t)W) : (fwd ? 1 : -1);
-  const int64_t q0 = VERT ? (int64_t)(fwd ? 0 : H - 1) * W + ln : (int64_t)ln * W + (fwd ? 0 : W - 1);
-  const float *cost_p = a.cost + (int64_t)n * HW * C;
-  const float *x1_p = (VERT ? a.msg_out : a.msg_in) + ((int64_t)n * 4 + (VERT ? RIGHT : UP)) * HW * C;
-  const float *x2_p = (VERT ? a.msg_out : a.msg_in) + ((int64_t)n * 4 + (VERT ? LEFT : DOWN)) * HW * C;
-  float *out_p = a.msg_out + ((int64_t)n * 4 + rslot) * HW * C;
-  uint8_t *arg_p = a.argmin ? a.argmin + ((int64_t)n * 4 + rslot) * HW * C : nullptr;
-  const int64_t rs_base = ((int64_t)n * 4 + rslot) * HW;
-  const float *edge_p = a.edge + ((int64_t)n * 4 + dir) * HW;
+  const int64_t HWC = HW * C;
+  // pixel at position p of the scanline: lbase + p * pstride.  Wave-uniform byte offsets of a
+  // position in planes of C floats, C bytes and one float per pixel; the scalar offsets of the
+  // sender and receiver positions advance by a signed step per sender (always in range).
+  const int lbase = VERT ? ln : ln * W;
+  const uint32_t pstride = VERT ? (uint32_t)W : 1u;
+  const uint32_t st_f = pstride * (uint32_t)C * 4u, st_b = pstride * (uint32_t)C, st_p = pstride * 4u;
+  const uint32_t p_first = fwd ? 0u : (uint32_t)nsteps;
+  const uint32_t d_f = fwd ? st_f : 0u - st_f, d_b = fwd ? st_b : 0u - st_b, d_p = fwd ? st_p : 0u - st_p;
+  auto pos = [&](int s) -> uint32_t { return (uint32_t)(fwd ? s : nsteps - s); };
+  // per-lane byte offsets of the scanline's pixel 0: loads, stores (dropped past C or past L)
+  const uint32_t v_ld = (uint32_t)(lbase * C + cl) * 4u;
+  const uint32_t v_st = st_ok ? (uint32_t)(lbase * C + c) * 4u : kBufDrop;
+  const uint32_t v_stb = st_ok ? (uint32_t)(lbase * C + c) : kBufDrop;
+  const uint32_t v_e = (uint32_t)lbase * 4u;
+  const uint32_t v_pp = (line_ok && c == 0) ? (uint32_t)lbase * 4u : kBufDrop;
+  const uint32_t plane = (uint32_t)(HWC * 4);
+  const float *xb = VERT ? a.msg_out : a.msg_in;
+  const auto cost_r = make_rsrc(a.cost + (int64_t)n * HWC, plane);
+  const auto x1_r = make_rsrc(xb + ((int64_t)n * 4 + (VERT ? RIGHT : UP)) * HWC, plane);
+  const auto x2_r = make_rsrc(xb + ((int64_t)n * 4 + (VERT ? LEFT : DOWN)) * HWC, plane);
+  const auto mi_r = make_rsrc(a.msg_in + ((int64_t)n * 4 + rslot) * HWC, plane);
+  const auto out_r = make_rsrc(a.msg_out + ((int64_t)n * 4 + rslot) * HWC, plane);
+  const auto arg_r = a.argmin ? make_rsrc(a.argmin + ((int64_t)n * 4 + rslot) * HWC, plane / 4u)
+                              : make_rsrc(a.msg_out, 0u);
+  const auto edge_r = make_rsrc(a.edge + ((int64_t)n * 4 + dir) * HW, (uint32_t)HW * 4u);
+  const auto sc_r = a.scale ? make_rsrc(a.scale + ((int64_t)n * 4 + rslot) * HW, (uint32_t)HW * 4u)
+                            : make_rsrc(a.msg_out, 0u);
 
   // this label's column of the direction's matrix
   float Mc[CM];
 #pragma unroll
   for (int l = 0; l < CM; ++l) Mc[l] = (l < C && lab) ? a.jump[((int64_t)dir * C + l) * C + c] : 0.f;
 
-  // the first sender's chain slot is never written by this pass
-  float chain = a.msg_in[(rs_base + q0) * C + cl];
-  if (st_ok) {
-    out_p[q0 * C + c] = chain;
-    if (arg_p) arg_p[q0 * C + c] = 0;
+  // the label of the minimum is 0 in this branch at every position of the scanline
+  if (a.gargmin) {
+    const auto ga_r = make_rsrc(a.gargmin + ((int64_t)n * 4 + rslot) * HW, (uint32_t)HW);
+    for (int p = c; p < len; p += SEG)
+      buf_st8(ga_r, line_ok ? (uint32_t)lbase + (uint32_t)p * pstride : kBufDrop, 0u, 0);
   }
-  if (line_ok && c == 0) {
-    if (a.scale) a.scale[rs_base + q0] = 0.f;
-    if (a.gargmin) a.gargmin[rs_base + q0] = 0;
-  }
+  // the first sender's chain slot is never written by this pass: it keeps the input message
+  float u = buf_ld(mi_r, v_ld, p_first * st_f);
+  buf_st(out_r, v_st, p_first * st_f, u);
+  buf_st8(arg_r, v_stb, p_first * st_b, 0);
+  buf_st(sc_r, v_pp, p_first * st_p, 0.f);
+  float k = 0.f;  // shifted frame minus the normalised frame, per scanline
 
   float pc[PF], p1[PF], p2[PF], pw[PF];
-  auto load_step = [&](int s, int i) {
-    const int64_t q = q0 + (int64_t)s * dq;
-    pc[i] = cost_p[q * C + cl];
-    p1[i] = x1_p[q * C + cl];
-    p2[i] = x2_p[q * C + cl];
-    pw[i] = edge_p[q];
+  auto load_at = [&](uint32_t of, uint32_t op, int i) {
+    pc[i] = buf_ld(cost_r, v_ld, of);
+    p1[i] = buf_ld(x1_r, v_ld, of);
+    p2[i] = buf_ld(x2_r, v_ld, of);
+    pw[i] = buf_ld(edge_r, v_e, op);
   };
+  auto load_step = [&](int s, int i) { load_at(pos(s) * st_f, pos(s) * st_p, i); };
 #pragma unroll
   for (int i = 0; i < PF; ++i) {
     load_step(i < nsteps ? i : (nsteps > 0 ? nsteps - 1 : 0), i);
     __builtin_amdgcn_sched_barrier(0);  // keep the ring slots in issue order
   }
-  for (int s0 = 0; s0 < nsteps; s0 += PF) {
+  // receiver offsets of the next step
+  uint32_t o_f = p_first * st_f + d_f, o_b = p_first * st_b + d_b, o_p = p_first * st_p + d_p;
+  auto step = [&](int i) {
+    // the chain: h -> LDS -> candidates -> minimum -> u
+    const float h = ((pc[i] + p1[i]) + p2[i]) + u;
+    const float hl = lab ? h : __builtin_inff();
+    lds_h[seg][c] = hl;
+    wave_lds_sync();
+    // the shift min_l h[l], over the lanes while the row makes its LDS round trip
+    const float shift = fminf(seg_reduce<SEG>(hl, OpMin()), 1e15f);  // all-infinite h: the clamp
+    const float w = pw[i];
+    float hv[CM], nv[CM];
 #pragma unroll
-    for (int i = 0; i < PF; ++i) {
-      const int s = s0 + i;
-      if (s < nsteps) {
-        float h = VERT ? ((pc[i] + p1[i]) + p2[i]) + chain : ((pc[i] + chain) + p1[i]) + p2[i];
-        if (c < CM) lds_h[seg][c] = lab ? h : __builtin_inff();
-        wave_lds_sync();
-        const float w = pw[i];
-        float nv[CM];
+    for (int l = 0; l < CM; ++l) hv[l] = lds_h[seg][l];
+    wave_lds_sync();  // the reads are done before the next step's write
 #pragma unroll
-        for (int l = 0; l < CM; ++l) nv[l] = fmaf(Mc[l], w, lds_h[seg][l]);
-        wave_lds_sync();  // the reads are done before the next step's write
-        // minimum: a min3 tree over the candidates and the initial 1e15
-        float m[CM + 1];
+    for (int l = 0; l < CM; ++l) nv[l] = fmaf(Mc[l], w, hv[l]);
+    // minimum: a min3 tree over the candidates and the initial 1e15
+    float m[CM + 1];
 #pragma unroll
-        for (int l = 0; l < CM; ++l) m[l] = nv[l];
-        m[CM] = 1e15f;
-        int cnt = CM + 1;
+    for (int l = 0; l < CM; ++l) m[l] = nv[l];
+    m[CM] = 1e15f;
+    int cnt = CM + 1;
 #pragma unroll
-        for (int r = 0; r < 6; ++r) {
-          if (cnt > 1) {
-            int o = 0;
+    for (int r = 0; r < 6; ++r) {
+      if (cnt > 1) {
+        int o = 0;
 #pragma unroll
-            for (int t = 0; t < CM + 1; t += 3) {
-              if (t < cnt) {
-                float v = m[t];
-                if (t + 1 < cnt) v = fminf(v, m[t + 1]);
-                if (t + 2 < cnt) v = fminf(v, m[t + 2]);
-                m[o++] = v;
-              }
-            }
-            cnt = o;
+        for (int t = 0; t < CM + 1; t += 3) {
+          if (t < cnt) {
+            float v = m[t];
+            if (t + 1 < cnt) v = fminf(v, m[t + 1]);
+            if (t + 2 < cnt) v = fminf(v, m[t + 2]);
+            m[o++] = v;
           }
         }
-        const float msg = m[0];
-        int arg = 0;
-#pragma unroll
-        for (int l = 0; l < CM; ++l) arg = nv[l] == msg ? l : arg;
-        // normalisation: m - log sum_c exp(m), max-shifted
-        float mx = seg_reduce<SEG>(lab ? msg : -__builtin_inff(), OpMax());
-        float se = seg_reduce<SEG>(lab ? fast_exp(msg - mx) : 0.f, OpSum());
-        const float lse = mx + fast_log(se);
-        chain = msg - lse;
-        const int64_t qq = q0 + (int64_t)(s + 1) * dq;
-        if (st_ok) {
-          out_p[qq * C + c] = chain;
-          if (arg_p) arg_p[qq * C + c] = (uint8_t)arg;
-        }
-        if (line_ok && c == 0) {
-          if (a.scale) a.scale[rs_base + qq] = fast_exp(lse);
-          if (a.gargmin) a.gargmin[rs_base + qq] = 0;
-        }
+        cnt = o;
       }
-      load_step(s + PF < nsteps ? s + PF : nsteps - 1, i);
+    }
+    const float msg = m[0];
+    u = msg - shift;
+    // off the chain: normalisation, argmin (the last label equal to the minimum), stores
+    const float mx = seg_reduce<SEG>(lab ? msg : -__builtin_inff(), OpMax());
+    const float se = seg_reduce<SEG>(lab ? fast_exp(msg - mx) : 0.f, OpSum());
+    const float lse = mx + fast_log(se);
+    int am[CM];
+#pragma unroll
+    for (int l = 0; l < CM; ++l) am[l] = nv[l] == msg ? l : 0;
+    int acnt = CM;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (acnt > 1) {
+        int o = 0;
+#pragma unroll
+        for (int t = 0; t < CM; t += 3) {
+          if (t < acnt) {
+            int v = am[t];
+            if (t + 1 < acnt) v = max(v, am[t + 1]);
+            if (t + 2 < acnt) v = max(v, am[t + 2]);
+            am[o++] = v;
+          }
+        }
+        acnt = o;
+      }
+    }
+    buf_st(out_r, v_st, o_f, msg - lse);
+    buf_st8(arg_r, v_stb, o_b, (uint8_t)am[0]);
+    buf_st(sc_r, v_pp, o_p, fast_exp(lse - k));
+    k = lse - shift;
+    o_f += d_f;
+    o_b += d_b;
+    o_p += d_p;
+  };
+  // main loop: the ring refills stay within the scanline (s + PF <= nsteps - 1)
+  int s0 = 0;
+  uint32_t l_f = pos(PF < nsteps ? PF : 0) * st_f, l_p = pos(PF < nsteps ? PF : 0) * st_p;
+  for (; s0 + 2 * PF <= nsteps; s0 += PF) {
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      step(i);
+      // refill ring slot i (its old contents are dead now: no register copies, no drain)
+      load_at(l_f, l_p, i);
+      l_f += d_f;
+      l_p += d_p;
+    }
+  }
+  // the last groups: refills clamped to the last sender, steps past the end skipped
+  for (; s0 < nsteps; s0 += PF) {
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      if (s0 + i < nsteps) {
+        step(i);
+        load_step(min(s0 + i + PF, nsteps - 1), i);
+      }
     }
   }
 }
@@ -1504,6 +1575,10 @@ void launch_stereo_w64_any(const FwdArgs &a, bool vert, hipStream_t st) {
 }
 
 bool stereo_w64_ok(const FwdArgs &a) { return a.C > 16 && a.delta < a.C; }
+// the semantic-global scanline kernel: full min-plus branch, C <= 32, [H,W,C] planes below 2^31 bytes
+bool semg_fast(const FwdArgs &a) {
+  return a.delta >= a.C && a.C <= 32 && (int64_t)a.H * a.W * a.C * 4 < (1LL << 31) && !generic_only();
+}
 
 void launch_stereo_w64_all(const FwdArgs &a, bool vert, hipStream_t st) {
   const int C = a.C;
@@ -1601,8 +1676,8 @@ static int lbp_fwd_impl(int kind, const float *cost, const float *jump, int64_t 
         else launch_kind<BPL_LBP_STEREO>(a, vert, st, shm);
         break;
       case BPL_LBP_SEMANTIC_GLOBAL:
-        if (a.delta >= a.C && a.C <= 20 && !generic_only()) launch_semg<20>(a, vert, st);
-        else if (a.delta >= a.C && a.C <= 32 && !generic_only()) launch_semg<32>(a, vert, st);
+        if (semg_fast(a) && a.C <= 20) launch_semg<20>(a, vert, st);
+        else if (semg_fast(a)) launch_semg<32>(a, vert, st);
         else launch_kind<BPL_LBP_SEMANTIC_GLOBAL>(a, vert, st, shm);
         break;
       default:
