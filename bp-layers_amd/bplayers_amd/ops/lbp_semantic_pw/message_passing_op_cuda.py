@@ -38,6 +38,7 @@ class LBPMinSumFunction(torch.autograd.Function):
         ctx.save_for_backward(cost, jump_in, edge, messages, messages_argmin, message_scale)
         ctx.single_pw = bool(single_pw)
         ctx.grad_mode = grad_mode
+        ctx.delta = int(delta)
         return messages
 
     @staticmethod
@@ -47,7 +48,7 @@ class LBPMinSumFunction(torch.autograd.Function):
         # reference mode: grad_messages = in_grad passes through without a copy
         grad_cost, grad_jump, grad_edge, grad_message = lbp.backward_minsum(
             cost, jump.contiguous(), edge, g, messages, messages_argmin, message_scale,
-            mode=ctx.grad_mode, grad_messages_out=g if ctx.grad_mode == 'reference' else None)
+            mode=ctx.grad_mode, delta=ctx.delta, grad_messages_out=g if ctx.grad_mode == 'reference' else None)
         return (grad_cost, None, None, grad_edge, grad_message, None, unpack_pairwise_grad(grad_jump, ctx.single_pw),
                 None, None)
 
@@ -65,6 +66,7 @@ class LBPMinSumExactChainFunction(torch.autograd.Function):
         ctx.save_for_backward(cost, jump_in, edge, *[x for r in recs for x in r])
         ctx.single_pw = bool(single_pw)
         ctx.iters = int(max_iter)
+        ctx.delta = int(delta)
         return out
 
     @staticmethod
@@ -72,7 +74,7 @@ class LBPMinSumExactChainFunction(torch.autograd.Function):
         cost, jump_in, edge, *flat = ctx.saved_tensors
         recs = [tuple(flat[3 * i:3 * i + 3]) for i in range(ctx.iters)]
         grad_cost, grad_jump, grad_edge, grad_message = _lbp_op.exact_chain_backward(
-            nat.LBP_SEMANTIC_GLOBAL, cost, jump_in, edge, in_grad, recs)
+            nat.LBP_SEMANTIC_GLOBAL, cost, jump_in, edge, in_grad, recs, delta=ctx.delta)
         return grad_cost, grad_edge, grad_message, None, unpack_pairwise_grad(grad_jump, ctx.single_pw), None, None
 
 

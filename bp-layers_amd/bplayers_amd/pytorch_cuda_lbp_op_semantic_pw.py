@@ -11,9 +11,10 @@ def forward_minsum(cost, jump, edge, messages, delta, *, out=None):
 
 
 def backward_minsum(cost, jump, edge, in_grad, messages, messages_argmin, message_scale, *, mode="reference",
-                    in_h_row_ends=False, out_h_row_ends=False, grad_messages_out=None):
+                    delta=None, in_h_row_ends=False, out_h_row_ends=False, grad_messages_out=None):
     """Gradients [grad_unary, grad_pairwise, grad_edge, grad_messages] of one iteration
-    (``in_h_row_ends`` / ``out_h_row_ends``: chained exact iterations, see _lbp_op)."""
+    (``delta``: the forward's, when known; ``in_h_row_ends`` / ``out_h_row_ends``: chained exact
+    iterations, see _lbp_op)."""
     return _lbp_op.backward_minsum(KIND, cost, jump, edge, in_grad, messages, messages_argmin, message_scale, mode=mode,
-                                   in_h_row_ends=in_h_row_ends, out_h_row_ends=out_h_row_ends,
+                                   delta=delta, in_h_row_ends=in_h_row_ends, out_h_row_ends=out_h_row_ends,
                                    grad_messages_out=grad_messages_out)

@@ -1107,21 +1107,32 @@ __global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(B
 // ---------------------------------------------------------------------------------------
 // Semantic global pairwise, 16 < C <= CM (20 or 32): the adjoint of one pass with
 // lbp_bwd_phase's semantics, for either forward branch.  Two scanlines per wave, label
-// c = lane & 31.  Per step the receiver gradients g(c) and their routing targets (the argmin,
-// or the label of the minimum for a truncation-term winner) go through LDS; sender label l
-// gathers g(c) over the receivers routed to it, in label order.  The matrix gradient entry
-// [r][c] += w g(c) (r the argmin; a truncation winner credits [0][C-1] in the exact mode, as a
-// segment sum added by the lane owning column C-1, and [label of the minimum][c] in the
-// reference mode) accumulates in an LDS copy of the segment's partial matrix; column c is
-// touched only by lane c, so the sums are ordered and deterministic.  The partial is written
-// once at the end for lbp_bwd_reduce_partials.
+// c = lane & 31.
+// Routing: receiver c writes its gradient g(c) into cell [r][c] of the segment's routing matrix
+// in LDS (r its routing target: the argmin, or the label of the minimum for a truncation-term
+// winner) after clearing the cell it wrote one step earlier; sender label l then sums row l in
+// receiver-label order.  The row holds g(c) where c was routed to l and zero elsewhere, so the
+// sum is the routed gradient with the additions of a label-order gather (bit for bit).
+// The matrix gradient entry [r][c] += w g(c) (r the argmin; a truncation winner credits
+// [0][C-1] in the exact mode, as a segment sum added by the lane owning column C-1, and [label
+// of the minimum][c] in the reference mode) accumulates in an LDS copy of the segment's partial
+// matrix; column c is touched only by lane c, so the sums are ordered and deterministic.  Its
+// read and that of the matrix entry for the edge gradient are issued at the top of the step,
+// off the step-to-step chain (in_grad + chain -> normalisation sum -> routing -> chain).  The
+// partial is written once at the end for lbp_bwd_reduce_partials.
+// TRUNC = false: the forward ran the full min-plus branch (delta >= C), so no argmin marks a
+// truncation-term winner and the label-of-the-minimum bytes are not read.
+// Planes are addressed through buffer resources (every [H,W,C] plane below 2^31 bytes) with
+// scalar offsets that advance per step; lanes past C or past the last scanline store to an
+// offset past the resource (dropped).
 // ---------------------------------------------------------------------------------------
-template <int CM, bool VERT, int PF>
+template <int CM, bool VERT, int PF, bool TRUNC>
 __global__ __launch_bounds__(64) void lbp_bwd_semg(BwdArgs a) {
   constexpr int SEG = 32, SPW = 2;
-  __shared__ __align__(16) int2 lds_ag[SPW][CM];  // {routing target, gradient bits}
-  __shared__ float lds_mat[CM * CM];       // the direction's matrix [r][c], stride CM
-  __shared__ float lds_part[SPW][CM * CM];  // the segment's matrix-gradient partial [r][c]
+  constexpr int RS = CM + 4;  // routing-matrix row stride (16-byte rows)
+  __shared__ __align__(16) float lds_r[SPW][SEG * RS + SEG];  // rows 0..31, then a dummy row
+  __shared__ float lds_mat[CM * CM];                  // the direction's matrix [r][c], stride CM
+  __shared__ float lds_part[SPW][CM * CM + SEG];      // matrix-gradient partial [r][c], dummy slots
   const int H = a.H, W = a.W, C = a.C;
   const int L = VERT ? W : H;
   const int len = VERT ? H : W;
@@ -1145,52 +1156,70 @@ __global__ __launch_bounds__(64) void lbp_bwd_semg(BwdArgs a) {
   const bool st_ok = line_ok && lab;
   const int cl = lab ? c : C - 1;
   const int64_t HW = (int64_t)H * W;
-  const int64_t dq = VERT ? (fwd ? (int64_t)W : -(int64_t)W) : (fwd ? 1 : -1);
-  const int64_t q0 = VERT ? (int64_t)(fwd ? 0 : H - 1) * W + ln : (int64_t)ln * W + (fwd ? 0 : W - 1);
-  const int64_t img = (int64_t)n * HW * C;
-  const int64_t rsl = ((int64_t)n * 4 + rslot) * HW;
-  float *Xo = a.X[dir] + img;
-  const float *XDp = a.X[DOWN] + img;
-  const float *XUp = a.X[UP] + img;
-  const float *ig_p = a.in_grad + rsl * C;
-  const float *m_p = a.msgs + rsl * C;
-  const uint8_t *am_p = a.argmin + rsl * C;
-  const uint8_t *gl_p = a.gargmin + rsl;
-  const float *edge_p = a.edge + ((int64_t)n * 4 + dir) * HW;
-  float *ge_p = a.grad_edge + ((int64_t)n * 4 + dir) * HW;
+  const int64_t HWC = HW * C;
   const bool exact = a.mode == BPL_GRAD_EXACT;
+  // pixel at position p of the scanline: lbase + p * pstride; wave-uniform byte offsets of a
+  // position in planes of C floats, C bytes, one float and one byte per pixel
+  const int lbase = VERT ? ln : ln * W;
+  const uint32_t pstride = VERT ? (uint32_t)W : 1u;
+  const uint32_t st_f = pstride * (uint32_t)C * 4u, st_b = pstride * (uint32_t)C, st_p = pstride * 4u;
+  // one adjoint step moves the positions by -dpos (the pass runs forward in dpos)
+  const uint32_t d_f = fwd ? st_f : 0u - st_f, d_b = fwd ? st_b : 0u - st_b;
+  const uint32_t d_p = fwd ? st_p : 0u - st_p, d_1 = fwd ? pstride : 0u - pstride;
+  auto pos = [&](int s) -> uint32_t { return (uint32_t)(fwd ? s : nsteps - s); };
+  const uint32_t v_ld = (uint32_t)(lbase * C + cl) * 4u;
+  const uint32_t v_ldb = (uint32_t)(lbase * C + cl);
+  const uint32_t v_st = st_ok ? (uint32_t)(lbase * C + c) * 4u : kBufDrop;
+  const uint32_t v_e = (uint32_t)lbase * 4u;
+  const uint32_t v_pp = (line_ok && c == 0) ? (uint32_t)lbase * 4u : kBufDrop;
+  const uint32_t v_p1 = (uint32_t)lbase;
+  const uint32_t plane = (uint32_t)(HWC * 4);
+  const int64_t img = (int64_t)n * HWC;
+  const int64_t rsl = ((int64_t)n * 4 + rslot) * HW;
+  // in_grad: with chained U/D slots the DOWN slot's rows 1..H-2 are the UP slot's, so the
+  // resource spans [UP plane, DOWN plane] and only the row-0 receiver (the first adjoint step of
+  // the UP pass) reads the DOWN plane
+  const bool uds_dn = VERT && a.uds_in && rslot == DOWN;
+  const auto ig_r = (!VERT && a.igz) ? make_rsrc(a.in_grad, 0u)
+                  : uds_dn ? make_rsrc(a.in_grad + ((int64_t)n * 4 + UP) * HWC, 2u * plane)
+                           : make_rsrc(a.in_grad + rsl * C, plane);
+  const auto m_r = make_rsrc(a.msgs + rsl * C, plane);
+  const auto am_r = make_rsrc(a.argmin + rsl * C, plane / 4u);
+  const auto gl_r = make_rsrc(TRUNC ? a.gargmin + rsl : a.argmin, TRUNC ? (uint32_t)HW : 0u);
+  const auto xd_r = make_rsrc(a.X[DOWN] + img, VERT ? 0u : plane);
+  const auto xu_r = make_rsrc(a.X[UP] + img, VERT ? 0u : plane);
+  const auto xo_r = make_rsrc(a.X[dir] + img, plane);
+  const auto edge_r = make_rsrc(a.edge + ((int64_t)n * 4 + dir) * HW, (uint32_t)HW * 4u);
+  const auto ge_r = make_rsrc(a.grad_edge + ((int64_t)n * 4 + dir) * HW, (uint32_t)HW * 4u);
 
   // the last position of the pass sends nothing: zero its outputs
-  {
-    const int64_t q = q0 + (int64_t)nsteps * dq;
-    if (st_ok) Xo[q * C + c] = 0.f;
-    if (line_ok && c == 0) ge_p[q] = 0.f;
-  }
+  buf_st(xo_r, v_st, pos(nsteps) * st_f, 0.f);
+  buf_st(ge_r, v_pp, pos(nsteps) * st_p, 0.f);
   const float *Md = a.jump + (int64_t)dir * C * C;
   for (int i = lane; i < CM * CM; i += kWave) {
     const int r = i / CM, cc = i - r * CM;
     lds_mat[i] = (r < C && cc < C) ? Md[r * C + cc] : 0.f;
-    lds_part[0][i] = 0.f;
-    lds_part[1][i] = 0.f;
   }
+  for (int i = c; i < CM * CM + SEG; i += SEG) lds_part[seg][i] = 0.f;
+  for (int i = c; i < SEG * RS + SEG; i += SEG) lds_r[seg][i] = 0.f;
   wave_lds_sync();
 
   float pg[PF], pm[PF], pd[PF], pu[PF], pw[PF];
   int pa[PF], pl[PF];
+  // ring slot i of adjoint step s: the receiver's (position s + 1) in_grad, message, argmin,
+  // label of the minimum and vertical h-gradients, the sender's (position s) edge weight
+  auto load_at = [&](uint32_t rf, uint32_t rb, uint32_t r1, uint32_t qp, uint32_t igo, int i) {
+    pg[i] = buf_ld(ig_r, v_ld, igo);
+    pm[i] = buf_ld(m_r, v_ld, rf);
+    pa[i] = (int)__builtin_amdgcn_raw_buffer_load_b8(am_r, v_ldb, rb, 0);
+    pl[i] = TRUNC ? (int)__builtin_amdgcn_raw_buffer_load_b8(gl_r, v_p1, r1, 0) : 0;
+    pd[i] = VERT ? 0.f : buf_ld(xd_r, v_ld, rf);
+    pu[i] = VERT ? 0.f : buf_ld(xu_r, v_ld, rf);
+    pw[i] = buf_ld(edge_r, v_e, qp);
+  };
   auto load_step = [&](int s, int i) {
-    const int64_t q = q0 + (int64_t)s * dq;
-    const int64_t qr = q + dq;
-    pg[i] = (!VERT && a.igz) ? 0.f : ig_p[qr * C + cl - ig_shift(a, rslot, qr)];
-    pm[i] = m_p[qr * C + cl];
-    pa[i] = (int)am_p[qr * C + cl];
-    pl[i] = (int)gl_p[qr];
-    if (!VERT) {
-      pd[i] = XDp[qr * C + cl];
-      pu[i] = XUp[qr * C + cl];
-    } else {
-      pd[i] = pu[i] = 0.f;
-    }
-    pw[i] = edge_p[q];
+    const uint32_t pr = pos(s + 1);
+    load_at(pr * st_f, pr * st_b, pr * pstride, pos(s) * st_p, pr * st_f + (uds_dn && pr == 0u ? plane : 0u), i);
   };
   if (nsteps > 0) {
 #pragma unroll
@@ -1201,46 +1230,88 @@ __global__ __launch_bounds__(64) void lbp_bwd_semg(BwdArgs a) {
     }
   }
   float acc = 0.f;
-  for (int t0 = 0; t0 < nsteps; t0 += PF) {
+  int prev = SEG * RS + c;  // the routing cell written one step earlier (the dummy row at first)
+  // sender offsets of the current adjoint step
+  uint32_t o_f = pos(nsteps - 1) * st_f, o_p = pos(nsteps - 1) * st_p;
+  auto step = [&](int i) {
+    const int A = pa[i];
+    const bool tr = TRUNC && A == kFromTrunc;  // the value came from the truncation term
+    const int r = tr ? pl[i] : A;              // routing target and reference-mode matrix row
+    const bool to_tr = tr && exact;
+    // LDS reads off the chain: the matrix entry of the edge gradient, the partial's entry
+    // (24-bit multiplies: labels are below 256; the empty asm statements keep the values
+    // computed on every lane, so the selects stay selects instead of branches)
+    const int rcm = __mul24(r, CM);
+    const float mv = to_tr ? lds_mat[C - 1] : lds_mat[rcm + cl];
+    const int pidx = (st_ok && !to_tr) ? rcm + c : CM * CM + c;
+    const float pold = lds_part[seg][pidx];
+    // normalisation Jacobian: g = t - softmax(msg) * sum(t), t = in_grad + chain
+    const float tt = pg[i] + acc;
+    const float ts = seg_reduce<SEG>(lab ? tt : 0.f, OpSum());
+    float gk = tt - fast_exp(pm[i]) * ts;
+    if (!VERT) gk += pd[i] + pu[i];
+    asm volatile("" : "+v"(gk));
+    gk = lab ? gk : 0.f;
+    // route: clear last step's cell, write [r][c]; sender c sums row c in label order
+    int cell = __mul24(r, RS) + c;
+    asm volatile("" : "+v"(cell));
+    cell = lab ? cell : SEG * RS + c;
+    lds_r[seg][prev] = 0.f;
+    lds_r[seg][cell] = gk;
+    prev = cell;
+    wave_lds_sync();
+    float row[CM];
+#pragma unroll
+    for (int l = 0; l < CM; ++l) row[l] = lds_r[seg][c * RS + l];
+    wave_lds_sync();  // the row is read before the next step's writes
+    float gh = 0.f;
+#pragma unroll
+    for (int l = 0; l < CM; ++l) gh += row[l];
+    acc = gh;
+    buf_st(xo_r, v_st, o_f, gh);
+    // parameter gradients of (dir, sender): edge sum_c M[r][c] g(c) and matrix entry [r][c]
+    const float w = pw[i];
+    const float ge = seg_reduce<SEG>(lab ? mv * gk : 0.f, OpSum());
+    buf_st(ge_r, v_pp, o_p, ge);
+    lds_part[seg][pidx] = pold + w * gk;
+    if constexpr (TRUNC) {
+      if (exact) {
+        const float fsum = seg_reduce<SEG>(lab && tr ? gk : 0.f, OpSum());
+        if (line_ok && c == C - 1) lds_part[seg][C - 1] += w * fsum;
+      }
+    }
+    o_f -= d_f;
+    o_p -= d_p;
+  };
+  // main loop: the ring refills stay within the scanline (s - PF >= 0)
+  int t0 = 0;
+  {
+    const int s = nsteps - 1 - PF;  // the first refill's sender position
+    const uint32_t pr = pos(s >= 0 ? s + 1 : 0), pq = pos(s >= 0 ? s : 0);
+    uint32_t l_f = pr * st_f, l_b = pr * st_b, l_1 = pr * pstride, l_p = pq * st_p;
+    for (; t0 + 2 * PF <= nsteps; t0 += PF) {
+#pragma unroll
+      for (int i = 0; i < PF; ++i) {
+        step(i);
+        // refill ring slot i (its old contents are dead now: no register copies, no drain)
+        load_at(l_f, l_b, l_1, l_p, l_f, i);
+        l_f -= d_f;
+        l_b -= d_b;
+        l_1 -= d_1;
+        l_p -= d_p;
+      }
+    }
+  }
+  // the last groups: refills clamped to the first sender, steps past the end skipped
+  for (; t0 < nsteps; t0 += PF) {
 #pragma unroll
     for (int i = 0; i < PF; ++i) {
       const int t = t0 + i;
-      const int s = nsteps - 1 - t;  // adjoint order
       if (t < nsteps) {
-        const int64_t q = q0 + (int64_t)s * dq;
-        // normalisation Jacobian: g = t - softmax(msg) * sum(t), t = in_grad + chain
-        const float tt = pg[i] + acc;
-        const float ts = seg_reduce<SEG>(lab ? tt : 0.f, OpSum());
-        float gk = tt - fast_exp(pm[i]) * ts;
-        if (!VERT) gk += pd[i] + pu[i];
-        gk = lab ? gk : 0.f;
-        const int A = pa[i];
-        const int gl = pl[i];
-        const bool tr = A == kFromTrunc;   // the value came from the truncation term
-        const int r = tr ? gl : A;         // routing target and reference-mode matrix row
-        if (c < CM) {
-          lds_ag[seg][c] = make_int2(lab ? r : -1, __float_as_int(gk));
-        }
-        wave_lds_sync();
-        // route: sender label c gathers the receivers routed to it, in label order
-        const float gh = routed_gather<CM>(lds_ag[seg], c);  // two entries per 16-byte read
-        // parameter gradients of (dir, sender): edge sum_c M[r][c] g(c) and matrix entry
-        // [r][c] (a truncation winner uses the truncation entry [0][C-1] in the exact mode)
-        const float w = pw[i];
-        const bool to_tr = tr && exact;
-        const float mv = to_tr ? lds_mat[C - 1] : lds_mat[r * CM + cl];
-        const float ge = seg_reduce<SEG>(lab ? mv * gk : 0.f, OpSum());
-        if (line_ok && c == 0) ge_p[q] = ge;
-        if (st_ok && !to_tr) lds_part[seg][r * CM + c] += w * gk;
-        if (exact) {
-          const float fsum = seg_reduce<SEG>(lab && tr ? gk : 0.f, OpSum());
-          if (line_ok && c == C - 1) lds_part[seg][C - 1] += w * fsum;
-        }
-        wave_lds_sync();  // the gathers are done before the next step's writes
-        acc = gh;
-        if (st_ok) Xo[q * C + c] = gh;
+        step(i);
+        const int s = nsteps - 1 - t;
+        load_step(s - PF >= 0 ? s - PF : 0, i);
       }
-      load_step(s - PF >= 0 ? s - PF : 0, i);
     }
   }
   // the segment's partial matrix, in lbp_bwd_reduce_partials' layout
@@ -1256,8 +1327,12 @@ template <int CM>
 void launch_semg(const BwdArgs &a, bool vert, int ngroups, hipStream_t st) {
   const int L = vert ? a.W : a.H;
   const dim3 grid((unsigned)(ngroups * 2 * ((L + 1) / 2)));
-  if (vert) hipLaunchKernelGGL((lbp_bwd_semg<CM, true, 4>), grid, dim3(kWave), 0, st, a);
-  else hipLaunchKernelGGL((lbp_bwd_semg<CM, false, 4>), grid, dim3(kWave), 0, st, a);
+  // the forward's full min-plus branch (delta >= C) marks no truncation-term winners
+  const bool trunc = !(a.delta >= a.C);
+  if (vert && trunc) hipLaunchKernelGGL((lbp_bwd_semg<CM, true, 4, true>), grid, dim3(kWave), 0, st, a);
+  else if (vert) hipLaunchKernelGGL((lbp_bwd_semg<CM, true, 4, false>), grid, dim3(kWave), 0, st, a);
+  else if (trunc) hipLaunchKernelGGL((lbp_bwd_semg<CM, false, 4, true>), grid, dim3(kWave), 0, st, a);
+  else hipLaunchKernelGGL((lbp_bwd_semg<CM, false, 4, false>), grid, dim3(kWave), 0, st, a);
 }
 
 // Semantic per-pixel pairwise, 16 < C <= 32 or C <= 20 (both forward branches, both modes):
@@ -1644,7 +1719,9 @@ void launch_stereo(const BwdArgs &a, bool vert, bool pair, int ngroups, hipStrea
 int seg_width(int C) { return C <= 16 ? 16 : C <= 32 ? 32 : 64; }
 // the semantic-global kernel for 16 < C <= 32: its partials use the 32-lane segments of the
 // generic layout
-bool semg_ok(const BwdArgs &a) { return a.C > 16 && a.C <= 32 && !generic_only(); }
+bool semg_ok(const BwdArgs &a) {
+  return a.C > 16 && a.C <= 32 && (int64_t)a.H * a.W * a.C * 4 < (1LL << 31) && !generic_only();
+}
 
 // BPL_GRAD_ACCUMULATE: the kernels write this call's grad_jump and grad_edge into workspace
 // copies, which lbp_bwd_accumulate then adds to the caller's tensors (stereo jumps of up to

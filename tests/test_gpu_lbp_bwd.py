@@ -100,8 +100,10 @@ CASES = [
 ]
 
 
-# only the stereo backward takes the forward's delta, so the semantic cases run without it
-RUNS = [(c, pd) for c in CASES for pd in ((False, True) if c[0] == "stereo" else (False,))]
+# the stereo and semantic-global backward calls take the forward's delta (the semantic-pixel
+# module's has no such argument): with it, delta >= C tells the semantic-global kernel that no
+# argmin marks a truncation-term winner
+RUNS = [(c, pd) for c in CASES for pd in ((False, True) if c[0] in ("stereo", "semantic_global") else (False,))]
 
 
 @pytest.mark.parametrize("case,pass_delta", RUNS,
@@ -110,7 +112,8 @@ RUNS = [(c, pd) for c in CASES for pd in ((False, True) if c[0] == "stereo" else
 @pytest.mark.parametrize("mode", ["exact", "reference"])
 def test_lbp_backward_matches_oracle(case, mode, pass_delta):
     """'delta' passes the forward's delta, which selects the stereo fast step where it applies
-    (delta <= 3, C > 32); 'no-delta' runs the general kernels."""
+    (delta <= 3, C > 32) and the semantic-global step without truncation-term winners
+    (delta >= C); 'no-delta' runs the general kernels."""
     kind, N, H, W, C, delta, extra = case
     print(f"case {case} mode={mode} pass_delta={pass_delta}", flush=True)
     run_case(kind, N, H, W, C, delta, mode, seed=100 + C + H, pass_delta=pass_delta, **extra)
