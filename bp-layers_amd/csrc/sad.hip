@@ -635,10 +635,14 @@ __global__ __launch_bounds__(512) void sad_flow_fwd_tiled_k(const float *__restr
 //      bit s of mask[k][target] says "the source at target - (u, v) with label k and argmin
 //      s lands here"; LDS OR is order-free, so the masks are deterministic.  The first
 //      landing of each label (usually the only one) is resolved into registers, the others
-//      (displacement, label and gradient) into a per-thread LDS list of FM_XCAP entries; then
-//      channel chunk by chunk with f0 of the region staged in LDS, the first landings are
-//      summed label by label, then the listed ones in list order (label, then argmin
-//      ascending), then -- rarely -- the landings past the list, found again from the masks.
+//      (displacement, label and gradient) into the block's LDS pool of FM_XCAP * FM_NT entries,
+//      each thread's extra landings in one contiguous range (start = exclusive prefix sum of the
+//      threads' counts, so the placement is deterministic; a label whose argmins are far from
+//      the flow lands about Poisson(1) times per target, which gives ~9 extra landings per
+//      pixel, and the pool holds 12 per pixel on average); then channel chunk by chunk with f0
+//      of the region staged in LDS, the first landings are summed label by label, then the
+//      pooled ones in order (label, then argmin ascending), then -- rarely, when the block's
+//      pool is full -- the landings past it, found again from the masks.
 // The u-volume's partial sums pass through df0 / df1 (same thread).  Own terms whose argmin
 // lies outside the window read f1 from global memory in place.
 constexpr int FM_TX = 32, FM_TY = 8, FM_NT = FM_TX * FM_TY, FM_KMAX = 32, FM_CH = 4, FM_XCAP = 12;
@@ -687,9 +691,9 @@ __global__ __launch_bounds__(FM_NT) void sad_flow_bwd_lds_k(const float *__restr
   const int K = sws + 1, hs = sws / 2, KL = 2 * hs + 1, E = K - 1 - hs;
   const int RR = FM_TY + 2 * E, RC = FM_TX + 2 * E, RN = RR * RC;
   uint32_t *mask = fm_lds;                                                       // [KL][FM_NT]
-  float *xg = reinterpret_cast<float *>(fm_lds + KL * FM_NT);                    // [FM_XCAP][FM_NT]
+  float *xg = reinterpret_cast<float *>(fm_lds + KL * FM_NT);                    // pool [FM_XCAP * FM_NT]
   float *reg = reinterpret_cast<float *>(fm_lds + (KL + FM_XCAP) * FM_NT);       // [CH][RR][RC]
-  uint32_t *xo = reinterpret_cast<uint32_t *>(reg + CH * RN);  // [FM_XCAP][FM_NT]: v+32 | (u+32) << 6 | k << 12
+  uint32_t *xo = reinterpret_cast<uint32_t *>(reg + CH * RN);  // pool: v+32 | (u+32) << 6 | k << 12
   const int t = threadIdx.x;
   const int ty = t / FM_TX, tx = t - ty * FM_TX;
   const int x0 = blockIdx.x * FM_TX, y0 = blockIdx.y * FM_TY, n = blockIdx.z;
@@ -804,35 +808,61 @@ __global__ __launch_bounds__(FM_NT) void sad_flow_bwd_lds_k(const float *__restr
       }
     }
     __syncthreads();
-    // first landing per label into registers, the others into the list
+    // first landing per label into registers; count the others
     int foff[FM_KMAX];
     float fg[FM_KMAX];
-    int nx = 0;  // landings past the first of their label (the list holds FM_XCAP of them)
+    int nx = 0;  // landings past the first of their label
 #pragma unroll
     for (int k = 0; k < FM_KMAX; ++k) {
-      uint32_t m = inside && k < KL ? mask[k * FM_NT + t] : 0u;
+      const uint32_t m = inside && k < KL ? mask[k * FM_NT + t] : 0u;
       const int e = (m != 0u ? __builtin_ctz(m) : hs) - hs;
       const int u = part ? e : k - hs, v = part ? k - hs : e;
       const float g = buf_ld(rgr, m != 0u ? (uint32_t)(((ipix - v * W - u) * K + k) * 4) : kBufDrop, 0);
       foff[k] = m != 0u ? own - v * RC - u : FM_NONE;
       fg[k] = m != 0u ? g : 0.f;
-      m &= m - 1u;
-      while (m != 0u) {
-        const int e2 = __builtin_ctz(m) - hs;
+      nx += __builtin_popcount(m & (m - 1u));
+    }
+    // the thread's range of the pool: exclusive prefix sum of the counts over the block (in the
+    // region buffer, which is restaged before its next read)
+    int *scan = reinterpret_cast<int *>(reg);
+    scan[t] = nx;
+    __syncthreads();
+    for (int d = 1; d < FM_NT; d <<= 1) {
+      const int add = t >= d ? scan[t - d] : 0;
+      __syncthreads();
+      scan[t] += add;
+      __syncthreads();
+    }
+    const int x0p = scan[t] - nx;                              // first pool entry of the thread
+    const int nfit = min(nx, max(0, FM_XCAP * FM_NT - x0p));  // of its landings in the pool
+    // the extra landings into the pool in order (label, then argmin ascending)
+    {
+      int i = 0;
+      for (int k = 0; k < KL && i < nfit; ++k) {
+        uint32_t m = inside ? mask[k * FM_NT + t] : 0u;
         m &= m - 1u;
-        const int u2 = part ? e2 : k - hs, v2 = part ? k - hs : e2;
-        if (nx < FM_XCAP) xo[nx * FM_NT + t] = (uint32_t)(v2 + 32) | (uint32_t)(u2 + 32) << 6 | (uint32_t)k << 12;
-        ++nx;
+        while (m != 0u && i < nfit) {
+          const int e2 = __builtin_ctz(m) - hs;
+          m &= m - 1u;
+          const int u2 = part ? e2 : k - hs, v2 = part ? k - hs : e2;
+          xo[x0p + i] = (uint32_t)(v2 + 32) | (uint32_t)(u2 + 32) << 6 | (uint32_t)k << 12;
+          ++i;
+        }
       }
     }
-    // the listed landings' gradients, loads in flight together
+    // the pooled landings' gradients, four loads in flight at a time
+    for (int i0 = 0; i0 < nfit; i0 += 4) {
+      float g4[4];
 #pragma unroll
-    for (int i = 0; i < FM_XCAP; ++i) {
-      const bool ok = i < nx;
-      const uint32_t w = ok ? xo[i * FM_NT + t] : 0u;
-      const int v = (int)(w & 63u) - 32, u = (int)((w >> 6) & 63u) - 32, k = (int)(w >> 12);
-      const float g = buf_ld(rgr, ok ? (uint32_t)(((ipix - v * W - u) * K + k) * 4) : kBufDrop, 0);
-      if (ok) xg[i * FM_NT + t] = g;
+      for (int q = 0; q < 4; ++q) {
+        const bool ok = i0 + q < nfit;
+        const uint32_t w = ok ? xo[x0p + i0 + q] : 0u;
+        const int v = (int)(w & 63u) - 32, u = (int)((w >> 6) & 63u) - 32, k = (int)(w >> 12);
+        g4[q] = buf_ld(rgr, ok ? (uint32_t)(((ipix - v * W - u) * K + k) * 4) : kBufDrop, 0);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (i0 + q < nfit) xg[x0p + i0 + q] = g4[q];
     }
     for (int c0 = 0; c0 < C; c0 += CH) {
       const int nc = min(CH, C - c0);
@@ -857,15 +887,14 @@ __global__ __launch_bounds__(FM_NT) void sad_flow_bwd_lds_k(const float *__restr
             for (int j = 0; j < CH; ++j) r1[j] -= sgn_eps(reg[j * RN + off] - b[j]) * g;
           }
         }
-        const int nl = min(nx, FM_XCAP);
-        for (int i = 0; i < nl; ++i) {
-          const uint32_t w = xo[i * FM_NT + t];
+        for (int i = 0; i < nfit; ++i) {
+          const uint32_t w = xo[x0p + i];
           const int off = own - ((int)(w & 63u) - 32) * RC - ((int)((w >> 6) & 63u) - 32);
-          const float g = xg[i * FM_NT + t];
+          const float g = xg[x0p + i];
 #pragma unroll
           for (int j = 0; j < CH; ++j) r1[j] -= sgn_eps(reg[j * RN + off] - b[j]) * g;
         }
-        if (nx > FM_XCAP) {  // rare: the landings past the list, in the same order
+        if (nx > nfit) {  // rare (the block's pool is full): the landings past it, in the same order
           int ix = 0;
           for (int k = 0; k < KL; ++k) {
             uint32_t m = mask[k * FM_NT + t];
@@ -873,7 +902,7 @@ __global__ __launch_bounds__(FM_NT) void sad_flow_bwd_lds_k(const float *__restr
             while (m != 0u) {
               const int e = __builtin_ctz(m) - hs;
               m &= m - 1u;
-              if (ix++ < FM_XCAP) continue;
+              if (ix++ < nfit) continue;
               const int u = part ? e : k - hs, v = part ? k - hs : e;
               const float g = gg[(p - (int64_t)v * W - u) * K + k];
               const int o2 = own - v * RC - u;
