@@ -635,23 +635,37 @@ __global__ __launch_bounds__(512) void sad_flow_fwd_tiled_k(const float *__restr
 //      bit s of mask[k][target] says "the source at target - (u, v) with label k and argmin
 //      s lands here"; LDS OR is order-free, so the masks are deterministic.  The first
 //      landing of each label (usually the only one) is resolved into registers, the others
-//      (displacement, label and gradient) into the block's LDS pool of FM_XCAP * FM_NT entries,
-//      each thread's extra landings in one contiguous range (start = exclusive prefix sum of the
-//      threads' counts, so the placement is deterministic; a label whose argmins are far from
-//      the flow lands about Poisson(1) times per target, which gives ~9 extra landings per
-//      pixel, and the pool holds 12 per pixel on average); then channel chunk by chunk with f0
-//      of the region staged in LDS, the first landings are summed label by label, then the
-//      pooled ones in order (label, then argmin ascending), then -- rarely, when the block's
-//      pool is full -- the landings past it, found again from the masks.
+//      (displacement, label and gradient) into the block's LDS pool, each thread's extra
+//      landings in one contiguous range (start = exclusive prefix sum of the threads' counts, so
+//      the placement is deterministic).  Extra landings are uneven: where the argmins of a
+//      label are far from the flow they land about Poisson(1) times per target, and where the
+//      flow converges many sources share a target (the 256x512, sws 24 workload: 10.7 extra
+//      landings per pixel on average, up to 155; up to 5184 in one block), so the pool takes
+//      what the LDS budget leaves (6144 entries at sws 24).  Then, channel chunk by chunk with
+//      f0 of the region staged in LDS (in the masks' memory), the first landings are summed
+//      label by label, then the pooled ones in order (label, then argmin ascending); a thread
+//      whose landings overflow the pool adds the rest after the last chunk, channel by
+//      channel, found again from the argmin volume (they come last in each channel's sum).
 // The u-volume's partial sums pass through df0 / df1 (same thread).  Own terms whose argmin
 // lies outside the window read f1 from global memory in place.
-constexpr int FM_TX = 32, FM_TY = 8, FM_NT = FM_TX * FM_TY, FM_KMAX = 32, FM_CH = 4, FM_XCAP = 12;
+constexpr int FM_TX = 32, FM_TY = 8, FM_NT = FM_TX * FM_TY, FM_KMAX = 32, FM_CH = 4;
 constexpr int FM_NONE = -1;  // own term without a match inside the image
+// LDS per block: two blocks (eight waves) share a CU
+constexpr int FM_LDS_BUDGET = 78 * 1024;
 
-size_t flow_bwd_lds_bytes(int sws, int ch) {
+// bytes of the LDS area shared by the landing masks [KL][FM_NT] and the staged region [CH][RR][RC]
+int flow_bwd_union_bytes(int sws, int ch) {
   const int K = sws + 1, hs = sws / 2, KL = 2 * hs + 1, E = K - 1 - hs;
-  return sizeof(uint32_t) * KL * FM_NT + (sizeof(float) + sizeof(uint32_t)) * FM_XCAP * FM_NT +
-         sizeof(float) * (size_t)ch * (FM_TY + 2 * E) * (FM_TX + 2 * E);
+  const int m = 4 * KL * FM_NT, r = 4 * ch * (FM_TY + 2 * E) * (FM_TX + 2 * E);
+  return (m > r ? m : r + 15) & ~15;
+}
+// entries of the pool of extra landings (a float gradient and a packed offset each)
+int flow_bwd_pool(int sws, int ch) {
+  const int left = (FM_LDS_BUDGET - flow_bwd_union_bytes(sws, ch)) / 8;
+  return left > 4 * FM_NT ? left : 4 * FM_NT;
+}
+size_t flow_bwd_lds_bytes(int sws, int ch) {
+  return (size_t)flow_bwd_union_bytes(sws, ch) + 8 * (size_t)flow_bwd_pool(sws, ch);
 }
 
 // region rows of nc channel planes (buffer resource over them) -> reg[CH][RR][RC]; a wave
@@ -686,14 +700,15 @@ __global__ __launch_bounds__(FM_NT) void sad_flow_bwd_lds_k(const float *__restr
                                                              const float *__restrict__ gu, const float *__restrict__ gv,
                                                              const float *__restrict__ us, const float *__restrict__ vs,
                                                              float *__restrict__ df0, float *__restrict__ df1, int C,
-                                                             int H, int W, int sws) {
+                                                             int H, int W, int sws, int ubytes, int pool) {
   extern __shared__ __align__(16) uint32_t fm_lds[];
   const int K = sws + 1, hs = sws / 2, KL = 2 * hs + 1, E = K - 1 - hs;
   const int RR = FM_TY + 2 * E, RC = FM_TX + 2 * E, RN = RR * RC;
-  uint32_t *mask = fm_lds;                                                       // [KL][FM_NT]
-  float *xg = reinterpret_cast<float *>(fm_lds + KL * FM_NT);                    // pool [FM_XCAP * FM_NT]
-  float *reg = reinterpret_cast<float *>(fm_lds + (KL + FM_XCAP) * FM_NT);       // [CH][RR][RC]
-  uint32_t *xo = reinterpret_cast<uint32_t *>(reg + CH * RN);  // pool: v+32 | (u+32) << 6 | k << 12
+  // the masks and the staged region share the first ubytes (never live together)
+  uint32_t *mask = fm_lds;                                                 // [KL][FM_NT]
+  float *reg = reinterpret_cast<float *>(fm_lds);                          // [CH][RR][RC]
+  float *xg = reinterpret_cast<float *>(fm_lds + ubytes / 4);              // pool gradients [pool]
+  uint32_t *xo = reinterpret_cast<uint32_t *>(xg + pool);  // pool offsets: v+32 | (u+32) << 6 | k << 12
   const int t = threadIdx.x;
   const int ty = t / FM_TX, tx = t - ty * FM_TX;
   const int x0 = blockIdx.x * FM_TX, y0 = blockIdx.y * FM_TY, n = blockIdx.z;
@@ -823,8 +838,8 @@ __global__ __launch_bounds__(FM_NT) void sad_flow_bwd_lds_k(const float *__restr
       nx += __builtin_popcount(m & (m - 1u));
     }
     // the thread's range of the pool: exclusive prefix sum of the counts over the block (in the
-    // region buffer, which is restaged before its next read)
-    int *scan = reinterpret_cast<int *>(reg);
+    // pool's gradient area, written only after every thread has read its sum)
+    int *scan = reinterpret_cast<int *>(xg);
     scan[t] = nx;
     __syncthreads();
     for (int d = 1; d < FM_NT; d <<= 1) {
@@ -833,8 +848,9 @@ __global__ __launch_bounds__(FM_NT) void sad_flow_bwd_lds_k(const float *__restr
       scan[t] += add;
       __syncthreads();
     }
-    const int x0p = scan[t] - nx;                              // first pool entry of the thread
-    const int nfit = min(nx, max(0, FM_XCAP * FM_NT - x0p));  // of its landings in the pool
+    const int x0p = scan[t] - nx;                     // first pool entry of the thread
+    const int nfit = min(nx, max(0, pool - x0p));     // of its landings in the pool
+    __syncthreads();
     // the extra landings into the pool in order (label, then argmin ascending)
     {
       int i = 0;
@@ -894,26 +910,34 @@ __global__ __launch_bounds__(FM_NT) void sad_flow_bwd_lds_k(const float *__restr
 #pragma unroll
           for (int j = 0; j < CH; ++j) r1[j] -= sgn_eps(reg[j * RN + off] - b[j]) * g;
         }
-        if (nx > nfit) {  // rare (the block's pool is full): the landings past it, in the same order
-          int ix = 0;
-          for (int k = 0; k < KL; ++k) {
-            uint32_t m = mask[k * FM_NT + t];
-            m &= m - 1u;
-            while (m != 0u) {
-              const int e = __builtin_ctz(m) - hs;
-              m &= m - 1u;
-              if (ix++ < nfit) continue;
-              const int u = part ? e : k - hs, v = part ? k - hs : e;
-              const float g = gg[(p - (int64_t)v * W - u) * K + k];
-              const int o2 = own - v * RC - u;
-#pragma unroll
-              for (int j = 0; j < CH; ++j) r1[j] -= sgn_eps(reg[j * RN + o2] - b[j]) * g;
-            }
-          }
-        }
 #pragma unroll
         for (int j = 0; j < CH; ++j)
           if (j < nc) df1[((int64_t)n * C + c0 + j) * HW + pix] = r1[j];
+      }
+    }
+    // rare (the block's pool is full): the landings past it, in the same order -- label, then
+    // argmin ascending, the first of each label and the pooled ones skipped -- after every chunk
+    if (inside && nx > nfit) {
+      const float *st = (part ? vs : us) + img * K;
+      int ix = 0;
+      for (int k = 0; k < KL; ++k) {
+        bool first = true;
+        for (int sv = 0; sv < K; ++sv) {
+          const int e = sv - hs;
+          const int u = part ? e : k - hs, v = part ? k - hs : e;
+          const int ys = y - v, xs = x - u;
+          if (ys < 0 || ys >= H || xs < 0 || xs >= W || (int)st[((int64_t)ys * W + xs) * K + k] != sv) continue;
+          if (first) {
+            first = false;
+            continue;
+          }
+          if (ix++ < nfit) continue;
+          const float g = gg[(img + (int64_t)ys * W + xs) * K + k];
+          for (int c = 0; c < C; ++c) {
+            const int64_t o = ((int64_t)n * C + c) * HW;
+            df1[o + pix] -= sgn_eps(f0[o + (int64_t)ys * W + xs] - f1[o + pix]) * g;
+          }
+        }
       }
     }
   }
@@ -1174,6 +1198,7 @@ extern "C" int bpl_sad_flow_bwd(const float *f0, const float *f1, const float *g
                      H * W * (sws + 1 > FM_CH ? sws + 1 : FM_CH) * 4 < (1LL << 31);
   if (masks) {
     const size_t shm = flow_bwd_lds_bytes(sws, FM_CH);
+    const int ubytes = flow_bwd_union_bytes(sws, FM_CH), pool = flow_bwd_pool(sws, FM_CH);
     if (shm > 64 * 1024) {
       const int rc = check_hip(hipFuncSetAttribute((const void *)sad_flow_bwd_lds_k<FM_CH>,
                                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm),
@@ -1182,7 +1207,7 @@ extern "C" int bpl_sad_flow_bwd(const float *f0, const float *f1, const float *g
     }
     dim3 grid((unsigned)((W + FM_TX - 1) / FM_TX), (unsigned)((H + FM_TY - 1) / FM_TY), (unsigned)N);
     hipLaunchKernelGGL(sad_flow_bwd_lds_k<FM_CH>, grid, dim3(FM_NT), shm, st, f0, f1, grad_u, grad_v, u_star, v_star,
-                       df0, df1, (int)C, (int)H, (int)W, sws);
+                       df0, df1, (int)C, (int)H, (int)W, sws, ubytes, pool);
   } else {
     hipLaunchKernelGGL(sad_flow_bwd_search_k, dim3(grid_for(N * H * W, FB_THREADS)), dim3(FB_THREADS), 0, st, f0, f1,
                        grad_u, grad_v, u_star, v_star, df0, df1, (int)N, (int)C, (int)H, (int)W, sws);

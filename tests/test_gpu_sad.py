@@ -94,6 +94,10 @@ def check_flow_forward(got, exp, C, sws, bu=0, bv=0):
     (1, 5, 17, 29, 6, "offset"),    # block-offset argmins land outside the window
     (2, 3, 13, 21, 5, "random"),    # arbitrary argmins, negatives included
     (1, 2, 12, 40, 33, "random"),
+    # sources converging on every K-th row (u volume) and column (v volume): ~2h extra landings
+    # per label on those targets, past the block's pool of extra landings
+    (1, 4, 30, 40, 24, "converge"),
+    (2, 3, 21, 37, 8, "converge"),
 ])
 def test_flow_sad_backward(N, C, H, W, sws, stars):
     from bplayers_amd import pytorch_cuda_flow_mp_sad_op as op
@@ -105,6 +109,14 @@ def test_flow_sad_backward(N, C, H, W, sws, stars):
         _, _, us, vs = S.flow_sad_forward(f0, f1, sws)
     elif stars == "offset":
         _, _, us, vs = S.flow_sad_forward(f0, f1, sws, 2, -1, 1, 1)
+    elif stars == "converge":
+        hs = sws // 2
+        ys = np.arange(H)[:, None, None]
+        xs = np.arange(W)[None, :, None]
+        # label s of the other displacement e = s - hs lands row ys + e (u volume) or column
+        # xs + e (v volume) on the nearest target line hs + K * m
+        us = np.broadcast_to(np.clip(hs + ((ys // K) * K + hs - ys), 0, 2 * hs), (N, H, W, K)).astype(np.float64)
+        vs = np.broadcast_to(np.clip(hs + ((xs // K) * K + hs - xs), 0, 2 * hs), (N, H, W, K)).astype(np.float64)
     else:
         us = rng.integers(-3, 2 * K + 2, size=(N, H, W, K)).astype(np.float64)
         vs = rng.integers(-3, 2 * K + 2, size=(N, H, W, K)).astype(np.float64)
