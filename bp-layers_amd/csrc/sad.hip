@@ -708,7 +708,8 @@ __global__ __launch_bounds__(FM_NT) void sad_flow_bwd_lds_k(const float *__restr
   uint32_t *mask = fm_lds;                                                 // [KL][FM_NT]
   float *reg = reinterpret_cast<float *>(fm_lds);                          // [CH][RR][RC]
   float *xg = reinterpret_cast<float *>(fm_lds + ubytes / 4);              // pool gradients [pool]
-  uint32_t *xo = reinterpret_cast<uint32_t *>(xg + pool);  // pool offsets: v+32 | (u+32) << 6 | k << 12
+  // pool offsets: v+32 | (u+32) << 6 | k << 12, then the source's region offset
+  uint32_t *xo = reinterpret_cast<uint32_t *>(xg + pool);
   const int t = threadIdx.x;
   const int ty = t / FM_TX, tx = t - ty * FM_TX;
   const int x0 = blockIdx.x * FM_TX, y0 = blockIdx.y * FM_TY, n = blockIdx.z;
@@ -866,19 +867,25 @@ __global__ __launch_bounds__(FM_NT) void sad_flow_bwd_lds_k(const float *__restr
         }
       }
     }
-    // the pooled landings' gradients, four loads in flight at a time
+    // the pooled landings' gradients, four loads in flight at a time; each entry's packed
+    // displacement is then replaced by the source's region offset (the thread's own slots)
     for (int i0 = 0; i0 < nfit; i0 += 4) {
       float g4[4];
+      int o4[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const bool ok = i0 + q < nfit;
         const uint32_t w = ok ? xo[x0p + i0 + q] : 0u;
         const int v = (int)(w & 63u) - 32, u = (int)((w >> 6) & 63u) - 32, k = (int)(w >> 12);
         g4[q] = buf_ld(rgr, ok ? (uint32_t)(((ipix - v * W - u) * K + k) * 4) : kBufDrop, 0);
+        o4[q] = own - v * RC - u;
       }
 #pragma unroll
       for (int q = 0; q < 4; ++q)
-        if (i0 + q < nfit) xg[x0p + i0 + q] = g4[q];
+        if (i0 + q < nfit) {
+          xg[x0p + i0 + q] = g4[q];
+          xo[x0p + i0 + q] = (uint32_t)o4[q];
+        }
     }
     for (int c0 = 0; c0 < C; c0 += CH) {
       const int nc = min(CH, C - c0);
@@ -904,8 +911,7 @@ __global__ __launch_bounds__(FM_NT) void sad_flow_bwd_lds_k(const float *__restr
           }
         }
         for (int i = 0; i < nfit; ++i) {
-          const uint32_t w = xo[x0p + i];
-          const int off = own - ((int)(w & 63u) - 32) * RC - ((int)((w >> 6) & 63u) - 32);
+          const int off = (int)xo[x0p + i];
           const float g = xg[x0p + i];
 #pragma unroll
           for (int j = 0; j < CH; ++j) r1[j] -= sgn_eps(reg[j * RN + off] - b[j]) * g;
