@@ -1673,10 +1673,14 @@ bool stereo_fast_ok(const BwdArgs &a) {
   return a.delta >= 0 && a.delta <= 3 && !a.accum && (a.K == 6 || a.K == 8) &&
          (int64_t)a.K * a.H * a.W * 4 < (1LL << 31) && (int64_t)a.H * a.W * a.C * 4 < (1LL << 31);
 }
+#ifndef BPL_BWD_PF_V
+#define BPL_BWD_PF_V 4
+#endif
 template <int LPL, bool FULL>
 void launch_stereo_w64(const BwdArgs &a, bool vert, bool pair, int ngroups, hipStream_t st) {
-  // prefetch depth 4 (the horizontal kernels: launch_bwd_pair, launch_bwd_h_unpaired)
-  constexpr int PF = 4;
+  // prefetch depth of the vertical phase (the horizontal kernels: launch_bwd_pair,
+  // launch_bwd_h_unpaired)
+  constexpr int PF = BPL_BWD_PF_V;
   const bool fast = stereo_fast_ok(a);
   auto go = [&](auto fk) {
     constexpr int FK = decltype(fk)::value;

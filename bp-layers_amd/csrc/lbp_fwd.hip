@@ -1524,7 +1524,7 @@ __global__ __launch_bounds__(4 * kWave) void lbp_fwd_stereo_pair(FwdArgs a) {
 
 // prefetch depth (steps) of the stereo kernels' register ring, per phase
 #ifndef BPL_FWD_PF_H
-#define BPL_FWD_PF_H 6
+#define BPL_FWD_PF_H 8
 #endif
 #ifndef BPL_FWD_PF_V
 #define BPL_FWD_PF_V 4
