@@ -490,8 +490,9 @@ template <int CM>
 void launch_semg(const FwdArgs &a, bool vert, hipStream_t st) {
   const int L = vert ? a.W : a.H;
   const dim3 grid((unsigned)(a.N * 2 * ((L + 1) / 2)));
-  if (vert) hipLaunchKernelGGL((lbp_fwd_semg<CM, true, 4>), grid, dim3(kWave), 0, st, a);
-  else hipLaunchKernelGGL((lbp_fwd_semg<CM, false, 4>), grid, dim3(kWave), 0, st, a);
+  // an 8-step ring (4 steps: 0.413 / 0.263 ms instead of 0.400 / 0.253 at 2x512x1024x19)
+  if (vert) hipLaunchKernelGGL((lbp_fwd_semg<CM, true, 8>), grid, dim3(kWave), 0, st, a);
+  else hipLaunchKernelGGL((lbp_fwd_semg<CM, false, 8>), grid, dim3(kWave), 0, st, a);
 }
 
 
