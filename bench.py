@@ -623,10 +623,15 @@ def kernel_class(name):
 # the forward, read and their gradient written in the backward)
 PHASE_B_PL_SEMANTIC = {"lbp_fwd_horizontal": (22.0, 8.0), "lbp_fwd_vertical": (22.0, 8.0),
                        "lbp_bwd_vertical": (22.0, 16.0), "lbp_bwd_horizontal": (30.0, 16.0)}
+# per-pixel matrices: each pass reads its direction's C x C matrix per image and pixel (4C B/pl) in the
+# forward, and reads it and writes its gradient in the backward (8C B/pl), C = 19
+PHASE_B_PL_SEMANTIC_PIXEL = {"lbp_fwd_horizontal": (22.0 + 2 * 76.0, 8.0), "lbp_fwd_vertical": (22.0 + 2 * 76.0, 8.0),
+                             "lbp_bwd_vertical": (22.0 + 4 * 76.0, 16.0), "lbp_bwd_horizontal": (30.0 + 4 * 76.0, 16.0)}
 PMC_WORKLOADS = {  # pixel*labels of one LBP iteration, labels, canonical phase bytes, output file
     "stereo_fb_384x1248_d192": (384 * 1248 * 192, 192, PHASE_B_PL, TRAFFIC_FILE),
     "semantic_fb_2x512x1024_c19": (2 * 512 * 1024 * 19, 19, PHASE_B_PL_SEMANTIC, None),
     "flow_fb_256x512_r12": (256 * 512 * 25, 25, PHASE_B_PL, None),  # one volume (u or v) per launch
+    "semantic_pixel_fb_2x512x1024_c19": (2 * 512 * 1024 * 19, 19, PHASE_B_PL_SEMANTIC_PIXEL, None),
 }
 
 

@@ -498,15 +498,19 @@ void launch_semg(const FwdArgs &a, bool vert, hipStream_t st) {
 
 // ---------------------------------------------------------------------------------------
 // Semantic per-pixel pairwise, full min-plus branch (delta >= C), C <= CM (20 or 32): the
-// lane layout and step of lbp_fwd_semg, with the sender's matrix column M_q[dir][.][c] read
-// per step into the prefetch ring (one coalesced row of C floats per l).  IMGS: the two
+// lane layout and step of lbp_fwd_semg (shifted frame, branch-free buffer-addressed steps), with
+// the sender's matrix column M_q[dir][.][c] read per step into the prefetch ring: CM loads that
+// share the step's scalar offset, each with a per-lane offset computed once (row l of the
+// sender's C x C block), so a step carries no address arithmetic for them.  IMGS: the two
 // half-waves are the two images of the same scanline (N == 2), so both read the same matrix
-// addresses in one instruction; otherwise two scanlines of one image.
+// addresses in one instruction; otherwise two scanlines of one image.  Tensors are addressed
+// as whole buffers (the messages of all N images below 2^31 bytes, a direction's matrix plane
+// too), the image and slot parts in the per-lane offsets.
 // ---------------------------------------------------------------------------------------
 template <int CM, bool VERT, int PF, bool IMGS>
 __global__ __launch_bounds__(64) void lbp_fwd_semp(FwdArgs a) {
   constexpr int SEG = 32, SPW = 2;
-  __shared__ float lds_h[SPW][CM];
+  __shared__ float lds_h[SPW][SEG];  // a row per segment: every lane writes its slot
   const int H = a.H, W = a.W, C = a.C;
   const int L = VERT ? W : H;
   const int len = VERT ? H : W;
@@ -529,96 +533,169 @@ __global__ __launch_bounds__(64) void lbp_fwd_semp(FwdArgs a) {
   const bool lab = c < C;
   const bool st_ok = line_ok && lab;
   const int cl = lab ? c : C - 1;  // label loaded by this lane
-  const int64_t HW = (int64_t)H * W;
-  const int64_t dq = VERT ? (fwd ? (int64_t)W : -(int64_t)W) : (fwd ? 1 : -1);
-  const int64_t q0 = VERT ? (int64_t)(fwd ? 0 : H - 1) * W + ln : (int64_t)ln * W + (fwd ? 0 : W - 1);
-  const float *cost_p = a.cost + (int64_t)n * HW * C;
-  const float *x1_p = (VERT ? a.msg_out : a.msg_in) + ((int64_t)n * 4 + (VERT ? RIGHT : UP)) * HW * C;
-  const float *x2_p = (VERT ? a.msg_out : a.msg_in) + ((int64_t)n * 4 + (VERT ? LEFT : DOWN)) * HW * C;
-  float *out_p = a.msg_out + ((int64_t)n * 4 + rslot) * HW * C;
-  uint8_t *arg_p = a.argmin ? a.argmin + ((int64_t)n * 4 + rslot) * HW * C : nullptr;
-  const int64_t rs_base = ((int64_t)n * 4 + rslot) * HW;
-  const float *edge_p = a.edge + ((int64_t)n * 4 + dir) * HW;
-  const float *mat_p = a.jump + (int64_t)dir * HW * C * C + cl;  // + (q * C + l) * C
+  const uint32_t HWu = (uint32_t)H * (uint32_t)W, HWCu = HWu * (uint32_t)C;
+  // pixel at position p of the scanline: lbase + p * pstride; wave-uniform byte offsets of a
+  // position in planes of C floats, C bytes, one float per pixel and C x C floats
+  const uint32_t lbase = VERT ? (uint32_t)ln : (uint32_t)ln * (uint32_t)W;
+  const uint32_t pstride = VERT ? (uint32_t)W : 1u;
+  const uint32_t st_f = pstride * (uint32_t)C * 4u, st_b = pstride * (uint32_t)C, st_p = pstride * 4u;
+  const uint32_t st_m = st_f * (uint32_t)C;
+  const uint32_t p_first = fwd ? 0u : (uint32_t)nsteps;
+  const uint32_t d_f = fwd ? st_f : 0u - st_f, d_b = fwd ? st_b : 0u - st_b, d_p = fwd ? st_p : 0u - st_p;
+  const uint32_t d_m = fwd ? st_m : 0u - st_m;
+  auto pos = [&](int s) -> uint32_t { return (uint32_t)(fwd ? s : nsteps - s); };
+  // per-lane byte offsets of the scanline's pixel 0 in the image's planes
+  const uint32_t lpix = lbase * (uint32_t)C;
+  auto slot_f = [&](int sl, int lbl) { return (((uint32_t)n * 4u + (uint32_t)sl) * HWCu + lpix + (uint32_t)lbl) * 4u; };
+  const uint32_t v_cost = ((uint32_t)n * HWCu + lpix + (uint32_t)cl) * 4u;
+  const uint32_t v_x1 = slot_f(VERT ? RIGHT : UP, cl), v_x2 = slot_f(VERT ? LEFT : DOWN, cl);
+  const uint32_t v_mi = slot_f(rslot, cl);
+  const uint32_t v_st = st_ok ? slot_f(rslot, c) : kBufDrop;
+  const uint32_t v_stb = st_ok ? ((uint32_t)n * 4u + (uint32_t)rslot) * HWCu + lpix + (uint32_t)c : kBufDrop;
+  const uint32_t v_e = (((uint32_t)n * 4u + (uint32_t)dir) * HWu + lbase) * 4u;
+  const uint32_t v_pp = (line_ok && c == 0) ? (((uint32_t)n * 4u + (uint32_t)rslot) * HWu + lbase) * 4u : kBufDrop;
+  const uint32_t N4 = (uint32_t)a.N * 4u;
+  const float *xb = VERT ? a.msg_out : a.msg_in;
+  const auto cost_r = make_rsrc(a.cost, (uint32_t)a.N * HWCu * 4u);
+  const auto x_r = make_rsrc(xb, N4 * HWCu * 4u);
+  const auto mi_r = make_rsrc(a.msg_in, N4 * HWCu * 4u);
+  const auto out_r = make_rsrc(a.msg_out, N4 * HWCu * 4u);
+  const auto arg_r = a.argmin ? make_rsrc(a.argmin, N4 * HWCu) : make_rsrc(a.msg_out, 0u);
+  const auto edge_r = make_rsrc(a.edge, N4 * HWu * 4u);
+  const auto sc_r = a.scale ? make_rsrc(a.scale, N4 * HWu * 4u) : make_rsrc(a.msg_out, 0u);
+  const auto mat_r = make_rsrc(a.jump + (int64_t)dir * HWu * C * C, HWCu * (uint32_t)C * 4u);
+  // row l of the sender's C x C block, label cl (rows past C read a neighbouring block or
+  // past the plane, whose values meet h = +inf only)
+  uint32_t v_m[CM];
+#pragma unroll
+  for (int l = 0; l < CM; ++l) v_m[l] = ((lpix + (uint32_t)l) * (uint32_t)C + (uint32_t)cl) * 4u;
 
-  float chain = a.msg_in[(rs_base + q0) * C + cl];
-  if (st_ok) {
-    out_p[q0 * C + c] = chain;
-    if (arg_p) arg_p[q0 * C + c] = 0;
+  // the label of the minimum is 0 in this branch at every position of the scanline
+  if (a.gargmin) {
+    const auto ga_r = make_rsrc(a.gargmin, N4 * HWu);
+    const uint32_t gb = ((uint32_t)n * 4u + (uint32_t)rslot) * HWu + lbase;
+    for (int p = c; p < len; p += SEG) buf_st8(ga_r, line_ok ? gb + (uint32_t)p * pstride : kBufDrop, 0u, 0);
   }
-  if (line_ok && c == 0) {
-    if (a.scale) a.scale[rs_base + q0] = 0.f;
-    if (a.gargmin) a.gargmin[rs_base + q0] = 0;
-  }
+  // the first sender's chain slot is never written by this pass: it keeps the input message
+  float u = buf_ld(mi_r, v_mi, p_first * st_f);
+  buf_st(out_r, v_st, p_first * st_f, u);
+  buf_st8(arg_r, v_stb, p_first * st_b, 0);
+  buf_st(sc_r, v_pp, p_first * st_p, 0.f);
+  float k = 0.f;  // shifted frame minus the normalised frame, per scanline
 
   float pc[PF], p1[PF], p2[PF], pw[PF], pm[PF][CM];
-  auto load_step = [&](int s, int i) {
-    const int64_t q = q0 + (int64_t)s * dq;
-    pc[i] = cost_p[q * C + cl];
-    p1[i] = x1_p[q * C + cl];
-    p2[i] = x2_p[q * C + cl];
-    pw[i] = edge_p[q];
+  auto load_at = [&](uint32_t of, uint32_t op, uint32_t om, int i) {
+    pc[i] = buf_ld(cost_r, v_cost, of);
+    p1[i] = buf_ld(x_r, v_x1, of);
+    p2[i] = buf_ld(x_r, v_x2, of);
+    pw[i] = buf_ld(edge_r, v_e, op);
 #pragma unroll
-    for (int l = 0; l < CM; ++l) pm[i][l] = mat_p[(q * C + (l < C ? l : C - 1)) * C];
+    for (int l = 0; l < CM; ++l) pm[i][l] = buf_ld(mat_r, v_m[l], om);
   };
+  auto load_step = [&](int s, int i) { load_at(pos(s) * st_f, pos(s) * st_p, pos(s) * st_m, i); };
 #pragma unroll
   for (int i = 0; i < PF; ++i) {
     load_step(i < nsteps ? i : (nsteps > 0 ? nsteps - 1 : 0), i);
     __builtin_amdgcn_sched_barrier(0);  // keep the ring slots in issue order
   }
-  for (int s0 = 0; s0 < nsteps; s0 += PF) {
+  // receiver offsets of the next step
+  uint32_t o_f = p_first * st_f + d_f, o_b = p_first * st_b + d_b, o_p = p_first * st_p + d_p;
+  auto step = [&](int i) {
+    // the chain: h -> LDS -> candidates -> minimum -> u
+    const float h = ((pc[i] + p1[i]) + p2[i]) + u;
+    const float hl = lab ? h : __builtin_inff();
+    lds_h[seg][c] = hl;
+    wave_lds_sync();
+    // the shift min_l h[l], over the lanes while the row makes its LDS round trip
+    const float shift = fminf(seg_reduce<SEG>(hl, OpMin()), 1e15f);  // all-infinite h: the clamp
+    const float w = pw[i];
+    float hv[CM], nv[CM];
 #pragma unroll
-    for (int i = 0; i < PF; ++i) {
-      const int s = s0 + i;
-      if (s < nsteps) {
-        float h = VERT ? ((pc[i] + p1[i]) + p2[i]) + chain : ((pc[i] + chain) + p1[i]) + p2[i];
-        if (c < CM) lds_h[seg][c] = lab ? h : __builtin_inff();
-        wave_lds_sync();
-        const float w = pw[i];
-        float nv[CM];
+    for (int l = 0; l < CM; ++l) hv[l] = lds_h[seg][l];
+    wave_lds_sync();  // the reads are done before the next step's write
 #pragma unroll
-        for (int l = 0; l < CM; ++l) nv[l] = fmaf(pm[i][l], w, lds_h[seg][l]);
-        wave_lds_sync();  // the reads are done before the next step's write
-        float m[CM + 1];
+    for (int l = 0; l < CM; ++l) nv[l] = fmaf(pm[i][l], w, hv[l]);
+    // minimum: a min3 tree over the candidates and the initial 1e15
+    float m[CM + 1];
 #pragma unroll
-        for (int l = 0; l < CM; ++l) m[l] = nv[l];
-        m[CM] = 1e15f;
-        int cnt = CM + 1;
+    for (int l = 0; l < CM; ++l) m[l] = nv[l];
+    m[CM] = 1e15f;
+    int cnt = CM + 1;
 #pragma unroll
-        for (int r = 0; r < 6; ++r) {
-          if (cnt > 1) {
-            int o = 0;
+    for (int r = 0; r < 6; ++r) {
+      if (cnt > 1) {
+        int o = 0;
 #pragma unroll
-            for (int t = 0; t < CM + 1; t += 3) {
-              if (t < cnt) {
-                float v = m[t];
-                if (t + 1 < cnt) v = fminf(v, m[t + 1]);
-                if (t + 2 < cnt) v = fminf(v, m[t + 2]);
-                m[o++] = v;
-              }
-            }
-            cnt = o;
+        for (int t = 0; t < CM + 1; t += 3) {
+          if (t < cnt) {
+            float v = m[t];
+            if (t + 1 < cnt) v = fminf(v, m[t + 1]);
+            if (t + 2 < cnt) v = fminf(v, m[t + 2]);
+            m[o++] = v;
           }
         }
-        const float msg = m[0];
-        int arg = 0;
-#pragma unroll
-        for (int l = 0; l < CM; ++l) arg = nv[l] == msg ? l : arg;
-        float mx = seg_reduce<SEG>(lab ? msg : -__builtin_inff(), OpMax());
-        float se = seg_reduce<SEG>(lab ? fast_exp(msg - mx) : 0.f, OpSum());
-        const float lse = mx + fast_log(se);
-        chain = msg - lse;
-        const int64_t qq = q0 + (int64_t)(s + 1) * dq;
-        if (st_ok) {
-          out_p[qq * C + c] = chain;
-          if (arg_p) arg_p[qq * C + c] = (uint8_t)arg;
-        }
-        if (line_ok && c == 0) {
-          if (a.scale) a.scale[rs_base + qq] = fast_exp(lse);
-          if (a.gargmin) a.gargmin[rs_base + qq] = 0;
-        }
+        cnt = o;
       }
-      load_step(s + PF < nsteps ? s + PF : nsteps - 1, i);
+    }
+    const float msg = m[0];
+    u = msg - shift;
+    // off the chain: normalisation, argmin (the last label equal to the minimum), stores
+    const float mx = seg_reduce<SEG>(lab ? msg : -__builtin_inff(), OpMax());
+    const float se = seg_reduce<SEG>(lab ? fast_exp(msg - mx) : 0.f, OpSum());
+    const float lse = mx + fast_log(se);
+    int am[CM];
+#pragma unroll
+    for (int l = 0; l < CM; ++l) am[l] = nv[l] == msg ? l : 0;
+    int acnt = CM;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (acnt > 1) {
+        int o = 0;
+#pragma unroll
+        for (int t = 0; t < CM; t += 3) {
+          if (t < acnt) {
+            int v = am[t];
+            if (t + 1 < acnt) v = max(v, am[t + 1]);
+            if (t + 2 < acnt) v = max(v, am[t + 2]);
+            am[o++] = v;
+          }
+        }
+        acnt = o;
+      }
+    }
+    buf_st(out_r, v_st, o_f, msg - lse);
+    buf_st8(arg_r, v_stb, o_b, (uint8_t)am[0]);
+    buf_st(sc_r, v_pp, o_p, fast_exp(lse - k));
+    k = lse - shift;
+    o_f += d_f;
+    o_b += d_b;
+    o_p += d_p;
+  };
+  // main loop: the ring refills stay within the scanline (s + PF <= nsteps - 1)
+  int s0 = 0;
+  {
+    const uint32_t pr = pos(PF < nsteps ? PF : 0);
+    uint32_t l_f = pr * st_f, l_p = pr * st_p, l_m = pr * st_m;
+    for (; s0 + 2 * PF <= nsteps; s0 += PF) {
+#pragma unroll
+      for (int i = 0; i < PF; ++i) {
+        step(i);
+        // refill ring slot i (its old contents are dead now: no register copies, no drain)
+        load_at(l_f, l_p, l_m, i);
+        l_f += d_f;
+        l_p += d_p;
+        l_m += d_m;
+      }
+    }
+  }
+  // the last groups: refills clamped to the last sender, steps past the end skipped
+  for (; s0 < nsteps; s0 += PF) {
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      if (s0 + i < nsteps) {
+        step(i);
+        load_step(min(s0 + i + PF, nsteps - 1), i);
+      }
     }
   }
 }
@@ -1576,6 +1653,13 @@ void launch_stereo_w64_any(const FwdArgs &a, bool vert, hipStream_t st) {
 }
 
 bool stereo_w64_ok(const FwdArgs &a) { return a.C > 16 && a.delta < a.C; }
+// the semantic per-pixel scanline kernel: full min-plus branch, C <= 32, the messages of all
+// images and a direction's C x C matrix plane below 2^31 bytes
+bool semp_fast(const FwdArgs &a) {
+  const int64_t hwc = (int64_t)a.H * a.W * a.C;
+  return a.delta >= a.C && a.C <= 32 && (int64_t)a.N * 4 * hwc * 4 < (1LL << 31) &&
+         hwc * a.C * 4 < (1LL << 31) && !generic_only();
+}
 // the semantic-global scanline kernel: full min-plus branch, C <= 32, [H,W,C] planes below 2^31 bytes
 bool semg_fast(const FwdArgs &a) {
   return a.delta >= a.C && a.C <= 32 && (int64_t)a.H * a.W * a.C * 4 < (1LL << 31) && !generic_only();
@@ -1682,8 +1766,8 @@ static int lbp_fwd_impl(int kind, const float *cost, const float *jump, int64_t 
         else launch_kind<BPL_LBP_SEMANTIC_GLOBAL>(a, vert, st, shm);
         break;
       default:
-        if (a.delta >= a.C && a.C <= 20 && !generic_only()) launch_semp<20, 4>(a, vert, st);
-        else if (a.delta >= a.C && a.C <= 32 && !generic_only()) launch_semp<32, 2>(a, vert, st);
+        if (semp_fast(a) && a.C <= 20) launch_semp<20, 4>(a, vert, st);
+        else if (semp_fast(a)) launch_semp<32, 2>(a, vert, st);
         else launch_kind<BPL_LBP_SEMANTIC_PIXEL>(a, vert, st, shm);
         break;
     }
