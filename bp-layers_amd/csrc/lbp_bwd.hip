@@ -1336,19 +1336,26 @@ void launch_semg(const BwdArgs &a, bool vert, int ngroups, hipStream_t st) {
 }
 
 // Semantic per-pixel pairwise, 16 < C <= 32 or C <= 20 (both forward branches, both modes):
-// the adjoint step of lbp_bwd_semg, with the per-pixel parameter gradients written per step:
-// the sender's C x C matrix gradient (lane c writes column c: w g(c) at row argmin(c) -- row
-// gargmin for a truncation winner in the reference mode; in the exact mode a truncation winner
-// credits the truncation entry [0][C-1], which the lane owning column C-1 adds) and the edge
-// gradient (sum_c M[r][c] g(c) in the exact mode from the matrix column read per step; zero in
-// the reference mode, as the reference build).  IMGS (N == 2): the half-waves are the two images
-// of the same scanline and the batch-shared matrix gradient is the sum image 0 + image 1 formed
-// in registers (permlane32 swap), each half-wave writing half of the rows; otherwise (N == 1)
-// two scanlines of one image, each writing its full slices.
+// the adjoint step of lbp_bwd_semg (routing through the segment's sender x receiver matrix in
+// LDS), with the per-pixel parameter gradients written per step: the sender's C x C matrix
+// gradient (lane c writes column c: w g(c) at row argmin(c) -- row gargmin for a truncation
+// winner in the reference mode; in the exact mode a truncation winner credits the truncation
+// entry [0][C-1], which the lane owning column C-1 adds) and the edge gradient (sum_c
+// M[r][c] g(c) in the exact mode from the matrix column read per step; zero in the reference
+// mode, as the reference build).  IMGS (N == 2): the half-waves are the two images of the same
+// scanline and the batch-shared matrix gradient is the sum image 0 + image 1 formed in
+// registers (permlane32 swap), each half-wave writing half of the rows; otherwise (N == 1) two
+// scanlines of one image, each writing its full slices.  Tensors are addressed as whole buffers
+// (the [N,4,H,W,C] planes of all images below 2^31 bytes, a direction's matrix plane too) with
+// per-lane offsets for the image, slot and matrix-row parts computed once and one scalar offset
+// per position, so the steps carry no 64-bit address arithmetic; lanes that must not store get
+// an offset past the buffer (dropped).
 template <int CM, bool VERT, int PF, bool IMGS, bool EXACT>
 __global__ __launch_bounds__(64) void lbp_bwd_semp(BwdArgs a) {
   constexpr int SEG = 32, SPW = 2;
-  __shared__ __align__(16) int2 lds_ag[SPW][CM];  // {routing target, gradient bits}
+  constexpr int RS = CM + 4;                          // routing-matrix row stride (16-byte rows)
+  constexpr int NR = IMGS ? (CM + 1) / 2 : CM;        // matrix-gradient rows a lane writes per step
+  __shared__ __align__(16) float lds_r[SPW][SEG * RS + SEG];  // rows 0..31, then a dummy row
   const int H = a.H, W = a.W, C = a.C;
   const int L = VERT ? W : H;
   const int len = VERT ? H : W;
@@ -1371,54 +1378,89 @@ __global__ __launch_bounds__(64) void lbp_bwd_semp(BwdArgs a) {
   const bool lab = c < C;
   const bool st_ok = line_ok && lab;
   const int cl = lab ? c : C - 1;
-  const int64_t HW = (int64_t)H * W;
-  const int64_t dq = VERT ? (fwd ? (int64_t)W : -(int64_t)W) : (fwd ? 1 : -1);
-  const int64_t q0 = VERT ? (int64_t)(fwd ? 0 : H - 1) * W + ln : (int64_t)ln * W + (fwd ? 0 : W - 1);
-  const int64_t img = (int64_t)n * HW * C;
-  const int64_t rsl = ((int64_t)n * 4 + rslot) * HW;
-  float *Xo = a.X[dir] + img;
-  const float *XDp = a.X[DOWN] + img;
-  const float *XUp = a.X[UP] + img;
-  const float *ig_p = a.in_grad + rsl * C;
-  const float *m_p = a.msgs + rsl * C;
-  const uint8_t *am_p = a.argmin + rsl * C;
-  const uint8_t *gl_p = a.gargmin + rsl;
-  const float *edge_p = a.edge + ((int64_t)n * 4 + dir) * HW;
-  float *ge_p = a.grad_edge + ((int64_t)n * 4 + dir) * HW;
-  const float *mat_p = EXACT ? a.jump + (int64_t)dir * HW * C * C + cl : nullptr;  // + (q * C + r) * C
-  float *gm_p = a.grad_jump + (int64_t)dir * HW * C * C + cl;
+  const uint32_t HWu = (uint32_t)H * (uint32_t)W, HWCu = HWu * (uint32_t)C;
+  const uint32_t lbase = VERT ? (uint32_t)ln : (uint32_t)ln * (uint32_t)W;
+  const uint32_t pstride = VERT ? (uint32_t)W : 1u;
+  const uint32_t st_f = pstride * (uint32_t)C * 4u, st_b = pstride * (uint32_t)C, st_p = pstride * 4u;
+  const uint32_t st_m = st_f * (uint32_t)C;
+  // one adjoint step moves the positions by -dpos (the pass runs forward in dpos)
+  const uint32_t d_f = fwd ? st_f : 0u - st_f, d_b = fwd ? st_b : 0u - st_b, d_p = fwd ? st_p : 0u - st_p;
+  const uint32_t d_1 = fwd ? pstride : 0u - pstride, d_m = fwd ? st_m : 0u - st_m;
+  auto pos = [&](int s) -> uint32_t { return (uint32_t)(fwd ? s : nsteps - s); };
+  const uint32_t lpix = lbase * (uint32_t)C;
+  const uint32_t N4 = (uint32_t)a.N * 4u;
+  auto slot_f = [&](int sl, int lbl) { return (((uint32_t)n * 4u + (uint32_t)sl) * HWCu + lpix + (uint32_t)lbl) * 4u; };
+  // in_grad: with chained U/D slots the DOWN slot's rows 1..H-2 are the UP slot's; only the
+  // row-0 receiver (the first adjoint step of the UP pass) reads the DOWN slot (UP + 1)
+  const bool uds_dn = VERT && a.uds_in && rslot == DOWN;
+  const uint32_t v_ig = slot_f(uds_dn ? UP : rslot, cl);
+  const uint32_t v_ms = slot_f(rslot, cl);
+  const uint32_t v_am = ((uint32_t)n * 4u + (uint32_t)rslot) * HWCu + lpix + (uint32_t)cl;
+  const uint32_t v_gl = ((uint32_t)n * 4u + (uint32_t)rslot) * HWu + lbase;
+  const uint32_t v_x = ((uint32_t)n * HWCu + lpix + (uint32_t)cl) * 4u;
+  const uint32_t v_xs = st_ok ? ((uint32_t)n * HWCu + lpix + (uint32_t)c) * 4u : kBufDrop;
+  const uint32_t v_e = (((uint32_t)n * 4u + (uint32_t)dir) * HWu + lbase) * 4u;
+  const uint32_t v_ge = (line_ok && c == 0) ? v_e : kBufDrop;
+  const auto ig_r = (!VERT && a.igz) ? make_rsrc(a.in_grad, 0u) : make_rsrc(a.in_grad, N4 * HWCu * 4u);
+  const auto m_r = make_rsrc(a.msgs, N4 * HWCu * 4u);
+  const auto am_r = make_rsrc(a.argmin, N4 * HWCu);
+  const auto gl_r = make_rsrc(a.gargmin, N4 * HWu);
+  const auto xd_r = make_rsrc(a.X[DOWN], VERT ? 0u : (uint32_t)a.N * HWCu * 4u);
+  const auto xu_r = make_rsrc(a.X[UP], VERT ? 0u : (uint32_t)a.N * HWCu * 4u);
+  const auto xo_r = make_rsrc(a.X[dir], (uint32_t)a.N * HWCu * 4u);
+  const auto edge_r = make_rsrc(a.edge, N4 * HWu * 4u);
+  const auto ge_r = make_rsrc(a.grad_edge, N4 * HWu * 4u);
+  const uint32_t mplane = HWCu * (uint32_t)C * 4u;
+  const auto mat_r = make_rsrc(EXACT ? a.jump + (int64_t)dir * HWu * C * C : a.grad_jump, EXACT ? mplane : 0u);
+  const auto gm_r = make_rsrc(a.grad_jump + (int64_t)dir * HWu * C * C, mplane);
+  // matrix rows: row l of the sender's block, label cl (rows past C are never used); the
+  // gradient rows this lane writes (IMGS: half of them per half-wave)
+  const int rlo = IMGS ? (seg ? C / 2 : 0) : 0, rhi = IMGS ? (seg ? C : C / 2) : C;
+  uint32_t v_mm[EXACT ? CM : 1], v_gm[NR];
+#pragma unroll
+  for (int l = 0; l < (EXACT ? CM : 1); ++l) v_mm[l] = ((lpix + (uint32_t)l) * (uint32_t)C + (uint32_t)cl) * 4u;
+#pragma unroll
+  for (int jr = 0; jr < NR; ++jr) {
+    const int rr = rlo + jr;
+    v_gm[jr] = (line_ok && lab && rr < rhi) ? ((lpix + (uint32_t)rr) * (uint32_t)C + (uint32_t)c) * 4u : kBufDrop;
+  }
 
   // the last position of the pass sends nothing: zero its outputs (and its matrix gradient,
   // once per scanline)
   {
-    const int64_t q = q0 + (int64_t)nsteps * dq;
-    if (st_ok) Xo[q * C + c] = 0.f;
-    if (line_ok && c == 0) ge_p[q] = 0.f;
-    if (line_ok && lab && (!IMGS || seg == 0))
-      for (int r = 0; r < C; ++r) gm_p[(q * C + r) * C] = 0.f;
+    const uint32_t pl = pos(nsteps);
+    buf_st(xo_r, v_xs, pl * st_f, 0.f);
+    buf_st(ge_r, v_ge, pl * st_p, 0.f);
+    const bool zr = line_ok && lab && (!IMGS || seg == 0);
+    for (int rr = 0; rr < C; ++rr)
+      buf_st(gm_r, zr ? ((lpix + (uint32_t)rr) * (uint32_t)C + (uint32_t)c) * 4u : kBufDrop, pl * st_m, 0.f);
   }
+  for (int i = c; i < SEG * RS + SEG; i += SEG) lds_r[seg][i] = 0.f;
+  wave_lds_sync();
 
   constexpr int PJ = EXACT ? PF : 1;
   float pg[PF], pm[PF], pd[PF], pu[PF], pw[PF], pj[PJ][CM];
   int pa[PF], pl[PF];
-  auto load_step = [&](int s, int i) {
-    const int64_t q = q0 + (int64_t)s * dq;
-    const int64_t qr = q + dq;
-    pg[i] = (!VERT && a.igz) ? 0.f : ig_p[qr * C + cl - ig_shift(a, rslot, qr)];
-    pm[i] = m_p[qr * C + cl];
-    pa[i] = (int)am_p[qr * C + cl];
-    pl[i] = (int)gl_p[qr];
-    if (!VERT) {
-      pd[i] = XDp[qr * C + cl];
-      pu[i] = XUp[qr * C + cl];
-    } else {
-      pd[i] = pu[i] = 0.f;
-    }
-    pw[i] = edge_p[q];
+  // ring slot i of adjoint step s: the receiver's (position s + 1) in_grad, message, argmin,
+  // label of the minimum and vertical h-gradients, the sender's (position s) edge weight and
+  // matrix column
+  auto load_at = [&](uint32_t rf, uint32_t rb, uint32_t r1, uint32_t qp, uint32_t qm, uint32_t igo, int i) {
+    pg[i] = buf_ld(ig_r, v_ig, igo);
+    pm[i] = buf_ld(m_r, v_ms, rf);
+    pa[i] = (int)__builtin_amdgcn_raw_buffer_load_b8(am_r, v_am, rb, 0);
+    pl[i] = (int)__builtin_amdgcn_raw_buffer_load_b8(gl_r, v_gl, r1, 0);
+    pd[i] = VERT ? 0.f : buf_ld(xd_r, v_x, rf);
+    pu[i] = VERT ? 0.f : buf_ld(xu_r, v_x, rf);
+    pw[i] = buf_ld(edge_r, v_e, qp);
     if (EXACT) {
 #pragma unroll
-      for (int l = 0; l < CM; ++l) pj[EXACT ? i : 0][l] = mat_p[(q * C + (l < C ? l : C - 1)) * C];
+      for (int l = 0; l < CM; ++l) pj[EXACT ? i : 0][l] = buf_ld(mat_r, v_mm[EXACT ? l : 0], qm);
     }
+  };
+  auto load_step = [&](int s, int i) {
+    const uint32_t pr = pos(s + 1), pq = pos(s);
+    load_at(pr * st_f, pr * st_b, pr * pstride, pq * st_p, pq * st_m,
+            pr * st_f + (uds_dn && pr == 0u ? HWCu * 4u : 0u), i);
   };
   if (nsteps > 0) {
 #pragma unroll
@@ -1429,80 +1471,120 @@ __global__ __launch_bounds__(64) void lbp_bwd_semp(BwdArgs a) {
     }
   }
   float acc = 0.f;
-  for (int t0 = 0; t0 < nsteps; t0 += PF) {
+  int prev = SEG * RS + c;  // the routing cell written one step earlier (the dummy row at first)
+  // sender offsets of the current adjoint step
+  uint32_t o_f = pos(nsteps - 1) * st_f, o_p = pos(nsteps - 1) * st_p, o_m = pos(nsteps - 1) * st_m;
+  auto step = [&](int i) {
+    const int A = pa[i];
+    const int gl = pl[i];
+    const bool tr = A == kFromTrunc;
+    const int r = tr ? gl : A;
+    // normalisation Jacobian: g = t - softmax(msg) * sum(t), t = in_grad + chain
+    const float tt = pg[i] + acc;
+    const float ts = seg_reduce<SEG>(lab ? tt : 0.f, OpSum());
+    float gk = tt - fast_exp(pm[i]) * ts;
+    if (!VERT) gk += pd[i] + pu[i];
+    asm volatile("" : "+v"(gk));
+    gk = lab ? gk : 0.f;
+    // route: clear last step's cell, write [r][c]; sender c sums row c in label order
+    int cell = __mul24(r, RS) + c;
+    asm volatile("" : "+v"(cell));
+    cell = lab ? cell : SEG * RS + c;
+    lds_r[seg][prev] = 0.f;
+    lds_r[seg][cell] = gk;
+    prev = cell;
+    wave_lds_sync();
+    float row[CM];
+#pragma unroll
+    for (int l = 0; l < CM; ++l) row[l] = lds_r[seg][c * RS + l];
+    wave_lds_sync();  // the row is read before the next step's writes
+    float gh = 0.f;
+#pragma unroll
+    for (int l = 0; l < CM; ++l) gh += row[l];
+    acc = gh;
+    buf_st(xo_r, v_xs, o_f, gh);
+    const float w = pw[i];
+    const bool to_tr = tr && EXACT;
+    // edge gradient
+    float ge = 0.f;
+    if (EXACT) {
+      // the truncation entry M[0][C-1] is the row-0 slot of the segment's lane C-1
+      const int j0 = __float_as_int(pj[EXACT ? i : 0][0]);
+      const float jt0 = __int_as_float(__builtin_amdgcn_readlane(j0, C - 1));
+      const float jt1 = __int_as_float(__builtin_amdgcn_readlane(j0, SEG + C - 1));
+      float mv = 0.f;
+#pragma unroll
+      for (int l = 0; l < CM; ++l) mv = l == A ? pj[EXACT ? i : 0][l] : mv;
+      mv = to_tr ? (seg ? jt1 : jt0) : mv;
+      ge = seg_reduce<SEG>(lab ? mv * gk : 0.f, OpSum());
+    }
+    buf_st(ge_r, v_ge, o_p, ge);
+    // matrix gradient column c of the sender
+    const float val = to_tr ? 0.f : w * gk;
+    float ex = 0.f;
+    if (EXACT) {
+      const float fsum = seg_reduce<SEG>(lab && tr ? gk : 0.f, OpSum());
+      ex = w * fsum;
+    }
+    const bool own_ex = EXACT && c == C - 1;
+    if (IMGS) {
+      float v0, v1, x0, x1;
+      int r0, r1;
+      swap_halves(val, v0, v1);
+      swap_halves(ex, x0, x1);
+      swap_halves(r, r0, r1);
+#pragma unroll
+      for (int jr = 0; jr < NR; ++jr) {
+        const int rr = rlo + jr;
+        float e0 = rr == r0 ? v0 : 0.f, e1 = rr == r1 ? v1 : 0.f;
+        if (own_ex && rr == 0) {
+          e0 += x0;
+          e1 += x1;
+        }
+        buf_st(gm_r, v_gm[jr], o_m, e0 + e1);
+      }
+    } else {
+#pragma unroll
+      for (int rr = 0; rr < NR; ++rr) {
+        float e = rr == r ? val : 0.f;
+        if (own_ex && rr == 0) e += ex;
+        buf_st(gm_r, v_gm[rr], o_m, e);
+      }
+    }
+    o_f -= d_f;
+    o_p -= d_p;
+    o_m -= d_m;
+  };
+  // main loop: the ring refills stay within the scanline (s - PF >= 0)
+  int t0 = 0;
+  {
+    const int s = nsteps - 1 - PF;  // the first refill's sender position
+    const uint32_t pr = pos(s >= 0 ? s + 1 : 0), pq = pos(s >= 0 ? s : 0);
+    uint32_t l_f = pr * st_f, l_b = pr * st_b, l_1 = pr * pstride, l_p = pq * st_p, l_m = pq * st_m;
+    for (; t0 + 2 * PF <= nsteps; t0 += PF) {
+#pragma unroll
+      for (int i = 0; i < PF; ++i) {
+        step(i);
+        // refill ring slot i (its old contents are dead now: no register copies, no drain)
+        load_at(l_f, l_b, l_1, l_p, l_m, l_f, i);
+        l_f -= d_f;
+        l_b -= d_b;
+        l_1 -= d_1;
+        l_p -= d_p;
+        l_m -= d_m;
+      }
+    }
+  }
+  // the last groups: refills clamped to the first sender, steps past the end skipped
+  for (; t0 < nsteps; t0 += PF) {
 #pragma unroll
     for (int i = 0; i < PF; ++i) {
       const int t = t0 + i;
-      const int s = nsteps - 1 - t;  // adjoint order
       if (t < nsteps) {
-        const int64_t q = q0 + (int64_t)s * dq;
-        const float tt = pg[i] + acc;
-        const float ts = seg_reduce<SEG>(lab ? tt : 0.f, OpSum());
-        float gk = tt - fast_exp(pm[i]) * ts;
-        if (!VERT) gk += pd[i] + pu[i];
-        gk = lab ? gk : 0.f;
-        const int A = pa[i];
-        const int gl = pl[i];
-        const bool tr = A == kFromTrunc;
-        const int r = tr ? gl : A;
-        if (c < CM) {
-          lds_ag[seg][c] = make_int2(lab ? r : -1, __float_as_int(gk));
-        }
-        wave_lds_sync();
-        const float gh = routed_gather<CM>(lds_ag[seg], c);  // two entries per 16-byte read
-        wave_lds_sync();  // the gathers are done before the next step's writes
-        const float w = pw[i];
-        const bool to_tr = tr && EXACT;
-        // edge gradient
-        float ge = 0.f;
-        if (EXACT) {
-          // the truncation entry M[0][C-1] is the row-0 slot of the segment's lane C-1
-          const int j0 = __float_as_int(pj[EXACT ? i : 0][0]);
-          const float jt0 = __int_as_float(__builtin_amdgcn_readlane(j0, C - 1));
-          const float jt1 = __int_as_float(__builtin_amdgcn_readlane(j0, SEG + C - 1));
-          float mv = 0.f;
-#pragma unroll
-          for (int l = 0; l < CM; ++l) mv = l == A ? pj[EXACT ? i : 0][l] : mv;
-          mv = to_tr ? (seg ? jt1 : jt0) : mv;
-          ge = seg_reduce<SEG>(lab ? mv * gk : 0.f, OpSum());
-        }
-        if (line_ok && c == 0) ge_p[q] = ge;
-        // matrix gradient column c of the sender q
-        const float val = to_tr ? 0.f : w * gk;
-        float ex = 0.f;
-        if (EXACT) {
-          const float fsum = seg_reduce<SEG>(lab && tr ? gk : 0.f, OpSum());
-          ex = w * fsum;
-        }
-        const bool own_ex = EXACT && c == C - 1;
-        if (IMGS) {
-          float v0, v1, x0, x1;
-          int r0, r1;
-          swap_halves(val, v0, v1);
-          swap_halves(ex, x0, x1);
-          swap_halves(r, r0, r1);
-          const int rlo = seg ? C / 2 : 0, rhi = seg ? C : C / 2;
-          if (line_ok && lab) {
-            for (int rr = rlo; rr < rhi; ++rr) {
-              float e0 = rr == r0 ? v0 : 0.f, e1 = rr == r1 ? v1 : 0.f;
-              if (own_ex && rr == 0) {
-                e0 += x0;
-                e1 += x1;
-              }
-              gm_p[(q * C + rr) * C] = e0 + e1;
-            }
-          }
-        } else if (line_ok && lab) {
-          for (int rr = 0; rr < C; ++rr) {
-            float e = rr == r ? val : 0.f;
-            if (own_ex && rr == 0) e += ex;
-            gm_p[(q * C + rr) * C] = e;
-          }
-        }
-        acc = gh;
-        if (st_ok) Xo[q * C + c] = gh;
+        step(i);
+        const int s = nsteps - 1 - t;
+        load_step(s - PF >= 0 ? s - PF : 0, i);
       }
-      load_step(s - PF >= 0 ? s - PF : 0, i);
     }
   }
 }
@@ -1527,7 +1609,11 @@ void launch_semp(const BwdArgs &a, bool vert, hipStream_t st) {
 }
 
 // the per-pixel kernels cover C <= 32 for batches of one or two images
-bool semp_ok(const BwdArgs &a) { return a.C <= 32 && a.N <= 2 && !generic_only(); }
+bool semp_ok(const BwdArgs &a) {
+  const int64_t hwc = (int64_t)a.H * a.W * a.C;
+  return a.C <= 32 && a.N <= 2 && (int64_t)a.N * 4 * hwc * 4 < (1LL << 31) && hwc * a.C * 4 < (1LL << 31) &&
+         !generic_only();
+}
 
 // y-boundary terms of the input-message gradient after the paired horizontal phase (which
 // wrote X_R + X_L into the U and D slots): U slot of row 0 = (in_grad_U + X_D) + (X_R + X_L),
