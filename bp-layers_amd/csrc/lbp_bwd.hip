@@ -65,11 +65,6 @@ __device__ __forceinline__ int64_t sl_px(int n, int s, int y, int x, int H, int 
 __device__ __forceinline__ int ig_slot(const BwdArgs &a, int rslot, int y) {
   return (a.uds_in && rslot == DOWN && y != 0 && y != a.H - 1) ? UP : rslot;
 }
-// the same as an element offset from the rslot plane, for in-plane pixel index qr
-__device__ __forceinline__ int64_t ig_shift(const BwdArgs &a, int rslot, int64_t qr) {
-  const int64_t W = a.W, H = a.H;
-  return (a.uds_in && rslot == DOWN && qr >= W && qr < (H - 1) * W) ? H * W * a.C : 0;
-}
 
 template <int KIND, int SEG, int LPL, bool VERT, int PF>
 __global__ __launch_bounds__(64) void lbp_bwd_phase(BwdArgs a) {

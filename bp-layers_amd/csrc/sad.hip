@@ -357,7 +357,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void s
 // the f1 rows y+ov+v-h of the tile (plus the horizontal halo) are staged in LDS channel chunk
 // by chunk; S(u,v) for all u is accumulated in registers, then cv_v[v] (min over u) is
 // written and the running cv_u (min over v) updated -- both with first-index ties.
-constexpr int FL_TY = 4, FL_TX = 32, FL_CB = 8, FL_G = 4, FL_KMAX = 25;
+constexpr int FL_TY = 4, FL_TX = 32, FL_CB = 8, FL_G = 4;
 
 // Block: 4 rows x 32 pixels x 4 groups of vertical displacements (512 threads).  Group g owns
 // the displacements vi = g, g+4, ...; the 4 groups advance together, so one staging of the f1
@@ -719,7 +719,6 @@ __global__ __launch_bounds__(FM_NT) void sad_flow_bwd_lds_k(const float *__restr
   const int64_t HW = (int64_t)H * W;
   const int64_t img = (int64_t)n * HW;
   const int64_t pix = inside ? (int64_t)y * W + x : 0;
-  const int64_t p = img + pix;
   const int own = (ty + E) * RC + (tx + E);  // own pixel in the region
   const int ipix = (int)pix;                 // own pixel in the image
   // the argmin and gradient volumes of this image as buffers: 32-bit offsets, dropped
