@@ -433,28 +433,13 @@ __global__ __launch_bounds__(64) void lbp_fwd_semg(FwdArgs a) {
     const float mx = seg_reduce<SEG>(lab ? msg : -__builtin_inff(), OpMax());
     const float se = seg_reduce<SEG>(lab ? fast_exp(msg - mx) : 0.f, OpSum());
     const float lse = mx + fast_log(se);
-    int am[CM];
+    // (a chain of selects in label order: the scheduler interleaves it with the next step's
+    // chain; a max tree over per-label candidates took more instructions)
+    int arg = 0;
 #pragma unroll
-    for (int l = 0; l < CM; ++l) am[l] = nv[l] == msg ? l : 0;
-    int acnt = CM;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      if (acnt > 1) {
-        int o = 0;
-#pragma unroll
-        for (int t = 0; t < CM; t += 3) {
-          if (t < acnt) {
-            int v = am[t];
-            if (t + 1 < acnt) v = max(v, am[t + 1]);
-            if (t + 2 < acnt) v = max(v, am[t + 2]);
-            am[o++] = v;
-          }
-        }
-        acnt = o;
-      }
-    }
+    for (int l = 0; l < CM; ++l) arg = nv[l] == msg ? l : arg;
     buf_st(out_r, v_st, o_f, msg - lse);
-    buf_st8(arg_r, v_stb, o_b, (uint8_t)am[0]);
+    buf_st8(arg_r, v_stb, o_b, (uint8_t)arg);
     buf_st(sc_r, v_pp, o_p, fast_exp(lse - k));
     k = lse - shift;
     o_f += d_f;
@@ -643,28 +628,13 @@ __global__ __launch_bounds__(64) void lbp_fwd_semp(FwdArgs a) {
     const float mx = seg_reduce<SEG>(lab ? msg : -__builtin_inff(), OpMax());
     const float se = seg_reduce<SEG>(lab ? fast_exp(msg - mx) : 0.f, OpSum());
     const float lse = mx + fast_log(se);
-    int am[CM];
+    // (a chain of selects in label order: the scheduler interleaves it with the next step's
+    // chain; a max tree over per-label candidates took more instructions)
+    int arg = 0;
 #pragma unroll
-    for (int l = 0; l < CM; ++l) am[l] = nv[l] == msg ? l : 0;
-    int acnt = CM;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      if (acnt > 1) {
-        int o = 0;
-#pragma unroll
-        for (int t = 0; t < CM; t += 3) {
-          if (t < acnt) {
-            int v = am[t];
-            if (t + 1 < acnt) v = max(v, am[t + 1]);
-            if (t + 2 < acnt) v = max(v, am[t + 2]);
-            am[o++] = v;
-          }
-        }
-        acnt = o;
-      }
-    }
+    for (int l = 0; l < CM; ++l) arg = nv[l] == msg ? l : arg;
     buf_st(out_r, v_st, o_f, msg - lse);
-    buf_st8(arg_r, v_stb, o_b, (uint8_t)am[0]);
+    buf_st8(arg_r, v_stb, o_b, (uint8_t)arg);
     buf_st(sc_r, v_pp, o_p, fast_exp(lse - k));
     k = lse - shift;
     o_f += d_f;
