@@ -316,27 +316,6 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
 }
 
-// Routed gather over one segment's (routing target, gradient bits) pairs in LDS: the sum, in
-// label order, of the gradients whose target is c.  Every pair is read unconditionally, all
-// reads issued before the first compare (an empty asm takes each loaded value), so the
-// compiler cannot turn the selects into branches around per-entry reads, each of which waits
-// for its own LDS round trip.
-template <int CM>
-__device__ __forceinline__ float routed_gather(const int2 *pairs, int c) {
-  typedef int v4i __attribute__((ext_vector_type(4)));
-  v4i v[CM / 2];
-#pragma unroll
-  for (int j = 0; j < CM / 2; ++j) v[j] = *reinterpret_cast<const v4i *>(pairs + 2 * j);
-  float gh = 0.f;
-#pragma unroll
-  for (int j = 0; j < CM / 2; ++j) {
-    asm volatile("" : "+v"(v[j]));
-    gh += v[j].x == c ? __int_as_float(v[j].y) : 0.f;
-    gh += v[j].z == c ? __int_as_float(v[j].w) : 0.f;
-  }
-  return gh;
-}
-
 // Workgroups of one launch are dealt to the 8 XCDs round-robin (workgroup b runs on XCD
 // b mod 8).  The remap gives each XCD a contiguous range of logical indices, so that
 // neighbouring scanlines, which share cache lines of per-pixel parameters, share one L2.
