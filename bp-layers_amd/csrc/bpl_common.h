@@ -346,8 +346,10 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, ui
 __device__ __forceinline__ float buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
   return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
 }
+// AUX: the store's cache-policy bits (0: default)
+template <int AUX = 0>
 __device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, float v) {
-  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, voff, soff, 0);
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, voff, soff, AUX);
 }
 __device__ __forceinline__ void buf_st8(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, uint8_t v) {
   __builtin_amdgcn_raw_buffer_store_b8(v, r, voff, soff, 0);
@@ -371,24 +373,24 @@ __device__ __forceinline__ void buf_ld_row(__amdgpu_buffer_rsrc_t r, uint32_t vo
     d[2] = __uint_as_float(v[2]); d[3] = __uint_as_float(v[3]);
   }
 }
-template <int LPL>
+template <int LPL, int AUX = 0>
 __device__ __forceinline__ void buf_st_row(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, const float *s) {
   static_assert(LPL >= 1 && LPL <= 4, "1 to 4 floats per lane");
   if constexpr (LPL == 1) {
-    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(s[0]), r, voff, soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(s[0]), r, voff, soff, AUX);
   } else if constexpr (LPL == 2) {
     decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)) v;
     v[0] = __float_as_uint(s[0]); v[1] = __float_as_uint(s[1]);
-    __builtin_amdgcn_raw_buffer_store_b64(v, r, voff, soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(v, r, voff, soff, AUX);
   } else if constexpr (LPL == 3) {
     decltype(__builtin_amdgcn_raw_buffer_load_b96(r, 0, 0, 0)) v;
     v[0] = __float_as_uint(s[0]); v[1] = __float_as_uint(s[1]); v[2] = __float_as_uint(s[2]);
-    __builtin_amdgcn_raw_buffer_store_b96(v, r, voff, soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b96(v, r, voff, soff, AUX);
   } else {
     decltype(__builtin_amdgcn_raw_buffer_load_b128(r, 0, 0, 0)) v;
     v[0] = __float_as_uint(s[0]); v[1] = __float_as_uint(s[1]);
     v[2] = __float_as_uint(s[2]); v[3] = __float_as_uint(s[3]);
-    __builtin_amdgcn_raw_buffer_store_b128(v, r, voff, soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b128(v, r, voff, soff, AUX);
   }
 }
 
