@@ -387,6 +387,12 @@ __device__ __forceinline__ int shl1_i(int v, int fill) {
 // HS (PAIR only, the chained-iteration flags as compile-time bits): bit 0 = in_grad's LEFT and
 // RIGHT slots are zero inside the sweep (not loaded), bit 1 = grad_msg's LEFT and RIGHT slots
 // are written at the row ends only.  The other kernels read the same flags from a.igz / a.osp.
+// cache policy of the stereo backward's row stores (h-gradients, grad_cost, grad_msg):
+// non-temporal (2), each plane being read back only after the phase has streamed its own inputs
+// (same-run A/B at 384x1248x192: vertical 0.585 -> 0.569 ms, horizontal 0.932 -> 0.901 ms)
+#ifndef BPL_BWD_ST_AUX
+#define BPL_BWD_ST_AUX 2
+#endif
 template <int LPL, bool VERT, bool FULL, int FASTK, bool PAIR, int PF, int HS = 0>
 __global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(BwdArgs a) {
   static_assert(!(PAIR && VERT), "pairing is for the horizontal phase");
@@ -485,10 +491,11 @@ __global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(B
   };
   auto st_row_b = [&](__amdgpu_buffer_rsrc_t r, uint32_t soff, const float *v) {
     if (FULL) {
-      buf_st_row<LPL>(r, (uint32_t)c0 * 4u, soff, v);
+      buf_st_row<LPL, BPL_BWD_ST_AUX>(r, (uint32_t)c0 * 4u, soff, v);
     } else {
 #pragma unroll
-      for (int k = 0; k < LPL; ++k) buf_st(r, valid[k] ? (uint32_t)(c0 + k) * 4u : kBufDrop, soff, v[k]);
+      for (int k = 0; k < LPL; ++k)
+        buf_st<BPL_BWD_ST_AUX>(r, valid[k] ? (uint32_t)(c0 + k) * 4u : kBufDrop, soff, v[k]);
     }
   };
 
