@@ -343,10 +343,11 @@ constexpr uint32_t kBufDrop = 0x7FFFFFFFu;
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, uint32_t bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(uniform_ptr(base), (short)0, (int)bytes, 0x00020000);
 }
+// AUX: the access's cache-policy bits (0: default; 2: non-temporal)
+template <int AUX = 0>
 __device__ __forceinline__ float buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
-  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, AUX));
 }
-// AUX: the store's cache-policy bits (0: default)
 template <int AUX = 0>
 __device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, float v) {
   __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, voff, soff, AUX);
@@ -356,19 +357,19 @@ __device__ __forceinline__ void buf_st8(__amdgpu_buffer_rsrc_t r, uint32_t voff,
 }
 // LPL consecutive floats per lane (one dword / dwordx2 / dwordx3 / dwordx4 access) at
 // base + voff + soff; voff must be a multiple of 4.
-template <int LPL>
+template <int LPL, int AUX = 0>
 __device__ __forceinline__ void buf_ld_row(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, float *d) {
   static_assert(LPL >= 1 && LPL <= 4, "1 to 4 floats per lane");
   if constexpr (LPL == 1) {
-    d[0] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+    d[0] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, AUX));
   } else if constexpr (LPL == 2) {
-    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, AUX);
     d[0] = __uint_as_float(v[0]); d[1] = __uint_as_float(v[1]);
   } else if constexpr (LPL == 3) {
-    const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, voff, soff, 0);
+    const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, voff, soff, AUX);
     d[0] = __uint_as_float(v[0]); d[1] = __uint_as_float(v[1]); d[2] = __uint_as_float(v[2]);
   } else {
-    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, AUX);
     d[0] = __uint_as_float(v[0]); d[1] = __uint_as_float(v[1]);
     d[2] = __uint_as_float(v[2]); d[3] = __uint_as_float(v[3]);
   }

@@ -774,12 +774,23 @@ __device__ __forceinline__ float shl1_inf(float v) {  // lane i <- lane i+1, lan
 #ifndef BPL_FWD_ST_AUX_V
 #define BPL_FWD_ST_AUX_V 2
 #endif
+// cache policy of their row loads, per phase: default.  Non-temporal loads speed the horizontal
+// phase up (0.794 -> 0.775 ms) but slow the vertical one that follows it, which reads the base
+// plane the horizontal phase has just written, by more (0.474 -> 0.503 ms; same-run A/B at
+// 384x1248x192), and slow it directly when used there (0.474 -> 0.493 ms)
+#ifndef BPL_FWD_LD_AUX_H
+#define BPL_FWD_LD_AUX_H 0
+#endif
+#ifndef BPL_FWD_LD_AUX_V
+#define BPL_FWD_LD_AUX_V 0
+#endif
 template <int LPL, bool VERT, bool FULL, bool OUTS, int PF, bool BV = false, bool PAIR = false>
 __device__ __forceinline__ void fwd_sweep_p1p2(const FwdArgs &a, int n, int dir, int rslot, int nj, int64_t q0,
                                                int64_t dq, int nsteps, int lane, int c0, const bool *valid,
                                                int Tb = 0) {
   static_assert(!(BV && !VERT) && !(PAIR && VERT), "base plane: read by vertical, written by horizontal sweeps");
   constexpr int kMsgAux = VERT ? BPL_FWD_ST_AUX_V : BPL_FWD_ST_AUX_H;  // messages' and argmins' store policy
+  constexpr int kLdAux = VERT ? BPL_FWD_LD_AUX_V : BPL_FWD_LD_AUX_H;  // row loads' policy
   constexpr int ND = (3 + LPL - 1) / LPL;
   const int H = a.H, W = a.W, C = a.C, K = a.K;
   const int64_t HW = (int64_t)H * W;
@@ -814,10 +825,10 @@ __device__ __forceinline__ void fwd_sweep_p1p2(const FwdArgs &a, int n, int dir,
   const uint32_t dpix = (uint32_t)((int)dq * 4);
   auto ld_row = [&](__amdgpu_buffer_rsrc_t r, uint32_t plane, uint32_t soff, float *d) {
     if (FULL) {
-      buf_ld_row<LPL>(r, vl[0] + plane, soff, d);
+      buf_ld_row<LPL, kLdAux>(r, vl[0] + plane, soff, d);
     } else {
 #pragma unroll
-      for (int k = 0; k < LPL; ++k) d[k] = buf_ld(r, vl[k] + plane, soff);
+      for (int k = 0; k < LPL; ++k) d[k] = buf_ld<kLdAux>(r, vl[k] + plane, soff);
     }
   };
   auto st_plane = [&](__amdgpu_buffer_rsrc_t r, uint32_t plane, uint32_t soff, const float *v) {
