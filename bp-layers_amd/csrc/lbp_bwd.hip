@@ -398,6 +398,16 @@ __device__ __forceinline__ int shl1_i(int v, int fill) {
 #ifndef BPL_BWD_LD_AUX
 #define BPL_BWD_LD_AUX 2
 #endif
+// cache policy of its argmin loads: default (non-temporal: vertical 0.553 -> 0.563 ms)
+#ifndef BPL_BWD_AM_AUX
+#define BPL_BWD_AM_AUX 0
+#endif
+// cache policy of the semantic per-pixel kernels' matrix-row loads and matrix-gradient stores:
+// default (non-temporal made every phase 8-10% slower: the two images of a wave and neighbouring
+// scanlines share those lines)
+#ifndef BPL_SEMP_AUX
+#define BPL_SEMP_AUX 0
+#endif
 template <int LPL, bool VERT, bool FULL, int FASTK, bool PAIR, int PF, int HS = 0>
 __global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(BwdArgs a) {
   static_assert(!(PAIR && VERT), "pairing is for the horizontal phase");
@@ -532,11 +542,12 @@ __global__ __launch_bounds__(PAIR ? 4 * kWave : kWave) void lbp_bwd_stereo_w64(B
   // LPL = 3 is unaligned on odd lanes, as the global form above)
   auto ld_args_b = [&](uint32_t soff, uint32_t *d) {
     if (FULL && LPL >= 2) {
-      d[0] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b16(am_rs, voff[0], soff, 0);
-      if (LPL == 3) d[1] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(am_rs, voff[0] + 2u, soff, 0);
+      d[0] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b16(am_rs, voff[0], soff, BPL_BWD_AM_AUX);
+      if (LPL == 3) d[1] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(am_rs, voff[0] + 2u, soff, BPL_BWD_AM_AUX);
     } else {
 #pragma unroll
-      for (int k = 0; k < NRAW; ++k) d[k] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(am_rs, voff[k], soff, 0);
+      for (int k = 0; k < NRAW; ++k)
+        d[k] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(am_rs, voff[k], soff, BPL_BWD_AM_AUX);
     }
   };
   auto arg_of = [&](const uint32_t *d, int k) -> int {
@@ -1440,7 +1451,7 @@ __global__ __launch_bounds__(64) void lbp_bwd_semp(BwdArgs a) {
     buf_st(ge_r, v_ge, pl * st_p, 0.f);
     const bool zr = line_ok && lab && (!IMGS || seg == 0);
     for (int rr = 0; rr < C; ++rr)
-      buf_st(gm_r, zr ? ((lpix + (uint32_t)rr) * (uint32_t)C + (uint32_t)c) * 4u : kBufDrop, pl * st_m, 0.f);
+      buf_st<BPL_SEMP_AUX>(gm_r, zr ? ((lpix + (uint32_t)rr) * (uint32_t)C + (uint32_t)c) * 4u : kBufDrop, pl * st_m, 0.f);
   }
   for (int i = c; i < SEG * RS + SEG; i += SEG) lds_r[seg][i] = 0.f;
   wave_lds_sync();
@@ -1461,7 +1472,7 @@ __global__ __launch_bounds__(64) void lbp_bwd_semp(BwdArgs a) {
     pw[i] = buf_ld(edge_r, v_e, qp);
     if (EXACT) {
 #pragma unroll
-      for (int l = 0; l < CM; ++l) pj[EXACT ? i : 0][l] = buf_ld(mat_r, v_mm[EXACT ? l : 0], qm);
+      for (int l = 0; l < CM; ++l) pj[EXACT ? i : 0][l] = buf_ld<BPL_SEMP_AUX>(mat_r, v_mm[EXACT ? l : 0], qm);
     }
   };
   auto load_step = [&](int s, int i) {
@@ -1548,14 +1559,14 @@ __global__ __launch_bounds__(64) void lbp_bwd_semp(BwdArgs a) {
           e0 += x0;
           e1 += x1;
         }
-        buf_st(gm_r, v_gm[jr], o_m, e0 + e1);
+        buf_st<BPL_SEMP_AUX>(gm_r, v_gm[jr], o_m, e0 + e1);
       }
     } else {
 #pragma unroll
       for (int rr = 0; rr < NR; ++rr) {
         float e = rr == r ? val : 0.f;
         if (own_ex && rr == 0) e += ex;
-        buf_st(gm_r, v_gm[rr], o_m, e);
+        buf_st<BPL_SEMP_AUX>(gm_r, v_gm[rr], o_m, e);
       }
     }
     o_f -= d_f;

@@ -17,6 +17,12 @@
 
 #include "bpl_common.h"
 
+// cache policy of the semantic per-pixel forward's matrix-row loads: default (non-temporal
+// made both phases 8-11% slower)
+#ifndef BPL_SEMP_AUX
+#define BPL_SEMP_AUX 0
+#endif
+
 namespace bpl {
 namespace {
 
@@ -575,7 +581,7 @@ __global__ __launch_bounds__(64) void lbp_fwd_semp(FwdArgs a) {
     p2[i] = buf_ld(x_r, v_x2, of);
     pw[i] = buf_ld(edge_r, v_e, op);
 #pragma unroll
-    for (int l = 0; l < CM; ++l) pm[i][l] = buf_ld(mat_r, v_m[l], om);
+    for (int l = 0; l < CM; ++l) pm[i][l] = buf_ld<BPL_SEMP_AUX>(mat_r, v_m[l], om);
   };
   auto load_step = [&](int s, int i) { load_at(pos(s) * st_f, pos(s) * st_p, pos(s) * st_m, i); };
 #pragma unroll
