@@ -622,7 +622,8 @@ __global__ __launch_bounds__(512) void sad_flow_fwd_tiled_k(const float *__restr
 // the reference's atomics): for stored argmins inside the window (0 <= s < K) the sources
 // landing on a pixel lie within K - 1 pixels of it, and are found as below; argmins outside
 // the window (only from block offsets) are added afterwards by sad_flow_bwd_far_k.  Terms
-// are summed per pixel in a fixed order: u-volume then v-volume, label ascending, e ascending.
+// are summed per pixel in a fixed order: u-volume then v-volume, label ascending, e ascending
+// (df1's extra landings in pieces whose partial sums are added in that order).
 
 // K <= 32.  Block = 8 x 32 pixels of one image.  Every source a pixel's terms read lies within
 // E = K - 1 - h pixels of it (in-window argmins), so the block works on a region of the tile
@@ -641,10 +642,13 @@ __global__ __launch_bounds__(512) void sad_flow_fwd_tiled_k(const float *__restr
 //      label are far from the flow they land about Poisson(1) times per target, and where the
 //      flow converges many sources share a target (the 256x512, sws 24 workload: 10.7 extra
 //      landings per pixel on average, up to 155; up to 5184 in one block), so the pool takes
-//      what the LDS budget leaves (6144 entries at sws 24).  Then, channel chunk by chunk with
+//      what the LDS budget leaves (5760 entries at sws 24).  Then, channel chunk by chunk with
 //      f0 of the region staged in LDS (in the masks' memory), the first landings are summed
-//      label by label, then the pooled ones in order (label, then argmin ascending); a thread
-//      whose landings overflow the pool adds the rest after the last chunk, channel by
+//      label by label, then the pooled ones in order (label, then argmin ascending): each
+//      thread sums the first L of its own (L = the block's pooled landings / 256, rounded up)
+//      and the rest go in pieces of L to helper threads, whose partial sums the owner adds in
+//      the pieces' order, so that no thread sums more than 2L however the landings fall; a
+//      thread whose landings overflow the pool adds the rest after the last chunk, channel by
 //      channel, found again from the argmin volume (they come last in each channel's sum).
 // The u-volume's partial sums pass through df0 / df1 (same thread).  Own terms whose argmin
 // lies outside the window read f1 from global memory in place.
@@ -659,13 +663,16 @@ int flow_bwd_union_bytes(int sws, int ch) {
   const int m = 4 * KL * FM_NT, r = 4 * ch * (FM_TY + 2 * E) * (FM_TX + 2 * E);
   return (m > r ? m : r + 15) & ~15;
 }
+// the pooled landings' pieces: one partial sum per channel of the chunk and one descriptor
+// per piece, at most one piece per thread
+int flow_bwd_piece_bytes(int ch) { return FM_NT * (4 * ch + 4); }
 // entries of the pool of extra landings (a float gradient and a packed offset each)
 int flow_bwd_pool(int sws, int ch) {
-  const int left = (FM_LDS_BUDGET - flow_bwd_union_bytes(sws, ch)) / 8;
+  const int left = (FM_LDS_BUDGET - flow_bwd_union_bytes(sws, ch) - flow_bwd_piece_bytes(ch)) / 8;
   return left > 4 * FM_NT ? left : 4 * FM_NT;
 }
 size_t flow_bwd_lds_bytes(int sws, int ch) {
-  return (size_t)flow_bwd_union_bytes(sws, ch) + 8 * (size_t)flow_bwd_pool(sws, ch);
+  return (size_t)flow_bwd_union_bytes(sws, ch) + flow_bwd_piece_bytes(ch) + 8 * (size_t)flow_bwd_pool(sws, ch);
 }
 
 // region rows of nc channel planes (buffer resource over them) -> reg[CH][RR][RC]; a wave
@@ -707,7 +714,9 @@ __global__ __launch_bounds__(FM_NT) void sad_flow_bwd_lds_k(const float *__restr
   // the masks and the staged region share the first ubytes (never live together)
   uint32_t *mask = fm_lds;                                                 // [KL][FM_NT]
   float *reg = reinterpret_cast<float *>(fm_lds);                          // [CH][RR][RC]
-  float *xg = reinterpret_cast<float *>(fm_lds + ubytes / 4);              // pool gradients [pool]
+  float *psum = reinterpret_cast<float *>(fm_lds + ubytes / 4);            // piece partials [FM_NT][CH]
+  uint32_t *desc = reinterpret_cast<uint32_t *>(psum + FM_NT * CH);        // piece descriptors [FM_NT]
+  float *xg = reinterpret_cast<float *>(desc + FM_NT);                     // pool gradients [pool]
   // pool offsets: v+32 | (u+32) << 6 | k << 12, then the source's region offset
   uint32_t *xo = reinterpret_cast<uint32_t *>(xg + pool);
   const int t = threadIdx.x;
@@ -850,6 +859,7 @@ __global__ __launch_bounds__(FM_NT) void sad_flow_bwd_lds_k(const float *__restr
     }
     const int x0p = scan[t] - nx;                     // first pool entry of the thread
     const int nfit = min(nx, max(0, pool - x0p));     // of its landings in the pool
+    const int npool = min(scan[FM_NT - 1], pool);     // pooled landings of the block
     __syncthreads();
     // the extra landings into the pool in order (label, then argmin ascending)
     {
@@ -886,14 +896,35 @@ __global__ __launch_bounds__(FM_NT) void sad_flow_bwd_lds_k(const float *__restr
           xo[x0p + i0 + q] = (uint32_t)o4[q];
         }
     }
+    // Pieces: a thread sums the first L of its pooled landings itself and hands the rest, in
+    // pieces of L, to the threads numbered by the pieces' order; with L = ceil(npool / FM_NT)
+    // the pieces number fewer than npool / L <= FM_NT (one per thread at most), and no thread
+    // sums more than 2L pooled landings per chunk, however unevenly they fall on the pixels.
+    // Piece descriptors: first pool entry | length << 16 | owner thread << 24.
+    const int L = max(1, (npool + FM_NT - 1) / FM_NT);
+    const int nown = min(nfit, L);
+    const int npc = (nfit - nown + L - 1) / L;
+    int *pscan = reinterpret_cast<int *>(psum);
+    pscan[t] = npc;
+    __syncthreads();
+    for (int d = 1; d < FM_NT; d <<= 1) {
+      const int add = t >= d ? pscan[t - d] : 0;
+      __syncthreads();
+      pscan[t] += add;
+      __syncthreads();
+    }
+    const int ps = pscan[t] - npc;         // the thread's first piece
+    const int npieces = pscan[FM_NT - 1];  // the block's pieces
+    for (int i = 0; i < npc; ++i)
+      desc[ps + i] = (uint32_t)(x0p + nown + i * L) | (uint32_t)min(L, nfit - nown - i * L) << 16 | (uint32_t)t << 24;
     for (int c0 = 0; c0 < C; c0 += CH) {
       const int nc = min(CH, C - c0);
-      __syncthreads();
+      __syncthreads();  // the previous chunk's region and partials are read; descriptors written
       stage_region(reg, make_rsrc(f0 + ((int64_t)n * C + c0) * HW, (uint32_t)(nc * HW * 4)), nc, CH * RR, RR, RC, ry0,
                    rx0, H, W, t);
       __syncthreads();
+      float b[CH] = {}, r1[CH] = {};
       if (inside) {
-        float b[CH], r1[CH];
 #pragma unroll
         for (int j = 0; j < CH; ++j) {
           const int64_t o = ((int64_t)n * C + c0 + min(j, nc - 1)) * HW + pix;
@@ -909,11 +940,37 @@ __global__ __launch_bounds__(FM_NT) void sad_flow_bwd_lds_k(const float *__restr
             for (int j = 0; j < CH; ++j) r1[j] -= sgn_eps(reg[j * RN + off] - b[j]) * g;
           }
         }
-        for (int i = 0; i < nfit; ++i) {
+        for (int i = 0; i < nown; ++i) {
           const int off = (int)xo[x0p + i];
           const float g = xg[x0p + i];
 #pragma unroll
           for (int j = 0; j < CH; ++j) r1[j] -= sgn_eps(reg[j * RN + off] - b[j]) * g;
+        }
+      }
+      if (t < npieces) {  // the piece numbered t, for its owner's pixel
+        const uint32_t dd = desc[t];
+        const int s0 = (int)(dd & 0xFFFFu), len = (int)((dd >> 16) & 0xFFu), o = (int)(dd >> 24);
+        const int64_t po = (int64_t)(y0 + o / FM_TX) * W + (x0 + o % FM_TX);
+        float bo[CH], pp[CH];
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+          bo[j] = f1[((int64_t)n * C + c0 + min(j, nc - 1)) * HW + po];
+          pp[j] = 0.f;
+        }
+        for (int i = 0; i < len; ++i) {
+          const int off = (int)xo[s0 + i];
+          const float g = xg[s0 + i];
+#pragma unroll
+          for (int j = 0; j < CH; ++j) pp[j] += sgn_eps(reg[j * RN + off] - bo[j]) * g;
+        }
+#pragma unroll
+        for (int j = 0; j < CH; ++j) psum[t * CH + j] = pp[j];
+      }
+      __syncthreads();  // the partials are written
+      if (inside) {
+        for (int q = ps; q < ps + npc; ++q) {
+#pragma unroll
+          for (int j = 0; j < CH; ++j) r1[j] -= psum[q * CH + j];
         }
 #pragma unroll
         for (int j = 0; j < CH; ++j)
