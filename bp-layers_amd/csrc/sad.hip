@@ -702,7 +702,9 @@ __device__ __forceinline__ void stage_region(float *reg, __amdgpu_buffer_rsrc_t 
   }
 }
 
-template <int CH>
+// KM >= K: the size of the per-label register arrays (16, 25 or 32: fewer registers for
+// smaller windows)
+template <int CH, int KM>
 __global__ __launch_bounds__(FM_NT) void sad_flow_bwd_lds_k(const float *__restrict__ f0, const float *__restrict__ f1,
                                                              const float *__restrict__ gu, const float *__restrict__ gv,
                                                              const float *__restrict__ us, const float *__restrict__ vs,
@@ -739,11 +741,11 @@ __global__ __launch_bounds__(FM_NT) void sad_flow_bwd_lds_k(const float *__restr
   for (int part = 0; part < 2; ++part) {
     const __amdgpu_buffer_rsrc_t rst = make_rsrc((part ? vs : us) + img * K, vb);
     const __amdgpu_buffer_rsrc_t rgr = make_rsrc((part ? gv : gu) + img * K, vb);
-    int toff[FM_KMAX];
-    float tg[FM_KMAX];
+    int toff[KM];
+    float tg[KM];
     bool far = false;
 #pragma unroll
-    for (int k = 0; k < FM_KMAX; ++k) {
+    for (int k = 0; k < KM; ++k) {
       const uint32_t o = inside && k < KL ? (uint32_t)((ipix * K + k) * 4) : kBufDrop;
       const int sv = (int)buf_ld(rst, o, 0);
       const float g = buf_ld(rgr, o, 0);
@@ -771,7 +773,7 @@ __global__ __launch_bounds__(FM_NT) void sad_flow_bwd_lds_k(const float *__restr
           r0[j] = part ? df0[o] : 0.f;
         }
 #pragma unroll
-        for (int k = 0; k < FM_KMAX; ++k) {
+        for (int k = 0; k < KM; ++k) {
           const int off = toff[k];
           const float g = tg[k];
           if (off >= 0) {
@@ -833,11 +835,11 @@ __global__ __launch_bounds__(FM_NT) void sad_flow_bwd_lds_k(const float *__restr
     }
     __syncthreads();
     // first landing per label into registers; count the others
-    int foff[FM_KMAX];
-    float fg[FM_KMAX];
+    int foff[KM];
+    float fg[KM];
     int nx = 0;  // landings past the first of their label
 #pragma unroll
-    for (int k = 0; k < FM_KMAX; ++k) {
+    for (int k = 0; k < KM; ++k) {
       const uint32_t m = inside && k < KL ? mask[k * FM_NT + t] : 0u;
       const int e = (m != 0u ? __builtin_ctz(m) : hs) - hs;
       const int u = part ? e : k - hs, v = part ? k - hs : e;
@@ -932,7 +934,7 @@ __global__ __launch_bounds__(FM_NT) void sad_flow_bwd_lds_k(const float *__restr
           r1[j] = part ? df1[o] : 0.f;
         }
 #pragma unroll
-        for (int k = 0; k < FM_KMAX; ++k) {
+        for (int k = 0; k < KM; ++k) {
           const int off = foff[k];
           if (off >= 0) {
             const float g = fg[k];
@@ -1261,15 +1263,23 @@ extern "C" int bpl_sad_flow_bwd(const float *f0, const float *f1, const float *g
   if (masks) {
     const size_t shm = flow_bwd_lds_bytes(sws, FM_CH);
     const int ubytes = flow_bwd_union_bytes(sws, FM_CH), pool = flow_bwd_pool(sws, FM_CH);
-    if (shm > 64 * 1024) {
-      const int rc = check_hip(hipFuncSetAttribute((const void *)sad_flow_bwd_lds_k<FM_CH>,
-                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm),
-                               "bpl_sad_flow_bwd: raising the LDS limit");
-      if (rc != BPL_OK) return rc;
-    }
-    dim3 grid((unsigned)((W + FM_TX - 1) / FM_TX), (unsigned)((H + FM_TY - 1) / FM_TY), (unsigned)N);
-    hipLaunchKernelGGL(sad_flow_bwd_lds_k<FM_CH>, grid, dim3(FM_NT), shm, st, f0, f1, grad_u, grad_v, u_star, v_star,
-                       df0, df1, (int)C, (int)H, (int)W, sws, ubytes, pool);
+    const dim3 grid((unsigned)((W + FM_TX - 1) / FM_TX), (unsigned)((H + FM_TY - 1) / FM_TY), (unsigned)N);
+    auto go = [&](auto km) -> int {
+      constexpr int KM = decltype(km)::value;
+      if (shm > 64 * 1024) {
+        const int rc = check_hip(hipFuncSetAttribute((const void *)sad_flow_bwd_lds_k<FM_CH, KM>,
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm),
+                                 "bpl_sad_flow_bwd: raising the LDS limit");
+        if (rc != BPL_OK) return rc;
+      }
+      hipLaunchKernelGGL((sad_flow_bwd_lds_k<FM_CH, KM>), grid, dim3(FM_NT), shm, st, f0, f1, grad_u, grad_v, u_star,
+                         v_star, df0, df1, (int)C, (int)H, (int)W, sws, ubytes, pool);
+      return BPL_OK;
+    };
+    const int rc = sws + 1 <= 16   ? go(std::integral_constant<int, 16>())
+                   : sws + 1 <= 25 ? go(std::integral_constant<int, 25>())
+                                   : go(std::integral_constant<int, 32>());
+    if (rc != BPL_OK) return rc;
   } else {
     hipLaunchKernelGGL(sad_flow_bwd_search_k, dim3(grid_for(N * H * W, FB_THREADS)), dim3(FB_THREADS), 0, st, f0, f1,
                        grad_u, grad_v, u_star, v_star, df0, df1, (int)N, (int)C, (int)H, (int)W, sws);
