@@ -316,15 +316,6 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
 }
 
-// Workgroups of one launch are dealt to the 8 XCDs round-robin (workgroup b runs on XCD
-// b mod 8).  The remap gives each XCD a contiguous range of logical indices, so that
-// neighbouring scanlines, which share cache lines of per-pixel parameters, share one L2.
-__device__ __forceinline__ int xcd_remap(int b, int nb) {
-  const int q = nb >> 3, r = nb & 7;
-  const int x = b & 7, i = b >> 3;
-  return x * q + min(x, r) + i;
-}
-
 // Uniform (SGPR) copy of a wave-uniform 64-bit pointer: a buffer resource built from it is
 // then known to be uniform (otherwise every buffer access becomes a readfirstlane loop).
 __device__ __forceinline__ void *uniform_ptr(const void *p) {

@@ -253,8 +253,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void s
   __shared__ float G1[SB_DC * SB_GP];      // [k - d0][x]
   __shared__ float F1[SB_CB * SB_SPAN];    // [channel][j]: f1 at x0 - (mind + d0 + SB_DC - 1) + j
   __shared__ float F0[SB_CB * SB_SPAN];    // [channel][j]: f0 at x0 + mind + d0 + j
-  const int x0 = blockIdx.x * SB_TX;
-  const int y = blockIdx.y;
+  // rows go to the XCDs in contiguous bands: the f0 and f1 columns that neighbouring tiles of a
+  // row both stage are read through one L2
+  const int gx = (int)gridDim.x;
+  const int tb = xcd_contiguous_block((int)blockIdx.x + gx * (int)blockIdx.y, gx * (int)gridDim.y);
+  const int x0 = (tb % gx) * SB_TX;
+  const int y = tb / gx;
   const int n = blockIdx.z;
   const int t = threadIdx.x;
   const int64_t HW = (int64_t)H * W;
@@ -384,8 +388,12 @@ __global__ __launch_bounds__(512) void sad_flow_fwd_k(const float *__restrict__ 
   auto m_v = reinterpret_cast<float (*)[NP][K]>(lds);
   auto m_i = reinterpret_cast<unsigned char (*)[NP][K]>(lds + sizeof(float) * (G - 1) * NP * K);
   const int hs = sws / 2;
-  const int x0 = blockIdx.x * FL_TX;
-  const int y0 = blockIdx.y * FL_TY;
+  // tiles go to the XCDs in contiguous ranges (bands of tile rows): the f1 rows and halo
+  // columns that neighbouring tiles both stage are read through one L2
+  const int gx = (int)gridDim.x;
+  const int tb = xcd_contiguous_block((int)blockIdx.x + gx * (int)blockIdx.y, gx * (int)gridDim.y);
+  const int x0 = (tb % gx) * FL_TX;
+  const int y0 = (tb / gx) * FL_TY;
   const int n = blockIdx.z;
   const int t = threadIdx.x;
   const int g = t / (FL_TY * FL_TX);
@@ -723,7 +731,11 @@ __global__ __launch_bounds__(FM_NT) void sad_flow_bwd_lds_k(const float *__restr
   uint32_t *xo = reinterpret_cast<uint32_t *>(xg + pool);
   const int t = threadIdx.x;
   const int ty = t / FM_TX, tx = t - ty * FM_TX;
-  const int x0 = blockIdx.x * FM_TX, y0 = blockIdx.y * FM_TY, n = blockIdx.z;
+  // tiles go to the XCDs in contiguous ranges (bands of tile rows), so that the halo rows
+  // that neighbouring tiles both stage are read through one L2
+  const int gx = (int)gridDim.x;
+  const int tb = xcd_contiguous_block((int)blockIdx.x + gx * (int)blockIdx.y, gx * (int)gridDim.y);
+  const int x0 = (tb % gx) * FM_TX, y0 = (tb / gx) * FM_TY, n = blockIdx.z;
   const int y = y0 + ty, x = x0 + tx;
   const int ry0 = y0 - E, rx0 = x0 - E;  // region origin
   const bool inside = y < H && x < W;
