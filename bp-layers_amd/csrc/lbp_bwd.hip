@@ -2099,12 +2099,13 @@ extern "C" int bpl_lbp_bwd_noalloc(int kind, int mode, const float *jump, int64_
 
 }  // namespace
 
-// Prefetch depth 12: the paired kernel has one wave per SIMD, so registers are spare, and a
-// shallower ring leaves its steps waiting on memory (384x1248x192: 1.205 ms per launch with 4
-// steps, 1.147 with 8, 1.084 with 12).  HS carries the chained-iteration flags.
+// Prefetch depth 8: the paired kernel has one wave per SIMD, so registers are spare; with
+// default-policy row loads a shallower ring left its steps waiting on memory (384x1248x192:
+// 1.205 ms per launch with 4 steps, 1.147 with 8, 1.084 with 12), with non-temporal ones 8 and
+// 12 steps are within 0.5% (8 ahead) and 16 is slower.  HS carries the chained-iteration flags.
 template <int LPL>
 void launch_bwd_pair(const BwdArgs &a, bool full, int fk, int ngroups, hipStream_t st) {
-  constexpr int PFH = 12;
+  constexpr int PFH = 8;
   const dim3 grid((unsigned)(ngroups * ((a.H + 1) / 2)));  // two rows per workgroup
   const int hs = (a.igz ? 1 : 0) | (a.osp ? 2 : 0);
   auto go = [&](auto full_c, auto fk_c) {
@@ -2136,8 +2137,8 @@ void launch_bwd_pair(const BwdArgs &a, bool full, int fk, int ngroups, hipStream
 }
 template void launch_bwd_pair<BPL_BWD_PAIR_LPL>(const BwdArgs &, bool, int, int, hipStream_t);
 
-// Unpaired horizontal adjoints (one wave per row and pass): the fast step with the paired
-// kernel's 12-step ring (one wave per SIMD here too), the general step with 4.
+// Unpaired horizontal adjoints (one wave per row and pass): the fast step with a 12-step ring
+// (one wave per SIMD here too), the general step with 4.
 template <int LPL>
 void launch_bwd_h_unpaired(const BwdArgs &a, bool full, int fk, int ngroups, hipStream_t st) {
   const dim3 grid((unsigned)(ngroups * 2 * a.H));
