@@ -783,7 +783,8 @@ __device__ __forceinline__ float shl1_inf(float v) {  // lane i <- lane i+1, lan
 // cache policy of their row loads, per phase: default.  Non-temporal loads speed the horizontal
 // phase up (0.794 -> 0.775 ms) but slow the vertical one that follows it, which reads the base
 // plane the horizontal phase has just written, by more (0.474 -> 0.503 ms; same-run A/B at
-// 384x1248x192), and slow it directly when used there (0.474 -> 0.493 ms)
+// 384x1248x192), and slow it directly when used there (0.474 -> 0.493 ms); for the cost plane
+// alone: horizontal 0.795 -> 0.789 ms, vertical 0.475 -> 0.478 ms, 0.1% per step (not kept)
 #ifndef BPL_FWD_LD_AUX_H
 #define BPL_FWD_LD_AUX_H 0
 #endif
